@@ -1,0 +1,198 @@
+/*
+ * lbvh.h -- C ABI of the MI355X LBVH builder (liblbvh.so).
+ *
+ * Drop-in for the scripts/bvh build path of AddisonPrairie/WebGPU-LVBH-demo.
+ * Each entry point below names the reference interface it replaces.  All
+ * arguments are plain pointers and sizes: `const float *tri` etc. are DEVICE
+ * pointers (from lbvh_alloc or any hipMalloc on the context's device) unless
+ * the comment says "host".  Work is enqueued on the context's HIP stream and
+ * returns immediately; call lbvh_synchronize() (or lbvh_download, which
+ * synchronises) before reading results on the host.
+ *
+ * Buffer layouts are the reference's, so the demo's path tracer reads the
+ * outputs unchanged:
+ *   triangle   48 B  v0.xyz pad | v1.xyz pad | v2.xyz pad
+ *                    (aabb-z-idx.js:153-157; filled at build-bvh.js:14-21)
+ *   AABB       32 B  min.xyz pad | max.xyz pad          (aabb-z-idx.js:159-162)
+ *   key        u32   30-bit Morton code                  (aabb-z-idx.js:197-208)
+ *   index      i32   sorted position -> input triangle  (radix-sort.js:499)
+ *   parents    i32[2N]  [0,N-1) internal, [N,2N) leaves (radix-tree.js:146-156)
+ *   BVH node   64 B  l_min.xyz l_child | l_max.xyz f_1 |
+ *                    r_min.xyz r_child | r_max.xyz f_2   (bvh-up-pass.js:161-178,
+ *                    pathtracer.js:441-450).  Root = node 0; child >= 0 is an
+ *                    internal node, child < 0 is leaf -(child+1) in O_TRI;
+ *                    node N-1 is all zero.
+ * Children are written in Karras order (l = the split's left child, f_1 = 2,
+ * f_2 = 0); the reference's slot order is thread-arrival order
+ * (bvh-up-pass.js:230-252) and the path tracer reads both slots alike
+ * (pathtracer.js:526-565).
+ *
+ * Every function returns an lbvh_status; details via lbvh_last_error().
+ * Nothing throws across this boundary.
+ */
+#ifndef LBVH_H
+#define LBVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LBVH_OK = 0,
+  LBVH_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad mode, n too large */
+  LBVH_ERR_SIZE_MISMATCH = 2,    /* buffer size does not match n */
+  LBVH_ERR_OUT_OF_BUDGET = 3,    /* request exceeds the context's HBM budget */
+  LBVH_ERR_OUT_OF_MEMORY = 4,    /* the device allocator failed */
+  LBVH_ERR_DEVICE = 5,           /* a HIP runtime call failed */
+  LBVH_ERR_NO_DEVICE = 6,        /* no such device ordinal */
+  LBVH_ERR_BUSY = 7,             /* serving queue full / no free slot */
+  LBVH_ERR_NOT_FOUND = 8         /* unknown handle / request id */
+} lbvh_status;
+
+/* Morton division mode for rel = (cen - min) / (max - min)
+ * (aabb-z-idx.js:192).  The reference's result depends on how the WebGPU
+ * stack lowers fp32 division; both lowerings are provided. */
+typedef enum {
+  LBVH_DIV_MUL_RCP = 0, /* d * fl(1/e)  (fdiv lowered to fmul + frcp) */
+  LBVH_DIV_IEEE = 1     /* correctly rounded d / e */
+} lbvh_div_mode;
+
+/* Largest triangle count of one build: keys carry 30 bits, indices are i32
+ * and the sort's look-back counts are 32-bit. */
+#define LBVH_MAX_TRIS (1u << 30)
+
+typedef struct lbvh_ctx lbvh_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Replaces initBVHBuild(device) (build-bvh.js:1-10) and the per-stage
+ * init*Kernel(device) calls: one context owns a HIP stream, the scratch
+ * arena the stages share, and an HBM budget (bytes; 0 = no limit) that
+ * caps lbvh_alloc plus scratch. */
+int lbvh_create(int device_ordinal, uint64_t hbm_budget_bytes,
+                lbvh_ctx **out_ctx);
+int lbvh_destroy(lbvh_ctx *ctx);
+const char *lbvh_last_error(const lbvh_ctx *ctx);
+const char *lbvh_status_string(int status);
+int lbvh_set_div_mode(lbvh_ctx *ctx, int div_mode);
+int lbvh_get_div_mode(const lbvh_ctx *ctx);
+/* The context's hipStream_t, for callers that record events or chain work. */
+void *lbvh_stream(lbvh_ctx *ctx);
+int lbvh_synchronize(lbvh_ctx *ctx);
+/* Bytes currently charged to the budget (allocations + scratch). */
+uint64_t lbvh_bytes_in_use(const lbvh_ctx *ctx);
+/* Releases the scratch arena (it regrows on the next call that needs it). */
+int lbvh_trim(lbvh_ctx *ctx);
+
+/* ---- device buffers (GPUDevice.createBuffer / GPUBuffer.destroy) -------- */
+
+/* Replaces device.createBuffer({size}) (e.g. build-bvh.js:14-18): device
+ * memory charged to the context's budget. */
+int lbvh_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out_dptr);
+/* Replaces GPUBuffer.destroy() (build-bvh.js:59-63). */
+int lbvh_free(lbvh_ctx *ctx, void *dptr);
+/* Host -> device copy (the mappedAtCreation upload, build-bvh.js:20-21).
+ * Synchronous with respect to the host buffer. */
+int lbvh_upload(lbvh_ctx *ctx, void *dst_dev, const void *src_host,
+                uint64_t bytes);
+/* Device -> host copy after all queued work (the reference never reads
+ * back; tests and callers that want the bytes do).  Synchronises. */
+int lbvh_download(lbvh_ctx *ctx, void *dst_host, const void *src_dev,
+                  uint64_t bytes);
+
+/* ---- per-stage entry points (the execute() shims) ----------------------- */
+
+/* Scene bounds: fp32 min/max over every vertex coordinate, written to
+ * bounds_out[0..5] (device) as min.xyz, max.xyz.  The reference takes
+ * bounds from the caller (parse-obj.js:10-53 -> aabb-z-idx.js:113-124);
+ * lbvh_build uses this reduction when the caller passes none. */
+int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                float *bounds_out);
+
+/* Replaces initAABB_ZidxKernel(device).execute(TRI, N, bounds)
+ * (aabb-z-idx.js:57-148, WGSL :177-208): per-triangle AABB (32 B) and
+ * 30-bit Morton key of the AABB centre.  bounds_host = 6 host floats
+ * (min.xyz, max.xyz) or NULL to use the GPU bounds reduction. */
+int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                     const float *bounds_host, float *aabb_out,
+                     uint32_t *keys_out);
+
+/* Replaces initRadixSortKernel(device).execute(keyBuf, N)
+ * (radix-sort.js:161-500): stable LSD sort of u32 keys.  Destructive like
+ * the reference: the sorted keys replace `keys`; idx_out[j] is the input
+ * position of the key now at j. */
+int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n);
+
+/* Replaces initRadixTreeKernel(device).execute(sortedKeyBuf, N)
+ * (radix-tree.js:50-120, WGSL :137-238): Karras radix tree over the sorted
+ * keys, ties broken by sorted position; writes the 2N-entry parents array
+ * (slots 0 and N-1 are 0). */
+int lbvh_radix_tree(lbvh_ctx *ctx, const uint32_t *sorted_keys, uint32_t n,
+                    int32_t *parents_out);
+
+/* Replaces initBVHUpPassKernel(device).execute(IDX, AABB, PARENT, N)
+ * (bvh-up-pass.js:70-156, WGSL :200-280): bottom-up node boxes; writes
+ * N 64-byte nodes. */
+int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
+                 const int32_t *parents, uint32_t n, void *bvh_out);
+
+/* Replaces initRearrangeKernel(device).execute(I_TRI, IDX, N)
+ * (rearrange.js:57-139, WGSL :162-169): tri_out[i] = tri_in[idx[i]]. */
+int lbvh_rearrange(lbvh_ctx *ctx, const float *tri_in, const int32_t *idx,
+                   uint32_t n, float *tri_out);
+
+/* ---- fused build and refit --------------------------------------------- */
+
+/* Replaces build(TRI_ARRAY, NUM_TRIS, MODEL_BOUNDS) (build-bvh.js:12-66)
+ * once the triangles are on the device: bvh_out (64*N bytes) and
+ * tri_out (48*N bytes) receive BVH_BUFFER and O_TRIANGLE_BUFFER.
+ * bounds_host = 6 host floats (MODEL_BOUNDS rounded to fp32) or NULL for
+ * the GPU bounds reduction.  The reference's intermediates (AABB, keys,
+ * index, parents; freed at build-bvh.js:59-63) live in the context's
+ * scratch; pass sorted_keys_out / perm_out (4*N bytes each, or NULL) to keep
+ * the sorted keys and permutation a later lbvh_refit needs. */
+int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
+               const float *bounds_host, void *bvh_out, float *tri_out,
+               uint32_t *sorted_keys_out, int32_t *perm_out);
+
+/* Refit: no counterpart in the reference, which rebuilds from scratch; it
+ * re-runs the up pass (bvh-up-pass.js:200-280) and rearrange
+ * (rearrange.js:162-169) over new vertex positions `tri` (input order,
+ * same N) with the topology of an earlier lbvh_build kept fixed through its
+ * sorted_keys and perm. */
+int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
+               const uint32_t *sorted_keys, const int32_t *perm,
+               void *bvh_out, float *tri_out);
+
+/* ---- profiling ------------------------------------------------------------
+ * No counterpart in the reference (which times build() with Date.now() at
+ * main.js:87-90).  When enabled, lbvh_build and lbvh_refit record a HIP event
+ * pair around each stage on the context's stream; lbvh_profile_read waits
+ * for them and returns the accumulated device time per stage. */
+typedef enum {
+  LBVH_STAGE_BOUNDS = 0,
+  LBVH_STAGE_MORTON = 1, /* leaf AABB + Morton key (+ digit histograms) */
+  LBVH_STAGE_SORT0 = 2,  /* onesweep digit passes 0..3 */
+  LBVH_STAGE_SORT1 = 3,
+  LBVH_STAGE_SORT2 = 4,
+  LBVH_STAGE_SORT3 = 5,
+  LBVH_STAGE_TREE = 6,   /* fused leaf gather + radix tree + node boxes */
+  LBVH_STAGE_COUNT = 7
+} lbvh_stage;
+
+typedef struct {
+  double ms[LBVH_STAGE_COUNT];        /* summed device milliseconds */
+  uint64_t launches[LBVH_STAGE_COUNT];
+} lbvh_profile;
+
+int lbvh_profile_enable(lbvh_ctx *ctx, int enable);
+int lbvh_profile_read(lbvh_ctx *ctx, lbvh_profile *out, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBVH_H */

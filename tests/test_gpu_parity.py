@@ -1,0 +1,277 @@
+"""GPU parity: every C-ABI entry point against the CPU oracle, bit-exact.
+
+Integer outputs (keys, permutation, parents, child ids) and the fp32 boxes
+(exact min/max) must match to the bit; the BVH is compared in the canonical
+child order both sides write.  Sizes stay where the oracle finishes in
+seconds, plus one full-size (16,777,216-triangle) comparison.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def sort_keys(rng, n, bits=30, dup=0.3):
+    keys = rng.integers(0, 2 ** bits, size=n, dtype=np.uint64).astype(np.uint32)
+    if n > 1:
+        m = rng.random(n) < dup
+        keys[m] = keys[rng.integers(0, n, size=int(m.sum()))]
+    return keys
+
+
+def mesh_cases(lb):
+    rng = np.random.default_rng(7)
+    zeros = rng.uniform(-1, 1, size=(3000, 12)).astype(np.float32)
+    zeros[:, 3::4] = np.float32(3.1415)
+    mask = rng.random((3000, 12)) < 0.2
+    mask[:, 3::4] = False
+    zeros[mask] = 0.0
+    same = np.tile(lb.wl_uniform(9, 1, 0.01), (777, 1))  # all keys equal
+    return {
+        "n1": lb.wl_uniform(21, 1, 0.01),
+        "n2": lb.wl_uniform(22, 2, 0.01),
+        "n3": lb.wl_uniform(23, 3, 0.01),
+        "n5": lb.wl_uniform(24, 5, 0.01),
+        "n255": lb.wl_uniform(25, 255, 0.01),
+        "n256": lb.wl_uniform(26, 256, 0.01),
+        "n257": lb.wl_uniform(27, 257, 0.01),
+        "n4097": lb.wl_clustered(28, 4097, 0.3),
+        "zeros3000": zeros,
+        "allsame777": same,
+        "clustered100k": lb.wl_clustered(29, 100003, 0.05),
+        "uniform-64k": lb.workload("uniform-64k"),
+    }
+
+
+# ---------------------------------------------------------------- stages
+
+@pytest.mark.parametrize("n", [1, 2, 1000, 65536, 1000003])
+def test_bounds(ctx, lb, ora, n):
+    say(f"bounds n={n}")
+    tri = lb.wl_clustered(n, n, 0.05)
+    d = ctx.to_device(tri)
+    out = ctx.alloc(24)
+    ctx.bounds(d, n, out)
+    got = out.read(np.float32, 6)
+    assert np.array_equal(got, ora.bounds(tri))
+    d.destroy(); out.destroy()
+
+
+@pytest.mark.parametrize("div_mode", [0, 1])
+@pytest.mark.parametrize("host_bounds", [True, False])
+def test_aabb_morton(ctx, lb, ora, div_mode, host_bounds):
+    for name, tri in mesh_cases(lb).items():
+        say(f"aabb_morton {name} div={div_mode} host_bounds={host_bounds}")
+        n = len(tri)
+        b = lb.wl_bounds(tri)
+        ctx.set_div_mode(div_mode)
+        d = ctx.to_device(tri)
+        aabb = ctx.alloc(32 * n)
+        keys = ctx.alloc(4 * n)
+        ctx.aabb_morton(d, n, b if host_bounds else None, aabb, keys)
+        ga = aabb.read(np.float32, 8 * n).reshape(n, 8)
+        gk = keys.read(np.uint32, n)
+        wa, wk = ora.aabb_z_idx(tri, b, div_mode)
+        assert np.array_equal(gk, wk), name
+        assert np.array_equal(ga[:, [0, 1, 2, 4, 5, 6]], wa[:, [0, 1, 2, 4, 5, 6]]), name
+        for x in (d, aabb, keys):
+            x.destroy()
+    ctx.set_div_mode(0)
+
+
+def test_aabb_morton_caller_bounds_narrower_than_mesh(ctx, lb, ora):
+    """OBJ-style input: bounds exclude two huge floor triangles
+    (parse-obj.js:65-78), whose quantised coordinates saturate."""
+    tri = lb.wl_uniform(31, 5000, 0.01)
+    b = lb.wl_bounds(tri)
+    floor = np.array([[-1e4, -1e4, 0.01, 3.1415, 1e4, -1e4, 0.01, 3.1415, -1e4, 1e4, 0.01, 3.1415],
+                      [1e4, -1e4, 0.01, 3.1415, 1e4, 1e4, 0.01, 3.1415, -1e4, 1e4, 0.01, 3.1415]],
+                     np.float32)
+    tri = np.concatenate([tri, floor])
+    n = len(tri)
+    for mode in (0, 1):
+        ctx.set_div_mode(mode)
+        bvh, otri, sk, pm = ctx.build_arrays(tri, b, keep_topology=True)
+        wb, wo, wsk, wpm = ora.build(tri, b, mode)
+        assert np.array_equal(sk, wsk) and np.array_equal(pm, wpm)
+        assert np.array_equal(bvh, wb)
+    ctx.set_div_mode(0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65536, 1000003, 4194304])
+def test_sort_is_stable(ctx, n):
+    say(f"sort n={n}")
+    rng = np.random.default_rng(n)
+    keys = sort_keys(rng, n)
+    dk = ctx.to_device(keys)
+    di = ctx.alloc(4 * n)
+    ctx.sort(dk, di, n)
+    gk = dk.read(np.uint32, n)
+    gi = di.read(np.int32, n)
+    want = np.argsort(keys, kind="stable")
+    assert np.array_equal(gi, want)
+    assert np.array_equal(gk, keys[want])
+    dk.destroy(); di.destroy()
+
+
+def test_sort_heavy_duplicates_and_constant_digits(ctx):
+    for n, keys in [(300000, np.zeros(300000, np.uint32)),
+                    (300000, np.full(300000, 0x3FFFFFFF, np.uint32)),
+                    (123457, (np.arange(123457, dtype=np.uint32) % 3) << 24),
+                    (200000, np.arange(200000, dtype=np.uint32)[::-1].copy())]:
+        say(f"sort special n={n}")
+        dk = ctx.to_device(keys)
+        di = ctx.alloc(4 * n)
+        ctx.sort(dk, di, n)
+        want = np.argsort(keys, kind="stable")
+        assert np.array_equal(di.read(np.int32, n), want)
+        assert np.array_equal(dk.read(np.uint32, n), keys[want])
+        dk.destroy(); di.destroy()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 100, 4097, 250000])
+def test_radix_tree(ctx, ora, n):
+    say(f"radix_tree n={n}")
+    rng = np.random.default_rng(n + 5)
+    keys = np.sort(sort_keys(rng, n, bits=int(rng.integers(2, 31))))
+    dk = ctx.to_device(keys)
+    dp = ctx.alloc(8 * n)
+    ctx.radix_tree(dk, n, dp)
+    assert np.array_equal(dp.read(np.int32, 2 * n), ora.radix_tree(keys))
+    dk.destroy(); dp.destroy()
+
+
+def test_stage_pipeline_matches_oracle(ctx, lb, ora):
+    """The reference's stage sequence (build-bvh.js:24-56) through the
+    per-stage entry points."""
+    for name, tri in mesh_cases(lb).items():
+        say(f"stage pipeline {name}")
+        n = len(tri)
+        b = lb.wl_bounds(tri)
+        d = ctx.to_device(tri)
+        aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+        idx, par = ctx.alloc(4 * n), ctx.alloc(8 * n)
+        bvh, otri = ctx.alloc(64 * n), ctx.alloc(48 * n)
+        ctx.aabb_morton(d, n, b, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.rearrange(d, idx, n, otri)
+        wb, wo, wsk, wpm = ora.build(tri, b, 0)
+        assert np.array_equal(keys.read(np.uint32, n), wsk), name
+        assert np.array_equal(idx.read(np.int32, n), wpm), name
+        assert np.array_equal(par.read(np.int32, 2 * n), ora.radix_tree(wsk)), name
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb), name
+        assert np.array_equal(otri.read(np.uint32, 12 * n), wo.view(np.uint32).ravel()), name
+        for x in (d, aabb, keys, idx, par, bvh, otri):
+            x.destroy()
+
+
+# ---------------------------------------------------------------- fused build
+
+@pytest.mark.parametrize("div_mode", [0, 1])
+def test_build_matches_oracle(ctx, lb, ora, div_mode):
+    ctx.set_div_mode(div_mode)
+    for name, tri in mesh_cases(lb).items():
+        for with_bounds in (True, False):
+            say(f"build {name} div={div_mode} bounds={with_bounds}")
+            b = lb.wl_bounds(tri) if with_bounds else None
+            bvh, otri, sk, pm = ctx.build_arrays(tri, b, keep_topology=True)
+            wb, wo, wsk, wpm = ora.build(tri, b, div_mode)
+            assert np.array_equal(sk, wsk), name
+            assert np.array_equal(pm, wpm), name
+            assert np.array_equal(bvh, wb), name
+            assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32)), name
+    ctx.set_div_mode(0)
+
+
+@pytest.mark.parametrize("div_mode", [0, 1])
+def test_build_clustered_1m(ctx, lb, ora, div_mode):
+    say(f"build clustered-1m div={div_mode}")
+    ctx.set_div_mode(div_mode)
+    tri = lb.workload("clustered-1m")
+    b = lb.wl_bounds(tri)
+    bvh, otri, sk, pm = ctx.build_arrays(tri, b, keep_topology=True)
+    wb, wo, wsk, wpm = ora.build(tri, b, div_mode)
+    ctx.set_div_mode(0)
+    assert np.array_equal(sk, wsk)
+    assert np.array_equal(pm, wpm)
+    assert np.array_equal(bvh, wb)
+    assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
+
+
+def test_build_is_deterministic(ctx, lb):
+    tri = lb.workload("uniform-64k")
+    first = ctx.build_arrays(tri, None)[0]
+    for k in range(20):
+        say(f"determinism rep {k}")
+        assert np.array_equal(ctx.build_arrays(tri, None)[0], first)
+
+
+def test_build_heightfield_16m_full_size(ctx, lb, ora):
+    """The 16,777,216-triangle heightfield against the oracle, plus
+    size-independent checks: sorted keys, a permutation, root box = the
+    scene bounds, every child box inside its parent's."""
+    say("build heightfield-16m")
+    tri = lb.workload("heightfield-16m")
+    n = len(tri)
+    bvh, otri, sk, pm = ctx.build_arrays(tri, None, keep_topology=True)
+    assert np.all(sk[1:] >= sk[:-1])
+    assert np.array_equal(np.sort(pm), np.arange(n, dtype=np.int32))
+    f, i = ora.bvh_fields(bvh, n)
+    b = ora.bounds(tri)
+    root_min = np.minimum(f[0, 0:3], f[0, 8:11])
+    root_max = np.maximum(f[0, 4:7], f[0, 12:15])
+    assert np.array_equal(root_min, np.where(b[:3] == 0, np.float32(-1e-8), b[:3]))
+    assert np.array_equal(root_max, np.where(b[3:] == 0, np.float32(1e-8), b[3:]))
+    say("oracle heightfield-16m")
+    wb, wo, wsk, wpm = ora.build(tri, None, 0)
+    assert np.array_equal(sk, wsk)
+    assert np.array_equal(pm, wpm)
+    assert np.array_equal(bvh, wb)
+    assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
+
+
+# ---------------------------------------------------------------- refit
+
+def test_refit_matches_oracle(ctx, lb, ora):
+    for n in (1, 2, 3, 1000, 65536, 300001):
+        say(f"refit n={n}")
+        tri = lb.wl_clustered(40 + n, n, 0.05)
+        bvh, otri, sk, pm = ctx.build_arrays(tri, None, keep_topology=True)
+        moved = lb.wl_jitter(41 + n, tri, 0.001)
+        d = ctx.to_device(moved)
+        dk, dp = ctx.to_device(sk), ctx.to_device(pm)
+        rb, ro = ctx.alloc(64 * n), ctx.alloc(48 * n)
+        ctx.refit(d, n, dk, dp, rb, ro)
+        wb, wo = ora.refit(moved, sk, pm)
+        assert np.array_equal(rb.read(np.uint8, 64 * n), wb)
+        assert np.array_equal(ro.read(np.uint32, 12 * n), wo.view(np.uint32).ravel())
+        # refit with the original vertices reproduces the build
+        d2 = ctx.to_device(tri)
+        ctx.refit(d2, n, dk, dp, rb, ro)
+        assert np.array_equal(rb.read(np.uint8, 64 * n), bvh)
+        for x in (d, dk, dp, rb, ro, d2):
+            x.destroy()
+
+
+# ---------------------------------------------------------------- budget
+
+def test_budget_refusal_leaves_context_usable(lb):
+    tri = lb.wl_uniform(50, 200000, 0.01)
+    with lb.Context(0, hbm_budget=40 << 20) as c:
+        d = c.to_device(tri)
+        bvh, otri = c.alloc(64 * 200000), c.alloc(48 * 200000)
+        with pytest.raises(lb.LBVHError) as e:
+            c.build(d, 200000, None, bvh, otri)
+        assert e.value.code == 3
+        for x in (d, bvh, otri):
+            x.destroy()
+        small = lb.wl_uniform(51, 1000, 0.01)
+        got = c.build_arrays(small, None)[0]
+    with lb.Context(0) as c2:
+        assert np.array_equal(got, c2.build_arrays(small, None)[0])

@@ -1,0 +1,219 @@
+// k_bounds_morton.hip -- scene-bounds reduction and leaf AABB / Morton keys.
+//
+// Reference: scripts/bvh/kernels/aabb-z-idx.js (WGSL compute_aabb_z_idx
+// :177-195, morton_code :197-199, split_3 :202-208) and the caller-side
+// bounds of scripts/obj/parse-obj.js:10-53.
+//
+// Both kernels are HBM-bound streams over the 48-byte triangles (three
+// 16-byte loads per triangle).  Morton arithmetic uses the explicitly
+// rounded intrinsics so that every fp32 operation rounds once, as in WGSL.
+#include "device_util.h"
+
+namespace lbvh {
+
+__device__ __forceinline__ uint32_t split_3(uint32_t u) {
+  uint32_t x = u;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// WGSL u32(f32): truncate toward zero, saturate to [0, 2^32-1]; NaN -> 0.
+__device__ __forceinline__ uint32_t f32_to_u32_sat(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-wide min/max of 6 values; result valid in thread 0.
+__device__ void block_minmax(float v[6]) {
+  __shared__ float s[4][6];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v[c] = wave_min(v[c]);
+    v[3 + c] = wave_max(v[3 + c]);
+  }
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) s[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      for (int c = 0; c < 3; ++c) {
+        v[c] = fminf(v[c], s[w][c]);
+        v[3 + c] = fmaxf(v[3 + c], s[w][3 + c]);
+      }
+  }
+  __syncthreads();
+}
+
+// Scene bounds: fp32 min/max over all vertex coordinates.  Each block
+// reduces a grid-stride share into partials[2*b, 2*b+1]; the last block to
+// finish (arrival counter, agent-scope acq_rel) folds the partials into
+// ctl->bounds and resets the counter for the next operation.
+__global__ __launch_bounds__(kMapThreads) void k_bounds(
+    const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
+    Ctl *ctl, int bump_epoch) {
+  if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
+                 c = tri[3 * (size_t)i + 2];
+    v[0] = fminf(v[0], fminf(a.x, fminf(b.x, c.x)));
+    v[1] = fminf(v[1], fminf(a.y, fminf(b.y, c.y)));
+    v[2] = fminf(v[2], fminf(a.z, fminf(b.z, c.z)));
+    v[3] = fmaxf(v[3], fmaxf(a.x, fmaxf(b.x, c.x)));
+    v[4] = fmaxf(v[4], fmaxf(a.y, fmaxf(b.y, c.y)));
+    v[5] = fmaxf(v[5], fmaxf(a.z, fmaxf(b.z, c.z)));
+  }
+  block_minmax(v);
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = make_float4(v[0], v[1], v[2], 0.0f);
+    partials[2 * blockIdx.x + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+    uint32_t done = __hip_atomic_fetch_add(&ctl->bounds_done, 1u,
+                                           __ATOMIC_ACQ_REL,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (done == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  float w[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    const float4 lo = partials[2 * b], hi = partials[2 * b + 1];
+    w[0] = fminf(w[0], lo.x); w[1] = fminf(w[1], lo.y); w[2] = fminf(w[2], lo.z);
+    w[3] = fmaxf(w[3], hi.x); w[4] = fmaxf(w[4], hi.y); w[5] = fmaxf(w[5], hi.z);
+  }
+  block_minmax(w);
+  if (threadIdx.x == 0) {
+    ctl->bounds[0] = w[0]; ctl->bounds[1] = w[1]; ctl->bounds[2] = w[2];
+    ctl->bounds[3] = 0.0f;
+    ctl->bounds[4] = w[3]; ctl->bounds[5] = w[4]; ctl->bounds[6] = w[5];
+    ctl->bounds[7] = 0.0f;
+    __hip_atomic_store(&ctl->bounds_done, 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// compute_aabb_z_idx (aabb-z-idx.js:177-195) over a grid-stride share:
+//   box  = min/max of the three vertices            (:186-187)
+//   cen  = (max + min) * .5f                          (:191)
+//   rel  = (cen - bmin) / (bmax - bmin)               (:192, per div mode)
+//   key  = morton(u32(rel * 1023.99f))                (:194-208)
+// Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
+// fused build, the four digit histograms of the keys (the sort's upfront
+// histogram, saving a second read of the keys).
+__global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
+    const float4 *__restrict__ tri, uint32_t n, MortonParams p,
+    int bounds_from_ctl, float4 *__restrict__ aabb,
+    uint32_t *__restrict__ keys, Ctl *ctl, int histogram, int bump_epoch) {
+  __shared__ uint32_t s_hist[kSortPasses][kRadix];
+  if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
+  if (histogram) {
+    for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
+      (&s_hist[0][0])[k] = 0u;
+    __syncthreads();
+  }
+  float bmin[3], ext[3], rcp[3];
+  if (bounds_from_ctl) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bmin[c] = ctl->bounds[c];
+      ext[c] = __fsub_rn(ctl->bounds[4 + c], bmin[c]);
+      rcp[c] = __fdiv_rn(1.0f, ext[c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bmin[c] = p.bmin[c];
+      ext[c] = p.ext[c];
+      rcp[c] = p.rcp[c];
+    }
+  }
+  const bool ieee = (p.div_mode == 1);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
+                 c = tri[3 * (size_t)i + 2];
+    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
+                         fminf(a.z, fminf(b.z, c.z))};
+    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
+                         fmaxf(a.z, fmaxf(b.z, c.z))};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+      const float d = __fsub_rn(cen, bmin[k]);
+      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
+      q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
+    }
+    const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+    keys[i] = key;
+    if (aabb) {
+      aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
+      aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
+    }
+    if (histogram) hist_add(s_hist, key);
+  }
+  if (histogram) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x) {
+      const uint32_t v = (&s_hist[0][0])[k];
+      if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
+    }
+  }
+}
+
+__global__ void k_copy_bounds(const Ctl *ctl, float *out) {
+  const int t = threadIdx.x;
+  if (t < 3) out[t] = ctl->bounds[t];
+  else if (t < 6) out[t] = ctl->bounds[t + 1];
+}
+
+__global__ void k_bump_epoch(Ctl *ctl) { ctl->epoch += 1; }
+
+void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
+                   bool bump_epoch, hipStream_t s) {
+  uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
+  if (blocks > (uint32_t)kBoundsBlocks) blocks = kBoundsBlocks;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
+                     partials, ctl, bump_epoch ? 1 : 0);
+}
+
+void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
+                        bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
+                        Ctl *ctl, bool histogram, bool bump_epoch,
+                        hipStream_t s) {
+  uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
+  if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_aabb_morton, dim3(blocks), dim3(kMapThreads), 0, s,
+                     tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
+                     histogram ? 1 : 0, bump_epoch ? 1 : 0);
+}
+
+void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_bounds, dim3(1), dim3(64), 0, s, ctl, out);
+}
+
+void launch_bump_epoch(Ctl *ctl, hipStream_t s) {
+  hipLaunchKernelGGL(k_bump_epoch, dim3(1), dim3(1), 0, s, ctl);
+}
+
+}  // namespace lbvh

@@ -1,0 +1,631 @@
+// lbvh_api.hip -- host side of the C ABI declared in include/lbvh.h.
+//
+// A context owns one HIP stream (default priority), a device control block
+// (Ctl) and a grow-only scratch arena.  All device memory it hands out or
+// keeps is charged against the context's HBM budget.  Entry points validate
+// their arguments on the host, enqueue kernels and return; nothing here
+// blocks on the GPU except lbvh_upload/lbvh_download/lbvh_synchronize.
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lbvh.h"
+#include "lbvh_internal.h"
+
+using namespace lbvh;
+
+namespace {
+
+struct DevBuf {
+  void *ptr = nullptr;
+  uint64_t bytes = 0;
+};
+
+}  // namespace
+
+struct lbvh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;  // 0 = unlimited
+  uint64_t in_use = 0;
+  int div_mode = LBVH_DIV_MUL_RCP;
+  std::string last_error;
+  std::unordered_map<void *, uint64_t> user_allocs;
+  std::recursive_mutex mu;
+  // device state
+  Ctl *ctl = nullptr;
+  uint64_t epoch_ops = 0;  // epoch bumps issued since the last rewind
+  DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
+  DevBuf status;                        // look-back words (zeroed)
+  DevBuf gside, gtag;                   // fused up-pass exchange (tags zeroed)
+  DevBuf ctag;                          // per-stage up-pass tags (zeroed)
+  DevBuf partials;                      // bounds partials
+  // profiling: event pairs bracketing each stage, read back on demand
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  lbvh_profile acc{};
+};
+
+namespace {
+
+int fail(lbvh_ctx *ctx, int status, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return status;
+}
+
+int hip_fail(lbvh_ctx *ctx, hipError_t e, const char *what) {
+  return fail(ctx, LBVH_ERR_DEVICE,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                          \
+  do {                                              \
+    hipError_t e_ = (expr);                         \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+bool fits_budget(const lbvh_ctx *ctx, uint64_t extra) {
+  return ctx->budget == 0 || ctx->in_use + extra <= ctx->budget;
+}
+
+int dev_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out) {
+  if (!fits_budget(ctx, bytes))
+    return fail(ctx, LBVH_ERR_OUT_OF_BUDGET,
+                "allocation of " + std::to_string(bytes) +
+                    " bytes exceeds the HBM budget (" +
+                    std::to_string(ctx->in_use) + " of " +
+                    std::to_string(ctx->budget) + " in use)");
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, LBVH_ERR_OUT_OF_MEMORY,
+                "hipMalloc(" + std::to_string(bytes) + ") failed: " +
+                    hipGetErrorString(e));
+  }
+  ctx->in_use += bytes;
+  *out = p;
+  return LBVH_OK;
+}
+
+void dev_release(lbvh_ctx *ctx, DevBuf &b) {
+  if (b.ptr) {
+    (void)hipFree(b.ptr);
+    ctx->in_use -= b.bytes;
+  }
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+// Grow-only scratch buffer.  Buffers whose contents carry epoch stamps are
+// zeroed when (re)allocated, which keeps every stale stamp older than the
+// current epoch.
+int ensure(lbvh_ctx *ctx, DevBuf &b, uint64_t bytes, bool zero) {
+  if (b.bytes >= bytes) return LBVH_OK;
+  // Grow with headroom so that slowly increasing sizes do not reallocate on
+  // every call.
+  uint64_t want = bytes + bytes / 4;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  dev_release(ctx, b);
+  void *p = nullptr;
+  int st = dev_alloc(ctx, want, &p);
+  if (st == LBVH_ERR_OUT_OF_BUDGET && want > bytes) {
+    want = bytes;
+    st = dev_alloc(ctx, want, &p);
+  }
+  if (st != LBVH_OK) return st;
+  b.ptr = p;
+  b.bytes = want;
+  if (zero) HIP_TRY(ctx, hipMemsetAsync(p, 0, want, ctx->stream));
+  return LBVH_OK;
+}
+
+// Epoch stamps are 28 bits wide in the look-back words.  Long before they
+// wrap, every stamped buffer is cleared and the epoch restarts.
+constexpr uint64_t kEpochRewind = 1ull << 26;
+
+int begin_op(lbvh_ctx *ctx) {
+  if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (DevBuf *b : {&ctx->status, &ctx->gtag, &ctx->ctag})
+    if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
+                              ctx->stream));
+  ctx->epoch_ops = 1;
+  return LBVH_OK;
+}
+
+// Stage bracketing for lbvh_profile_*: a StageScope records one event before
+// the stage's launches and one after them when profiling is enabled.
+hipEvent_t take_event(lbvh_ctx *ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageScope {
+  lbvh_ctx *ctx;
+  int stage;
+  hipEvent_t a = nullptr;
+  StageScope(lbvh_ctx *c, int s) : ctx(c), stage(s) {
+    if (ctx->prof && (a = take_event(ctx)))
+      (void)hipEventRecord(a, ctx->stream);
+  }
+  ~StageScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(ctx);
+    if (!b) {
+      ctx->ev_pool.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->marks.push_back({stage, a, b});
+  }
+};
+
+int check_launch(lbvh_ctx *ctx, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(ctx, e, what);
+  return LBVH_OK;
+}
+
+int check_n(lbvh_ctx *ctx, uint32_t n) {
+  if (n > LBVH_MAX_TRIS)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                "triangle count " + std::to_string(n) + " exceeds the maximum " +
+                    std::to_string(LBVH_MAX_TRIS));
+  return LBVH_OK;
+}
+
+int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
+  int st;
+  if ((st = ensure(ctx, ctx->keys_a, 4ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->keys_b, 4ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->idx_a, 4ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
+  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix;
+  return ensure(ctx, ctx->status, 8ull * words, true);
+}
+
+int ensure_partials(lbvh_ctx *ctx) {
+  return ensure(ctx, ctx->partials, 32ull * kBoundsBlocks, false);
+}
+
+// The four onesweep passes: in -> tmp -> out -> tmp -> out.  `in` may equal
+// `out` (the stand-alone sort is in place, like the reference's).
+void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
+              int32_t *vtmp, int32_t *vout, uint32_t n) {
+  uint64_t *status = static_cast<uint64_t *>(ctx->status.ptr);
+  const uint32_t *src[kSortPasses] = {in, tmp, out, tmp};
+  uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
+  const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
+  int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
+  for (int p = 0; p < kSortPasses; ++p) {
+    StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
+    launch_onesweep_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, ctx->ctl,
+                         status, ctx->stream);
+  }
+}
+
+MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
+  MortonParams p{};
+  for (int c = 0; c < 3; ++c) {
+    p.bmin[c] = bounds[c];
+    volatile float ext = bounds[3 + c] - bounds[c];  // one fp32 rounding
+    p.ext[c] = ext;
+    volatile float rcp = 1.0f / p.ext[c];
+    p.rcp[c] = rcp;
+  }
+  p.div_mode = ctx->div_mode;
+  return p;
+}
+
+#define CTX_GUARD(ctx)                                        \
+  if (!(ctx)) return LBVH_ERR_INVALID_ARGUMENT;               \
+  std::lock_guard<std::recursive_mutex> lock_((ctx)->mu);     \
+  (void)hipSetDevice((ctx)->device)
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char *lbvh_status_string(int status) {
+  switch (status) {
+    case LBVH_OK: return "ok";
+    case LBVH_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case LBVH_ERR_SIZE_MISMATCH: return "size mismatch";
+    case LBVH_ERR_OUT_OF_BUDGET: return "out of HBM budget";
+    case LBVH_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case LBVH_ERR_DEVICE: return "device error";
+    case LBVH_ERR_NO_DEVICE: return "no such device";
+    case LBVH_ERR_BUSY: return "busy";
+    case LBVH_ERR_NOT_FOUND: return "not found";
+    default: return "unknown status";
+  }
+}
+
+int lbvh_create(int device_ordinal, uint64_t hbm_budget_bytes,
+                lbvh_ctx **out_ctx) {
+  if (!out_ctx) return LBVH_ERR_INVALID_ARGUMENT;
+  *out_ctx = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device_ordinal < 0 ||
+      device_ordinal >= count) {
+    (void)hipGetLastError();
+    return LBVH_ERR_NO_DEVICE;
+  }
+  lbvh_ctx *ctx = new (std::nothrow) lbvh_ctx();
+  if (!ctx) return LBVH_ERR_OUT_OF_MEMORY;
+  ctx->device = device_ordinal;
+  ctx->budget = hbm_budget_bytes;
+  if (hipSetDevice(device_ordinal) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+          hipSuccess) {
+    delete ctx;
+    return LBVH_ERR_DEVICE;
+  }
+  void *p = nullptr;
+  int st = dev_alloc(ctx, sizeof(Ctl), &p);
+  if (st != LBVH_OK ||
+      hipMemsetAsync(p, 0, sizeof(Ctl), ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    if (p) (void)hipFree(p);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return st != LBVH_OK ? st : LBVH_ERR_DEVICE;
+  }
+  ctx->ctl = static_cast<Ctl *>(p);
+  *out_ctx = ctx;
+  return LBVH_OK;
+}
+
+int lbvh_destroy(lbvh_ctx *ctx) {
+  if (!ctx) return LBVH_ERR_INVALID_ARGUMENT;
+  {
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
+                      &ctx->status, &ctx->gside, &ctx->gtag, &ctx->ctag,
+                      &ctx->partials})
+      dev_release(ctx, *b);
+    for (auto &m : ctx->marks) {
+      ctx->ev_pool.push_back(m.a);
+      ctx->ev_pool.push_back(m.b);
+    }
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (auto &kv : ctx->user_allocs) (void)hipFree(kv.first);
+    ctx->user_allocs.clear();
+    (void)hipFree(ctx->ctl);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return LBVH_OK;
+}
+
+const char *lbvh_last_error(const lbvh_ctx *ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int lbvh_set_div_mode(lbvh_ctx *ctx, int div_mode) {
+  CTX_GUARD(ctx);
+  if (div_mode != LBVH_DIV_MUL_RCP && div_mode != LBVH_DIV_IEEE)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "unknown division mode");
+  ctx->div_mode = div_mode;
+  return LBVH_OK;
+}
+
+int lbvh_get_div_mode(const lbvh_ctx *ctx) {
+  return ctx ? ctx->div_mode : -1;
+}
+
+void *lbvh_stream(lbvh_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int lbvh_synchronize(lbvh_ctx *ctx) {
+  CTX_GUARD(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  uint32_t fault = 0;
+  HIP_TRY(ctx, hipMemcpy(&fault, &ctx->ctl->fault, 4, hipMemcpyDeviceToHost));
+  if (fault) {
+    HIP_TRY(ctx, hipMemset(&ctx->ctl->fault, 0, 4));
+    return fail(ctx, LBVH_ERR_DEVICE,
+                "a sort look-back wait exceeded its bound; results of the "
+                "operations since the last synchronize are invalid");
+  }
+  return LBVH_OK;
+}
+
+uint64_t lbvh_bytes_in_use(const lbvh_ctx *ctx) {
+  return ctx ? ctx->in_use : 0;
+}
+
+int lbvh_trim(lbvh_ctx *ctx) {
+  CTX_GUARD(ctx);
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
+                    &ctx->status, &ctx->gside, &ctx->gtag, &ctx->ctag,
+                    &ctx->partials})
+    dev_release(ctx, *b);
+  return LBVH_OK;
+}
+
+int lbvh_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out_dptr) {
+  CTX_GUARD(ctx);
+  if (!out_dptr) return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null output");
+  *out_dptr = nullptr;
+  void *p = nullptr;
+  int st = dev_alloc(ctx, bytes, &p);
+  if (st != LBVH_OK) return st;
+  ctx->user_allocs[p] = bytes;
+  *out_dptr = p;
+  return LBVH_OK;
+}
+
+int lbvh_free(lbvh_ctx *ctx, void *dptr) {
+  CTX_GUARD(ctx);
+  if (!dptr) return LBVH_OK;
+  auto it = ctx->user_allocs.find(dptr);
+  if (it == ctx->user_allocs.end())
+    return fail(ctx, LBVH_ERR_NOT_FOUND, "pointer was not allocated by lbvh_alloc");
+  // Work already queued may still read the buffer.
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipFree(dptr));
+  ctx->in_use -= it->second;
+  ctx->user_allocs.erase(it);
+  return LBVH_OK;
+}
+
+int lbvh_upload(lbvh_ctx *ctx, void *dst_dev, const void *src_host,
+                uint64_t bytes) {
+  CTX_GUARD(ctx);
+  if (bytes == 0) return LBVH_OK;
+  if (!dst_dev || !src_host)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  HIP_TRY(ctx, hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LBVH_OK;
+}
+
+int lbvh_download(lbvh_ctx *ctx, void *dst_host, const void *src_dev,
+                  uint64_t bytes) {
+  CTX_GUARD(ctx);
+  if (bytes == 0) return LBVH_OK;
+  if (!dst_host || !src_dev)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  HIP_TRY(ctx, hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LBVH_OK;
+}
+
+// ---- per-stage entry points ----------------------------------------------
+
+int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                float *bounds_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (!bounds_out || (n && !tri))
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if (n == 0)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bounds of an empty mesh");
+  if ((st = ensure_partials(ctx))) return st;
+  if ((st = begin_op(ctx))) return st;
+  launch_bounds(reinterpret_cast<const float4 *>(tri), n,
+                static_cast<float4 *>(ctx->partials.ptr), ctx->ctl, true,
+                ctx->stream);
+  launch_copy_bounds(ctx->ctl, bounds_out, ctx->stream);
+  return check_launch(ctx, "lbvh_bounds");
+}
+
+int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                     const float *bounds_host, float *aabb_out,
+                     uint32_t *keys_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!tri || !keys_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = begin_op(ctx))) return st;
+  MortonParams p{};
+  if (bounds_host) {
+    p = morton_params(ctx, bounds_host);
+  } else {
+    if ((st = ensure_partials(ctx))) return st;
+    p.div_mode = ctx->div_mode;
+    launch_bounds(reinterpret_cast<const float4 *>(tri), n,
+                  static_cast<float4 *>(ctx->partials.ptr), ctx->ctl, true,
+                  ctx->stream);
+  }
+  launch_aabb_morton(reinterpret_cast<const float4 *>(tri), n, p,
+                     bounds_host == nullptr,
+                     reinterpret_cast<float4 *>(aabb_out), keys_out, ctx->ctl,
+                     false, bounds_host != nullptr, ctx->stream);
+  return check_launch(ctx, "lbvh_aabb_morton");
+}
+
+int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!keys || !idx_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if (n == 1) {
+    HIP_TRY(ctx, hipMemsetAsync(idx_out, 0, 4, ctx->stream));
+    return LBVH_OK;
+  }
+  if ((st = ensure_sort_scratch(ctx, n))) return st;
+  if ((st = begin_op(ctx))) return st;
+  launch_histogram(keys, n, ctx->ctl, ctx->stream);
+  run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
+           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n);
+  return check_launch(ctx, "lbvh_sort");
+}
+
+int lbvh_radix_tree(lbvh_ctx *ctx, const uint32_t *sorted_keys, uint32_t n,
+                    int32_t *parents_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!sorted_keys || !parents_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  launch_radix_tree(sorted_keys, n, parents_out, ctx->stream);
+  return check_launch(ctx, "lbvh_radix_tree");
+}
+
+int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
+                 const int32_t *parents, uint32_t n, void *bvh_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!idx || !aabb || !parents || !bvh_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = ensure(ctx, ctx->ctag, 4ull * n, true))) return st;
+  if ((st = begin_op(ctx))) return st;
+  launch_bump_epoch(ctx->ctl, ctx->stream);
+  launch_up_pass(idx, reinterpret_cast<const float4 *>(aabb), parents, n,
+                 static_cast<float4 *>(bvh_out),
+                 static_cast<uint32_t *>(ctx->ctag.ptr), ctx->ctl, ctx->stream);
+  return check_launch(ctx, "lbvh_up_pass");
+}
+
+int lbvh_rearrange(lbvh_ctx *ctx, const float *tri_in, const int32_t *idx,
+                   uint32_t n, float *tri_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!tri_in || !idx || !tri_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  launch_rearrange(reinterpret_cast<const float4 *>(tri_in), idx, n,
+                   reinterpret_cast<float4 *>(tri_out), ctx->stream);
+  return check_launch(ctx, "lbvh_rearrange");
+}
+
+// ---- fused build / refit ----------------------------------------------------
+
+int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
+               const float *bounds_host, void *bvh_out, float *tri_out,
+               uint32_t *sorted_keys_out, int32_t *perm_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!tri || !bvh_out || !tri_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = ensure_sort_scratch(ctx, n))) return st;
+  if ((st = ensure(ctx, ctx->gside, 64ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
+  if (!bounds_host && (st = ensure_partials(ctx))) return st;
+  if ((st = begin_op(ctx))) return st;
+
+  const float4 *t4 = reinterpret_cast<const float4 *>(tri);
+  uint32_t *keys = static_cast<uint32_t *>(ctx->keys_a.ptr);
+  uint32_t *skeys = sorted_keys_out ? sorted_keys_out : keys;
+  int32_t *perm = perm_out ? perm_out : static_cast<int32_t *>(ctx->idx_a.ptr);
+  MortonParams p{};
+  if (bounds_host) {
+    p = morton_params(ctx, bounds_host);
+  } else {
+    p.div_mode = ctx->div_mode;
+    StageScope sc(ctx, LBVH_STAGE_BOUNDS);
+    launch_bounds(t4, n, static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
+                  true, ctx->stream);
+  }
+  const bool sort = n > 1;
+  {
+    StageScope sc(ctx, LBVH_STAGE_MORTON);
+    launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
+                       ctx->ctl, sort, bounds_host != nullptr, ctx->stream);
+  }
+  if (sort) {
+    run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), skeys,
+             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n);
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(perm, 0, 4, ctx->stream));
+    if (skeys != keys)
+      HIP_TRY(ctx, hipMemcpyAsync(skeys, keys, 4, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+  }
+  {
+    StageScope sc(ctx, LBVH_STAGE_TREE);
+    launch_tree_fused(t4, skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
+                      static_cast<float4 *>(bvh_out),
+                      static_cast<GSide *>(ctx->gside.ptr),
+                      static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
+                      ctx->stream);
+  }
+  return check_launch(ctx, "lbvh_build");
+}
+
+int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
+               const uint32_t *sorted_keys, const int32_t *perm, void *bvh_out,
+               float *tri_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!tri || !sorted_keys || !perm || !bvh_out || !tri_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = ensure(ctx, ctx->gside, 64ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
+  if ((st = begin_op(ctx))) return st;
+  launch_bump_epoch(ctx->ctl, ctx->stream);
+  StageScope sc(ctx, LBVH_STAGE_TREE);
+  launch_tree_fused(reinterpret_cast<const float4 *>(tri), sorted_keys, perm, n,
+                    reinterpret_cast<float4 *>(tri_out),
+                    static_cast<float4 *>(bvh_out),
+                    static_cast<GSide *>(ctx->gside.ptr),
+                    static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
+                    ctx->stream);
+  return check_launch(ctx, "lbvh_refit");
+}
+
+// ---- profiling ---------------------------------------------------------------
+
+int lbvh_profile_enable(lbvh_ctx *ctx, int enable) {
+  CTX_GUARD(ctx);
+  ctx->prof = enable != 0;
+  return LBVH_OK;
+}
+
+int lbvh_profile_read(lbvh_ctx *ctx, lbvh_profile *out, int reset) {
+  CTX_GUARD(ctx);
+  if (!ctx->marks.empty()) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &m : ctx->marks) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+        ctx->acc.ms[m.stage] += ms;
+        ctx->acc.launches[m.stage] += 1;
+      }
+      ctx->ev_pool.push_back(m.a);
+      ctx->ev_pool.push_back(m.b);
+    }
+    ctx->marks.clear();
+  }
+  if (out) *out = ctx->acc;
+  if (reset) ctx->acc = lbvh_profile{};
+  return LBVH_OK;
+}
+
+}  // extern "C"

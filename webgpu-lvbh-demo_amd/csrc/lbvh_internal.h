@@ -1,0 +1,106 @@
+// lbvh_internal.h -- shared definitions of the HIP kernels and the host API.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbvh {
+
+// ---- sizes and launch shapes --------------------------------------------
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;  // 256 digit values per pass
+constexpr int kSortPasses = 4;           // 30-bit keys: bits 0-7,8-15,16-23,24-29
+constexpr int kSortThreads = 256;        // 4 wave64 per tile
+constexpr int kSortItems = 16;           // keys per thread
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
+
+constexpr int kTreeThreads = 256;  // leaves per tile of the fused up pass
+
+constexpr int kMapThreads = 256;  // triangle-parallel kernels
+constexpr int kBoundsBlocks = 1024;  // grid of the bounds reduction
+constexpr int kMortonBlocksMax = 2048;  // grid-stride Morton + histogram
+constexpr uint32_t kSpinCap = 1u << 22;  // look-back polls before giving up
+
+// ---- device control block ------------------------------------------------
+//
+// One per context, zeroed at allocation.  Every counter in it returns to its
+// initial value by the end of the operation that uses it, so back-to-back
+// operations (and graph replays) need no host-side reset:
+//   epoch       bumped once per operation by its first kernel; stamps the
+//               look-back words and arrival tags of that operation
+//   tile_ctr    dynamic tile id of a sort pass; the block that draws the last
+//               id puts it back to 0
+//   done_ctr    completion count of the last sort pass; the last block zeroes
+//               the digit histograms and the counter
+//   bounds_done completion count of the bounds reduction (reset by the last
+//               block)
+//   fault       sticky flag: a bounded wait gave up (never set in a correct
+//               run; read by lbvh_synchronize)
+struct Ctl {
+  uint32_t epoch;
+  uint32_t tile_ctr;
+  uint32_t done_ctr;
+  uint32_t bounds_done;
+  uint32_t fault;
+  uint32_t pad0[3];
+  float bounds[8];                       // min.xyz, 0, max.xyz, 0
+  uint32_t hist[kSortPasses][kRadix];    // global digit counts per pass
+};
+
+// Exchange record of one side of a split in the fused up pass: the side's box
+// and the far end of its leaf range.  Two sides per split (64 B).
+struct __align__(16) GSide {
+  float4 lo;  // min.xyz, far end (int bits)
+  float4 hi;  // max.xyz, unused
+};
+
+// Morton quantisation parameters (uniform block of aabb-z-idx.js:164-169).
+struct MortonParams {
+  float bmin[3];
+  float ext[3];  // max - min
+  float rcp[3];  // fl(1 / ext)
+  int div_mode;  // lbvh_div_mode
+};
+
+// ---- kernel launchers (each .hip file owns its kernels) --------------------
+
+// k_bounds_morton.hip
+void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
+                   bool bump_epoch, hipStream_t s);
+void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
+                        bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
+                        Ctl *ctl, bool histogram, bool bump_epoch,
+                        hipStream_t s);
+void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
+void launch_bump_epoch(Ctl *ctl, hipStream_t s);
+
+// k_sort.hip
+void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
+                      hipStream_t s);
+// One onesweep digit pass (pass 0 takes the input position as payload and
+// ignores vals_in; pass kSortPasses-1 clears the histograms when done).
+// The four passes sort keys_in into keys_out through keys_tmp:
+//   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
+void launch_onesweep_pass(int pass, const uint32_t *keys_in,
+                          const int32_t *vals_in, uint32_t *keys_out,
+                          int32_t *vals_out, uint32_t n, Ctl *ctl,
+                          uint64_t *status, hipStream_t s);
+inline uint32_t sort_tiles(uint32_t n) {
+  return (n + kSortTile - 1) / kSortTile;
+}
+
+// k_tree.hip
+void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
+                       hipStream_t s);
+void launch_up_pass(const int32_t *idx, const float4 *aabb,
+                    const int32_t *parents, uint32_t n, float4 *nodes,
+                    uint32_t *tags, const Ctl *ctl, hipStream_t s);
+void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
+                      float4 *out, hipStream_t s);
+void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
+                       const int32_t *perm, uint32_t n, float4 *otri,
+                       float4 *nodes, GSide *gside, uint32_t *gtag,
+                       const Ctl *ctl, hipStream_t s);
+
+}  // namespace lbvh

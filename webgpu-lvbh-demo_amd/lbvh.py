@@ -1,0 +1,360 @@
+"""ctypes binding of liblbvh.so (include/lbvh.h) and liblbvh_workloads.so.
+
+Used by the Python tests and bench.py; the Node.js entry point
+(bvh/index.js) binds the same C ABI through the N-API addon.
+
+The binding is a thin mirror of the C ABI: every call goes to the HIP
+library; if the library is missing, loading raises instead of falling back
+to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "liblbvh.so")
+WL_PATH = os.path.join(LIB_DIR, "liblbvh_workloads.so")
+
+DIV_MUL_RCP = 0
+DIV_IEEE = 1
+
+STATUS = {
+    0: "ok",
+    1: "invalid argument",
+    2: "size mismatch",
+    3: "out of HBM budget",
+    4: "out of device memory",
+    5: "device error",
+    6: "no such device",
+    7: "busy",
+    8: "not found",
+}
+
+TRI_BYTES = 48
+NODE_BYTES = 64
+AABB_BYTES = 32
+
+# Every symbol include/lbvh.h declares, with (restype, argtypes).
+_vp, _u32, _u64, _i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+SIGNATURES = {
+    "lbvh_create": (_i32, [_i32, _u64, C.POINTER(_vp)]),
+    "lbvh_destroy": (_i32, [_vp]),
+    "lbvh_last_error": (C.c_char_p, [_vp]),
+    "lbvh_status_string": (C.c_char_p, [_i32]),
+    "lbvh_set_div_mode": (_i32, [_vp, _i32]),
+    "lbvh_get_div_mode": (_i32, [_vp]),
+    "lbvh_stream": (_vp, [_vp]),
+    "lbvh_synchronize": (_i32, [_vp]),
+    "lbvh_bytes_in_use": (_u64, [_vp]),
+    "lbvh_trim": (_i32, [_vp]),
+    "lbvh_alloc": (_i32, [_vp, _u64, C.POINTER(_vp)]),
+    "lbvh_free": (_i32, [_vp, _vp]),
+    "lbvh_upload": (_i32, [_vp, _vp, _vp, _u64]),
+    "lbvh_download": (_i32, [_vp, _vp, _vp, _u64]),
+    "lbvh_bounds": (_i32, [_vp, _vp, _u32, _vp]),
+    "lbvh_aabb_morton": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp]),
+    "lbvh_sort": (_i32, [_vp, _vp, _vp, _u32]),
+    "lbvh_radix_tree": (_i32, [_vp, _vp, _u32, _vp]),
+    "lbvh_up_pass": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp]),
+    "lbvh_rearrange": (_i32, [_vp, _vp, _vp, _u32, _vp]),
+    "lbvh_build": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "lbvh_refit": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "lbvh_profile_enable": (_i32, [_vp, _i32]),
+    "lbvh_profile_read": (_i32, [_vp, _vp, _i32]),
+}
+
+STAGES = ["bounds", "morton", "sort0", "sort1", "sort2", "sort3", "tree"]
+
+
+class Profile(C.Structure):
+    _fields_ = [("ms", C.c_double * len(STAGES)),
+                ("launches", C.c_uint64 * len(STAGES))]
+
+WL_SIGNATURES = {
+    "lbvh_wl_uniform": (None, [_u64, _u32, C.c_double, _vp]),
+    "lbvh_wl_clustered": (None, [_u64, _u32, C.c_double, _vp]),
+    "lbvh_wl_heightfield": (None, [_u64, _u32, _u32, _vp]),
+    "lbvh_wl_log_uniform_sizes": (None, [_u64, _u32, _u32, _u32, _vp]),
+    "lbvh_wl_jitter": (None, [_u64, _u32, C.c_double, _vp, _vp]),
+    "lbvh_wl_bounds": (None, [_vp, _u32, _vp]),
+    "lbvh_wl_uniform01": (None, [_u64, _u64, _u32, _vp]),
+}
+
+_lib = None
+_wl = None
+
+
+def _bind(lib, sigs):
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load_library() -> C.CDLL:
+    """Loads liblbvh.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it first (python -c "
+                f"'import __graft_entry__ as g; g.build()')")
+        _lib = _bind(C.CDLL(LIB_PATH), SIGNATURES)
+    return _lib
+
+
+def load_workloads() -> C.CDLL:
+    global _wl
+    if _wl is None:
+        if not os.path.exists(WL_PATH):
+            raise RuntimeError(f"{WL_PATH} is missing: build it first")
+        _wl = _bind(C.CDLL(WL_PATH), WL_SIGNATURES)
+    return _wl
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class LBVHError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class DeviceBuffer:
+    """Device allocation from lbvh_alloc (GPUBuffer stand-in)."""
+
+    def __init__(self, ctx: "Context", ptr: int, size: int):
+        self.ctx, self.ptr, self.size = ctx, ptr, size
+
+    def destroy(self):
+        if self.ptr:
+            self.ctx._check(self.ctx.lib.lbvh_free(self.ctx.handle, self.ptr))
+            self.ptr = 0
+
+    def read(self, dtype=np.uint8, count: Optional[int] = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        if count is None:
+            count = self.size // dt.itemsize
+        out = np.empty(count, dtype=dt)
+        self.ctx.download(out, self)
+        return out
+
+    def write(self, arr: np.ndarray):
+        self.ctx.upload(self, arr)
+
+
+def _dp(b) -> Optional[int]:
+    if b is None:
+        return None
+    return b.ptr if isinstance(b, DeviceBuffer) else int(b)
+
+
+class Context:
+    """One lbvh_ctx: a HIP stream, scratch arena and HBM budget."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.lbvh_create(device, hbm_budget, C.byref(h))
+        if st != 0:
+            raise LBVHError(st, f"lbvh_create(device={device})")
+        self.handle = h.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.lbvh_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- plumbing --------------------------------------------------------
+    def _check(self, st: int):
+        if st != 0:
+            raise LBVHError(st, self.lib.lbvh_last_error(self.handle).decode())
+
+    def last_error(self) -> str:
+        return self.lib.lbvh_last_error(self.handle).decode()
+
+    @property
+    def stream(self) -> int:
+        return self.lib.lbvh_stream(self.handle)
+
+    def synchronize(self):
+        self._check(self.lib.lbvh_synchronize(self.handle))
+
+    def bytes_in_use(self) -> int:
+        return self.lib.lbvh_bytes_in_use(self.handle)
+
+    def trim(self):
+        self._check(self.lib.lbvh_trim(self.handle))
+
+    def set_div_mode(self, mode: int):
+        self._check(self.lib.lbvh_set_div_mode(self.handle, mode))
+
+    def profile_enable(self, on: bool = True):
+        self._check(self.lib.lbvh_profile_enable(self.handle, int(on)))
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """Per-stage device time: {stage: (total_ms, launches)}."""
+        p = Profile()
+        self._check(self.lib.lbvh_profile_read(self.handle, C.byref(p), int(reset)))
+        return {s: (p.ms[i], int(p.launches[i])) for i, s in enumerate(STAGES)}
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        p = C.c_void_p()
+        self._check(self.lib.lbvh_alloc(self.handle, nbytes, C.byref(p)))
+        return DeviceBuffer(self, p.value or 0, nbytes)
+
+    def upload(self, buf, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        self._check(self.lib.lbvh_upload(self.handle, _dp(buf), _ptr(arr), arr.nbytes))
+
+    def download(self, out: np.ndarray, buf):
+        self._check(self.lib.lbvh_download(self.handle, _ptr(out), _dp(buf), out.nbytes))
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        b = self.alloc(arr.nbytes)
+        if arr.nbytes:
+            self.upload(b, arr)
+        return b
+
+    # ---- per-stage entry points (device buffers) ---------------------------
+    def bounds(self, tri, n: int, out):
+        self._check(self.lib.lbvh_bounds(self.handle, _dp(tri), n, _dp(out)))
+
+    def aabb_morton(self, tri, n: int, bounds: Optional[Sequence[float]], aabb, keys):
+        b = None if bounds is None else np.asarray(bounds, dtype=np.float32)
+        self._check(self.lib.lbvh_aabb_morton(self.handle, _dp(tri), n, _ptr(b),
+                                              _dp(aabb), _dp(keys)))
+
+    def sort(self, keys, idx, n: int):
+        self._check(self.lib.lbvh_sort(self.handle, _dp(keys), _dp(idx), n))
+
+    def radix_tree(self, skeys, n: int, parents):
+        self._check(self.lib.lbvh_radix_tree(self.handle, _dp(skeys), n, _dp(parents)))
+
+    def up_pass(self, idx, aabb, parents, n: int, bvh):
+        self._check(self.lib.lbvh_up_pass(self.handle, _dp(idx), _dp(aabb),
+                                          _dp(parents), n, _dp(bvh)))
+
+    def rearrange(self, tri, idx, n: int, out):
+        self._check(self.lib.lbvh_rearrange(self.handle, _dp(tri), _dp(idx), n, _dp(out)))
+
+    def build(self, tri, n: int, bounds: Optional[Sequence[float]], bvh, otri,
+              skeys=None, perm=None):
+        b = None if bounds is None else np.asarray(bounds, dtype=np.float32)
+        self._check(self.lib.lbvh_build(self.handle, _dp(tri), n, _ptr(b), _dp(bvh),
+                                        _dp(otri), _dp(skeys), _dp(perm)))
+
+    def refit(self, tri, n: int, skeys, perm, bvh, otri):
+        self._check(self.lib.lbvh_refit(self.handle, _dp(tri), n, _dp(skeys),
+                                        _dp(perm), _dp(bvh), _dp(otri)))
+
+    # ---- host-array conveniences (tests) -----------------------------------
+    def build_arrays(self, tri: np.ndarray, bounds=None, keep_topology=False):
+        """Uploads `tri` (n x 12 float32), runs lbvh_build, returns host arrays
+        (bvh bytes as uint8 [n*64], otri float32 [n,12], and optionally the
+        sorted keys and permutation)."""
+        tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 12)
+        n = tri.shape[0]
+        dtri = self.to_device(tri)
+        bvh = self.alloc(NODE_BYTES * n)
+        otri = self.alloc(TRI_BYTES * n)
+        sk = self.alloc(4 * n) if keep_topology else None
+        pm = self.alloc(4 * n) if keep_topology else None
+        try:
+            self.build(dtri, n, bounds, bvh, otri, sk, pm)
+            out = [bvh.read(np.uint8, NODE_BYTES * n),
+                   otri.read(np.float32, 12 * n).reshape(n, 12)]
+            if keep_topology:
+                out += [sk.read(np.uint32, n), pm.read(np.int32, n)]
+            return tuple(out)
+        finally:
+            for b in (dtri, bvh, otri, sk, pm):
+                if b is not None:
+                    b.destroy()
+
+
+# ---- workloads -------------------------------------------------------------
+
+def wl_uniform(seed: int, n: int, half_extent: float = 0.01) -> np.ndarray:
+    out = np.empty((n, 12), dtype=np.float32)
+    load_workloads().lbvh_wl_uniform(seed, n, half_extent, _ptr(out))
+    return out
+
+
+def wl_clustered(seed: int, n: int, dup_fraction: float = 0.05) -> np.ndarray:
+    out = np.empty((n, 12), dtype=np.float32)
+    load_workloads().lbvh_wl_clustered(seed, n, dup_fraction, _ptr(out))
+    return out
+
+
+def wl_heightfield(seed: int, nx: int, ny: int) -> np.ndarray:
+    out = np.empty((2 * nx * ny, 12), dtype=np.float32)
+    load_workloads().lbvh_wl_heightfield(seed, nx, ny, _ptr(out))
+    return out
+
+
+def wl_log_uniform_sizes(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.uint32)
+    load_workloads().lbvh_wl_log_uniform_sizes(seed, count, lo, hi, _ptr(out))
+    return out
+
+
+def wl_jitter(seed: int, tri: np.ndarray, sigma: float) -> np.ndarray:
+    tri = np.ascontiguousarray(tri, dtype=np.float32)
+    out = np.empty_like(tri)
+    load_workloads().lbvh_wl_jitter(seed, tri.shape[0], sigma, _ptr(tri), _ptr(out))
+    return out
+
+
+def wl_bounds(tri: np.ndarray) -> np.ndarray:
+    tri = np.ascontiguousarray(tri, dtype=np.float32)
+    out = np.empty(6, dtype=np.float32)
+    load_workloads().lbvh_wl_bounds(_ptr(tri), tri.shape[0], _ptr(out))
+    return out
+
+
+def wl_uniform01(seed: int, first: int, count: int) -> np.ndarray:
+    out = np.empty(count, dtype=np.float64)
+    load_workloads().lbvh_wl_uniform01(seed, first, count, _ptr(out))
+    return out
+
+
+# Named workloads of the benchmark configurations.
+PARITY_SEED = 1
+CLUSTERED_SEED = 2
+HEIGHTFIELD_SEED = 3
+BATCH_SEED = 4
+
+
+def workload(name: str) -> np.ndarray:
+    if name == "uniform-64k":
+        return wl_uniform(PARITY_SEED, 65536, 0.01)
+    if name == "clustered-1m":
+        return wl_clustered(CLUSTERED_SEED, 1 << 20, 0.05)
+    if name == "heightfield-16m":
+        return wl_heightfield(HEIGHTFIELD_SEED, 4096, 2048)
+    raise KeyError(name)
