@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heightfield-16m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tree-tile", type=int, default=0,
+                    help="leaves per block of the fused up pass (256/512/1024)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus "
@@ -124,6 +126,8 @@ def main():
     tri = lb.workload(args.workload)
     n = tri.shape[0]
     ctx = lb.Context(0)
+    if args.tree_tile:
+        ctx.set_tuning(lb.TUNE_TREE_TILE, args.tree_tile)
     d_tri = ctx.to_device(tri)
     bvh = ctx.alloc(64 * n)
     otri = ctx.alloc(48 * n)
@@ -186,6 +190,7 @@ def main():
         "dtype": "fp32+u32",
         "data": "synthetic",
         "config": {"workload": args.workload, "triangles": n,
+                   "tree_tile": ctx.get_tuning(lb.TUNE_TREE_TILE),
                    "bounds": "GPU reduction", "div_mode": "mul_rcp",
                    "inputs": "resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),

@@ -79,6 +79,14 @@ const char *lbvh_last_error(const lbvh_ctx *ctx);
 const char *lbvh_status_string(int status);
 int lbvh_set_div_mode(lbvh_ctx *ctx, int div_mode);
 int lbvh_get_div_mode(const lbvh_ctx *ctx);
+/* Launch-shape tuning (no counterpart in the reference, whose workgroup and
+ * digit sizes are constants, e.g. WG_SIZE = 64 at aabb-z-idx.js:4).  Values
+ * are validated; results do not depend on them. */
+typedef enum {
+  LBVH_TUNE_TREE_TILE = 0 /* leaves per block of the fused up pass: 256|512|1024 */
+} lbvh_tunable;
+int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value);
+int lbvh_get_tuning(const lbvh_ctx *ctx, int param);
 /* The context's hipStream_t, for callers that record events or chain work. */
 void *lbvh_stream(lbvh_ctx *ctx);
 int lbvh_synchronize(lbvh_ctx *ctx);

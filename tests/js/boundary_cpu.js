@@ -1,0 +1,40 @@
+'use strict'
+// Node-side boundary checks that need no GPU: the module and addon load,
+// every reference entry point is exported, host-side conversions round like
+// the reference's Float32Array.set / DataView.setFloat32, and a missing
+// device surfaces as an Error with the lbvh_status code.
+const assert = require('assert')
+const path = require('path')
+const bvh = require(path.join(__dirname, '..', '..', 'webgpu-lvbh-demo_amd', 'bvh'))
+
+for (const name of ['initBVHBuild', 'initAABB_ZidxKernel', 'initRadixSortKernel',
+  'initRadixTreeKernel', 'initBVHUpPassKernel', 'initRearrangeKernel', 'createDevice']) {
+  assert.strictEqual(typeof bvh[name], 'function', name)
+}
+
+const g = bvh.installGlobals({})
+assert.strictEqual(g.initBVHBuild, bvh.initBVHBuild)
+
+// f64 -> f32 rounding of TRI_ARRAY and MODEL_BOUNDS
+const arr = [0.1, 1 / 3, 3.1415, 1e-8, -2.5e38, 7, 8, 9, 10, 11, 12, 13]
+const f = bvh._internal.toFloat32(arr, 1)
+const want = new Float32Array(12)
+want.set(arr)
+assert.deepStrictEqual(Array.from(f), Array.from(want))
+assert.strictEqual(f[0], Math.fround(0.1))
+const b = bvh._internal.toBounds({ min: [0.1, -1 / 3, 2], max: [1.1, 5.5, 1e10] })
+assert.strictEqual(b[1], Math.fround(-1 / 3))
+assert.strictEqual(b[5], Math.fround(1e10))
+assert.strictEqual(bvh._internal.toBounds(null), null)
+const typed = new Float32Array(24)
+assert.strictEqual(bvh._internal.toFloat32(typed, 2), typed)
+
+let threw = false
+try {
+  bvh.createDevice({ ordinal: 4096 })
+} catch (e) {
+  threw = true
+  assert.strictEqual(e.code, 6)
+}
+assert.ok(threw, 'createDevice on a missing device must throw')
+console.log('boundary_cpu ok')

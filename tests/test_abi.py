@@ -4,6 +4,7 @@ crashing."""
 import ctypes as C
 import os
 import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -69,9 +70,16 @@ def test_create_reports_missing_device(lb):
     assert st == 6 and not h.value
 
 
-def test_addon_exports_napi_module(lb):
-    path = os.path.join(lb.LIB_DIR, "lbvh.node")
-    if not os.path.exists("/usr/include/node/node_api.h"):
-        pytest.skip("Node headers absent")
-    syms = exported(path)
-    assert "napi_register_module_v1" in syms
+def test_node_boundary_loads_addon(lb):
+    """The Node.js module loads the N-API addon, exports the reference's entry
+    points, rounds host inputs like the reference and reports a missing
+    device as an Error carrying the status code (tests/js/boundary_cpu.js)."""
+    if not os.path.exists(os.path.join(lb.LIB_DIR, "lbvh.node")):
+        pytest.fail("lbvh.node was not built (make -C webgpu-lvbh-demo_amd addon)")
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed")
+    r = subprocess.run([node, os.path.join(ROOT, "tests", "js", "boundary_cpu.js")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "boundary_cpu ok" in r.stdout

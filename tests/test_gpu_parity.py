@@ -204,6 +204,23 @@ def test_build_clustered_1m(ctx, lb, ora, div_mode):
     assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
 
 
+@pytest.mark.parametrize("tile", [256, 512, 1024])
+def test_build_independent_of_tree_tile(ctx, lb, ora, tile):
+    """The fused up pass's tile size changes which splits merge in LDS and
+    which through HBM, never the result."""
+    say(f"tree tile {tile}")
+    ctx.set_tuning(lb.TUNE_TREE_TILE, tile)
+    try:
+        for name in ("clustered100k", "allsame777", "n257", "n2"):
+            tri = mesh_cases(lb)[name]
+            bvh = ctx.build_arrays(tri, None)[0]
+            assert np.array_equal(bvh, ora.build(tri, None, 0)[0]), name
+        tri = lb.workload("clustered-1m")
+        assert np.array_equal(ctx.build_arrays(tri, None)[0], ora.build(tri, None, 0)[0])
+    finally:
+        ctx.set_tuning(lb.TUNE_TREE_TILE, 512)
+
+
 def test_build_is_deterministic(ctx, lb):
     tri = lb.workload("uniform-64k")
     first = ctx.build_arrays(tri, None)[0]

@@ -1,0 +1,244 @@
+'use strict'
+// bvh/index.js -- the reference's scripts/bvh interface on MI355X.
+//
+// Drop-in for AddisonPrairie/WebGPU-LVBH-demo scripts/bvh:
+//   initBVHBuild(device)                 build-bvh.js:1-10
+//     .build(TRI_ARRAY, NUM_TRIS, MODEL_BOUNDS)
+//         -> Promise<{ BVH_BUFFER, O_TRIANGLE_BUFFER }>  build-bvh.js:12-66
+//   initAABB_ZidxKernel(device).execute(TRI, N, bounds)  aabb-z-idx.js:57-148
+//   initRadixSortKernel(device).execute(KEYS, N)         radix-sort.js:161-500
+//   initRadixTreeKernel(device).execute(SORTED_KEYS, N)  radix-tree.js:50-120
+//   initBVHUpPassKernel(device).execute(IDX, AABB, PARENT, N)
+//                                                        bvh-up-pass.js:70-156
+//   initRearrangeKernel(device).execute(I_TRI, IDX, N)   rearrange.js:57-139
+//
+// `device` comes from createDevice() instead of navigator.gpu; buffers are
+// LBVHBuffer handles with the reference's byte sizes, .size and .destroy(),
+// plus .read() (the reference never reads back).  All GPU work goes through
+// the N-API addon into liblbvh.so (HIP kernels); nothing runs on the CPU.
+// Written for Node 12: no optional chaining, no top-level await.
+
+const path = require('path')
+
+const addon = require(path.join(__dirname, '..', 'lib', 'lbvh.node'))
+
+const DIV_MODES = { mul_rcp: 0, ieee: 1 }
+
+class LBVHBuffer {
+  constructor (device, size, ptr) {
+    this.device = device
+    this.size = size
+    this.ptr = ptr
+  }
+
+  destroy () {
+    if (this.ptr !== null) {
+      addon.free(this.device.ctx, this.ptr)
+      this.ptr = null
+    }
+  }
+
+  // Device -> host copy after all queued work; resolves to an ArrayBuffer.
+  async read () {
+    await this.device.queue.onSubmittedWorkDone()
+    return addon.download(this.device.ctx, this.ptr, this.size)
+  }
+}
+
+class LBVHDevice {
+  constructor (options) {
+    const o = options || {}
+    this.ctx = addon.createContext(o.ordinal || 0, o.hbmBudget || 0)
+    this.setDivMode(o.divMode || 'mul_rcp')
+    const ctx = this.ctx
+    this.queue = {
+      // The context's stream drains off the JS thread.
+      onSubmittedWorkDone: function () { return addon.synchronize(ctx) }
+    }
+  }
+
+  setDivMode (mode) {
+    if (!(mode in DIV_MODES)) throw new TypeError(`unknown division mode ${mode}`)
+    addon.setDivMode(this.ctx, DIV_MODES[mode])
+    this.divMode = mode
+  }
+
+  // GPUDevice.createBuffer({ size }) (build-bvh.js:14-18); `data` optionally
+  // fills it (the mappedAtCreation upload, build-bvh.js:20-21).
+  createBuffer (descriptor, data) {
+    const size = descriptor.size
+    const buf = new LBVHBuffer(this, size, addon.alloc(this.ctx, size))
+    if (data !== undefined && size > 0) addon.upload(this.ctx, buf.ptr, data)
+    return buf
+  }
+
+  bytesInUse () { return addon.bytesInUse(this.ctx) }
+}
+
+function createDevice (options) {
+  return new LBVHDevice(options)
+}
+
+// TRI_ARRAY (JS Array of 12*N numbers, parse-obj.js:55-59) -> Float32Array,
+// rounding each value like Float32Array.set (build-bvh.js:20).  Typed arrays
+// and ArrayBuffers are used as they are.
+function toFloat32 (TRI_ARRAY, NUM_TRIS) {
+  if (TRI_ARRAY instanceof Float32Array) return TRI_ARRAY
+  if (TRI_ARRAY instanceof ArrayBuffer) return new Float32Array(TRI_ARRAY)
+  const out = new Float32Array(12 * NUM_TRIS)
+  out.set(TRI_ARRAY)
+  return out
+}
+
+// MODEL_BOUNDS {min:[x,y,z], max:[x,y,z]} -> Float32Array(6), rounded like
+// DataView.setFloat32 (aabb-z-idx.js:116-122); null -> GPU reduction.
+function toBounds (MODEL_BOUNDS) {
+  if (MODEL_BOUNDS === undefined || MODEL_BOUNDS === null) return null
+  const b = new Float32Array(6)
+  for (let c = 0; c < 3; c++) {
+    b[c] = MODEL_BOUNDS.min[c]
+    b[3 + c] = MODEL_BOUNDS.max[c]
+  }
+  return b
+}
+
+function initBVHBuild (device) {
+  return { build }
+
+  // build-bvh.js:12-66: upload, one fused device pipeline, free the input.
+  async function build (TRI_ARRAY, NUM_TRIS, MODEL_BOUNDS) {
+    const tris = toFloat32(TRI_ARRAY, NUM_TRIS)
+    const I_TRIANGLE_BUFFER = device.createBuffer({ size: NUM_TRIS * 48 }, tris)
+    const BVH_BUFFER = device.createBuffer({ size: NUM_TRIS * 64 })
+    const O_TRIANGLE_BUFFER = device.createBuffer({ size: NUM_TRIS * 48 })
+    if (NUM_TRIS > 0) {
+      addon.build(device.ctx, I_TRIANGLE_BUFFER.ptr, NUM_TRIS,
+        toBounds(MODEL_BOUNDS), BVH_BUFFER.ptr, O_TRIANGLE_BUFFER.ptr, null, null)
+    }
+    await device.queue.onSubmittedWorkDone()
+    I_TRIANGLE_BUFFER.destroy()
+    return { BVH_BUFFER, O_TRIANGLE_BUFFER }
+  }
+}
+
+// ---- per-stage shims (the reference's execute() functions) -----------------
+
+function initAABB_ZidxKernel (device) {
+  return { execute }
+
+  // aabb-z-idx.js:57-148
+  async function execute (TRIANGLE_BUFFER, size, bounds) {
+    if (TRIANGLE_BUFFER.size !== 48 * size) {
+      console.warn(`in AABB/Z-index: buffer size [ ${TRIANGLE_BUFFER.size} ] does not match requested size [ ${size} ]`)
+      return
+    }
+    const AABB_BUFFER = device.createBuffer({ size: size * 32 })
+    const Z_IDX_BUFFER = device.createBuffer({ size: size * 4 })
+    if (size > 0) {
+      addon.aabbMorton(device.ctx, TRIANGLE_BUFFER.ptr, size, toBounds(bounds),
+        AABB_BUFFER.ptr, Z_IDX_BUFFER.ptr)
+    }
+    await device.queue.onSubmittedWorkDone()
+    return { AABB_BUFFER, Z_IDX_BUFFER }
+  }
+}
+
+function initRadixSortKernel (device) {
+  return { execute }
+
+  // radix-sort.js:161-500: destructive, sorted keys replace valBuffer's.
+  // (The reference's guard calls console.warning, which does not exist; a
+  // warning is printed here instead.)
+  async function execute (valBuffer, size) {
+    if (valBuffer.size !== size * 4) {
+      console.warn(`in radix sort: buffer size [ ${valBuffer.size} ] does not match requested size [ ${size} ]`)
+      return
+    }
+    const IDX_BUFFER = device.createBuffer({ size: valBuffer.size })
+    if (size > 0) addon.sort(device.ctx, valBuffer.ptr, IDX_BUFFER.ptr, size)
+    await device.queue.onSubmittedWorkDone()
+    return { IDX_BUFFER }
+  }
+}
+
+function initRadixTreeKernel (device) {
+  return { execute }
+
+  // radix-tree.js:50-120
+  async function execute (KEY_BUFFER, SIZE) {
+    if (KEY_BUFFER.size !== 4 * SIZE) {
+      console.warn(`in radix tree: buffer size [ ${KEY_BUFFER.size} ] does not match requested size [ ${SIZE} ]`)
+      return
+    }
+    const PARENT_BUFFER = device.createBuffer({ size: SIZE * 8 })
+    if (SIZE > 0) addon.radixTree(device.ctx, KEY_BUFFER.ptr, SIZE, PARENT_BUFFER.ptr)
+    await device.queue.onSubmittedWorkDone()
+    return { PARENT_BUFFER }
+  }
+}
+
+function initBVHUpPassKernel (device) {
+  return { execute }
+
+  // bvh-up-pass.js:70-156 (the reference has no size guard here)
+  async function execute (IDX_BUFFER, AABB_BUFFER, PARENT_BUFFER, size) {
+    const BVH_BUFFER = device.createBuffer({ size: size * 64 })
+    if (size > 0) {
+      addon.upPass(device.ctx, IDX_BUFFER.ptr, AABB_BUFFER.ptr, PARENT_BUFFER.ptr,
+        size, BVH_BUFFER.ptr)
+    }
+    await device.queue.onSubmittedWorkDone()
+    return { BVH_BUFFER }
+  }
+}
+
+function initRearrangeKernel (device) {
+  return { execute }
+
+  // rearrange.js:57-139
+  async function execute (I_TRIANGLE_BUFFER, INDEX_BUFFER, size) {
+    if (I_TRIANGLE_BUFFER.size !== 48 * size) {
+      console.warn(`in rearrange: triangle buffer size [ ${I_TRIANGLE_BUFFER.size} ] does not match requested size [ ${size} ]`)
+      return
+    }
+    if (INDEX_BUFFER.size !== 4 * size) {
+      console.warn(`in rearrange: index buffer size [ ${INDEX_BUFFER.size} ] does not match requested size [ ${size} ]`)
+      return
+    }
+    const O_TRIANGLE_BUFFER = device.createBuffer({ size: I_TRIANGLE_BUFFER.size })
+    if (size > 0) {
+      addon.rearrange(device.ctx, I_TRIANGLE_BUFFER.ptr, INDEX_BUFFER.ptr, size,
+        O_TRIANGLE_BUFFER.ptr)
+    }
+    await device.queue.onSubmittedWorkDone()
+    return { O_TRIANGLE_BUFFER }
+  }
+}
+
+// The reference loads its scripts as browser globals (index.html:3-12);
+// installGlobals() defines the same names on the global object.
+function installGlobals (target) {
+  const g = target || global
+  g.initBVHBuild = initBVHBuild
+  g.initAABB_ZidxKernel = initAABB_ZidxKernel
+  g.initRadixSortKernel = initRadixSortKernel
+  g.initRadixTreeKernel = initRadixTreeKernel
+  g.initBVHUpPassKernel = initBVHUpPassKernel
+  g.initRearrangeKernel = initRearrangeKernel
+  return g
+}
+
+module.exports = {
+  createDevice,
+  initBVHBuild,
+  initAABB_ZidxKernel,
+  initRadixSortKernel,
+  initRadixTreeKernel,
+  initBVHUpPassKernel,
+  initRearrangeKernel,
+  installGlobals,
+  LBVHBuffer,
+  LBVHDevice,
+  addon,
+  _internal: { toFloat32, toBounds }
+}

@@ -115,21 +115,77 @@ __device__ void write_single_leaf_node(int4 *nodes, const float mn[3],
   nodes[3] = make_int4(0, 0, 0, 0);
 }
 
+// Exchanges this side of split s through HBM.  Each side record is seven
+// 64-bit words (box min.xyz, max.xyz, far end), each carrying the operation's
+// epoch in its high half and written with a relaxed agent-scope atomic store.
+// The arrival tag is swapped relaxed too, so no cache-wide release/acquire
+// fence is issued: the second arriver instead polls each sibling word until
+// it shows the current epoch.  The sibling stored those words before its own
+// swap, which this thread has already observed, so every poll ends; the cap
+// only bounds the loop should that ever break (ctl->fault is then set).
+// Returns true, with the sibling's box and far end, for the second arriver.
+__device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
+                                              Ctl *ctl, uint32_t epoch,
+                                              int64_t s, int side,
+                                              int far_self, const float mn[3],
+                                              const float mx[3], float smn[3],
+                                              float smx[3], int &sfar) {
+  uint64_t *mine = gside[2 * s + side].w;
+  const uint64_t stamp = (uint64_t)epoch << 32;
+  const uint32_t v[7] = {__float_as_uint(mn[0]), __float_as_uint(mn[1]),
+                         __float_as_uint(mn[2]), __float_as_uint(mx[0]),
+                         __float_as_uint(mx[1]), __float_as_uint(mx[2]),
+                         (uint32_t)far_self};
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    __hip_atomic_store(mine + k, stamp | v[k], __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  // Stores issued and acknowledged before the swap (no cache maintenance at
+  // workgroup scope), so the sibling normally finds them on its first poll.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  const uint32_t old = __hip_atomic_exchange(gtag + s, epoch, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+  if (old != epoch) return false;
+  const uint64_t *other = gside[2 * s + (1 - side)].w;
+  uint32_t o[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    uint64_t w;
+    uint32_t spins = 0;
+    while (true) {
+      w = __hip_atomic_load(other + k, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      if ((uint32_t)(w >> 32) == epoch) break;
+      if (++spins > kSpinCap) {
+        atomicOr(&ctl->fault, 2u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    o[k] = (uint32_t)w;
+  }
+  smn[0] = __uint_as_float(o[0]); smn[1] = __uint_as_float(o[1]);
+  smn[2] = __uint_as_float(o[2]); smx[0] = __uint_as_float(o[3]);
+  smx[1] = __uint_as_float(o[4]); smx[2] = __uint_as_float(o[5]);
+  sfar = (int)o[6];
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Per-stage up pass (bvh_upward_pass, bvh-up-pass.js:200-280)
 // ---------------------------------------------------------------------------
 
 // One thread per leaf climbs through the parents array.  Each child writes
 // its half of the parent node (Karras order: a child is the left one when
-// the child numbered one above it has the same parent), then swaps the
-// build's epoch into the parent's arrival tag (agent-scope acq_rel).  The
-// first arriver stops; the second reads its sibling's half, merges and
-// climbs.  Unlike the reference there is no spin-wait: the acquire makes
-// the sibling's half visible.
+// the child numbered one above it has the same parent) and publishes its box
+// through the same epoch-stamped exchange as the fused kernel (arrive_global,
+// keyed by the parent's id); the first arriver stops, the second merges and
+// climbs.  Unlike the reference there is no zero-sentinel spin
+// (bvh-up-pass.js:265-268): the stamps say when the sibling's box is there.
 __global__ __launch_bounds__(kMapThreads) void k_up_pass(
     const int32_t *__restrict__ idx, const int4 *__restrict__ aabb,
-    const int32_t *__restrict__ par, uint32_t n, int4 *nodes, uint32_t *tags,
-    const Ctl *ctl) {
+    const int32_t *__restrict__ par, uint32_t n, int4 *nodes, GSide *gside,
+    uint32_t *tags, Ctl *ctl) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const uint32_t epoch = ctl->epoch;
@@ -167,16 +223,16 @@ __global__ __launch_bounds__(kMapThreads) void k_up_pass(
       node[2] = pack3(mn, id);
       node[3] = pack3(mx, 0);
     }
-    const uint32_t old = __hip_atomic_exchange(tags + p, epoch, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-    if (old != epoch) return;  // first arriver
-    const int4 omn = node[left ? 2 : 0], omx = node[left ? 3 : 1];
-    mn[0] = fminf(mn[0], __int_as_float(omn.x));
-    mn[1] = fminf(mn[1], __int_as_float(omn.y));
-    mn[2] = fminf(mn[2], __int_as_float(omn.z));
-    mx[0] = fmaxf(mx[0], __int_as_float(omx.x));
-    mx[1] = fmaxf(mx[1], __int_as_float(omx.y));
-    mx[2] = fmaxf(mx[2], __int_as_float(omx.z));
+    float smn[3], smx[3];
+    int unused;
+    if (!arrive_global(gside, tags, ctl, epoch, p, left ? 0 : 1, 0, mn, mx,
+                       smn, smx, unused))
+      return;  // first arriver
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], smn[k]);
+      mx[k] = fmaxf(mx[k], smx[k]);
+    }
     if (p == 0) return;  // root merged
     leaf = false;
     num = p;
@@ -206,11 +262,12 @@ __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
 }
 
+template <int T>
 struct TreeLds {
-  int delta[kTreeThreads + 1];         // delta(ts-1+k, ts+k), k = 0..T
-  float box[2][6][kTreeThreads];       // [side][min.xyz max.xyz][split slot]
-  int far[2][kTreeThreads];            // far end of the side's leaf range
-  uint32_t cnt[kTreeThreads];          // local arrivals per split slot
+  int delta[T + 1];         // delta(ts-1+k, ts+k), k = 0..T
+  float box[2][6][T];       // [side][min.xyz max.xyz][split slot]
+  int far[2][T];            // far end of the side's leaf range
+  uint32_t cnt[T];          // local arrivals per split slot
 };
 
 // State of one climbing thread: the completed node [l, r] and its box, plus
@@ -228,36 +285,14 @@ struct Climb {
   int pside;
 };
 
-// Exchanges this side of split s through HBM.  Returns true (with the
-// sibling's box and far end) for the second arriver.
-__device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
-                                              uint32_t epoch, int64_t s,
-                                              int side, int far_self,
-                                              const float mn[3],
-                                              const float mx[3], float smn[3],
-                                              float smx[3], int &sfar) {
-  int4 *mine = reinterpret_cast<int4 *>(gside + 2 * s + side);
-  mine[0] = pack3(mn, far_self);
-  mine[1] = pack3(mx, 0);
-  const uint32_t old = __hip_atomic_exchange(gtag + s, epoch, __ATOMIC_ACQ_REL,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-  if (old != epoch) return false;
-  const int4 *other = reinterpret_cast<const int4 *>(gside + 2 * s + (1 - side));
-  const int4 a = other[0], b = other[1];
-  smn[0] = __int_as_float(a.x); smn[1] = __int_as_float(a.y);
-  smn[2] = __int_as_float(a.z); sfar = a.w;
-  smx[0] = __int_as_float(b.x); smx[1] = __int_as_float(b.y);
-  smx[2] = __int_as_float(b.z);
-  return true;
-}
-
 // Climbs until the thread stops: first arrival at a split (locally pending
 // or globally done) or the root.  `resume` re-enters a pending local
 // arrival through HBM.
-__device__ void climb(Climb &c, TreeLds &L, const uint32_t *__restrict__ skeys,
-                      int64_t n, int64_t ts, int64_t te, int4 *nodes,
-                      GSide *gside, uint32_t *gtag, uint32_t epoch,
-                      bool resume) {
+template <int T>
+__device__ void climb(Climb &c, TreeLds<T> &L,
+                      const uint32_t *__restrict__ skeys, int64_t n,
+                      int64_t ts, int64_t te, int4 *nodes, GSide *gside,
+                      uint32_t *gtag, Ctl *ctl, uint32_t epoch, bool resume) {
   while (true) {
     int64_t s;
     int side, far_self;
@@ -312,8 +347,8 @@ __device__ void climb(Climb &c, TreeLds &L, const uint32_t *__restrict__ skeys,
       sfar = L.far[1 - side][slot];
     } else {
       c.global = true;
-      if (!arrive_global(gside, gtag, epoch, s, side, far_self, c.mn, c.mx, smn,
-                         smx, sfar))
+      if (!arrive_global(gside, gtag, ctl, epoch, s, side, far_self, c.mn,
+                         c.mx, smn, smx, sfar))
         return;
     }
     // Merge into the split-s node: children [L, s] and [s+1, R].
@@ -342,24 +377,25 @@ __device__ void climb(Climb &c, TreeLds &L, const uint32_t *__restrict__ skeys,
   }
 }
 
-// One block per tile of kTreeThreads sorted positions.  Phase 1: every
-// thread gathers its triangle into leaf order (O_TRI), takes its leaf box
-// and climbs, merging through LDS while both children of a split lie in the
-// tile; a split at a tile seam goes through HBM at once.  Phase 2 (after the
-// block barrier): a first arriver whose local sibling never came (the
-// sibling subtree reaches outside the tile) re-arrives through HBM.  Above
-// the seams every step is a global exchange.  No thread ever waits on
-// another, so there is no spinning.
-__global__ __launch_bounds__(kTreeThreads) void k_tree_fused(
+// One block per tile of T sorted positions.  Phase 1: every thread gathers
+// its triangle into leaf order (O_TRI), takes its leaf box and climbs,
+// merging through LDS while both children of a split lie in the tile; a
+// split at a tile seam goes through HBM at once.  Phase 2 (after the block
+// barrier): a first arriver whose local sibling never came (the sibling
+// subtree reaches outside the tile) re-arrives through HBM.  Above the seams
+// every step is a global exchange.  No thread waits for another thread's
+// progress, only for stores already issued.
+template <int T>
+__global__ __launch_bounds__(T) void k_tree_fused(
     const int4 *__restrict__ tri, const uint32_t *__restrict__ skeys,
     const int32_t *__restrict__ perm, uint32_t n, int4 *__restrict__ otri,
-    int4 *nodes, GSide *gside, uint32_t *gtag, const Ctl *ctl) {
-  __shared__ TreeLds L;
+    int4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl) {
+  __shared__ TreeLds<T> L;
   const int64_t t = threadIdx.x;
-  const int64_t ts = (int64_t)blockIdx.x * kTreeThreads;
+  const int64_t ts = (int64_t)blockIdx.x * T;
   const int64_t i = ts + t;
   const int64_t N = n;
-  const int64_t te = (ts + kTreeThreads < N) ? ts + kTreeThreads : N;
+  const int64_t te = (ts + T < N) ? ts + T : N;
   const uint32_t epoch = ctl->epoch;
 
   L.cnt[t] = 0u;
@@ -393,12 +429,12 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree_fused(
     int4 *last = nodes + 4 * (N - 1);
     last[0] = z; last[1] = z; last[2] = z; last[3] = z;
   }
-  if (i < N) climb(c, L, skeys, N, ts, te, nodes, gside, gtag, epoch, false);
+  if (i < N) climb<T>(c, L, skeys, N, ts, te, nodes, gside, gtag, ctl, epoch, false);
   __syncthreads();
   if (c.pend && L.cnt[c.ps - ts] == 1u) {
     c.pend = false;
     c.global = true;
-    climb(c, L, skeys, N, ts, te, nodes, gside, gtag, epoch, true);
+    climb<T>(c, L, skeys, N, ts, te, nodes, gside, gtag, ctl, epoch, true);
   }
 }
 
@@ -420,11 +456,11 @@ void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
 
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     const int32_t *parents, uint32_t n, float4 *nodes,
-                    uint32_t *tags, const Ctl *ctl, hipStream_t s) {
+                    GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s) {
   hipLaunchKernelGGL(k_up_pass, dim3(blocks_for(n, kMapThreads)),
                      dim3(kMapThreads), 0, s, idx,
                      reinterpret_cast<const int4 *>(aabb), parents, n,
-                     reinterpret_cast<int4 *>(nodes), tags, ctl);
+                     reinterpret_cast<int4 *>(nodes), gside, tags, ctl);
 }
 
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
@@ -437,13 +473,28 @@ void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
 
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, GSide *gside, uint32_t *gtag,
-                       const Ctl *ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_tree_fused, dim3(blocks_for(n, kTreeThreads)),
-                     dim3(kTreeThreads), 0, s,
-                     reinterpret_cast<const int4 *>(tri), skeys, perm, n,
-                     reinterpret_cast<int4 *>(otri),
-                     reinterpret_cast<int4 *>(nodes), gside, gtag, ctl);
+                       float4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl,
+                       int tile, hipStream_t s) {
+  const int4 *t4 = reinterpret_cast<const int4 *>(tri);
+  int4 *o4 = reinterpret_cast<int4 *>(otri);
+  int4 *n4 = reinterpret_cast<int4 *>(nodes);
+  switch (tile) {
+    case 256:
+      hipLaunchKernelGGL(k_tree_fused<256>, dim3(blocks_for(n, 256)),
+                         dim3(256), 0, s, t4, skeys, perm, n, o4, n4, gside,
+                         gtag, ctl);
+      break;
+    case 1024:
+      hipLaunchKernelGGL(k_tree_fused<1024>, dim3(blocks_for(n, 1024)),
+                         dim3(1024), 0, s, t4, skeys, perm, n, o4, n4, gside,
+                         gtag, ctl);
+      break;
+    default:
+      hipLaunchKernelGGL(k_tree_fused<512>, dim3(blocks_for(n, 512)),
+                         dim3(512), 0, s, t4, skeys, perm, n, o4, n4, gside,
+                         gtag, ctl);
+      break;
+  }
 }
 
 }  // namespace lbvh

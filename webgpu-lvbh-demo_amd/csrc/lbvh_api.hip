@@ -31,6 +31,7 @@ struct lbvh_ctx {
   uint64_t budget = 0;  // 0 = unlimited
   uint64_t in_use = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
+  int tree_tile = kTreeTileDefault;
   std::string last_error;
   std::unordered_map<void *, uint64_t> user_allocs;
   std::recursive_mutex mu;
@@ -333,6 +334,28 @@ int lbvh_get_div_mode(const lbvh_ctx *ctx) {
   return ctx ? ctx->div_mode : -1;
 }
 
+int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value) {
+  CTX_GUARD(ctx);
+  switch (param) {
+    case LBVH_TUNE_TREE_TILE:
+      if (value != 256 && value != 512 && value != 1024)
+        return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                    "tree tile must be 256, 512 or 1024");
+      ctx->tree_tile = value;
+      return LBVH_OK;
+    default:
+      return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "unknown tuning parameter");
+  }
+}
+
+int lbvh_get_tuning(const lbvh_ctx *ctx, int param) {
+  if (!ctx) return -1;
+  switch (param) {
+    case LBVH_TUNE_TREE_TILE: return ctx->tree_tile;
+    default: return -1;
+  }
+}
+
 void *lbvh_stream(lbvh_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int lbvh_synchronize(lbvh_ctx *ctx) {
@@ -500,10 +523,12 @@ int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
   if (!idx || !aabb || !parents || !bvh_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   if ((st = ensure(ctx, ctx->ctag, 4ull * n, true))) return st;
+  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
   if ((st = begin_op(ctx))) return st;
   launch_bump_epoch(ctx->ctl, ctx->stream);
   launch_up_pass(idx, reinterpret_cast<const float4 *>(aabb), parents, n,
                  static_cast<float4 *>(bvh_out),
+                 static_cast<GSide *>(ctx->gside.ptr),
                  static_cast<uint32_t *>(ctx->ctag.ptr), ctx->ctl, ctx->stream);
   return check_launch(ctx, "lbvh_up_pass");
 }
@@ -533,7 +558,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   if ((st = ensure_sort_scratch(ctx, n))) return st;
-  if ((st = ensure(ctx, ctx->gside, 64ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
   if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
   if ((st = begin_op(ctx))) return st;
@@ -572,7 +597,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
                       static_cast<float4 *>(bvh_out),
                       static_cast<GSide *>(ctx->gside.ptr),
                       static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
-                      ctx->stream);
+                      ctx->tree_tile, ctx->stream);
   }
   return check_launch(ctx, "lbvh_build");
 }
@@ -586,7 +611,7 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !sorted_keys || !perm || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = ensure(ctx, ctx->gside, 64ull * n, false))) return st;
+  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
   if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
   if ((st = begin_op(ctx))) return st;
   launch_bump_epoch(ctx->ctl, ctx->stream);
@@ -596,7 +621,7 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
                     static_cast<float4 *>(bvh_out),
                     static_cast<GSide *>(ctx->gside.ptr),
                     static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
-                    ctx->stream);
+                    ctx->tree_tile, ctx->stream);
   return check_launch(ctx, "lbvh_refit");
 }
 

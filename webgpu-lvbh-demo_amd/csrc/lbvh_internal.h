@@ -15,7 +15,7 @@ constexpr int kSortThreads = 256;        // 4 wave64 per tile
 constexpr int kSortItems = 16;           // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 
-constexpr int kTreeThreads = 256;  // leaves per tile of the fused up pass
+constexpr int kTreeTileDefault = 512;  // leaves per tile of the fused up pass
 
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
 constexpr int kBoundsBlocks = 1024;  // grid of the bounds reduction
@@ -49,10 +49,10 @@ struct Ctl {
 };
 
 // Exchange record of one side of a split in the fused up pass: the side's box
-// and the far end of its leaf range.  Two sides per split (64 B).
-struct __align__(16) GSide {
-  float4 lo;  // min.xyz, far end (int bits)
-  float4 hi;  // max.xyz, unused
+// (min.xyz, max.xyz) and the far end of its leaf range, one value per 64-bit
+// word with the operation's epoch in the high half.  Two sides per split.
+struct __align__(64) GSide {
+  uint64_t w[8];  // 7 used
 };
 
 // Morton quantisation parameters (uniform block of aabb-z-idx.js:164-169).
@@ -95,12 +95,13 @@ void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
                        hipStream_t s);
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     const int32_t *parents, uint32_t n, float4 *nodes,
-                    uint32_t *tags, const Ctl *ctl, hipStream_t s);
+                    GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s);
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                       float4 *out, hipStream_t s);
+// tile: 256, 512 or 1024 leaves per block.
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, GSide *gside, uint32_t *gtag,
-                       const Ctl *ctl, hipStream_t s);
+                       float4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl,
+                       int tile, hipStream_t s);
 
 }  // namespace lbvh

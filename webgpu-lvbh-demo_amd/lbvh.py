@@ -23,6 +23,8 @@ WL_PATH = os.path.join(LIB_DIR, "liblbvh_workloads.so")
 DIV_MUL_RCP = 0
 DIV_IEEE = 1
 
+TUNE_TREE_TILE = 0
+
 STATUS = {
     0: "ok",
     1: "invalid argument",
@@ -48,6 +50,8 @@ SIGNATURES = {
     "lbvh_status_string": (C.c_char_p, [_i32]),
     "lbvh_set_div_mode": (_i32, [_vp, _i32]),
     "lbvh_get_div_mode": (_i32, [_vp]),
+    "lbvh_set_tuning": (_i32, [_vp, _i32, _i32]),
+    "lbvh_get_tuning": (_i32, [_vp, _i32]),
     "lbvh_stream": (_vp, [_vp]),
     "lbvh_synchronize": (_i32, [_vp]),
     "lbvh_bytes_in_use": (_u64, [_vp]),
@@ -211,6 +215,12 @@ class Context:
 
     def set_div_mode(self, mode: int):
         self._check(self.lib.lbvh_set_div_mode(self.handle, mode))
+
+    def set_tuning(self, param: int, value: int):
+        self._check(self.lib.lbvh_set_tuning(self.handle, param, value))
+
+    def get_tuning(self, param: int) -> int:
+        return self.lib.lbvh_get_tuning(self.handle, param)
 
     def profile_enable(self, on: bool = True):
         self._check(self.lib.lbvh_profile_enable(self.handle, int(on)))
