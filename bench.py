@@ -63,6 +63,11 @@ class Hip:
         self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         self.h.hipEventDestroy.argtypes = [C.c_void_p]
 
+    def copy_d2d(self, dst, src, nbytes, stream):
+        f = self.h.hipMemcpyAsync
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        assert f(C.c_void_p(dst), C.c_void_p(src), nbytes, 3, C.c_void_p(stream)) == 0
+
     def event(self):
         e = C.c_void_p()
         assert self.h.hipEventCreate(C.byref(e)) == 0
@@ -114,6 +119,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tree-tile", type=int, default=0,
                     help="leaves per block of the fused up pass (256/512/1024)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run device-to-device copies of MIB MiB, the "
+                         "known-size reference for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus "
@@ -131,6 +139,15 @@ def main():
     d_tri = ctx.to_device(tri)
     bvh = ctx.alloc(64 * n)
     otri = ctx.alloc(48 * n)
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        for _ in range(3):
+            hip.copy_d2d(dst.ptr, src.ptr, nbytes, ctx.stream)
+        ctx.synchronize()
+        src.destroy()
+        dst.destroy()
+        log(f"copy calibration: 3 x {nbytes} bytes")
     log(f"warmup x{args.warmup}")
     for _ in range(args.warmup):
         ctx.build(d_tri, n, None, bvh, otri)

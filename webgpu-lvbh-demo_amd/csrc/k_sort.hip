@@ -29,6 +29,7 @@ constexpr int kWaves = kSortThreads / 64;
 static_assert(kSortThreads == kRadix, "one thread per digit in the look-back");
 constexpr uint32_t kFlagAggregate = 1u;
 constexpr uint32_t kFlagInclusive = 2u;
+constexpr int kLookback = 8;  // predecessor tiles probed per look-back step
 
 __device__ __forceinline__ uint64_t lb_pack(uint32_t stamp, uint32_t flag,
                                             uint32_t count) {
@@ -147,31 +148,46 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const uint32_t gdigit = block_exclusive_scan(gcount, s_wsum);
   s_start[d] = start;
 
-  // Decoupled look-back for digit d.
-  // Every earlier tile drew its id before this one and publishes its
-  // aggregate without waiting, so the wait ends; the spin cap only bounds
-  // the loop should that invariant ever break (the result is then wrong,
-  // and ctl->fault says so, but the kernel still exits).
+  // Decoupled look-back for digit d, kLookback predecessor words per probe:
+  // the loads of one probe are independent, so a walk over tiles that have
+  // only published aggregates advances kLookback tiles per memory round
+  // trip.  Every earlier tile drew its id before this one and publishes its
+  // aggregate without waiting, so the walk ends; the spin cap only bounds the
+  // loop should that invariant ever break (the result is then wrong and
+  // ctl->fault says so, but the kernel still exits).
   uint32_t excl = 0;
   if (tile > 0) {
     int64_t k = (int64_t)tile - 1;
     uint32_t spins = 0;
     while (k >= 0) {
-      const uint64_t w = __hip_atomic_load(status + (size_t)k * kRadix + d,
-                                           __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t hi = (uint32_t)(w >> 32);
-      if ((hi >> 2) != stamp) {
+      uint64_t w[kLookback];
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j)
+        w[j] = (k - j >= 0)
+                   ? __hip_atomic_load(status + (size_t)(k - j) * kRadix + d,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                   : 0ull;
+      int used = 0;
+      bool done = false;
+#pragma unroll
+      for (int j = 0; j < kLookback; ++j) {
+        if (done || used < j) continue;      // stopped earlier in this probe
+        if (k - j < 0) { done = true; continue; }
+        const uint32_t hi = (uint32_t)(w[j] >> 32);
+        if ((hi >> 2) != stamp) continue;  // not published yet
+        excl += (uint32_t)w[j];
+        used = j + 1;
+        if ((hi & 3u) == kFlagInclusive) done = true;
+      }
+      if (done) break;
+      k -= used;
+      if (used == 0) {
         if (++spins > kSpinCap) {
           atomicOr(&ctl->fault, 1u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += (uint32_t)w;
-      if ((hi & 3u) == kFlagInclusive) break;
-      --k;
     }
     __hip_atomic_store(my_word, lb_pack(stamp, kFlagInclusive, excl + total),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
