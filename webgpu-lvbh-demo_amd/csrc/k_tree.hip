@@ -255,108 +255,117 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 }
 
 // ---------------------------------------------------------------------------
-// Fused leaf + tree + up pass
+// Fused leaf + tree + up pass: multi-level agglomeration
 // ---------------------------------------------------------------------------
+//
+// Level 0 runs over the sorted leaves, one block per tile of T positions:
+// each thread gathers its triangle into leaf order (O_TRI), takes the leaf
+// box and climbs, merging in LDS whenever both children of a split lie in
+// the tile.  A climber stops when its parent split is outside the tile, or
+// when it arrives first at a split whose other child never completes in the
+// tile.  Those stopped subtrees are the tile's frontier: each has its parent
+// on one of the two root paths through the tile's edge gaps, and a root path
+// of the position-augmented keys has at most 64 nodes, so a tile has at most
+// kFrontierMax of them.  The frontier is written, in leaf order, as "items"
+// (leaf range + box) to a compact array whose offsets come from a
+// single-value decoupled look-back over the tiles.  Level L >= 1 runs the
+// same agglomeration over the items of level L-1, until one tile holds all
+// remaining items and completes the root.  No block waits on another
+// block's tree work; nodes are written exactly once, by the thread that
+// merges their children, as soon as their Karras id is known.
+
+constexpr int kFrontierMax = 128;
+constexpr uint32_t kLbAggregate = 1u, kLbInclusive = 2u;
 
 __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
 }
 
 template <int T>
-struct TreeLds {
-  int delta[T + 1];         // delta(ts-1+k, ts+k), k = 0..T
-  float box[2][6][T];       // [side][min.xyz max.xyz][split slot]
-  int far[2][T];            // far end of the side's leaf range
-  uint32_t cnt[T];          // local arrivals per split slot
+struct AggLds {
+  int dgap[T + 1];     // delta at the gap after item ts-1+k (leaf space)
+  int il[T], ir[T];    // leaf range of each item of the tile
+  float box[2][6][T];  // [side][min.xyz max.xyz][gap slot]
+  int far[2][T];       // far item of the side that arrived
+  uint32_t cnt[T];     // arrivals per gap slot
+  uint32_t wsum[16];
+  uint32_t tile, base;
 };
 
-// State of one climbing thread: the completed node [l, r] and its box, plus
-// the halves of that node once its children merged (written when its id is
-// known, i.e. once its parent's side is decided).
-struct Climb {
-  int64_t l, r;
+// A climbing subtree: items [ia, ib] covering leaves [l, r], its box, and the
+// two halves of its node while that node's id is still open.
+struct Agg {
+  int64_t ia, ib, l, r;
   float mn[3], mx[3];
   bool have_node;
   int4 h[4];
-  bool global;
-  // pending local arrival
-  bool pend;
-  int64_t ps;
-  int pside;
+  int stop;  // 0 root / inactive, 1 pending local arrival, 2 frontier
+  int64_t pgap;
 };
 
-// Climbs until the thread stops: first arrival at a split (locally pending
-// or globally done) or the root.  `resume` re-enters a pending local
-// arrival through HBM.
 template <int T>
-__device__ void climb(Climb &c, TreeLds<T> &L,
-                      const uint32_t *__restrict__ skeys, int64_t n,
-                      int64_t ts, int64_t te, int4 *nodes, GSide *gside,
-                      uint32_t *gtag, Ctl *ctl, uint32_t epoch, bool resume) {
+__device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
+                                            int64_t ts, int64_t te,
+                                            int4 *nodes) {
   while (true) {
-    int64_t s;
-    int side, far_self;
-    if (resume) {
-      resume = false;
-      s = c.ps;
-      side = c.pside;
-      far_self = side == 0 ? (int)c.l : (int)c.r;
-    } else {
-      auto delta = [&](int64_t x) -> int {  // x in [0, n-2]
-        if (x >= ts - 1 && x <= te - 1) return L.delta[x - ts + 1];
-        return delta_next(skeys[x], skeys[x + 1], (uint32_t)x);
-      };
-      const int dl = (c.l == 0) ? -1 : delta(c.l - 1);
-      const int dr = (c.r == n - 1) ? -1 : delta(c.r);
-      const bool root = (c.l == 0 && c.r == n - 1);
-      const bool is_left = dr > dl;
-      if (c.have_node) {
-        const int64_t K = root ? 0 : (is_left ? c.r : c.l);
-        int4 *node = nodes + 4 * K;
-        node[0] = c.h[0]; node[1] = c.h[1]; node[2] = c.h[2]; node[3] = c.h[3];
-        c.have_node = false;
-      }
-      if (root) return;
-      s = is_left ? c.r : c.l - 1;
-      side = is_left ? 0 : 1;
-      far_self = is_left ? (int)c.l : (int)c.r;
+    const int dl = L.dgap[c.ia - ts];      // gap before the first item
+    const int dr = L.dgap[c.ib - ts + 1];  // gap after the last item
+    const bool root = (c.l == 0 && c.r == n - 1);
+    const bool is_left = dr > dl;
+    if (c.have_node) {
+      const int64_t K = root ? 0 : (is_left ? c.r : c.l);
+      int4 *node = nodes + 4 * K;
+      node[0] = c.h[0]; node[1] = c.h[1]; node[2] = c.h[2]; node[3] = c.h[3];
+      c.have_node = false;
+    }
+    if (root) { c.stop = 0; return; }
+    const int64_t gap = is_left ? c.ib : c.ia - 1;  // item gap of the split
+    if (gap < ts || gap + 2 > te) {                 // other child off-tile
+      c.stop = 2;
+      return;
+    }
+    const int side = is_left ? 0 : 1;
+    const int64_t slot = gap - ts;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      L.box[side][k][slot] = c.mn[k];
+      L.box[side][3 + k][slot] = c.mx[k];
+    }
+    L.far[side][slot] = (int)(is_left ? c.ia : c.ib);
+    const uint32_t old = __hip_atomic_fetch_add(
+        &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) {
+      c.stop = 1;
+      c.pgap = gap;
+      return;
     }
     float smn[3], smx[3];
-    int sfar;
-    if (!c.global && s >= ts && s + 2 <= te) {
-      const int64_t slot = s - ts;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        L.box[side][k][slot] = c.mn[k];
-        L.box[side][3 + k][slot] = c.mx[k];
-      }
-      L.far[side][slot] = far_self;
-      const uint32_t old = __hip_atomic_fetch_add(
-          &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) {
-        c.pend = true;
-        c.ps = s;
-        c.pside = side;
-        return;
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        smn[k] = L.box[1 - side][k][slot];
-        smx[k] = L.box[1 - side][3 + k][slot];
-      }
-      sfar = L.far[1 - side][slot];
-    } else {
-      c.global = true;
-      if (!arrive_global(gside, gtag, ctl, epoch, s, side, far_self, c.mn,
-                         c.mx, smn, smx, sfar))
-        return;
+    for (int k = 0; k < 3; ++k) {
+      smn[k] = L.box[1 - side][k][slot];
+      smx[k] = L.box[1 - side][3 + k][slot];
     }
-    // Merge into the split-s node: children [L, s] and [s+1, R].
-    const bool self_left = (side == 0);
-    const int64_t Lr = self_left ? c.l : (int64_t)sfar;
-    const int64_t Rr = self_left ? (int64_t)sfar : c.r;
-    const float *lmn = self_left ? c.mn : smn, *lmx = self_left ? c.mx : smx;
-    const float *rmn = self_left ? smn : c.mn, *rmx = self_left ? smx : c.mx;
+    const int64_t sfar = L.far[1 - side][slot];
+    // Merge into the node split at leaf s: children [Lr, s] and [s+1, Rr].
+    const int64_t s = is_left ? c.r : c.l - 1;
+    int64_t Lr, Rr;
+    float lmn[3], lmx[3], rmn[3], rmx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lmn[k] = is_left ? c.mn[k] : smn[k];
+      lmx[k] = is_left ? c.mx[k] : smx[k];
+      rmn[k] = is_left ? smn[k] : c.mn[k];
+      rmx[k] = is_left ? smx[k] : c.mx[k];
+    }
+    if (is_left) {
+      Lr = c.l;
+      Rr = L.ir[sfar - ts];
+      c.ib = sfar;
+    } else {
+      Lr = L.il[sfar - ts];
+      Rr = c.r;
+      c.ia = sfar;
+    }
     const int lid = (Lr == s) ? -(int)(s + 1) : (int)s;
     const int rid = (Rr == s + 1) ? -(int)(s + 2) : (int)(s + 1);
     c.h[0] = pack3(lmn, lid);
@@ -364,77 +373,150 @@ __device__ void climb(Climb &c, TreeLds<T> &L,
     c.h[2] = pack3(rmn, rid);
     c.h[3] = pack3(rmx, 0);
     c.have_node = true;
-    float nmn[3], nmx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      nmn[k] = fminf(lmn[k], rmn[k]);
-      nmx[k] = fmaxf(lmx[k], rmx[k]);
+      c.mn[k] = fminf(lmn[k], rmn[k]);
+      c.mx[k] = fmaxf(lmx[k], rmx[k]);
     }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) { c.mn[k] = nmn[k]; c.mx[k] = nmx[k]; }
     c.l = Lr;
     c.r = Rr;
   }
 }
 
-// One block per tile of T sorted positions.  Phase 1: every thread gathers
-// its triangle into leaf order (O_TRI), takes its leaf box and climbs,
-// merging through LDS while both children of a split lie in the tile; a
-// split at a tile seam goes through HBM at once.  Phase 2 (after the block
-// barrier): a first arriver whose local sibling never came (the sibling
-// subtree reaches outside the tile) re-arrives through HBM.  Above the seams
-// every step is a global exchange.  No thread waits for another thread's
-// progress, only for stores already issued.
-template <int T>
-__global__ __launch_bounds__(T) void k_tree_fused(
+// One agglomeration level.  kLeaves: level 0 over the sorted leaves (gathers
+// O_TRI and the leaf boxes); otherwise the items of the level below.
+template <int T, bool kLeaves>
+__global__ __launch_bounds__(T) void k_agglomerate(
     const int4 *__restrict__ tri, const uint32_t *__restrict__ skeys,
     const int32_t *__restrict__ perm, uint32_t n, int4 *__restrict__ otri,
-    int4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl) {
-  __shared__ TreeLds<T> L;
-  const int64_t t = threadIdx.x;
-  const int64_t ts = (int64_t)blockIdx.x * T;
-  const int64_t i = ts + t;
+    const int4 *__restrict__ items_in, int4 *__restrict__ items_out,
+    uint32_t out_cap, uint32_t level, uint64_t *lstatus, int4 *nodes,
+    Ctl *ctl) {
+  __shared__ AggLds<T> L;
+  const int64_t tid = threadIdx.x;
   const int64_t N = n;
-  const int64_t te = (ts + T < N) ? ts + T : N;
-  const uint32_t epoch = ctl->epoch;
-
-  L.cnt[t] = 0u;
-  Climb c;
-  c.have_node = false;
-  c.global = false;
-  c.pend = false;
-  if (i < N) {
-    const uint32_t k0 = skeys[i];
-    L.delta[t + 1] = (i + 1 < N) ? delta_next(k0, skeys[i + 1], (uint32_t)i) : -1;
-    if (t == 0)
-      L.delta[0] = (ts == 0) ? -1
-                             : delta_next(skeys[ts - 1], k0, (uint32_t)(ts - 1));
-    const uint32_t pi = (uint32_t)perm[i];
-    const size_t src = 3 * (size_t)(pi < (uint32_t)N ? pi : 0u);
-    const int4 a = tri[src], b = tri[src + 1], d = tri[src + 2];
-    otri[3 * i] = a;
-    otri[3 * i + 1] = b;
-    otri[3 * i + 2] = d;
-    tri_box(a, b, d, c.mn, c.mx);
-    perturb(c.mn, c.mx);
-    c.l = c.r = i;
+  if (tid == 0) {
+    const uint32_t t = atomicAdd(&ctl->agg_tile_ctr[level], 1u);
+    if (t == gridDim.x - 1)
+      __hip_atomic_store(&ctl->agg_tile_ctr[level], 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    L.tile = t;
   }
+  const int64_t M = kLeaves ? N : (int64_t)ctl->level_count[level - 1];
+  const uint32_t stamp = (ctl->epoch * 8u + level) & 0x3FFFFFFFu;
   __syncthreads();
-  if (N == 1) {
-    if (i == 0) write_single_leaf_node(nodes, c.mn, c.mx);
+  const int64_t tile = L.tile;
+  const int64_t ts = tile * T;
+  if (ts >= M) {
+    if (tile == 0) ctl->level_count[level] = 0u;  // nothing at this level
     return;
   }
-  if (i == N - 1) {
-    const int4 z = make_int4(0, 0, 0, 0);
-    int4 *last = nodes + 4 * (N - 1);
-    last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+  const int64_t te = (ts + T < M) ? ts + T : M;
+  const int64_t k = ts + tid;
+  const bool active = k < te;
+
+  Agg c;
+  c.have_node = false;
+  c.stop = 0;
+  c.ia = c.ib = k;
+  L.cnt[tid] = 0u;
+  if (active) {
+    if (kLeaves) {
+      c.l = c.r = k;
+      const uint32_t pi = (uint32_t)perm[k];
+      const size_t src = 3 * (size_t)(pi < (uint32_t)N ? pi : 0u);
+      const int4 a = tri[src], b = tri[src + 1], d = tri[src + 2];
+      otri[3 * k] = a;
+      otri[3 * k + 1] = b;
+      otri[3 * k + 2] = d;
+      tri_box(a, b, d, c.mn, c.mx);
+      perturb(c.mn, c.mx);
+    } else {
+      const int4 a = items_in[2 * k], b = items_in[2 * k + 1];
+      c.l = a.x; c.r = a.y;
+      c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
+      c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
+      c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
+    }
+    L.il[tid] = (int)c.l;
+    L.ir[tid] = (int)c.r;
+    L.dgap[tid + 1] = (c.r + 1 < N)
+        ? delta_next(skeys[c.r], skeys[c.r + 1], (uint32_t)c.r) : -1;
+    if (tid == 0)
+      L.dgap[0] = (c.l > 0)
+          ? delta_next(skeys[c.l - 1], skeys[c.l], (uint32_t)(c.l - 1)) : -1;
   }
-  if (i < N) climb<T>(c, L, skeys, N, ts, te, nodes, gside, gtag, ctl, epoch, false);
   __syncthreads();
-  if (c.pend && L.cnt[c.ps - ts] == 1u) {
-    c.pend = false;
-    c.global = true;
-    climb<T>(c, L, skeys, N, ts, te, nodes, gside, gtag, ctl, epoch, true);
+  if (kLeaves) {
+    if (N == 1) {
+      if (k == 0) write_single_leaf_node(nodes, c.mn, c.mx);
+      return;
+    }
+    if (k == N - 1) {
+      const int4 z = make_int4(0, 0, 0, 0);
+      int4 *last = nodes + 4 * (N - 1);
+      last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+    }
+  }
+  if (active) agglomerate<T>(c, L, N, ts, te, nodes);
+  __syncthreads();
+
+  // Frontier: off-tile parents, and first arrivals whose sibling never came.
+  const bool out = active && (c.stop == 2 ||
+                              (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
+  const uint32_t rank = block_exclusive_scan(out ? 1u : 0u, L.wsum);
+  __shared__ uint32_t s_count;
+  if (tid == T - 1) s_count = rank + (out ? 1u : 0u);
+  __syncthreads();
+  const uint32_t count = s_count;
+  if (tid == 0) {
+    uint64_t *mine = lstatus + tile;
+    __hip_atomic_store(mine,
+                       ((uint64_t)((stamp << 2) | (tile == 0 ? kLbInclusive
+                                                             : kLbAggregate))
+                        << 32) | count,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t excl = 0;
+    int64_t j = tile - 1;
+    uint32_t spins = 0;
+    while (j >= 0) {
+      const uint64_t w = __hip_atomic_load(lstatus + j, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t hi = (uint32_t)(w >> 32);
+      if ((hi >> 2) != stamp) {
+        if (++spins > kSpinCap) {
+          atomicOr(&ctl->fault, 4u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += (uint32_t)w;
+      if ((hi & 3u) == kLbInclusive) break;
+      --j;
+    }
+    if (tile > 0)
+      __hip_atomic_store(mine,
+                         ((uint64_t)((stamp << 2) | kLbInclusive) << 32) |
+                             (excl + count),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (te == M) ctl->level_count[level] = excl + count;
+    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
+    L.base = excl;
+  }
+  __syncthreads();
+  if (out) {
+    const uint32_t dst = L.base + rank;
+    if (dst < out_cap) {
+      items_out[2 * (size_t)dst] =
+          make_int4((int)c.l, (int)c.r, __float_as_int(c.mn[0]),
+                    __float_as_int(c.mn[1]));
+      items_out[2 * (size_t)dst + 1] =
+          make_int4(__float_as_int(c.mn[2]), __float_as_int(c.mx[0]),
+                    __float_as_int(c.mx[1]), __float_as_int(c.mx[2]));
+    } else {
+      atomicOr(&ctl->fault, 8u);
+    }
   }
 }
 
@@ -471,28 +553,69 @@ void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                      reinterpret_cast<int4 *>(out));
 }
 
+// Level plan: level 0 over n leaves in ceil(n/T) tiles; level L+1 over at
+// most min(items of L, kFrontierMax * tiles of L) items, until one tile.
+int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap) {
+  uint64_t m = n;
+  uint64_t tiles = (m + tile - 1) / tile;
+  int levels = 0;
+  grid[levels] = (uint32_t)(tiles ? tiles : 1);
+  cap[levels] = 0;
+  ++levels;
+  while (tiles > 1 && levels < kMaxTreeLevels) {
+    const uint64_t bound = m < tiles * kFrontierMax ? m : tiles * kFrontierMax;
+    cap[levels - 1] = (uint32_t)bound;  // items written by the level below
+    m = bound;
+    tiles = (m + tile - 1) / tile;
+    grid[levels] = (uint32_t)tiles;
+    cap[levels] = 0;
+    ++levels;
+  }
+  return levels;
+}
+
+template <int T>
+static void launch_levels(const int4 *t4, const uint32_t *skeys,
+                          const int32_t *perm, uint32_t n, int4 *o4, int4 *n4,
+                          int4 *items_a, int4 *items_b, uint64_t *lstatus,
+                          Ctl *ctl, hipStream_t s) {
+  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
+  const int levels = tree_levels(n, T, grid, cap);
+  uint64_t *st = lstatus;
+  hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid[0]), dim3(T), 0, s,
+                     t4, skeys, perm, n, o4, (const int4 *)nullptr, items_a,
+                     cap[0], 0u, st, n4, ctl);
+  st += grid[0];
+  for (int L = 1; L < levels; ++L) {
+    const int4 *in = (L & 1) ? items_a : items_b;
+    int4 *out = (L & 1) ? items_b : items_a;
+    hipLaunchKernelGGL((k_agglomerate<T, false>), dim3(grid[L]), dim3(T), 0,
+                       s, (const int4 *)nullptr, skeys, perm, n,
+                       (int4 *)nullptr, in, out, cap[L], (uint32_t)L, st, n4,
+                       ctl);
+    st += grid[L];
+  }
+}
+
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl,
-                       int tile, hipStream_t s) {
+                       float4 *nodes, int4 *items_a, int4 *items_b,
+                       uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s) {
   const int4 *t4 = reinterpret_cast<const int4 *>(tri);
   int4 *o4 = reinterpret_cast<int4 *>(otri);
   int4 *n4 = reinterpret_cast<int4 *>(nodes);
   switch (tile) {
     case 256:
-      hipLaunchKernelGGL(k_tree_fused<256>, dim3(blocks_for(n, 256)),
-                         dim3(256), 0, s, t4, skeys, perm, n, o4, n4, gside,
-                         gtag, ctl);
+      launch_levels<256>(t4, skeys, perm, n, o4, n4, items_a, items_b,
+                         lstatus, ctl, s);
       break;
     case 1024:
-      hipLaunchKernelGGL(k_tree_fused<1024>, dim3(blocks_for(n, 1024)),
-                         dim3(1024), 0, s, t4, skeys, perm, n, o4, n4, gside,
-                         gtag, ctl);
+      launch_levels<1024>(t4, skeys, perm, n, o4, n4, items_a, items_b,
+                          lstatus, ctl, s);
       break;
     default:
-      hipLaunchKernelGGL(k_tree_fused<512>, dim3(blocks_for(n, 512)),
-                         dim3(512), 0, s, t4, skeys, perm, n, o4, n4, gside,
-                         gtag, ctl);
+      launch_levels<512>(t4, skeys, perm, n, o4, n4, items_a, items_b,
+                         lstatus, ctl, s);
       break;
   }
 }

@@ -40,7 +40,9 @@ struct lbvh_ctx {
   uint64_t epoch_ops = 0;  // epoch bumps issued since the last rewind
   DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
   DevBuf status;                        // look-back words (zeroed)
-  DevBuf gside, gtag;                   // fused up-pass exchange (tags zeroed)
+  DevBuf gside;                         // per-stage up-pass exchange
+  DevBuf items_a, items_b;              // tree agglomeration levels
+  DevBuf lstatus;                       // their look-back words (zeroed)
   DevBuf ctag;                          // per-stage up-pass tags (zeroed)
   DevBuf partials;                      // bounds partials
   // profiling: event pairs bracketing each stage, read back on demand
@@ -135,7 +137,7 @@ constexpr uint64_t kEpochRewind = 1ull << 26;
 int begin_op(lbvh_ctx *ctx) {
   if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->status, &ctx->gtag, &ctx->ctag})
+  for (DevBuf *b : {&ctx->status, &ctx->lstatus, &ctx->ctag})
     if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
   HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
                               ctx->stream));
@@ -198,6 +200,15 @@ int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
   if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
   const uint64_t words = (uint64_t)sort_tiles(n) * kRadix;
   return ensure(ctx, ctx->status, 8ull * words, true);
+}
+
+int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n) {
+  int st;
+  const uint64_t items = 32ull * tree_item_capacity(n, ctx->tree_tile);
+  if ((st = ensure(ctx, ctx->items_a, items, false))) return st;
+  if ((st = ensure(ctx, ctx->items_b, items, false))) return st;
+  return ensure(ctx, ctx->lstatus, 8ull * tree_status_words(n, ctx->tree_tile),
+                true);
 }
 
 int ensure_partials(lbvh_ctx *ctx) {
@@ -301,7 +312,8 @@ int lbvh_destroy(lbvh_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
-                      &ctx->status, &ctx->gside, &ctx->gtag, &ctx->ctag,
+                      &ctx->status, &ctx->gside, &ctx->items_a, &ctx->items_b,
+                      &ctx->lstatus, &ctx->ctag,
                       &ctx->partials})
       dev_release(ctx, *b);
     for (auto &m : ctx->marks) {
@@ -380,7 +392,8 @@ int lbvh_trim(lbvh_ctx *ctx) {
   CTX_GUARD(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
-                    &ctx->status, &ctx->gside, &ctx->gtag, &ctx->ctag,
+                    &ctx->status, &ctx->gside, &ctx->items_a, &ctx->items_b,
+                      &ctx->lstatus, &ctx->ctag,
                     &ctx->partials})
     dev_release(ctx, *b);
   return LBVH_OK;
@@ -558,8 +571,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   if ((st = ensure_sort_scratch(ctx, n))) return st;
-  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
-  if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
+  if ((st = ensure_tree_scratch(ctx, n))) return st;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
   if ((st = begin_op(ctx))) return st;
 
@@ -595,8 +607,9 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
     StageScope sc(ctx, LBVH_STAGE_TREE);
     launch_tree_fused(t4, skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
                       static_cast<float4 *>(bvh_out),
-                      static_cast<GSide *>(ctx->gside.ptr),
-                      static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
+                      static_cast<int4 *>(ctx->items_a.ptr),
+                      static_cast<int4 *>(ctx->items_b.ptr),
+                      static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
                       ctx->tree_tile, ctx->stream);
   }
   return check_launch(ctx, "lbvh_build");
@@ -611,16 +624,16 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !sorted_keys || !perm || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
-  if ((st = ensure(ctx, ctx->gtag, 4ull * n, true))) return st;
+  if ((st = ensure_tree_scratch(ctx, n))) return st;
   if ((st = begin_op(ctx))) return st;
   launch_bump_epoch(ctx->ctl, ctx->stream);
   StageScope sc(ctx, LBVH_STAGE_TREE);
   launch_tree_fused(reinterpret_cast<const float4 *>(tri), sorted_keys, perm, n,
                     reinterpret_cast<float4 *>(tri_out),
                     static_cast<float4 *>(bvh_out),
-                    static_cast<GSide *>(ctx->gside.ptr),
-                    static_cast<uint32_t *>(ctx->gtag.ptr), ctx->ctl,
+                    static_cast<int4 *>(ctx->items_a.ptr),
+                    static_cast<int4 *>(ctx->items_b.ptr),
+                    static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
                     ctx->tree_tile, ctx->stream);
   return check_launch(ctx, "lbvh_refit");
 }

@@ -15,7 +15,8 @@ constexpr int kSortThreads = 256;        // 4 wave64 per tile
 constexpr int kSortItems = 16;           // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 
-constexpr int kTreeTileDefault = 512;  // leaves per tile of the fused up pass
+constexpr int kTreeTileDefault = 1024;  // leaves per tile of the fused up pass
+constexpr int kMaxTreeLevels = 16;      // agglomeration levels (6 at 2^24, T=1024)
 
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
 constexpr int kBoundsBlocks = 1024;  // grid of the bounds reduction
@@ -35,8 +36,10 @@ constexpr uint32_t kSpinCap = 1u << 22;  // look-back polls before giving up
 //               the digit histograms and the counter
 //   bounds_done completion count of the bounds reduction (reset by the last
 //               block)
-//   fault       sticky flag: a bounded wait gave up (never set in a correct
-//               run; read by lbvh_synchronize)
+//   fault       sticky flag: a bounded wait gave up or a frontier overflowed
+//               (never set in a correct run; read by lbvh_synchronize)
+//   agg_tile_ctr / level_count: tile ids (self-resetting) and item counts of
+//               the tree's agglomeration levels
 struct Ctl {
   uint32_t epoch;
   uint32_t tile_ctr;
@@ -44,6 +47,8 @@ struct Ctl {
   uint32_t bounds_done;
   uint32_t fault;
   uint32_t pad0[3];
+  uint32_t agg_tile_ctr[kMaxTreeLevels];  // per-level dynamic tile ids
+  uint32_t level_count[kMaxTreeLevels];   // items produced by each level
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
   uint32_t hist[kSortPasses][kRadix];    // global digit counts per pass
 };
@@ -98,10 +103,28 @@ void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s);
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                       float4 *out, hipStream_t s);
-// tile: 256, 512 or 1024 leaves per block.
+// Fused leaf gather + radix tree + node boxes by multi-level agglomeration.
+// tile: 256, 512 or 1024 leaves per block.  items_a/items_b hold
+// tree_item_capacity(n, tile) 32-byte items each; lstatus holds
+// tree_status_words(n, tile) look-back words (zeroed once).
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, GSide *gside, uint32_t *gtag, Ctl *ctl,
-                       int tile, hipStream_t s);
+                       float4 *nodes, int4 *items_a, int4 *items_b,
+                       uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s);
+int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap);
+inline uint64_t tree_item_capacity(uint32_t n, int tile) {
+  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
+  const int levels = tree_levels(n, tile, grid, cap);
+  uint64_t m = 1;
+  for (int L = 0; L < levels; ++L) m = cap[L] > m ? cap[L] : m;
+  return m;
+}
+inline uint64_t tree_status_words(uint32_t n, int tile) {
+  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
+  const int levels = tree_levels(n, tile, grid, cap);
+  uint64_t w = 0;
+  for (int L = 0; L < levels; ++L) w += grid[L];
+  return w;
+}
 
 }  // namespace lbvh
