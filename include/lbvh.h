@@ -175,6 +175,19 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
                const uint32_t *sorted_keys, const int32_t *perm,
                void *bvh_out, float *tri_out);
 
+/* ---- consumer check ------------------------------------------------------ */
+
+/* Closest-hit ray casting through a built BVH the way the demo's path tracer
+ * traverses it (pathtracer.js:495-584: stack traversal from node 0, nearer
+ * child first; Moller-Trumbore at :587-619; slab test at :621-628), on the
+ * device.  rays: count x 8 floats (origin.xyz, pad, dir.xyz, pad); hits:
+ * count x 4 floats (normal.xyz facing the ray, dist; dist = -1 for a miss),
+ * the path tracer's RayHit.  bvh / tri are lbvh_build's outputs for n
+ * triangles.  The stack holds 64 entries (the path tracer's 32 can overflow
+ * on deeper trees). */
+int lbvh_raycast(lbvh_ctx *ctx, const void *bvh, const float *tri, uint32_t n,
+                 const float *rays, uint32_t count, float *hits);
+
 /* ---- profiling ------------------------------------------------------------
  * No counterpart in the reference (which times build() with Date.now() at
  * main.js:87-90).  When enabled, lbvh_build and lbvh_refit record a HIP event

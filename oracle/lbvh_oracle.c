@@ -15,6 +15,7 @@
  */
 #include "lbvh_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -471,4 +472,143 @@ void oracle_bruteforce_bvh(const float *tri, const int32_t *perm,
   }
   if (n >= 2)
     bf_nodes(tri, perm, sorted_keys, 0, (int32_t)n - 1, 0, bvh);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ray casting: pathtracer.js:495-628                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float x, y, z; } v3;
+static v3 v3sub(v3 a, v3 b) { v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static v3 v3cross(v3 a, v3 b) {
+  v3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  return r;
+}
+static float v3dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 v3at(const float *p) { v3 r = {p[0], p[1], p[2]}; return r; }
+
+/* tri_intersect (pathtracer.js:587-619), Moller-Trumbore; out[3] = t or -1 */
+static void tri_intersect(v3 o, v3 d, const float *t, float out[4]) {
+  v3 v0 = v3at(t), v1 = v3at(t + 4), v2 = v3at(t + 8);
+  v3 e1 = v3sub(v1, v0), e2 = v3sub(v2, v0);
+  v3 pvec = v3cross(d, e2);
+  float det = v3dot(e1, pvec);
+  out[0] = out[1] = out[2] = out[3] = -1.0f;
+  if (fabsf(det) < 1e-10f) return;
+  float idet = 1.0f / det;
+  v3 tvec = v3sub(o, v0);
+  float u = v3dot(tvec, pvec) * idet;
+  if (u < 0.f || u > 1.f) return;
+  v3 qvec = v3cross(tvec, e1);
+  float v = v3dot(d, qvec) * idet;
+  if (v < 0.f || u + v > 1.f) return;
+  v3 nrm = v3cross(e1, e2);
+  float len = sqrtf(v3dot(nrm, nrm));
+  out[0] = nrm.x / len; out[1] = nrm.y / len; out[2] = nrm.z / len;
+  out[3] = v3dot(e2, qvec) * idet;
+}
+
+/* aabb_intersect (pathtracer.js:621-628) */
+static float aabb_intersect(const float *lo, const float *hi, v3 o, v3 inv) {
+  float fx = (hi[0] - o.x) * inv.x, fy = (hi[1] - o.y) * inv.y,
+        fz = (hi[2] - o.z) * inv.z;
+  float nx = (lo[0] - o.x) * inv.x, ny = (lo[1] - o.y) * inv.y,
+        nz = (lo[2] - o.z) * inv.z;
+  float t0 = fmax32(fmin32(fx, nx), fmax32(fmin32(fy, ny), fmin32(fz, nz)));
+  float t1 = fmin32(fmax32(fx, nx), fmin32(fmax32(fy, ny), fmax32(fz, nz)));
+  return (t1 >= t0) ? ((t1 < 0.f) ? -1e30f : t0) : -1e30f;
+}
+
+static v3 ray_dir(const float *r) {
+  v3 d = {r[4], r[5], r[6]};
+  /* d += (abs(d) < 1e-5) * 1e-5 (pathtracer.js:502-503) */
+  if (fabsf(d.x) < 1e-5f) d.x = d.x + 1e-5f;
+  if (fabsf(d.y) < 1e-5f) d.y = d.y + 1e-5f;
+  if (fabsf(d.z) < 1e-5f) d.z = d.z + 1e-5f;
+  return d;
+}
+
+static void finish_hit(v3 d, v3 nrm, float dist, float *h) {
+  /* RayHit: normal facing the ray, dist -1 for a miss (:572-583) */
+  v3 neg = {-nrm.x, -nrm.y, -nrm.z};
+  float s = v3dot(d, neg) > 0.f ? 1.f : -1.f;
+  h[0] = s * nrm.x; h[1] = s * nrm.y; h[2] = s * nrm.z;
+  h[3] = (dist == 1e30f) ? -1.f : dist;
+}
+
+/* intersect_bvh (pathtracer.js:495-584): stack traversal from node 0,
+ * nearer valid child first.  The stack holds 64 entries (the path tracer's
+ * 32 can overflow on trees deeper than 32) and the loop is capped, since a
+ * single-triangle BVH's empty r-slot (child 0, zero box) can loop. */
+void oracle_raycast(const void *bvhp, const float *tri, uint32_t n,
+                    const float *rays, uint32_t count, float *hits) {
+  const bvh_node *bvh = (const bvh_node *)bvhp;
+  const uint64_t cap = 4ull * n + 256;
+  for (uint32_t q = 0; q < count; ++q) {
+    const float *r = rays + 8 * (size_t)q;
+    v3 o = {r[0], r[1], r[2]};
+    v3 d = ray_dir(r);
+    v3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    float dist = 1e30f;
+    v3 nrm = {0.f, 0.f, 0.f};
+    int32_t stack[64];
+    int sp = 0;
+    int32_t node = 0;
+    for (uint64_t step = 0; sp >= 0 && step < cap; ++step) {
+      if (node < 0) {
+        uint32_t t = (uint32_t)(-(node + 1));
+        if (t < n) {
+          float h[4];
+          tri_intersect(o, d, tri + 12 * (size_t)t, h);
+          if (h[3] > 0.f && h[3] < dist) {
+            nrm.x = h[0]; nrm.y = h[1]; nrm.z = h[2];
+            dist = fmin32(h[3], dist);
+          }
+        }
+        sp -= 1;
+        node = sp >= 0 ? stack[sp] : 0;
+      } else {
+        if ((uint32_t)node >= n) break;
+        const bvh_node *b = &bvh[node];
+        float ld = aabb_intersect(b->l_min, b->l_max, o, inv);
+        float rd = aabb_intersect(b->r_min, b->r_max, o, inv);
+        int lv = ld != -1e30f && ld < dist, rv = rd != -1e30f && rd < dist;
+        if (lv && rv) {
+          int32_t near = ld < rd ? b->l_child : b->r_child;
+          int32_t far = ld < rd ? b->r_child : b->l_child;
+          if (sp < 64) stack[sp] = far;
+          sp = sp + 1 < 64 ? sp + 1 : 64;
+          node = near;
+        } else if (lv) {
+          node = b->l_child;
+        } else if (rv) {
+          node = b->r_child;
+        } else {
+          sp -= 1;
+          node = sp >= 0 ? stack[sp] : 0;
+        }
+      }
+    }
+    finish_hit(d, nrm, dist, hits + 4 * (size_t)q);
+  }
+}
+
+void oracle_raycast_bruteforce(const float *tri, uint32_t n, const float *rays,
+                               uint32_t count, float *hits) {
+  for (uint32_t q = 0; q < count; ++q) {
+    const float *r = rays + 8 * (size_t)q;
+    v3 o = {r[0], r[1], r[2]};
+    v3 d = ray_dir(r);
+    float dist = 1e30f;
+    v3 nrm = {0.f, 0.f, 0.f};
+    for (uint32_t t = 0; t < n; ++t) {
+      float h[4];
+      tri_intersect(o, d, tri + 12 * (size_t)t, h);
+      if (h[3] > 0.f && h[3] < dist) {
+        nrm.x = h[0]; nrm.y = h[1]; nrm.z = h[2];
+        dist = h[3];
+      }
+    }
+    finish_hit(d, nrm, dist, hits + 4 * (size_t)q);
+  }
 }

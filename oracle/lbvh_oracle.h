@@ -51,6 +51,15 @@ void oracle_build(const float *tri, uint32_t n, const float *bounds_or_null,
 void oracle_refit(const float *tri, uint32_t n, const uint32_t *sorted_keys,
                   const int32_t *perm, void *bvh_out, float *tri_out);
 
+/* Closest-hit ray casting as the demo's path tracer does it
+ * (pathtracer.js:495-628).  rays: 8 floats each (origin.xyz, pad, dir.xyz,
+ * pad); hits: 4 floats each (normal.xyz, dist; dist = -1 for a miss). */
+void oracle_raycast(const void *bvh, const float *tri, uint32_t n,
+                    const float *rays, uint32_t count, float *hits);
+/* Same rays against every triangle, no BVH. */
+void oracle_raycast_bruteforce(const float *tri, uint32_t n, const float *rays,
+                               uint32_t count, float *hits);
+
 /* Independent brute-force checkers (no shared code with the restatement). */
 void oracle_bruteforce_tree(const uint32_t *sorted_keys, uint32_t n,
                             int32_t *parents_out);

@@ -21,6 +21,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
                          invariants
   up_pass                brute-force per-node range min/max, canonical order
   rearrange              numpy gather
+  raycast                against raycast_bruteforce (every triangle, no BVH)
+                         on hits/misses and closest distances
 """
 from __future__ import annotations
 
@@ -66,6 +68,8 @@ def lib():
             "oracle_canonicalize_bvh": (None, [vp, u32]),
             "oracle_build": (None, [vp, u32, vp, i32, vp, vp, vp, vp]),
             "oracle_refit": (None, [vp, u32, vp, vp, vp, vp]),
+            "oracle_raycast": (None, [vp, vp, u32, vp, u32, vp]),
+            "oracle_raycast_bruteforce": (None, [vp, u32, vp, u32, vp]),
             "oracle_bruteforce_tree": (None, [vp, u32, vp]),
             "oracle_bruteforce_bvh": (None, [vp, vp, vp, u32, vp]),
         }
@@ -192,6 +196,24 @@ def bruteforce_bvh(tri, perm, skeys):
     n = k.shape[0]
     out = np.zeros(64 * n, np.uint8)
     lib().oracle_bruteforce_bvh(_p(tri), _p(pm), _p(k), n, _p(out))
+    return out
+
+
+def raycast(bvh, otri, rays):
+    """Closest hits through the BVH, path-tracer traversal (n x 4 floats)."""
+    b = np.ascontiguousarray(bvh, dtype=np.uint8)
+    t = _tri(otri)
+    r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    out = np.empty((r.shape[0], 4), np.float32)
+    lib().oracle_raycast(_p(b), _p(t), t.shape[0], _p(r), r.shape[0], _p(out))
+    return out
+
+
+def raycast_bruteforce(tri, rays):
+    t = _tri(tri)
+    r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    out = np.empty((r.shape[0], 4), np.float32)
+    lib().oracle_raycast_bruteforce(_p(t), t.shape[0], _p(r), r.shape[0], _p(out))
     return out
 
 

@@ -292,3 +292,50 @@ def test_budget_refusal_leaves_context_usable(lb):
         got = c.build_arrays(small, None)[0]
     with lb.Context(0) as c2:
         assert np.array_equal(got, c2.build_arrays(small, None)[0])
+
+
+# ---------------------------------------------------------------- consumer check
+
+def _rays(seed, count, top=None):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros((count, 8), np.float32)
+    if top is None:
+        o = rng.uniform(-0.5, 1.5, (count, 3)).astype(np.float32)
+        t = rng.uniform(0, 1, (count, 3)).astype(np.float32)
+        rays[:, 0:3], rays[:, 4:7] = o, t - o
+    else:  # straight down onto a heightfield, plus some oblique rays
+        rays[:, 0] = rng.uniform(0, 1, count)
+        rays[:, 1] = rng.uniform(0, top, count)
+        rays[:, 2] = 1.0
+        rays[:, 4:7] = [0, 0, -1]
+        k = count // 2
+        rays[:k, 4:7] = rng.uniform(-0.3, 0.3, (k, 3)).astype(np.float32)
+        rays[:k, 6] = -1
+    return rays
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5000, 100003])
+def test_raycast_over_gpu_bvh(ctx, lb, ora, n):
+    """The path tracer's traversal (pathtracer.js:495-584) over the GPU-built
+    BVH and O_TRI finds the same closest hits as testing every triangle."""
+    say(f"raycast n={n}")
+    tri = lb.wl_clustered(80 + n, n, 0.05)
+    bvh, otri = ctx.build_arrays(tri, None)
+    rays = _rays(n, 4096)
+    got = ctx.raycast_arrays(bvh, otri, rays)
+    assert np.array_equal(got, ora.raycast(bvh, otri, rays))
+    if n <= 5000:
+        assert np.array_equal(got, ora.raycast_bruteforce(otri, rays))
+    else:
+        sample = rays[:256]
+        assert np.array_equal(got[:256], ora.raycast_bruteforce(otri, sample))
+
+
+def test_raycast_heightfield_16m(ctx, lb, ora):
+    say("raycast heightfield-16m")
+    tri = lb.workload("heightfield-16m")
+    bvh, otri = ctx.build_arrays(tri, None)
+    rays = _rays(5, 8192, top=0.5)
+    got = ctx.raycast_arrays(bvh, otri, rays)
+    assert (got[:, 3] > 0).mean() > 0.95   # rays from above hit the surface
+    assert np.array_equal(got, ora.raycast(bvh, otri, rays))

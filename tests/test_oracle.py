@@ -290,3 +290,31 @@ def test_refit_boxes_match_bruteforce(ora, lb):
     rbvh, rotri = ora.refit(moved, sk, pm)
     assert np.array_equal(rbvh, ora.bruteforce_bvh(moved, pm, sk))
     assert np.array_equal(rotri, moved[pm])
+
+
+# ---------------------------------------------------------------- consumer
+
+def make_rays(seed, count, lo=-0.5, hi=1.5):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (count, 3)).astype(np.float32)
+    t = rng.uniform(0, 1, (count, 3)).astype(np.float32)
+    rays = np.zeros((count, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = t - o
+    rays[: count // 8, 4:7] = [0, 0, -1]     # axis-aligned: exercises d += 1e-5
+    rays[: count // 8, 0:3] = np.c_[rng.uniform(0, 1, (count // 8, 2)), np.full(count // 8, 2)]
+    return rays
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 300, 20000])
+def test_raycast_traversal_equals_bruteforce(ora, lb, n):
+    """The path tracer's traversal over the oracle's BVH finds exactly the
+    closest hit that testing every triangle finds."""
+    tri = lb.wl_clustered(70 + n, n, 0.05)
+    bvh, otri, sk, pm = ora.build(tri)
+    rays = make_rays(n, 3000)
+    h1 = ora.raycast(bvh, otri, rays)
+    h2 = ora.raycast_bruteforce(otri, rays)
+    assert np.array_equal(h1, h2)
+    if n >= 300:
+        assert (h1[:, 3] > 0).sum() >= (50 if n >= 20000 else 5)

@@ -638,6 +638,24 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
   return check_launch(ctx, "lbvh_refit");
 }
 
+// ---- consumer check ------------------------------------------------------------
+
+int lbvh_raycast(lbvh_ctx *ctx, const void *bvh, const float *tri, uint32_t n,
+                 const float *rays, uint32_t count, float *hits) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (count == 0) return LBVH_OK;
+  if (!rays || !hits || (n && (!bvh || !tri)))
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if (n == 0) {
+    HIP_TRY(ctx, hipMemsetAsync(hits, 0, 16ull * count, ctx->stream));
+    return LBVH_OK;
+  }
+  launch_raycast(bvh, tri, n, rays, count, hits, ctx->stream);
+  return check_launch(ctx, "lbvh_raycast");
+}
+
 // ---- profiling ---------------------------------------------------------------
 
 int lbvh_profile_enable(lbvh_ctx *ctx, int enable) {

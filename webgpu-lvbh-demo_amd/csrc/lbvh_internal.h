@@ -112,6 +112,11 @@ void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        float4 *nodes, int4 *items_a, int4 *items_b,
                        uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s);
 int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap);
+
+// k_raycast.hip
+void launch_raycast(const void *nodes, const float *tri, uint32_t n,
+                    const float *rays, uint32_t count, float *hits,
+                    hipStream_t s);
 inline uint64_t tree_item_capacity(uint32_t n, int tile) {
   uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
   const int levels = tree_levels(n, tile, grid, cap);

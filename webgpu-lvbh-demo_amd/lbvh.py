@@ -68,6 +68,7 @@ SIGNATURES = {
     "lbvh_rearrange": (_i32, [_vp, _vp, _vp, _u32, _vp]),
     "lbvh_build": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     "lbvh_refit": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "lbvh_raycast": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     "lbvh_profile_enable": (_i32, [_vp, _i32]),
     "lbvh_profile_read": (_i32, [_vp, _vp, _i32]),
 }
@@ -281,6 +282,24 @@ class Context:
     def refit(self, tri, n: int, skeys, perm, bvh, otri):
         self._check(self.lib.lbvh_refit(self.handle, _dp(tri), n, _dp(skeys),
                                         _dp(perm), _dp(bvh), _dp(otri)))
+
+    def raycast(self, bvh, tri, n: int, rays, count: int, hits):
+        self._check(self.lib.lbvh_raycast(self.handle, _dp(bvh), _dp(tri), n,
+                                          _dp(rays), count, _dp(hits)))
+
+    def raycast_arrays(self, bvh: np.ndarray, otri: np.ndarray, rays: np.ndarray) -> np.ndarray:
+        """Host arrays in, closest hits (count x 4 float32) out."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        otri = np.ascontiguousarray(otri, dtype=np.float32).reshape(-1, 12)
+        n, count = otri.shape[0], rays.shape[0]
+        db, dt, dr = self.to_device(bvh), self.to_device(otri), self.to_device(rays)
+        dh = self.alloc(16 * count)
+        try:
+            self.raycast(db, dt, n, dr, count, dh)
+            return dh.read(np.float32, 4 * count).reshape(count, 4)
+        finally:
+            for b in (db, dt, dr, dh):
+                b.destroy()
 
     # ---- host-array conveniences (tests) -----------------------------------
     def build_arrays(self, tri: np.ndarray, bounds=None, keep_topology=False):
