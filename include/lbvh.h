@@ -175,6 +175,26 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
                const uint32_t *sorted_keys, const int32_t *perm,
                void *bvh_out, float *tri_out);
 
+/* ---- batched build ------------------------------------------------------- */
+
+/* Builds num_meshes independent LBVHs in one captured graph (no counterpart
+ * in the reference, which builds one mesh per call to build(),
+ * build-bvh.js:12-66).  The meshes' triangles are concatenated in `tri`;
+ * offsets (host, num_meshes + 1 entries, offsets[0] = 0, non-decreasing)
+ * gives mesh k's triangles [offsets[k], offsets[k+1]).  Every output keeps
+ * that layout: mesh k's BVH_BUFFER is nodes [offsets[k], offsets[k+1]) of
+ * bvh_out with node ids and leaf ids relative to the mesh (its root is its
+ * first node), its O_TRIANGLE_BUFFER the same range of tri_out, and
+ * sorted_keys_out / perm_out (optional) hold per-mesh sorted keys and
+ * mesh-relative permutations.  Each mesh's result is byte-identical to
+ * lbvh_build on that mesh alone.  bounds_host: 6 host floats per mesh, or
+ * NULL for per-mesh GPU bounds.  Repeated calls with the same offsets and
+ * buffers replay the captured graph. */
+int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
+                       const uint32_t *offsets, uint32_t num_meshes,
+                       const float *bounds_host, void *bvh_out, float *tri_out,
+                       uint32_t *sorted_keys_out, int32_t *perm_out);
+
 /* ---- consumer check ------------------------------------------------------ */
 
 /* Closest-hit ray casting through a built BVH the way the demo's path tracer

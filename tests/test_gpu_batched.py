@@ -1,0 +1,92 @@
+"""Batched builds (lbvh_build_batched): each mesh of a batch must come out
+byte-identical to a single-mesh build of that mesh, and graph replays must
+keep giving the same bytes with other work in between."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make_batch(lb, sizes, seed):
+    meshes = [lb.wl_uniform(seed + k, int(n), 0.01) if n else np.zeros((0, 12), np.float32)
+              for k, n in enumerate(sizes)]
+    offsets = np.zeros(len(sizes) + 1, np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    tri = np.concatenate(meshes) if offsets[-1] else np.zeros((0, 12), np.float32)
+    return meshes, offsets, tri
+
+
+def run_batch(ctx, tri, offsets, bounds=None):
+    total = int(offsets[-1])
+    d = ctx.to_device(tri)
+    bvh, otri = ctx.alloc(64 * total), ctx.alloc(48 * total)
+    sk, pm = ctx.alloc(4 * total), ctx.alloc(4 * total)
+    try:
+        ctx.build_batched(d, offsets, bounds, bvh, otri, sk, pm)
+        return (bvh.read(np.uint8, 64 * total), otri.read(np.float32, 12 * total).reshape(-1, 12),
+                sk.read(np.uint32, total), pm.read(np.int32, total))
+    finally:
+        for b in (d, bvh, otri, sk, pm):
+            b.destroy()
+
+
+@pytest.mark.parametrize("with_bounds", [False, True])
+def test_batch_meshes_match_oracle(ctx, lb, ora, with_bounds):
+    sizes = [1, 2, 0, 3, 255, 256, 257, 4096, 4097, 9000, 1, 33333, 5]
+    say(f"batch {len(sizes)} meshes bounds={with_bounds}")
+    meshes, offsets, tri = make_batch(lb, sizes, 100)
+    bounds = None
+    if with_bounds:
+        bounds = np.concatenate([lb.wl_bounds(m) if len(m) else np.zeros(6, np.float32)
+                                 for m in meshes])
+    bvh, otri, sk, pm = run_batch(ctx, tri, offsets, bounds)
+    for k, m in enumerate(meshes):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        if a == b:
+            continue
+        wb, wo, wsk, wpm = ora.build(m, None if bounds is None else bounds[6 * k:6 * k + 6], 0)
+        assert np.array_equal(sk[a:b], wsk), k
+        assert np.array_equal(pm[a:b], wpm), k
+        assert np.array_equal(bvh[64 * a:64 * b], wb), k
+        assert np.array_equal(otri[a:b].view(np.uint32), wo.view(np.uint32)), k
+
+
+def test_batch_graph_replays(ctx, lb):
+    """Replays of the captured graph, with eager work between them, give the
+    same bytes as per-mesh lbvh_build calls."""
+    rng = np.random.default_rng(3)
+    sizes = np.exp(rng.uniform(np.log(256), np.log(70000), 48)).astype(np.uint32)
+    meshes, offsets, tri = make_batch(lb, sizes, 500)
+    total = int(offsets[-1])
+    want = np.concatenate([ctx.build_arrays(m, None)[0] for m in meshes])
+    d = ctx.to_device(tri)
+    bvh, otri = ctx.alloc(64 * total), ctx.alloc(48 * total)
+    other = lb.wl_clustered(7, 50000, 0.05)
+    try:
+        for rep in range(4):
+            say(f"graph replay {rep}")
+            ctx.build_batched(d, offsets, None, bvh, otri)
+            got = bvh.read(np.uint8, 64 * total)
+            assert np.array_equal(got, want), rep
+            ctx.build_arrays(other, None)  # eager work between replays
+    finally:
+        for b in (d, bvh, otri):
+            b.destroy()
+
+
+def test_batch_log_uniform_256(ctx, lb):
+    """256 meshes, log-uniform sizes in [256, 262144] (the 2,048-mesh
+    benchmark's distribution), against per-mesh builds on the GPU."""
+    sizes = lb.wl_log_uniform_sizes(lb.BATCH_SEED, 256, 256, 262144)
+    say(f"batch 256 log-uniform meshes, {int(sizes.sum())} triangles")
+    meshes, offsets, tri = make_batch(lb, sizes, 1000)
+    bvh, otri, sk, pm = run_batch(ctx, tri, offsets)
+    for k in range(0, len(meshes), 17):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        wb, wo = ctx.build_arrays(meshes[k], None)
+        assert np.array_equal(bvh[64 * a:64 * b], wb), k
+        assert np.array_equal(otri[a:b], wo), k

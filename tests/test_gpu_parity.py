@@ -279,12 +279,16 @@ def test_refit_matches_oracle(ctx, lb, ora):
 # ---------------------------------------------------------------- budget
 
 def test_budget_refusal_leaves_context_usable(lb):
-    tri = lb.wl_uniform(50, 200000, 0.01)
-    with lb.Context(0, hbm_budget=40 << 20) as c:
+    n = 200000
+    tri = lb.wl_uniform(50, n, 0.01)
+    user = (48 + 64 + 48) * n
+    # room for the inputs and outputs plus 1 MiB: the build's scratch
+    # (16 bytes per triangle for keys and indices alone) cannot fit
+    with lb.Context(0, hbm_budget=user + (1 << 20) + (64 << 10)) as c:
         d = c.to_device(tri)
-        bvh, otri = c.alloc(64 * 200000), c.alloc(48 * 200000)
+        bvh, otri = c.alloc(64 * n), c.alloc(48 * n)
         with pytest.raises(lb.LBVHError) as e:
-            c.build(d, 200000, None, bvh, otri)
+            c.build(d, n, None, bvh, otri)
         assert e.value.code == 3
         for x in (d, bvh, otri):
             x.destroy()

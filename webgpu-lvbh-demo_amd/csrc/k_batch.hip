@@ -1,0 +1,173 @@
+// k_batch.hip -- per-mesh bounds and Morton keys of a batched build.
+//
+// A batched build treats S independent meshes, concatenated, in one set of
+// launches.  Work is cut into sort tiles of kSortTile triangles that never
+// straddle a mesh (SortSegs); each block handles one tile.  The arithmetic is
+// the single-mesh path's (aabb-z-idx.js:177-208), per mesh.
+#include "device_util.h"
+
+namespace lbvh {
+
+// Order-preserving u32 encoding of fp32, so unsigned atomicMin/atomicMax
+// compute float min/max exactly.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__device__ __forceinline__ uint32_t split3(uint32_t u) {
+  uint32_t x = u;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t to_u32_sat(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+// Bumps the epoch and resets the per-mesh digit histograms and (when the
+// GPU computes them) the per-mesh bounds accumulators.
+__global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
+    uint32_t *seg_hist, uint32_t *seg_ord, uint32_t num_segs, int init_bounds,
+    Ctl *ctl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t hwords = (uint64_t)num_segs * kSortPasses * kRadix;
+  for (uint64_t i = first; i < hwords; i += stride) seg_hist[i] = 0u;
+  if (init_bounds)
+    for (uint64_t i = first; i < 8ull * num_segs; i += stride)
+      seg_ord[i] = ((i & 7) < 3) ? 0xFFFFFFFFu : 0u;  // min slots, max slots
+}
+
+// Per-mesh fp32 min/max of the vertices, one sort tile per block.
+__global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
+    const float4 *__restrict__ tri, SortSegs segs, uint32_t *seg_ord) {
+  __shared__ float s[kSortThreads / 64][6];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t sg = segs.tile_seg[tile];
+  const uint32_t base = segs.seg_off[sg] + (tile - segs.seg_tile0[sg]) * kSortTile;
+  const uint32_t end = min(base + (uint32_t)kSortTile, segs.seg_off[sg + 1]);
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = base + threadIdx.x; i < end; i += kSortThreads) {
+    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
+                 c = tri[3 * (size_t)i + 2];
+    v[0] = fminf(v[0], fminf(a.x, fminf(b.x, c.x)));
+    v[1] = fminf(v[1], fminf(a.y, fminf(b.y, c.y)));
+    v[2] = fminf(v[2], fminf(a.z, fminf(b.z, c.z)));
+    v[3] = fmaxf(v[3], fmaxf(a.x, fmaxf(b.x, c.x)));
+    v[4] = fmaxf(v[4], fmaxf(a.y, fmaxf(b.y, c.y)));
+    v[5] = fmaxf(v[5], fmaxf(a.z, fmaxf(b.z, c.z)));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c] = fminf(v[c], __shfl_xor(v[c], o, 64));
+      v[3 + c] = fmaxf(v[3 + c], __shfl_xor(v[3 + c], o, 64));
+    }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int c = 0; c < 6; ++c) s[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    float r = s[0][c];
+    for (int w = 1; w < kSortThreads / 64; ++w)
+      r = c < 3 ? fminf(r, s[w][c]) : fmaxf(r, s[w][c]);
+    if (c < 3) atomicMin(&seg_ord[8 * sg + c], f2ord(r));
+    else atomicMax(&seg_ord[8 * sg + c], f2ord(r));
+  }
+}
+
+// Per-mesh leaf AABB centre -> Morton key, plus the mesh's digit histograms.
+// Bounds: host-provided fp32 (6 per mesh) or the GPU accumulators.
+__global__ __launch_bounds__(kSortThreads) void k_batch_morton(
+    const float4 *__restrict__ tri, SortSegs segs,
+    const float *__restrict__ seg_bounds, const uint32_t *__restrict__ seg_ord,
+    int div_mode, uint32_t *__restrict__ keys, uint32_t *seg_hist) {
+  __shared__ uint32_t s_hist[kSortPasses][kRadix];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t sg = segs.tile_seg[tile];
+  const uint32_t base = segs.seg_off[sg] + (tile - segs.seg_tile0[sg]) * kSortTile;
+  const uint32_t end = min(base + (uint32_t)kSortTile, segs.seg_off[sg + 1]);
+  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += kSortThreads)
+    (&s_hist[0][0])[k] = 0u;
+  float bmin[3], ext[3], rcp[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float lo, hi;
+    if (seg_bounds) {
+      lo = seg_bounds[6 * sg + c];
+      hi = seg_bounds[6 * sg + 3 + c];
+    } else {
+      lo = ord2f(seg_ord[8 * sg + c]);
+      hi = ord2f(seg_ord[8 * sg + 3 + c]);
+    }
+    bmin[c] = lo;
+    ext[c] = __fsub_rn(hi, lo);
+    rcp[c] = __fdiv_rn(1.0f, ext[c]);
+  }
+  __syncthreads();
+  const bool ieee = (div_mode == 1);
+  for (uint32_t i = base + threadIdx.x; i < end; i += kSortThreads) {
+    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
+                 c = tri[3 * (size_t)i + 2];
+    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
+                         fminf(a.z, fminf(b.z, c.z))};
+    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
+                         fmaxf(a.z, fmaxf(b.z, c.z))};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+      const float d = __fsub_rn(cen, bmin[k]);
+      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
+      q[k] = to_u32_sat(__fmul_rn(rel, 1023.99f));
+    }
+    const uint32_t key = split3(q[0]) | (split3(q[1]) << 1) | (split3(q[2]) << 2);
+    keys[i] = key;
+    hist_add(s_hist, key);
+  }
+  __syncthreads();
+  uint32_t *h = seg_hist + (size_t)sg * kSortPasses * kRadix;
+  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += kSortThreads) {
+    const uint32_t v = (&s_hist[0][0])[k];
+    if (v) atomicAdd(&h[k], v);
+  }
+}
+
+void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
+                           uint32_t num_segs, bool init_bounds, Ctl *ctl,
+                           hipStream_t s) {
+  uint64_t words = (uint64_t)num_segs * kSortPasses * kRadix;
+  uint32_t blocks = (uint32_t)((words + kMapThreads - 1) / kMapThreads);
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_batch_prologue, dim3(blocks), dim3(kMapThreads), 0, s,
+                     seg_hist, seg_ord, num_segs, init_bounds ? 1 : 0, ctl);
+}
+
+void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
+                         uint32_t tiles, uint32_t *seg_ord, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_bounds, dim3(tiles), dim3(kSortThreads), 0, s,
+                     tri, segs, seg_ord);
+}
+
+void launch_batch_morton(const float4 *tri, const SortSegs &segs,
+                         uint32_t tiles, const float *seg_bounds,
+                         const uint32_t *seg_ord, int div_mode, uint32_t *keys,
+                         uint32_t *seg_hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_morton, dim3(tiles), dim3(kSortThreads), 0, s,
+                     tri, segs, seg_bounds, seg_ord, div_mode, keys, seg_hist);
+}
+
+}  // namespace lbvh

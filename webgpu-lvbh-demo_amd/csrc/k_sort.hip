@@ -56,12 +56,16 @@ __global__ __launch_bounds__(kMapThreads) void k_histogram(
 
 // One 8-bit digit pass.  kFirst: the payload is the input position (the
 // reference's init_idx, radix-sort.js:527-534, folded into the first pass).
+// With segments (batched builds), every segment is sorted on its own: tiles
+// never straddle a segment, the look-back stops at the segment's first tile,
+// digit bases come from the segment's histograms, and payloads are
+// positions within the segment.
 template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t pass, uint32_t num_tiles, Ctl *ctl,
-    uint64_t *status, int last_pass) {
+    uint64_t *status, int last_pass, SortSegs segs) {
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[kWaves][kRadix];
@@ -83,14 +87,24 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   }
   for (int k = tid; k < kWaves * kRadix; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
-  const uint32_t gcount = ctl->hist[pass][tid];
   const uint32_t stamp = (ctl->epoch * 4u + pass) & 0x3FFFFFFFu;
   __syncthreads();
   const uint32_t tile = s_tile;
+  uint32_t seg_base = 0, seg_end = n, tile0 = 0;
+  const uint32_t *hist = ctl->hist[pass];
+  if (segs.tile_seg) {
+    const uint32_t sg = segs.tile_seg[tile];
+    seg_base = segs.seg_off[sg];
+    seg_end = segs.seg_off[sg + 1];
+    tile0 = segs.seg_tile0[sg];
+    hist = segs.seg_hist + (size_t)sg * kSortPasses * kRadix + pass * kRadix;
+  }
+  const uint32_t gcount = hist[tid];
+  const uint32_t tile_base = seg_base + (tile - tile0) * kSortTile;
 
   // Warp-striped load: wave w owns keys [wbase, wbase + 64*kSortItems), item
   // j of lane l sits at wbase + 64*j + l, so (j, l) order is input order.
-  const uint32_t wbase = tile * kSortTile + wave * 64 * kSortItems;
+  const uint32_t wbase = tile_base + wave * 64 * kSortItems;
   uint32_t key[kSortItems];
   int32_t val[kSortItems];
 #pragma unroll
@@ -98,9 +112,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t pos = wbase + 64 * j + lane;
     key[j] = 0u;
     val[j] = 0;
-    if (pos < n) {
+    if (pos < seg_end) {
       key[j] = keys_in[pos];
-      val[j] = kFirst ? (int32_t)pos : vals_in[pos];
+      val[j] = kFirst ? (int32_t)(pos - seg_base) : vals_in[pos];
     }
   }
 
@@ -110,7 +124,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   uint32_t rank[kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    const bool valid = (wbase + 64 * j + lane) < n;
+    const bool valid = (wbase + 64 * j + lane) < seg_end;
     const uint32_t d = (key[j] >> shift) & (kRadix - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -140,7 +154,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   }
   uint64_t *my_word = status + (size_t)tile * kRadix + d;
   __hip_atomic_store(my_word,
-                     lb_pack(stamp, tile == 0 ? kFlagInclusive : kFlagAggregate,
+                     lb_pack(stamp, tile == tile0 ? kFlagInclusive : kFlagAggregate,
                              total),
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
@@ -156,14 +170,14 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   // loop should that invariant ever break (the result is then wrong and
   // ctl->fault says so, but the kernel still exits).
   uint32_t excl = 0;
-  if (tile > 0) {
+  if (tile > tile0) {
     int64_t k = (int64_t)tile - 1;
     uint32_t spins = 0;
-    while (k >= 0) {
+    while (k >= (int64_t)tile0) {
       uint64_t w[kLookback];
 #pragma unroll
       for (int j = 0; j < kLookback; ++j)
-        w[j] = (k - j >= 0)
+        w[j] = (k - j >= (int64_t)tile0)
                    ? __hip_atomic_load(status + (size_t)(k - j) * kRadix + d,
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                    : 0ull;
@@ -172,7 +186,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 #pragma unroll
       for (int j = 0; j < kLookback; ++j) {
         if (done || used < j) continue;      // stopped earlier in this probe
-        if (k - j < 0) { done = true; continue; }
+        if (k - j < (int64_t)tile0) { done = true; continue; }
         const uint32_t hi = (uint32_t)(w[j] >> 32);
         if ((hi >> 2) != stamp) continue;  // not published yet
         excl += (uint32_t)w[j];
@@ -192,13 +206,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     __hip_atomic_store(my_word, lb_pack(stamp, kFlagInclusive, excl + total),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  s_gbase[d] = gdigit + excl - start;
+  s_gbase[d] = seg_base + gdigit + excl - start;
   __syncthreads();
 
   // Reorder the tile in LDS by (digit, input order).
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    if ((wbase + 64 * j + lane) < n) {
+    if ((wbase + 64 * j + lane) < seg_end) {
       const uint32_t dj = (key[j] >> shift) & (kRadix - 1);
       const uint32_t lp = s_start[dj] + s_whist[wave][dj] + rank[j];
       s_keys[lp] = key[j];
@@ -209,21 +223,21 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 
   // Write out: consecutive threads take consecutive tile positions, so each
   // digit run goes to consecutive global addresses.
-  const uint32_t tile_base = tile * kSortTile;
-  const uint32_t tile_n = (n - tile_base < (uint32_t)kSortTile)
-                              ? n - tile_base : (uint32_t)kSortTile;
+  const uint32_t tile_n = (seg_end - tile_base < (uint32_t)kSortTile)
+                              ? seg_end - tile_base : (uint32_t)kSortTile;
   for (uint32_t p = tid; p < tile_n; p += kSortThreads) {
     const uint32_t k = s_keys[p];
     const uint32_t g = s_gbase[(k >> shift) & (kRadix - 1)] + p;
-    if (g < n) {
+    if (g >= seg_base && g < seg_end) {
       keys_out[g] = k;
       vals_out[g] = s_vals[p];
     }
   }
 
-  if (last_pass) {
+  if (last_pass && !segs.tile_seg) {
     // The last block to finish clears the digit histograms for the next
-    // operation: every block read its counts before arriving here.
+    // operation: every block read its counts before arriving here.  (Batched
+    // builds clear their per-segment histograms up front.)
     if (tid == 0) {
       const uint32_t done = __hip_atomic_fetch_add(
           &ctl->done_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -252,17 +266,20 @@ void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
 void launch_onesweep_pass(int pass, const uint32_t *keys_in,
                           const int32_t *vals_in, uint32_t *keys_out,
                           int32_t *vals_out, uint32_t n, Ctl *ctl,
-                          uint64_t *status, hipStream_t s) {
-  const uint32_t tiles = sort_tiles(n);
+                          uint64_t *status, hipStream_t s,
+                          const SortSegs *segs, uint32_t seg_tiles) {
+  const SortSegs one{};
+  const SortSegs &sg = segs ? *segs : one;
+  const uint32_t tiles = segs ? seg_tiles : sort_tiles(n);
   const int last = (pass == kSortPasses - 1) ? 1 : 0;
   if (pass == 0)
     hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kSortThreads), 0, s,
                        keys_in, (const int32_t *)nullptr, keys_out, vals_out,
-                       n, 0u, tiles, ctl, status, last);
+                       n, 0u, tiles, ctl, status, last, sg);
   else
     hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kSortThreads), 0,
                        s, keys_in, vals_in, keys_out, vals_out, n,
-                       (uint32_t)pass, tiles, ctl, status, last);
+                       (uint32_t)pass, tiles, ctl, status, last, sg);
 }
 
 }  // namespace lbvh

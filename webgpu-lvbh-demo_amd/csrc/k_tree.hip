@@ -384,17 +384,19 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
 }
 
 // One agglomeration level.  kLeaves: level 0 over the sorted leaves (gathers
-// O_TRI and the leaf boxes); otherwise the items of the level below.
+// O_TRI and the leaf boxes); otherwise the items of the level below.  With
+// segments (batched builds) every segment is an independent mesh: its tiles
+// are consecutive, positions, node ids and the permutation are relative to
+// its first leaf, and its frontier items go to its own output region.
 template <int T, bool kLeaves>
 __global__ __launch_bounds__(T) void k_agglomerate(
-    const int4 *__restrict__ tri, const uint32_t *__restrict__ skeys,
-    const int32_t *__restrict__ perm, uint32_t n, int4 *__restrict__ otri,
-    const int4 *__restrict__ items_in, int4 *__restrict__ items_out,
-    uint32_t out_cap, uint32_t level, uint64_t *lstatus, int4 *nodes,
-    Ctl *ctl) {
+    const int4 *__restrict__ tri_all, const uint32_t *__restrict__ skeys_all,
+    const int32_t *__restrict__ perm_all, uint32_t n_all,
+    int4 *__restrict__ otri_all, const int4 *__restrict__ items_in,
+    int4 *__restrict__ items_out, uint32_t out_cap_all, uint32_t level,
+    uint64_t *lstatus, int4 *nodes_all, Ctl *ctl, TreeSegs segs) {
   __shared__ AggLds<T> L;
   const int64_t tid = threadIdx.x;
-  const int64_t N = n;
   if (tid == 0) {
     const uint32_t t = atomicAdd(&ctl->agg_tile_ctr[level], 1u);
     if (t == gridDim.x - 1)
@@ -402,15 +404,33 @@ __global__ __launch_bounds__(T) void k_agglomerate(
                          __HIP_MEMORY_SCOPE_AGENT);
     L.tile = t;
   }
-  const int64_t M = kLeaves ? N : (int64_t)ctl->level_count[level - 1];
   const uint32_t stamp = (ctl->epoch * 8u + level) & 0x3FFFFFFFu;
   __syncthreads();
-  const int64_t tile = L.tile;
+  const int64_t gtile = L.tile;
+  int64_t tile0 = 0, leaf_base = 0, N = n_all, M, in_base = 0, out_base = 0;
+  uint32_t out_cap = out_cap_all;
+  uint32_t *count_slot = &ctl->level_count[level];
+  if (segs.tile_seg) {
+    const uint32_t sg = segs.tile_seg[gtile];
+    tile0 = segs.seg_tile0[sg];
+    leaf_base = segs.seg_off[sg];
+    N = (int64_t)segs.seg_off[sg + 1] - leaf_base;
+    M = kLeaves ? N : (int64_t)segs.count_in[sg];
+    if (!kLeaves) in_base = segs.in_off[sg];
+    out_base = segs.out_off[sg];
+    out_cap = segs.out_cap[sg];
+    count_slot = &segs.count_out[sg];
+  } else {
+    M = kLeaves ? N : (int64_t)ctl->level_count[level - 1];
+  }
+  const int64_t tile = gtile - tile0;  // tile within the mesh
   const int64_t ts = tile * T;
   if (ts >= M) {
-    if (tile == 0) ctl->level_count[level] = 0u;  // nothing at this level
+    if (tile == 0) *count_slot = 0u;  // nothing at this level
     return;
   }
+  const uint32_t *skeys = skeys_all + leaf_base;
+  int4 *nodes = nodes_all + 4 * leaf_base;
   const int64_t te = (ts + T < M) ? ts + T : M;
   const int64_t k = ts + tid;
   const bool active = k < te;
@@ -423,16 +443,18 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   if (active) {
     if (kLeaves) {
       c.l = c.r = k;
-      const uint32_t pi = (uint32_t)perm[k];
-      const size_t src = 3 * (size_t)(pi < (uint32_t)N ? pi : 0u);
-      const int4 a = tri[src], b = tri[src + 1], d = tri[src + 2];
-      otri[3 * k] = a;
-      otri[3 * k + 1] = b;
-      otri[3 * k + 2] = d;
+      const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
+      const size_t src = 3 * (size_t)(leaf_base + (pi < (uint32_t)N ? pi : 0u));
+      const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
+      int4 *o = otri_all + 3 * (leaf_base + k);
+      o[0] = a;
+      o[1] = b;
+      o[2] = d;
       tri_box(a, b, d, c.mn, c.mx);
       perturb(c.mn, c.mx);
     } else {
-      const int4 a = items_in[2 * k], b = items_in[2 * k + 1];
+      const int4 a = items_in[2 * (in_base + k)],
+                 b = items_in[2 * (in_base + k) + 1];
       c.l = a.x; c.r = a.y;
       c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
       c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
@@ -470,16 +492,16 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   __syncthreads();
   const uint32_t count = s_count;
   if (tid == 0) {
-    uint64_t *mine = lstatus + tile;
+    uint64_t *mine = lstatus + gtile;
     __hip_atomic_store(mine,
                        ((uint64_t)((stamp << 2) | (tile == 0 ? kLbInclusive
                                                              : kLbAggregate))
                         << 32) | count,
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
-    int64_t j = tile - 1;
+    int64_t j = gtile - 1;
     uint32_t spins = 0;
-    while (j >= 0) {
+    while (j >= tile0) {
       const uint64_t w = __hip_atomic_load(lstatus + j, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t hi = (uint32_t)(w >> 32);
@@ -500,7 +522,7 @@ __global__ __launch_bounds__(T) void k_agglomerate(
                          ((uint64_t)((stamp << 2) | kLbInclusive) << 32) |
                              (excl + count),
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (te == M) ctl->level_count[level] = excl + count;
+    if (te == M) *count_slot = excl + count;
     if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
     L.base = excl;
   }
@@ -508,10 +530,11 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   if (out) {
     const uint32_t dst = L.base + rank;
     if (dst < out_cap) {
-      items_out[2 * (size_t)dst] =
+      int4 *io = items_out + 2 * (out_base + dst);
+      io[0] =
           make_int4((int)c.l, (int)c.r, __float_as_int(c.mn[0]),
                     __float_as_int(c.mn[1]));
-      items_out[2 * (size_t)dst + 1] =
+      io[1] =
           make_int4(__float_as_int(c.mn[2]), __float_as_int(c.mx[0]),
                     __float_as_int(c.mx[1]), __float_as_int(c.mx[2]));
     } else {
@@ -584,7 +607,7 @@ static void launch_levels(const int4 *t4, const uint32_t *skeys,
   uint64_t *st = lstatus;
   hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid[0]), dim3(T), 0, s,
                      t4, skeys, perm, n, o4, (const int4 *)nullptr, items_a,
-                     cap[0], 0u, st, n4, ctl);
+                     cap[0], 0u, st, n4, ctl, TreeSegs{});
   st += grid[0];
   for (int L = 1; L < levels; ++L) {
     const int4 *in = (L & 1) ? items_a : items_b;
@@ -592,8 +615,48 @@ static void launch_levels(const int4 *t4, const uint32_t *skeys,
     hipLaunchKernelGGL((k_agglomerate<T, false>), dim3(grid[L]), dim3(T), 0,
                        s, (const int4 *)nullptr, skeys, perm, n,
                        (int4 *)nullptr, in, out, cap[L], (uint32_t)L, st, n4,
-                       ctl);
+                       ctl, TreeSegs{});
     st += grid[L];
+  }
+}
+
+template <int T>
+static void launch_level_t(int level, uint32_t grid, const int4 *t4,
+                           const uint32_t *skeys, const int32_t *perm,
+                           uint32_t n, int4 *o4, const int4 *in, int4 *out,
+                           uint32_t cap, uint64_t *st, int4 *n4, Ctl *ctl,
+                           const TreeSegs &segs, hipStream_t s) {
+  if (level == 0)
+    hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid), dim3(T), 0, s, t4,
+                       skeys, perm, n, o4, in, out, cap, 0u, st, n4, ctl,
+                       segs);
+  else
+    hipLaunchKernelGGL((k_agglomerate<T, false>), dim3(grid), dim3(T), 0, s,
+                       t4, skeys, perm, n, o4, in, out, cap, (uint32_t)level,
+                       st, n4, ctl, segs);
+}
+
+void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
+                       const uint32_t *skeys, const int32_t *perm, uint32_t n,
+                       float4 *otri, const int4 *items_in, int4 *items_out,
+                       uint32_t out_cap, uint64_t *lstatus, float4 *nodes,
+                       Ctl *ctl, const TreeSegs &segs, hipStream_t s) {
+  const int4 *t4 = reinterpret_cast<const int4 *>(tri);
+  int4 *o4 = reinterpret_cast<int4 *>(otri);
+  int4 *n4 = reinterpret_cast<int4 *>(nodes);
+  switch (tile) {
+    case 256:
+      launch_level_t<256>(level, grid, t4, skeys, perm, n, o4, items_in,
+                          items_out, out_cap, lstatus, n4, ctl, segs, s);
+      break;
+    case 1024:
+      launch_level_t<1024>(level, grid, t4, skeys, perm, n, o4, items_in,
+                           items_out, out_cap, lstatus, n4, ctl, segs, s);
+      break;
+    default:
+      launch_level_t<512>(level, grid, t4, skeys, perm, n, o4, items_in,
+                          items_out, out_cap, lstatus, n4, ctl, segs, s);
+      break;
   }
 }
 

@@ -5,6 +5,7 @@
 // keeps is charged against the context's HBM budget.  Entry points validate
 // their arguments on the host, enqueue kernels and return; nothing here
 // blocks on the GPU except lbvh_upload/lbvh_download/lbvh_synchronize.
+#include <algorithm>
 #include <mutex>
 #include <new>
 #include <string>
@@ -45,6 +46,25 @@ struct lbvh_ctx {
   DevBuf lstatus;                       // their look-back words (zeroed)
   DevBuf ctag;                          // per-stage up-pass tags (zeroed)
   DevBuf partials;                      // bounds partials
+  // batched builds: scratch, and the plan (tile tables + captured graph) of
+  // the last batch layout
+  DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_status, b_hist, b_ord,
+      b_bounds, b_items_a, b_items_b, b_lstatus, b_table;
+  struct BatchPlan {
+    std::vector<uint32_t> offsets;
+    int tile = 0;
+    uint32_t S = 0, total = 0, sort_tiles = 0;
+    int levels = 0;
+    std::vector<uint32_t> level_tiles, level_cap;
+    // u32 offsets of the tables inside b_table
+    uint64_t o_seg_off = 0, o_tile_seg = 0, o_seg_tile0 = 0;
+    std::vector<uint64_t> o_ltile_seg, o_lseg_tile0, o_lin, o_lout, o_lcap,
+        o_lcount;
+    uint64_t items_cap = 0, lstatus_words = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> graph_key;
+  } plan;
   // profiling: event pairs bracketing each stage, read back on demand
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -137,7 +157,8 @@ constexpr uint64_t kEpochRewind = 1ull << 26;
 int begin_op(lbvh_ctx *ctx) {
   if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->status, &ctx->lstatus, &ctx->ctag})
+  for (DevBuf *b : {&ctx->status, &ctx->lstatus, &ctx->ctag, &ctx->b_status,
+                    &ctx->b_lstatus})
     if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
   HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
                               ctx->stream));
@@ -177,6 +198,21 @@ struct StageScope {
     ctx->marks.push_back({stage, a, b});
   }
 };
+
+std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
+  return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
+          &ctx->status,  &ctx->gside,    &ctx->items_a,  &ctx->items_b,
+          &ctx->lstatus, &ctx->ctag,     &ctx->partials, &ctx->b_keys_a,
+          &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_status,
+          &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_items_a,
+          &ctx->b_items_b, &ctx->b_lstatus, &ctx->b_table};
+}
+
+void drop_plan(lbvh_ctx *ctx) {
+  if (ctx->plan.exec) (void)hipGraphExecDestroy(ctx->plan.exec);
+  if (ctx->plan.graph) (void)hipGraphDestroy(ctx->plan.graph);
+  ctx->plan = lbvh_ctx::BatchPlan{};
+}
 
 int check_launch(lbvh_ctx *ctx, const char *what) {
   hipError_t e = hipGetLastError();
@@ -249,6 +285,132 @@ MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
   std::lock_guard<std::recursive_mutex> lock_((ctx)->mu);     \
   (void)hipSetDevice((ctx)->device)
 
+
+// ---- batched builds ---------------------------------------------------------
+
+// Builds (or reuses) the tile tables of a batch layout: sort tiles per mesh,
+// and for every agglomeration level the tiles, item regions and counts of
+// each mesh (the single-mesh level plan, per mesh).  Uploaded once.
+int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
+  auto &P = ctx->plan;
+  if (P.S == S && P.tile == ctx->tree_tile &&
+      std::equal(P.offsets.begin(), P.offsets.end(), offsets) &&
+      P.offsets.size() == (size_t)S + 1)
+    return LBVH_OK;
+  drop_plan(ctx);
+  P.offsets.assign(offsets, offsets + S + 1);
+  P.S = S;
+  P.tile = ctx->tree_tile;
+  P.total = offsets[S];
+  std::vector<uint32_t> tab;
+  auto put = [&](const std::vector<uint32_t> &v) {
+    const uint64_t at = tab.size();
+    tab.insert(tab.end(), v.begin(), v.end());
+    return at;
+  };
+  P.o_seg_off = put(P.offsets);
+  std::vector<uint32_t> tile_seg, seg_tile0(S, 0);
+  std::vector<int> levels_k(S, 0);
+  std::vector<std::vector<uint32_t>> grid_k(S), cap_k(S);
+  for (uint32_t k = 0; k < S; ++k) {
+    const uint32_t n = offsets[k + 1] - offsets[k];
+    seg_tile0[k] = (uint32_t)tile_seg.size();
+    for (uint32_t j = 0; j < sort_tiles(n); ++j) tile_seg.push_back(k);
+    if (n) {
+      grid_k[k].resize(kMaxTreeLevels);
+      cap_k[k].resize(kMaxTreeLevels);
+      levels_k[k] = tree_levels(n, P.tile, grid_k[k].data(), cap_k[k].data());
+      P.levels = std::max(P.levels, levels_k[k]);
+    }
+  }
+  P.sort_tiles = (uint32_t)tile_seg.size();
+  P.o_tile_seg = put(tile_seg);
+  P.o_seg_tile0 = put(seg_tile0);
+  std::vector<uint32_t> prev_out(S, 0);
+  for (int L = 0; L < P.levels; ++L) {
+    std::vector<uint32_t> ltile_seg, lseg_tile0(S, 0), lout(S, 0), lcap(S, 0);
+    uint32_t out_total = 0;
+    for (uint32_t k = 0; k < S; ++k) {
+      if (L >= levels_k[k]) continue;
+      lseg_tile0[k] = (uint32_t)ltile_seg.size();
+      for (uint32_t j = 0; j < grid_k[k][L]; ++j) ltile_seg.push_back(k);
+      lout[k] = out_total;
+      lcap[k] = cap_k[k][L];
+      out_total += cap_k[k][L];
+    }
+    P.level_tiles.push_back((uint32_t)ltile_seg.size());
+    P.level_cap.push_back(out_total);
+    P.o_ltile_seg.push_back(put(ltile_seg));
+    P.o_lseg_tile0.push_back(put(lseg_tile0));
+    P.o_lin.push_back(put(prev_out));
+    P.o_lout.push_back(put(lout));
+    P.o_lcap.push_back(put(lcap));
+    P.o_lcount.push_back(put(std::vector<uint32_t>(S, 0)));
+    prev_out = lout;
+    P.items_cap = std::max<uint64_t>(P.items_cap, out_total);
+    P.lstatus_words += ltile_seg.size();
+  }
+  P.items_cap = std::max<uint64_t>(P.items_cap, 1);
+  int st;
+  if ((st = ensure(ctx, ctx->b_table, 4ull * tab.size(), false))) return st;
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->b_table.ptr, tab.data(), 4ull * tab.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return LBVH_OK;
+}
+
+// Enqueues the whole batched pipeline (captured into a graph by the caller).
+void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
+                   void *bvh_out, float *tri_out, uint32_t *skeys,
+                   int32_t *perm) {
+  const auto &P = ctx->plan;
+  const uint32_t *T = static_cast<const uint32_t *>(ctx->b_table.ptr);
+  uint32_t *TW = static_cast<uint32_t *>(ctx->b_table.ptr);
+  uint32_t *hist = static_cast<uint32_t *>(ctx->b_hist.ptr);
+  uint32_t *ord = static_cast<uint32_t *>(ctx->b_ord.ptr);
+  const float4 *t4 = reinterpret_cast<const float4 *>(tri);
+  const SortSegs ss{T + P.o_tile_seg, T + P.o_seg_tile0, T + P.o_seg_off,
+                    hist};
+  launch_batch_prologue(hist, ord, P.S, !have_bounds, ctx->ctl, ctx->stream);
+  if (!have_bounds)
+    launch_batch_bounds(t4, ss, P.sort_tiles, ord, ctx->stream);
+  uint32_t *keys = static_cast<uint32_t *>(ctx->b_keys_a.ptr);
+  uint32_t *tmp = static_cast<uint32_t *>(ctx->b_keys_b.ptr);
+  uint32_t *out = skeys ? skeys : keys;
+  int32_t *vtmp = static_cast<int32_t *>(ctx->b_idx_b.ptr);
+  int32_t *vout = perm ? perm : static_cast<int32_t *>(ctx->b_idx_a.ptr);
+  launch_batch_morton(t4, ss, P.sort_tiles,
+                      have_bounds ? static_cast<const float *>(ctx->b_bounds.ptr)
+                                  : nullptr,
+                      ord, ctx->div_mode, keys, hist, ctx->stream);
+  uint64_t *status = static_cast<uint64_t *>(ctx->b_status.ptr);
+  const uint32_t *src[kSortPasses] = {keys, tmp, out, tmp};
+  uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
+  const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
+  int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
+  for (int p = 0; p < kSortPasses; ++p)
+    launch_onesweep_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
+                         ctx->ctl, status, ctx->stream, &ss, P.sort_tiles);
+  int4 *ia = static_cast<int4 *>(ctx->b_items_a.ptr);
+  int4 *ib = static_cast<int4 *>(ctx->b_items_b.ptr);
+  uint64_t *ls = static_cast<uint64_t *>(ctx->b_lstatus.ptr);
+  for (int L = 0; L < P.levels; ++L) {
+    if (P.level_tiles[L] == 0) break;
+    const TreeSegs ts{T + P.o_ltile_seg[L], T + P.o_lseg_tile0[L],
+                      T + P.o_seg_off,      L ? T + P.o_lin[L] : nullptr,
+                      T + P.o_lout[L],      T + P.o_lcap[L],
+                      L ? T + P.o_lcount[L - 1] : nullptr,
+                      TW + P.o_lcount[L]};
+    const int4 *in = (L & 1) ? ia : ib;
+    int4 *outi = (L & 1) ? ib : ia;
+    launch_tree_level(L, P.tile, P.level_tiles[L], t4, out, vout, P.total,
+                      reinterpret_cast<float4 *>(tri_out), in, outi, 0u, ls,
+                      static_cast<float4 *>(bvh_out), ctx->ctl, ts,
+                      ctx->stream);
+    ls += P.level_tiles[L];
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -311,11 +473,8 @@ int lbvh_destroy(lbvh_ctx *ctx) {
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
-                      &ctx->status, &ctx->gside, &ctx->items_a, &ctx->items_b,
-                      &ctx->lstatus, &ctx->ctag,
-                      &ctx->partials})
-      dev_release(ctx, *b);
+    drop_plan(ctx);
+    for (DevBuf *b : all_scratch(ctx)) dev_release(ctx, *b);
     for (auto &m : ctx->marks) {
       ctx->ev_pool.push_back(m.a);
       ctx->ev_pool.push_back(m.b);
@@ -391,11 +550,8 @@ uint64_t lbvh_bytes_in_use(const lbvh_ctx *ctx) {
 int lbvh_trim(lbvh_ctx *ctx) {
   CTX_GUARD(ctx);
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->keys_a, &ctx->keys_b, &ctx->idx_a, &ctx->idx_b,
-                    &ctx->status, &ctx->gside, &ctx->items_a, &ctx->items_b,
-                      &ctx->lstatus, &ctx->ctag,
-                    &ctx->partials})
-    dev_release(ctx, *b);
+  drop_plan(ctx);
+  for (DevBuf *b : all_scratch(ctx)) dev_release(ctx, *b);
   return LBVH_OK;
 }
 
@@ -636,6 +792,79 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
                     static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
                     ctx->tree_tile, ctx->stream);
   return check_launch(ctx, "lbvh_refit");
+}
+
+
+// ---- batched build -------------------------------------------------------------
+
+int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
+                       const uint32_t *offsets, uint32_t num_meshes,
+                       const float *bounds_host, void *bvh_out, float *tri_out,
+                       uint32_t *sorted_keys_out, int32_t *perm_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if (!offsets) return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null offsets");
+  if (num_meshes == 0) return LBVH_OK;
+  if (offsets[0] != 0)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+  for (uint32_t k = 0; k < num_meshes; ++k)
+    if (offsets[k + 1] < offsets[k])
+      return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "offsets must not decrease");
+  const uint32_t total = offsets[num_meshes];
+  if ((st = check_n(ctx, total))) return st;
+  if (total == 0) return LBVH_OK;
+  if (!tri || !bvh_out || !tri_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = make_plan(ctx, offsets, num_meshes))) return st;
+  const auto &P = ctx->plan;
+  if ((st = ensure(ctx, ctx->b_keys_a, 4ull * total, false))) return st;
+  if ((st = ensure(ctx, ctx->b_keys_b, 4ull * total, false))) return st;
+  if ((st = ensure(ctx, ctx->b_idx_a, 4ull * total, false))) return st;
+  if ((st = ensure(ctx, ctx->b_idx_b, 4ull * total, false))) return st;
+  if ((st = ensure(ctx, ctx->b_status, 8ull * kRadix * P.sort_tiles, true)))
+    return st;
+  if ((st = ensure(ctx, ctx->b_hist, 4ull * kSortPasses * kRadix * num_meshes,
+                   false)))
+    return st;
+  if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;
+  if ((st = ensure(ctx, ctx->b_items_a, 32ull * P.items_cap, false))) return st;
+  if ((st = ensure(ctx, ctx->b_items_b, 32ull * P.items_cap, false))) return st;
+  if ((st = ensure(ctx, ctx->b_lstatus, 8ull * std::max<uint64_t>(P.lstatus_words, 1),
+                   true)))
+    return st;
+  if (bounds_host) {
+    if ((st = ensure(ctx, ctx->b_bounds, 24ull * num_meshes, false))) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->b_bounds.ptr, bounds_host,
+                                24ull * num_meshes, hipMemcpyHostToDevice,
+                                ctx->stream));
+  }
+  if ((st = begin_op(ctx))) return st;
+  // Replay the captured graph when the batch layout, buffers and options
+  // match the last call; otherwise capture the pipeline anew.
+  std::vector<const void *> key = {
+      tri, bvh_out, tri_out, sorted_keys_out, perm_out,
+      reinterpret_cast<const void *>((uintptr_t)(bounds_host ? 1 : 0)),
+      reinterpret_cast<const void *>((uintptr_t)ctx->div_mode)};
+  for (DevBuf *b : all_scratch(ctx)) key.push_back(b->ptr);
+  auto &Pm = ctx->plan;
+  if (!Pm.exec || Pm.graph_key != key) {
+    if (Pm.exec) (void)hipGraphExecDestroy(Pm.exec);
+    if (Pm.graph) (void)hipGraphDestroy(Pm.graph);
+    Pm.exec = nullptr;
+    Pm.graph = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream,
+                                       hipStreamCaptureModeThreadLocal));
+    enqueue_batch(ctx, tri, bounds_host != nullptr, bvh_out, tri_out,
+                  sorted_keys_out, perm_out);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+    Pm.graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&Pm.exec, g, nullptr, nullptr, 0));
+    Pm.graph_key = key;
+  }
+  HIP_TRY(ctx, hipGraphLaunch(Pm.exec, ctx->stream));
+  return check_launch(ctx, "lbvh_build_batched");
 }
 
 // ---- consumer check ------------------------------------------------------------

@@ -87,10 +87,22 @@ void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
 // ignores vals_in; pass kSortPasses-1 clears the histograms when done).
 // The four passes sort keys_in into keys_out through keys_tmp:
 //   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
+// Segments of a batched sort (all null: one segment [0, n)).  Tiles of
+// kSortTile keys never straddle segments; tile_seg maps each tile to its
+// segment, seg_tile0 gives a segment's first tile, seg_off its first key
+// (S+1 entries), seg_hist its kSortPasses x kRadix digit counts.
+struct SortSegs {
+  const uint32_t *tile_seg;
+  const uint32_t *seg_tile0;
+  const uint32_t *seg_off;
+  const uint32_t *seg_hist;
+};
 void launch_onesweep_pass(int pass, const uint32_t *keys_in,
                           const int32_t *vals_in, uint32_t *keys_out,
                           int32_t *vals_out, uint32_t n, Ctl *ctl,
-                          uint64_t *status, hipStream_t s);
+                          uint64_t *status, hipStream_t s,
+                          const SortSegs *segs = nullptr,
+                          uint32_t seg_tiles = 0);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
@@ -103,6 +115,21 @@ void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s);
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                       float4 *out, hipStream_t s);
+// Segments of one agglomeration level of a batched build (all null: one
+// mesh).  Per tile: segment; per segment: first tile at this level, first
+// leaf (seg_off, S+1 entries), input item offset and count (levels >= 1),
+// output item offset, capacity and count.
+struct TreeSegs {
+  const uint32_t *tile_seg;
+  const uint32_t *seg_tile0;
+  const uint32_t *seg_off;
+  const uint32_t *in_off;
+  const uint32_t *out_off;
+  const uint32_t *out_cap;
+  const uint32_t *count_in;
+  uint32_t *count_out;
+};
+
 // Fused leaf gather + radix tree + node boxes by multi-level agglomeration.
 // tile: 256, 512 or 1024 leaves per block.  items_a/items_b hold
 // tree_item_capacity(n, tile) 32-byte items each; lstatus holds
@@ -112,6 +139,23 @@ void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        float4 *nodes, int4 *items_a, int4 *items_b,
                        uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s);
 int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap);
+// One agglomeration level with explicit segments (batched builds).
+void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
+                       const uint32_t *skeys, const int32_t *perm, uint32_t n,
+                       float4 *otri, const int4 *items_in, int4 *items_out,
+                       uint32_t out_cap, uint64_t *lstatus, float4 *nodes,
+                       Ctl *ctl, const TreeSegs &segs, hipStream_t s);
+
+// k_batch.hip
+void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
+                           uint32_t num_segs, bool init_bounds, Ctl *ctl,
+                           hipStream_t s);
+void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
+                         uint32_t tiles, uint32_t *seg_ord, hipStream_t s);
+void launch_batch_morton(const float4 *tri, const SortSegs &segs,
+                         uint32_t tiles, const float *seg_bounds,
+                         const uint32_t *seg_ord, int div_mode, uint32_t *keys,
+                         uint32_t *seg_hist, hipStream_t s);
 
 // k_raycast.hip
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,

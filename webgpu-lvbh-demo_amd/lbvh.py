@@ -69,6 +69,7 @@ SIGNATURES = {
     "lbvh_build": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     "lbvh_refit": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "lbvh_raycast": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
+    "lbvh_build_batched": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     "lbvh_profile_enable": (_i32, [_vp, _i32]),
     "lbvh_profile_read": (_i32, [_vp, _vp, _i32]),
 }
@@ -278,6 +279,14 @@ class Context:
         b = None if bounds is None else np.asarray(bounds, dtype=np.float32)
         self._check(self.lib.lbvh_build(self.handle, _dp(tri), n, _ptr(b), _dp(bvh),
                                         _dp(otri), _dp(skeys), _dp(perm)))
+
+    def build_batched(self, tri, offsets: np.ndarray, bounds, bvh, otri,
+                      skeys=None, perm=None):
+        off = np.ascontiguousarray(offsets, dtype=np.uint32)
+        b = None if bounds is None else np.ascontiguousarray(bounds, dtype=np.float32)
+        self._check(self.lib.lbvh_build_batched(
+            self.handle, _dp(tri), _ptr(off), len(off) - 1, _ptr(b), _dp(bvh),
+            _dp(otri), _dp(skeys), _dp(perm)))
 
     def refit(self, tri, n: int, skeys, perm, bvh, otri):
         self._check(self.lib.lbvh_refit(self.handle, _dp(tri), n, _dp(skeys),
