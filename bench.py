@@ -20,9 +20,10 @@ import argparse
 import ctypes as C
 import json
 import os
-import statistics
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "webgpu-lvbh-demo_amd")
@@ -98,6 +99,129 @@ def timed_builds(lb, ctx, hip, d_tri, n, bounds, bvh, otri, steps):
     return ms, wall
 
 
+def batched_bench(lb, hip, meshes=2048, steps=5, log=print):
+    """2,048 independent meshes, log-uniform in [256, 262144] triangles, each a
+    uniform mesh with its own seed: one graph-captured lbvh_build_batched vs a
+    loop of per-mesh lbvh_build calls on the same stream."""
+    sizes = lb.wl_log_uniform_sizes(lb.BATCH_SEED, meshes, 256, 262144)
+    offsets = np.zeros(meshes + 1, np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    total = int(offsets[-1])
+    log(f"batched: generating {meshes} meshes, {total} triangles")
+    tri = np.empty((total, 12), np.float32)
+    for k in range(meshes):
+        tri[offsets[k]:offsets[k + 1]] = lb.wl_uniform(lb.BATCH_SEED * 100000 + k,
+                                                       int(sizes[k]), 0.01)
+    ctx = lb.Context(0)
+    d = ctx.to_device(tri)
+    del tri
+    bvh, otri = ctx.alloc(64 * total), ctx.alloc(48 * total)
+    ctx.build_batched(d, offsets, None, bvh, otri)  # capture + warm
+    ctx.synchronize()
+    a, b = hip.event(), hip.event()
+    hip.record(a, ctx.stream)
+    for _ in range(steps):
+        ctx.build_batched(d, offsets, None, bvh, otri)
+    hip.record(b, ctx.stream)
+    batched_ms = hip.elapsed_ms(a, b) / steps
+    log("batched: per-mesh loop")
+    nodes, tris = bvh.ptr, otri.ptr
+
+    def loop():
+        for k in range(meshes):
+            o = int(offsets[k])
+            ctx.build(d.ptr + 48 * o, int(sizes[k]), None, nodes + 64 * o, tris + 48 * o)
+
+    loop()
+    ctx.synchronize()
+    hip.record(a, ctx.stream)
+    loop_steps = 2
+    for _ in range(loop_steps):
+        loop()
+    hip.record(b, ctx.stream)
+    loop_ms = hip.elapsed_ms(a, b) / loop_steps
+    for x in (d, bvh, otri):
+        x.destroy()
+    ctx.close()
+    return {"workload": f"{meshes} uniform meshes, log-uniform [256, 262144]",
+            "meshes": meshes, "triangles": total,
+            "batched_ms": round(batched_ms, 3), "loop_ms": round(loop_ms, 3),
+            "speedup": round(loop_ms / batched_ms, 2),
+            "batched_Mtri_s": round(total / (batched_ms * 1e-3) / 1e6, 1)}
+
+
+def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
+    """Poisson arrivals of build/refit requests: sizes log-uniform in
+    [1024, 4194304]; 70% refits of a recently built resident mesh (vertices
+    + N(0, 0.001)), 30% rebuilds of a new uniform mesh.  Meshes are
+    synthesised on the device from their seeds.  The arrival rate is 0.7x
+    the capacity measured on a back-to-back burst."""
+    rng = np.random.default_rng(5)
+    sizes = lb.wl_log_uniform_sizes(6, requests, 1024, 4194304)
+    is_refit = rng.random(requests) < 0.7
+    plan, built, next_mesh = [], [], 0
+    for i in range(requests):
+        if is_refit[i] and built:
+            m, n = built[int(rng.integers(max(0, len(built) - 32), len(built)))]
+            plan.append((lb.REQ_REFIT, m, lb.SRC_JITTER, n, 1000 + i, 0.001))
+        else:
+            n = int(sizes[i])
+            plan.append((lb.REQ_BUILD, next_mesh, lb.SRC_UNIFORM, n, 2000 + i, 0.01))
+            built.append((next_mesh, n))
+            next_mesh += 1
+    srv = lb.Server(0, budget, slots)
+    # capacity from a back-to-back burst (its own meshes, then evicted)
+    burst = 400
+    t0 = time.perf_counter()
+    for k in range(burst):
+        kind, m, src, n, seed, p = plan[k]
+        srv.submit(kind, 10 ** 9 + m, src, n, seed=seed, param=p)
+    srv.drain()
+    srv.poll()
+    capacity = burst / (time.perf_counter() - t0)
+    rate = 0.7 * capacity
+    for m, _ in built:
+        try:
+            srv.evict(10 ** 9 + m)
+        except lb.LBVHError:
+            pass
+    log(f"serving: capacity {capacity:.0f} req/s, offered {rate:.0f} req/s")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
+    done = []
+    t0 = time.perf_counter()
+    i = 0
+    while i < requests:
+        now = time.perf_counter() - t0
+        while i < requests and arrivals[i] <= now:
+            kind, m, src, n, seed, p = plan[i]
+            srv.submit(kind, m, src, n, seed=seed, param=p)
+            i += 1
+        done += srv.poll()
+        if i < requests:
+            wait = arrivals[i] - (time.perf_counter() - t0)
+            if wait > 2e-4:
+                time.sleep(min(wait, 2e-3) - 1e-4)
+    srv.drain()
+    wall = time.perf_counter() - t0
+    done += srv.poll()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([c["done_s"] - c["submit_s"] for c in done]) * 1e3
+    ok = np.array([c["status"] == 0 for c in done])
+    tris = sum(c["n"] for c in done if c["status"] == 0)
+    refits = sum(1 for c in done if c["kind"] == lb.REQ_REFIT)
+    return {"requests": len(done), "ok": int(ok.sum()), "refits": refits,
+            "rebuilds": len(done) - refits, "slots": slots,
+            "hbm_budget_GB": round(budget / 2 ** 30, 1),
+            "offered_req_s": round(rate, 1), "capacity_req_s": round(capacity, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "throughput_req_s": round(len(done) / wall, 1),
+            "throughput_Mtri_s": round(tris / wall / 1e6, 1),
+            "evictions": int(st["evictions"]),
+            "peak_GB": round(st["peak_bytes"] / 2 ** 30, 2)}
+
+
 def cpu_baseline(tri, reps=2):
     """The CPU oracle (single-threaded C restatement) on the same mesh."""
     sys.path.insert(0, ROOT)
@@ -119,6 +243,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tree-tile", type=int, default=0,
                     help="leaves per block of the fused up pass (256/512/1024)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB, the "
                          "known-size reference for HBM traffic counters")
@@ -153,6 +280,7 @@ def main():
         ctx.build(d_tri, n, None, bvh, otri)
     ctx.synchronize()
 
+    tile = ctx.get_tuning(lb.TUNE_TREE_TILE)
     log(f"timed x{args.steps}")
     ms, wall = timed_builds(lb, ctx, hip, d_tri, n, None, bvh, otri, args.steps)
     ms_per_step = ms / args.steps
@@ -183,6 +311,17 @@ def main():
     pipe_bytes = build_bytes(n)
     pipe_gbs = pipe_bytes / (ms_per_step * 1e-3) / 1e9
 
+    for b in (d_tri, bvh, otri):
+        b.destroy()
+    ctx.close()
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(lb, hip, log=log)
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench(lb, requests=args.serving_requests, log=log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle)")
@@ -207,7 +346,7 @@ def main():
         "dtype": "fp32+u32",
         "data": "synthetic",
         "config": {"workload": args.workload, "triangles": n,
-                   "tree_tile": ctx.get_tuning(lb.TUNE_TREE_TILE),
+                   "tree_tile": tile,
                    "bounds": "GPU reduction", "div_mode": "mul_rcp",
                    "inputs": "resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
@@ -219,11 +358,10 @@ def main():
         "stages": stages,
         "profiled_ms_per_step": round(pms / args.steps, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
-    for b in (d_tri, bvh, otri):
-        b.destroy()
-    ctx.close()
     print(json.dumps(result), flush=True)
 
 

@@ -195,6 +195,76 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                        const float *bounds_host, void *bvh_out, float *tri_out,
                        uint32_t *sorted_keys_out, int32_t *perm_out);
 
+/* ---- serving ------------------------------------------------------------- */
+
+/* A device-resident build/refit service (no counterpart in the reference,
+ * which builds once per dropped file, main.js:88).  Requests are admitted as
+ * execution slots (each with its own HIP stream and scratch) and HBM budget
+ * free up, so work from different requests overlaps on the GPU; requests on
+ * the same mesh run in submission order.  Built trees stay resident (the
+ * triangles, BVH_BUFFER, O_TRIANGLE_BUFFER, sorted keys and permutation,
+ * 168 bytes per triangle) until evicted: least recently used idle meshes
+ * are evicted when a new build needs room.  A request that can never fit
+ * the budget completes with LBVH_ERR_OUT_OF_BUDGET; a refit of a mesh that
+ * is not resident, or with a different triangle count, completes with
+ * LBVH_ERR_NOT_FOUND / LBVH_ERR_SIZE_MISMATCH.  Resident results are
+ * byte-identical to lbvh_build / lbvh_refit on the same input. */
+typedef struct lbvh_server lbvh_server;
+
+typedef enum {
+  LBVH_REQ_BUILD = 0, /* (re)build mesh_id from new triangles */
+  LBVH_REQ_REFIT = 1  /* refit resident mesh_id to new vertex positions */
+} lbvh_request_kind;
+
+/* Where a request's triangles come from. */
+typedef enum {
+  LBVH_SRC_HOST = 0,    /* `tri` points to 12*n host floats (copied at submit) */
+  LBVH_SRC_DEVICE = 1,  /* `tri` is a device pointer, read when the request runs */
+  LBVH_SRC_UNIFORM = 2, /* synthesised on the device: the uniform mesh of
+                           lbvh_wl_uniform(seed, n, param) (bit-identical) */
+  LBVH_SRC_JITTER = 3   /* the resident mesh's triangles + N(0, param^2) per
+                           coordinate, drawn on the device from seed (refits) */
+} lbvh_source;
+
+typedef struct {
+  uint64_t request_id;
+  uint64_t mesh_id;
+  int kind;
+  int status;        /* lbvh_status of the request */
+  uint32_t n;
+  uint32_t slot;
+  double submit_s;   /* host steady-clock seconds: submitted (arrival) */
+  double start_s;    /* admitted to a slot */
+  double done_s;     /* device work observed complete */
+} lbvh_completion;
+
+typedef struct {
+  uint64_t submitted, completed, failed, evictions;
+  uint64_t resident_meshes, resident_bytes, bytes_in_use, peak_bytes;
+  uint32_t queued, in_flight;
+} lbvh_server_stats;
+
+int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
+                       uint32_t slots, lbvh_server **out);
+/* Waits for in-flight work, then frees every resident mesh and slot. */
+int lbvh_server_destroy(lbvh_server *srv);
+int lbvh_server_set_div_mode(lbvh_server *srv, int div_mode);
+int lbvh_server_submit(lbvh_server *srv, int kind, uint64_t mesh_id,
+                       int source, const float *tri, uint32_t n,
+                       uint64_t seed, double param, uint64_t *out_request_id);
+/* Up to `max` completions not yet returned (oldest first). */
+int lbvh_server_poll(lbvh_server *srv, lbvh_completion *out, uint32_t max,
+                     uint32_t *out_count);
+/* Blocks until nothing is queued or in flight. */
+int lbvh_server_drain(lbvh_server *srv);
+/* Device buffers of a resident mesh (valid until it is rebuilt or evicted;
+ * read them only when no request on the mesh is in flight). */
+int lbvh_server_mesh(lbvh_server *srv, uint64_t mesh_id, void **bvh,
+                     float **tri_out, uint32_t *n);
+int lbvh_server_evict(lbvh_server *srv, uint64_t mesh_id);
+int lbvh_server_stats_get(lbvh_server *srv, lbvh_server_stats *out);
+const char *lbvh_server_last_error(lbvh_server *srv);
+
 /* ---- consumer check ------------------------------------------------------ */
 
 /* Closest-hit ray casting through a built BVH the way the demo's path tracer

@@ -1,0 +1,506 @@
+// server.hip -- device-resident LBVH build/refit service (lbvh_server_*).
+//
+// Continuous batching over execution slots: each slot is an lbvh_ctx with
+// its own HIP stream (default priority) and scratch, so requests admitted to
+// different slots overlap on the GPU.  A dispatcher thread admits queued
+// requests in arrival order as slots and HBM budget free up (requests on a
+// mesh that is busy, or that do not fit yet, wait without blocking refits
+// behind them), watches completion events, and evicts least recently used
+// idle meshes when a new build needs room.  All device work of a request is
+// enqueued on its slot's stream; the dispatcher never waits on the GPU.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/lbvh.h"
+#include "lbvh_internal.h"
+
+namespace lbvh {
+void launch_synth_uniform(uint64_t seed, uint32_t n, double half_extent,
+                          float *tri, hipStream_t s);
+void launch_synth_jitter(uint64_t seed, uint32_t n, double sigma, float *tri,
+                         hipStream_t s);
+}  // namespace lbvh
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Bytes a resident mesh holds: triangles 48, BVH 64, O_TRI 48, sorted keys
+// 4, permutation 4 per triangle.
+constexpr uint64_t kMeshBytesPerTri = 168;
+// Scratch a slot may need to grow to for a build / refit (keys, indices,
+// look-back words, agglomeration items, with the arena's 25% headroom).
+constexpr uint64_t kBuildScratchPerTri = 40;
+constexpr uint64_t kRefitScratchPerTri = 16;
+
+struct Mesh {
+  uint32_t n = 0;
+  void *buf = nullptr;  // one allocation: tri | bvh | otri | keys | perm
+  bool valid = false;   // holds a completed build
+  bool busy = false;    // a request on it is queued in a slot
+  uint64_t last_use = 0;
+  float *tri() const { return static_cast<float *>(buf); }
+  void *bvh() const { return static_cast<char *>(buf) + 48ull * n; }
+  float *otri() const {
+    return reinterpret_cast<float *>(static_cast<char *>(buf) + 112ull * n);
+  }
+  uint32_t *keys() const {
+    return reinterpret_cast<uint32_t *>(static_cast<char *>(buf) + 160ull * n);
+  }
+  int32_t *perm() const {
+    return reinterpret_cast<int32_t *>(static_cast<char *>(buf) + 164ull * n);
+  }
+};
+
+struct Request {
+  uint64_t id = 0, mesh_id = 0, seed = 0;
+  int kind = 0, source = 0;
+  uint32_t n = 0;
+  double param = 0.0, submit_s = 0.0;
+  const float *dev = nullptr;
+  std::vector<float> host;
+};
+
+struct Slot {
+  lbvh_ctx *ctx = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  Request req;
+  double start_s = 0.0;
+  int status = LBVH_OK;
+};
+
+}  // namespace
+
+struct lbvh_server {
+  int device = 0;
+  uint64_t budget = 0;
+  int div_mode = LBVH_DIV_MUL_RCP;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv;       // wakes the dispatcher
+  std::condition_variable idle_cv;  // wakes lbvh_server_drain
+  std::deque<Request> queue;
+  std::deque<lbvh_completion> done;
+  std::unordered_map<uint64_t, Mesh> meshes;
+  uint64_t next_id = 1, tick = 0, resident_bytes = 0, peak = 0;
+  lbvh_server_stats st{};
+  std::string last_error;
+  bool stop = false;
+  std::thread thr;
+};
+
+namespace {
+
+uint64_t slots_bytes(lbvh_server *s) {
+  uint64_t b = 0;
+  for (auto &sl : s->slots) b += lbvh_bytes_in_use(sl.ctx);
+  return b;
+}
+
+uint64_t in_use(lbvh_server *s) { return s->resident_bytes + slots_bytes(s); }
+
+// Mesh memory is stream-ordered (hipMallocAsync / hipFreeAsync): a plain
+// hipFree would synchronise the whole device and stall every other slot.
+// A mesh is only freed while idle (its last request completed).
+void free_mesh(lbvh_server *s, Mesh &m) {
+  if (m.buf) {
+    (void)hipFreeAsync(m.buf, static_cast<hipStream_t>(lbvh_stream(s->slots[0].ctx)));
+    s->resident_bytes -= kMeshBytesPerTri * m.n;
+  }
+  m.buf = nullptr;
+  m.n = 0;
+  m.valid = false;
+}
+
+// Device memory a request still needs beyond what is allocated.
+uint64_t request_need(lbvh_server *s, const Request &r, const Slot &sl) {
+  uint64_t need = 0;
+  if (r.kind == LBVH_REQ_BUILD) {
+    auto it = s->meshes.find(r.mesh_id);
+    if (it == s->meshes.end() || it->second.n != r.n)
+      need += kMeshBytesPerTri * r.n;
+  }
+  const uint64_t scratch =
+      (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
+      r.n + sizeof(lbvh::Ctl);
+  const uint64_t have = lbvh_bytes_in_use(sl.ctx);
+  if (scratch > have) need += scratch - have;
+  return need;
+}
+
+// Frees idle memory (LRU idle meshes other than `keep`, then idle slots'
+// scratch) until `need` more bytes fit the budget.  True if they fit.
+bool make_room(lbvh_server *s, uint64_t need, uint64_t keep) {
+  if (s->budget == 0) return true;
+  while (in_use(s) + need > s->budget) {
+    uint64_t victim = 0, oldest = UINT64_MAX;
+    for (auto &kv : s->meshes)
+      if (kv.first != keep && !kv.second.busy && kv.second.buf &&
+          kv.second.last_use < oldest) {
+        oldest = kv.second.last_use;
+        victim = kv.first;
+      }
+    if (oldest != UINT64_MAX) {
+      free_mesh(s, s->meshes[victim]);
+      s->meshes.erase(victim);
+      s->st.evictions += 1;
+      continue;
+    }
+    bool trimmed = false;
+    for (auto &sl : s->slots)
+      if (!sl.busy && lbvh_bytes_in_use(sl.ctx) > sizeof(lbvh::Ctl)) {
+        lbvh_trim(sl.ctx);
+        trimmed = true;
+      }
+    if (!trimmed) return false;
+  }
+  return true;
+}
+
+void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
+              double start) {
+  lbvh_completion c{};
+  c.request_id = r.id;
+  c.mesh_id = r.mesh_id;
+  c.kind = r.kind;
+  c.status = status;
+  c.n = r.n;
+  c.slot = slot;
+  c.submit_s = r.submit_s;
+  c.start_s = start;
+  c.done_s = now_s();
+  s->done.push_back(c);
+  s->st.completed += 1;
+  if (status != LBVH_OK) s->st.failed += 1;
+}
+
+// Enqueues request r on slot k.  Returns the synchronous status.
+int launch(lbvh_server *s, uint32_t k, Request &r) {
+  Slot &sl = s->slots[k];
+  hipStream_t stream = static_cast<hipStream_t>(lbvh_stream(sl.ctx));
+  Mesh &m = s->meshes[r.mesh_id];
+  if (r.kind == LBVH_REQ_BUILD && (m.buf == nullptr || m.n != r.n)) {
+    free_mesh(s, m);
+    void *p = nullptr;
+    if (hipMallocAsync(&p, kMeshBytesPerTri * r.n, stream) != hipSuccess) {
+      (void)hipGetLastError();
+      return LBVH_ERR_OUT_OF_MEMORY;
+    }
+    m.buf = p;
+    m.n = r.n;
+    s->resident_bytes += kMeshBytesPerTri * r.n;
+  }
+  m.busy = true;
+  m.last_use = ++s->tick;
+  switch (r.source) {
+    case LBVH_SRC_HOST:
+      if (hipMemcpyAsync(m.tri(), r.host.data(), 48ull * r.n,
+                         hipMemcpyHostToDevice, stream) != hipSuccess)
+        return LBVH_ERR_DEVICE;
+      break;
+    case LBVH_SRC_DEVICE:
+      if (hipMemcpyAsync(m.tri(), r.dev, 48ull * r.n, hipMemcpyDeviceToDevice,
+                         stream) != hipSuccess)
+        return LBVH_ERR_DEVICE;
+      break;
+    case LBVH_SRC_UNIFORM:
+      lbvh::launch_synth_uniform(r.seed, r.n, r.param, m.tri(), stream);
+      break;
+    case LBVH_SRC_JITTER:
+      lbvh::launch_synth_jitter(r.seed, r.n, r.param, m.tri(), stream);
+      break;
+  }
+  int st;
+  if (r.kind == LBVH_REQ_BUILD) {
+    m.valid = false;
+    st = lbvh_build(sl.ctx, m.tri(), r.n, nullptr, m.bvh(), m.otri(),
+                    m.keys(), m.perm());
+  } else {
+    st = lbvh_refit(sl.ctx, m.tri(), r.n, m.keys(), m.perm(), m.bvh(),
+                    m.otri());
+  }
+  if (st != LBVH_OK) return st;
+  if (hipEventRecord(sl.done, stream) != hipSuccess) return LBVH_ERR_DEVICE;
+  return LBVH_OK;
+}
+
+// Validates a queued request against the resident state.  LBVH_OK if it can
+// run, another status if it must fail.
+int check_request(lbvh_server *s, const Request &r) {
+  const uint64_t fixed = sizeof(lbvh::Ctl) * s->slots.size();
+  if (s->budget &&
+      kMeshBytesPerTri * r.n + kBuildScratchPerTri * r.n + fixed > s->budget)
+    return LBVH_ERR_OUT_OF_BUDGET;
+  if (r.kind == LBVH_REQ_REFIT || r.source == LBVH_SRC_JITTER) {
+    auto it = s->meshes.find(r.mesh_id);
+    if (it == s->meshes.end() || !it->second.valid) return LBVH_ERR_NOT_FOUND;
+    if (it->second.n != r.n) return LBVH_ERR_SIZE_MISMATCH;
+  }
+  return LBVH_OK;
+}
+
+void dispatcher(lbvh_server *s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lock(s->mu);
+  while (!s->stop) {
+    bool progress = false;
+    // Completions.
+    for (uint32_t k = 0; k < s->slots.size(); ++k) {
+      Slot &sl = s->slots[k];
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      // (Kernel-side bounded-wait faults are sticky per slot and reported
+      // by lbvh_server_drain, which keeps this path free of device reads.)
+      const int status = (q == hipSuccess) ? LBVH_OK : LBVH_ERR_DEVICE;
+      auto it = s->meshes.find(sl.req.mesh_id);
+      if (it != s->meshes.end()) {
+        it->second.busy = false;
+        if (sl.req.kind == LBVH_REQ_BUILD) it->second.valid = (status == LBVH_OK);
+      }
+      complete(s, sl.req, status, k, sl.start_s);
+      sl.busy = false;
+      progress = true;
+    }
+    // Admissions, in arrival order.  A mesh already in a slot holds back
+    // later requests on it; a request that does not fit yet holds back
+    // later requests that need memory, so large builds are not starved.
+    bool memory_blocked = false;
+    std::vector<uint64_t> held;  // meshes with an earlier request still queued
+    for (auto qi = s->queue.begin(); qi != s->queue.end();) {
+      uint32_t free_slot = UINT32_MAX;
+      for (uint32_t k = 0; k < s->slots.size(); ++k)
+        if (!s->slots[k].busy) { free_slot = k; break; }
+      if (free_slot == UINT32_MAX) break;
+      auto mi = s->meshes.find(qi->mesh_id);
+      if ((mi != s->meshes.end() && mi->second.busy) ||
+          std::find(held.begin(), held.end(), qi->mesh_id) != held.end()) {
+        held.push_back(qi->mesh_id);
+        ++qi;
+        continue;
+      }
+      int st = check_request(s, *qi);
+      if (st == LBVH_OK) {
+        const uint64_t need = request_need(s, *qi, s->slots[free_slot]);
+        if (need > 0 && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
+          memory_blocked = true;
+          held.push_back(qi->mesh_id);
+          ++qi;
+          continue;
+        }
+        Slot &sl = s->slots[free_slot];
+        sl.req = std::move(*qi);
+        qi = s->queue.erase(qi);
+        sl.start_s = now_s();
+        st = launch(s, free_slot, sl.req);
+        if (st == LBVH_OK) {
+          sl.busy = true;
+          s->peak = std::max(s->peak, in_use(s));
+        } else {
+          auto m = s->meshes.find(sl.req.mesh_id);
+          if (m != s->meshes.end()) m->second.busy = false;
+          complete(s, sl.req, st, free_slot, sl.start_s);
+        }
+      } else {
+        complete(s, *qi, st, UINT32_MAX, now_s());
+        qi = s->queue.erase(qi);
+      }
+      progress = true;
+    }
+    bool any_busy = false;
+    for (auto &sl : s->slots) any_busy |= sl.busy;
+    if (!any_busy && s->queue.empty()) s->idle_cv.notify_all();
+    if (progress) continue;
+    if (any_busy)
+      s->cv.wait_for(lock, std::chrono::microseconds(20));
+    else
+      s->cv.wait(lock);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
+                       uint32_t slots, lbvh_server **out) {
+  if (!out || slots == 0 || slots > 64) return LBVH_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  lbvh_server *s = new lbvh_server();
+  s->device = device_ordinal;
+  s->budget = hbm_budget_bytes;
+  s->slots.resize(slots);
+  for (auto &sl : s->slots) {
+    int st = lbvh_create(device_ordinal, 0, &sl.ctx);
+    if (st == LBVH_OK &&
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+      st = LBVH_ERR_DEVICE;
+    if (st != LBVH_OK) {
+      for (auto &x : s->slots) {
+        if (x.done) (void)hipEventDestroy(x.done);
+        if (x.ctx) lbvh_destroy(x.ctx);
+      }
+      delete s;
+      return st;
+    }
+  }
+  s->thr = std::thread(dispatcher, s);
+  *out = s;
+  return LBVH_OK;
+}
+
+int lbvh_server_destroy(lbvh_server *s) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  {
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  s->thr.join();
+  for (auto &sl : s->slots) {
+    lbvh_synchronize(sl.ctx);
+    (void)hipEventDestroy(sl.done);
+  }
+  for (auto &kv : s->meshes) free_mesh(s, kv.second);
+  for (auto &sl : s->slots) lbvh_destroy(sl.ctx);
+  delete s;
+  return LBVH_OK;
+}
+
+int lbvh_server_set_div_mode(lbvh_server *s, int div_mode) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  for (auto &sl : s->slots) {
+    const int st = lbvh_set_div_mode(sl.ctx, div_mode);
+    if (st != LBVH_OK) return st;
+  }
+  s->div_mode = div_mode;
+  return LBVH_OK;
+}
+
+int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
+                       const float *tri, uint32_t n, uint64_t seed,
+                       double param, uint64_t *out_request_id) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (kind != LBVH_REQ_BUILD && kind != LBVH_REQ_REFIT) {
+    s->last_error = "unknown request kind";
+    return LBVH_ERR_INVALID_ARGUMENT;
+  }
+  if (source < LBVH_SRC_HOST || source > LBVH_SRC_JITTER ||
+      (source == LBVH_SRC_JITTER && kind != LBVH_REQ_REFIT) ||
+      ((source == LBVH_SRC_HOST || source == LBVH_SRC_DEVICE) && !tri) ||
+      n == 0 || n > LBVH_MAX_TRIS) {
+    s->last_error = "invalid source, pointer or triangle count";
+    return LBVH_ERR_INVALID_ARGUMENT;
+  }
+  Request r;
+  r.id = s->next_id++;
+  r.mesh_id = mesh_id;
+  r.kind = kind;
+  r.source = source;
+  r.n = n;
+  r.seed = seed;
+  r.param = param;
+  r.submit_s = now_s();
+  if (source == LBVH_SRC_HOST) r.host.assign(tri, tri + 12ull * n);
+  if (source == LBVH_SRC_DEVICE) r.dev = tri;
+  if (out_request_id) *out_request_id = r.id;
+  s->queue.push_back(std::move(r));
+  s->st.submitted += 1;
+  s->cv.notify_all();
+  return LBVH_OK;
+}
+
+int lbvh_server_poll(lbvh_server *s, lbvh_completion *out, uint32_t max,
+                     uint32_t *out_count) {
+  if (!s || !out_count || (max && !out)) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  uint32_t k = 0;
+  while (k < max && !s->done.empty()) {
+    out[k++] = s->done.front();
+    s->done.pop_front();
+  }
+  *out_count = k;
+  return LBVH_OK;
+}
+
+int lbvh_server_drain(lbvh_server *s) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lock(s->mu);
+  s->cv.notify_all();
+  s->idle_cv.wait(lock, [s] {
+    if (!s->queue.empty()) return false;
+    for (auto &sl : s->slots)
+      if (sl.busy) return false;
+    return true;
+  });
+  for (auto &sl : s->slots) {
+    const int st = lbvh_synchronize(sl.ctx);  // also reports kernel faults
+    if (st != LBVH_OK) {
+      s->last_error = lbvh_last_error(sl.ctx);
+      return st;
+    }
+  }
+  return LBVH_OK;
+}
+
+int lbvh_server_mesh(lbvh_server *s, uint64_t mesh_id, void **bvh,
+                     float **tri_out, uint32_t *n) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  auto it = s->meshes.find(mesh_id);
+  if (it == s->meshes.end() || !it->second.valid) {
+    s->last_error = "mesh is not resident";
+    return LBVH_ERR_NOT_FOUND;
+  }
+  if (bvh) *bvh = it->second.bvh();
+  if (tri_out) *tri_out = it->second.otri();
+  if (n) *n = it->second.n;
+  return LBVH_OK;
+}
+
+int lbvh_server_evict(lbvh_server *s, uint64_t mesh_id) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  auto it = s->meshes.find(mesh_id);
+  if (it == s->meshes.end()) return LBVH_ERR_NOT_FOUND;
+  if (it->second.busy) return LBVH_ERR_BUSY;
+  free_mesh(s, it->second);
+  s->meshes.erase(it);
+  s->st.evictions += 1;
+  return LBVH_OK;
+}
+
+int lbvh_server_stats_get(lbvh_server *s, lbvh_server_stats *out) {
+  if (!s || !out) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  *out = s->st;
+  out->resident_meshes = 0;
+  for (auto &kv : s->meshes) out->resident_meshes += kv.second.buf ? 1 : 0;
+  out->resident_bytes = s->resident_bytes;
+  out->bytes_in_use = in_use(s);
+  out->peak_bytes = s->peak;
+  out->queued = (uint32_t)s->queue.size();
+  uint32_t busy = 0;
+  for (auto &sl : s->slots) busy += sl.busy ? 1 : 0;
+  out->in_flight = busy;
+  return LBVH_OK;
+}
+
+const char *lbvh_server_last_error(lbvh_server *s) {
+  return s ? s->last_error.c_str() : "null server";
+}
+
+}  // extern "C"

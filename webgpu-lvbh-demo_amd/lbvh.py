@@ -70,6 +70,17 @@ SIGNATURES = {
     "lbvh_refit": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "lbvh_raycast": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     "lbvh_build_batched": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "lbvh_server_create": (_i32, [_i32, _u64, _u32, C.POINTER(_vp)]),
+    "lbvh_server_destroy": (_i32, [_vp]),
+    "lbvh_server_set_div_mode": (_i32, [_vp, _i32]),
+    "lbvh_server_submit": (_i32, [_vp, _i32, _u64, _i32, _vp, _u32, _u64, C.c_double,
+                                  C.POINTER(_u64)]),
+    "lbvh_server_poll": (_i32, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    "lbvh_server_drain": (_i32, [_vp]),
+    "lbvh_server_mesh": (_i32, [_vp, _u64, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_u32)]),
+    "lbvh_server_evict": (_i32, [_vp, _u64]),
+    "lbvh_server_stats_get": (_i32, [_vp, _vp]),
+    "lbvh_server_last_error": (C.c_char_p, [_vp]),
     "lbvh_profile_enable": (_i32, [_vp, _i32]),
     "lbvh_profile_read": (_i32, [_vp, _vp, _i32]),
 }
@@ -333,6 +344,108 @@ class Context:
             for b in (dtri, bvh, otri, sk, pm):
                 if b is not None:
                     b.destroy()
+
+
+# ---- serving -----------------------------------------------------------------
+
+REQ_BUILD, REQ_REFIT = 0, 1
+SRC_HOST, SRC_DEVICE, SRC_UNIFORM, SRC_JITTER = 0, 1, 2, 3
+
+
+class Completion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("mesh_id", C.c_uint64),
+                ("kind", C.c_int), ("status", C.c_int), ("n", C.c_uint32),
+                ("slot", C.c_uint32), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("evictions", C.c_uint64),
+                ("resident_meshes", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("queued", C.c_uint32), ("in_flight", C.c_uint32)]
+
+
+class Server:
+    """lbvh_server: device-resident build/refit service."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, slots: int = 4):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.lbvh_server_create(device, hbm_budget, slots, C.byref(h))
+        if st != 0:
+            raise LBVHError(st, "lbvh_server_create")
+        self.handle = h.value
+        self._buf = (Completion * 1024)()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.lbvh_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            raise LBVHError(st, self.lib.lbvh_server_last_error(self.handle).decode())
+
+    def set_div_mode(self, mode: int):
+        self._check(self.lib.lbvh_server_set_div_mode(self.handle, mode))
+
+    def submit(self, kind: int, mesh_id: int, source: int, n: int, tri=None,
+               seed: int = 0, param: float = 0.0) -> int:
+        rid = C.c_uint64()
+        ptr = None
+        if source == SRC_HOST:
+            tri = np.ascontiguousarray(tri, dtype=np.float32)
+            ptr = _ptr(tri)
+        elif source == SRC_DEVICE:
+            ptr = _dp(tri)
+        self._check(self.lib.lbvh_server_submit(self.handle, kind, mesh_id, source, ptr,
+                                                n, seed, param, C.byref(rid)))
+        return rid.value
+
+    def poll(self):
+        got = C.c_uint32()
+        out = []
+        while True:
+            self._check(self.lib.lbvh_server_poll(self.handle, self._buf, 1024, C.byref(got)))
+            out += [dict((f, getattr(self._buf[i], f)) for f, _ in Completion._fields_)
+                    for i in range(got.value)]
+            if got.value < 1024:
+                return out
+
+    def drain(self):
+        self._check(self.lib.lbvh_server_drain(self.handle))
+
+    def mesh(self, mesh_id: int):
+        """(bvh device ptr, O_TRI device ptr, n) of a resident mesh."""
+        b, t, n = C.c_void_p(), C.c_void_p(), C.c_uint32()
+        self._check(self.lib.lbvh_server_mesh(self.handle, mesh_id, C.byref(b),
+                                              C.byref(t), C.byref(n)))
+        return b.value, t.value, n.value
+
+    def evict(self, mesh_id: int):
+        self._check(self.lib.lbvh_server_evict(self.handle, mesh_id))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._check(self.lib.lbvh_server_stats_get(self.handle, C.byref(st)))
+        return dict((f, getattr(st, f)) for f, _ in ServerStats._fields_)
+
+    def read_mesh(self, ctx: "Context", mesh_id: int):
+        """Host copies of a resident mesh's BVH bytes and O_TRI."""
+        b, t, n = self.mesh(mesh_id)
+        bvh = np.empty(64 * n, np.uint8)
+        otri = np.empty((n, 12), np.float32)
+        ctx.download(bvh, b)
+        ctx.download(otri, t)
+        return bvh, otri
 
 
 # ---- workloads -------------------------------------------------------------
