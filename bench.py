@@ -261,8 +261,12 @@ def main():
     tri = lb.workload(args.workload)
     n = tri.shape[0]
     ctx = lb.Context(0)
+    tuning = None
     if args.tree_tile:
         ctx.set_tuning(lb.TUNE_TREE_TILE, args.tree_tile)
+    else:
+        log("autotune")
+        tuning = ctx.autotune(1 << 16, n, reps=3)
     d_tri = ctx.to_device(tri)
     bvh = ctx.alloc(64 * n)
     otri = ctx.alloc(48 * n)
@@ -280,7 +284,7 @@ def main():
         ctx.build(d_tri, n, None, bvh, otri)
     ctx.synchronize()
 
-    tile = ctx.get_tuning(lb.TUNE_TREE_TILE)
+    tile = ctx.get_tuning_for(lb.TUNE_TREE_TILE, n)
     log(f"timed x{args.steps}")
     ms, wall = timed_builds(lb, ctx, hip, d_tri, n, None, bvh, otri, args.steps)
     ms_per_step = ms / args.steps
@@ -356,6 +360,8 @@ def main():
                               "achieved": round(pipe_gbs, 1), "unit": "GB/s",
                               "frac_of_hbm_peak": round(pipe_gbs / HBM_PEAK_GBS, 4)},
         "stages": stages,
+        "autotune_ms": None if tuning is None else
+        {str(k): {str(t): round(v, 4) for t, v in d.items()} for k, d in tuning.items()},
         "profiled_ms_per_step": round(pms / args.steps, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "batched": batched,

@@ -85,8 +85,21 @@ int lbvh_get_div_mode(const lbvh_ctx *ctx);
 typedef enum {
   LBVH_TUNE_TREE_TILE = 0 /* leaves per block of the fused up pass: 256|512|1024 */
 } lbvh_tunable;
+/* set_tuning fixes a value for every size; get_tuning returns the value a
+ * build of n triangles would use (n = 0: the fixed/default value). */
 int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value);
 int lbvh_get_tuning(const lbvh_ctx *ctx, int param);
+int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n);
+/* Autotuner: for every power-of-two size bucket 2^k in [n_min, n_max] it
+ * builds a device-synthesised uniform mesh of 2^k triangles with each
+ * variant of every tunable, times them with HIP events (median of `reps`
+ * after a warm-up) and keeps the fastest per bucket; later builds pick the
+ * entry of their size's bucket (or the nearest tuned one).  A fixed value
+ * from lbvh_set_tuning is cleared.  ms_out (optional) receives, per bucket
+ * and variant in order, the measured milliseconds: buckets x 3 tree tiles
+ * (256, 512, 1024).  Synchronous. */
+int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
+                  float *ms_out);
 /* The context's hipStream_t, for callers that record events or chain work. */
 void *lbvh_stream(lbvh_ctx *ctx);
 int lbvh_synchronize(lbvh_ctx *ctx);

@@ -221,6 +221,19 @@ def test_build_independent_of_tree_tile(ctx, lb, ora, tile):
         ctx.set_tuning(lb.TUNE_TREE_TILE, 512)
 
 
+def test_autotune_table_never_changes_results(lb, ora):
+    with lb.Context(0) as c:
+        times = c.autotune(1 << 10, 1 << 16, reps=2)
+        assert sorted(times) == [1 << k for k in range(10, 17)]
+        for bucket, per_tile in times.items():
+            assert set(per_tile) == {256, 512, 1024}
+            assert all(t > 0 for t in per_tile.values())
+            assert c.get_tuning_for(lb.TUNE_TREE_TILE, bucket) in (256, 512, 1024)
+        for n in (1, 700, 5000, 70000):
+            tri = lb.wl_clustered(90 + n, n, 0.05)
+            assert np.array_equal(c.build_arrays(tri, None)[0], ora.build(tri, None, 0)[0])
+
+
 def test_build_is_deterministic(ctx, lb):
     tri = lb.workload("uniform-64k")
     first = ctx.build_arrays(tri, None)[0]

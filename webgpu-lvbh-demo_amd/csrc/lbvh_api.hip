@@ -15,6 +15,11 @@
 #include "../../include/lbvh.h"
 #include "lbvh_internal.h"
 
+namespace lbvh {
+void launch_synth_uniform(uint64_t seed, uint32_t n, double half_extent,
+                          float *tri, hipStream_t s);
+}
+
 using namespace lbvh;
 
 namespace {
@@ -33,6 +38,8 @@ struct lbvh_ctx {
   uint64_t in_use = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
   int tree_tile = kTreeTileDefault;
+  bool tree_tile_fixed = false;  // set_tuning overrides the tuned table
+  int tune_tile[33] = {0};       // autotuned tree tile per log2 size bucket
   std::string last_error;
   std::unordered_map<void *, uint64_t> user_allocs;
   std::recursive_mutex mu;
@@ -116,6 +123,12 @@ int dev_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out) {
   ctx->in_use += bytes;
   *out = p;
   return LBVH_OK;
+}
+
+void dev_free_raw(lbvh_ctx *ctx, void *p, uint64_t bytes) {
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(p);
+  ctx->in_use -= bytes;
 }
 
 void dev_release(lbvh_ctx *ctx, DevBuf &b) {
@@ -238,13 +251,30 @@ int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
   return ensure(ctx, ctx->status, 8ull * words, true);
 }
 
-int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n) {
+int bucket_of(uint32_t n) {
+  int b = 0;
+  while ((2ull << b) <= n && b < 32) ++b;
+  return b;  // floor(log2 n)
+}
+
+// Tree tile for an n-triangle build: a fixed value, else the autotuned entry
+// of n's size bucket (or the nearest tuned bucket), else the default.
+int tile_for(const lbvh_ctx *ctx, uint32_t n) {
+  if (ctx->tree_tile_fixed || n == 0) return ctx->tree_tile;
+  const int b = bucket_of(n);
+  for (int d = 0; d <= 32; ++d) {
+    if (b - d >= 0 && ctx->tune_tile[b - d]) return ctx->tune_tile[b - d];
+    if (b + d <= 32 && ctx->tune_tile[b + d]) return ctx->tune_tile[b + d];
+  }
+  return ctx->tree_tile;
+}
+
+int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n, int tile) {
   int st;
-  const uint64_t items = 32ull * tree_item_capacity(n, ctx->tree_tile);
+  const uint64_t items = 32ull * tree_item_capacity(n, tile);
   if ((st = ensure(ctx, ctx->items_a, items, false))) return st;
   if ((st = ensure(ctx, ctx->items_b, items, false))) return st;
-  return ensure(ctx, ctx->lstatus, 8ull * tree_status_words(n, ctx->tree_tile),
-                true);
+  return ensure(ctx, ctx->lstatus, 8ull * tree_status_words(n, tile), true);
 }
 
 int ensure_partials(lbvh_ctx *ctx) {
@@ -293,14 +323,19 @@ MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
 // each mesh (the single-mesh level plan, per mesh).  Uploaded once.
 int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
   auto &P = ctx->plan;
-  if (P.S == S && P.tile == ctx->tree_tile &&
+  uint32_t big = 0;
+  for (uint32_t k = 0; k < S; ++k) big = std::max(big, offsets[k + 1] - offsets[k]);
+  if (P.S == S && P.tile == tile_for(ctx, big) &&
       std::equal(P.offsets.begin(), P.offsets.end(), offsets) &&
       P.offsets.size() == (size_t)S + 1)
     return LBVH_OK;
   drop_plan(ctx);
   P.offsets.assign(offsets, offsets + S + 1);
   P.S = S;
-  P.tile = ctx->tree_tile;
+  uint32_t largest = 0;
+  for (uint32_t k = 0; k < S; ++k)
+    largest = std::max(largest, offsets[k + 1] - offsets[k]);
+  P.tile = tile_for(ctx, largest);
   P.total = offsets[S];
   std::vector<uint32_t> tab;
   auto put = [&](const std::vector<uint32_t> &v) {
@@ -513,6 +548,7 @@ int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value) {
         return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
                     "tree tile must be 256, 512 or 1024");
       ctx->tree_tile = value;
+      ctx->tree_tile_fixed = true;
       return LBVH_OK;
     default:
       return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "unknown tuning parameter");
@@ -520,11 +556,82 @@ int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value) {
 }
 
 int lbvh_get_tuning(const lbvh_ctx *ctx, int param) {
+  return lbvh_get_tuning_for(ctx, param, 0);
+}
+
+int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n) {
   if (!ctx) return -1;
   switch (param) {
-    case LBVH_TUNE_TREE_TILE: return ctx->tree_tile;
+    case LBVH_TUNE_TREE_TILE: return tile_for(ctx, n);
     default: return -1;
   }
+}
+
+int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
+                  float *ms_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if (n_min == 0 || n_max < n_min || n_max > LBVH_MAX_TRIS || reps < 1)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bad autotune range");
+  static const int tiles[3] = {256, 512, 1024};
+  const int b0 = bucket_of(n_min), b1 = bucket_of(n_max);
+  const uint32_t nbig = 1u << b1;
+  void *tri = nullptr, *bvh = nullptr, *otri = nullptr;
+  if ((st = dev_alloc(ctx, 48ull * nbig, &tri))) return st;
+  if ((st = dev_alloc(ctx, 64ull * nbig, &bvh))) {
+    dev_free_raw(ctx, tri, 48ull * nbig);
+    return st;
+  }
+  if ((st = dev_alloc(ctx, 48ull * nbig, &otri))) {
+    dev_free_raw(ctx, tri, 48ull * nbig);
+    dev_free_raw(ctx, bvh, 64ull * nbig);
+    return st;
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const bool was_fixed = ctx->tree_tile_fixed;
+  ctx->tree_tile_fixed = true;  // force each variant while timing
+  const int saved = ctx->tree_tile;
+  int out_i = 0;
+  st = LBVH_OK;
+  for (int k = b0; k <= b1 && st == LBVH_OK; ++k) {
+    const uint32_t n = 1u << k;
+    launch_synth_uniform(0x5EED0000ull + k, n, 0.01, static_cast<float *>(tri),
+                         ctx->stream);
+    float best = 1e30f;
+    int best_tile = kTreeTileDefault;
+    for (int v = 0; v < 3 && st == LBVH_OK; ++v) {
+      ctx->tree_tile = tiles[v];
+      std::vector<float> ms;
+      for (int r = -1; r < reps && st == LBVH_OK; ++r) {
+        (void)hipEventRecord(a, ctx->stream);
+        st = lbvh_build(ctx, static_cast<float *>(tri), n, nullptr, bvh,
+                        static_cast<float *>(otri), nullptr, nullptr);
+        (void)hipEventRecord(b, ctx->stream);
+        if (st == LBVH_OK && hipEventSynchronize(b) == hipSuccess && r >= 0) {
+          float t = 0.0f;
+          (void)hipEventElapsedTime(&t, a, b);
+          ms.push_back(t);
+        }
+      }
+      if (ms.empty()) continue;
+      std::sort(ms.begin(), ms.end());
+      const float med = ms[ms.size() / 2];
+      if (ms_out) ms_out[out_i] = med;
+      ++out_i;
+      if (med < best) { best = med; best_tile = tiles[v]; }
+    }
+    if (st == LBVH_OK) ctx->tune_tile[k] = best_tile;
+  }
+  ctx->tree_tile = saved;
+  ctx->tree_tile_fixed = (st == LBVH_OK) ? false : was_fixed;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  dev_free_raw(ctx, tri, 48ull * nbig);
+  dev_free_raw(ctx, bvh, 64ull * nbig);
+  dev_free_raw(ctx, otri, 48ull * nbig);
+  return st;
 }
 
 void *lbvh_stream(lbvh_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
@@ -727,7 +834,8 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   if ((st = ensure_sort_scratch(ctx, n))) return st;
-  if ((st = ensure_tree_scratch(ctx, n))) return st;
+  const int tile = tile_for(ctx, n);
+  if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
   if ((st = begin_op(ctx))) return st;
 
@@ -766,7 +874,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
                       static_cast<int4 *>(ctx->items_a.ptr),
                       static_cast<int4 *>(ctx->items_b.ptr),
                       static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
-                      ctx->tree_tile, ctx->stream);
+                      tile, ctx->stream);
   }
   return check_launch(ctx, "lbvh_build");
 }
@@ -780,7 +888,8 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !sorted_keys || !perm || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = ensure_tree_scratch(ctx, n))) return st;
+  const int tile = tile_for(ctx, n);
+  if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
   if ((st = begin_op(ctx))) return st;
   launch_bump_epoch(ctx->ctl, ctx->stream);
   StageScope sc(ctx, LBVH_STAGE_TREE);
@@ -790,7 +899,7 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
                     static_cast<int4 *>(ctx->items_a.ptr),
                     static_cast<int4 *>(ctx->items_b.ptr),
                     static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
-                    ctx->tree_tile, ctx->stream);
+                    tile, ctx->stream);
   return check_launch(ctx, "lbvh_refit");
 }
 

@@ -52,6 +52,8 @@ SIGNATURES = {
     "lbvh_get_div_mode": (_i32, [_vp]),
     "lbvh_set_tuning": (_i32, [_vp, _i32, _i32]),
     "lbvh_get_tuning": (_i32, [_vp, _i32]),
+    "lbvh_get_tuning_for": (_i32, [_vp, _i32, _u32]),
+    "lbvh_autotune": (_i32, [_vp, _u32, _u32, _i32, _vp]),
     "lbvh_stream": (_vp, [_vp]),
     "lbvh_synchronize": (_i32, [_vp]),
     "lbvh_bytes_in_use": (_u64, [_vp]),
@@ -234,6 +236,20 @@ class Context:
 
     def get_tuning(self, param: int) -> int:
         return self.lib.lbvh_get_tuning(self.handle, param)
+
+    def get_tuning_for(self, param: int, n: int) -> int:
+        return self.lib.lbvh_get_tuning_for(self.handle, param, n)
+
+    TUNE_TILES = (256, 512, 1024)
+
+    def autotune(self, n_min: int, n_max: int, reps: int = 3) -> dict:
+        """Runs lbvh_autotune; returns {bucket size: {tile: ms}}."""
+        b0, b1 = n_min.bit_length() - 1, n_max.bit_length() - 1
+        ms = np.zeros((b1 - b0 + 1) * len(self.TUNE_TILES), np.float32)
+        self._check(self.lib.lbvh_autotune(self.handle, n_min, n_max, reps, _ptr(ms)))
+        ms = ms.reshape(-1, len(self.TUNE_TILES))
+        return {1 << (b0 + i): dict(zip(self.TUNE_TILES, map(float, ms[i])))
+                for i in range(ms.shape[0])}
 
     def profile_enable(self, on: bool = True):
         self._check(self.lib.lbvh_profile_enable(self.handle, int(on)))
