@@ -340,7 +340,11 @@ def test_raycast_over_gpu_bvh(ctx, lb, ora, n):
     bvh, otri = ctx.build_arrays(tri, None)
     rays = _rays(n, 4096)
     got = ctx.raycast_arrays(bvh, otri, rays)
-    assert np.array_equal(got, ora.raycast(bvh, otri, rays))
+    want = ora.raycast(bvh, otri, rays)
+    bad = np.where((got != want).any(axis=1))[0]
+    for q in bad[:8]:
+        say(f"ray {q}: {rays[q].tolist()} gpu {got[q].tolist()} cpu {want[q].tolist()}")
+    assert len(bad) == 0, f"{len(bad)} of {len(rays)} rays differ"
     if n <= 5000:
         assert np.array_equal(got, ora.raycast_bruteforce(otri, rays))
     else:
@@ -354,5 +358,11 @@ def test_raycast_heightfield_16m(ctx, lb, ora):
     bvh, otri = ctx.build_arrays(tri, None)
     rays = _rays(5, 8192, top=0.5)
     got = ctx.raycast_arrays(bvh, otri, rays)
-    assert (got[:, 3] > 0).mean() > 0.95   # rays from above hit the surface
-    assert np.array_equal(got, ora.raycast(bvh, otri, rays))
+    # every vertical ray from above the domain hits the surface (oblique
+    # ones may leave the 1 x 0.5 domain)
+    assert np.all(got[4096:, 3] > 0)
+    want = ora.raycast(bvh, otri, rays)
+    bad = np.where((got != want).any(axis=1))[0]
+    for q in bad[:8]:
+        say(f"ray {q}: {rays[q].tolist()} gpu {got[q].tolist()} cpu {want[q].tolist()}")
+    assert len(bad) == 0, f"{len(bad)} of {len(rays)} rays differ"

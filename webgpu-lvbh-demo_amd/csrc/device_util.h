@@ -26,6 +26,33 @@ __device__ __forceinline__ void hist_add(uint32_t (*s_hist)[kRadix],
   }
 }
 
+// XCD-aware tile id.  MI355X hands workgroups to its 8 XCDs round-robin by
+// id (block b runs on XCD b % 8), and each XCD has its own L2.  This
+// bijection on [0, tiles) gives each XCD one contiguous run of tiles, so
+// data that consecutive tiles share stays in one L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
+  const uint32_t q = tiles / kXcds, r = tiles % kXcds, x = b % kXcds;
+  return x * q + (x < r ? x : r) + b / kXcds;
+}
+
+// Keys [begin, end) of a sort tile inside its segment [seg_begin, seg_end)
+// (one segment [0, n) when segs.tile_seg is null).
+struct TileRange {
+  uint32_t begin, end, seg_begin, seg_end;
+};
+__device__ __forceinline__ TileRange sort_tile_range(uint32_t tile, uint32_t n,
+                                                     const SortSegs &segs) {
+  if (!segs.tile_seg) {
+    const uint32_t b = tile * (uint32_t)kSortTile;
+    return {b, min(b + (uint32_t)kSortTile, n), 0u, n};
+  }
+  const uint32_t sg = segs.tile_seg[tile];
+  const uint32_t sb = segs.seg_off[sg], se = segs.seg_off[sg + 1];
+  const uint32_t b = sb + (tile - segs.seg_tile0[sg]) * (uint32_t)kSortTile;
+  return {b, min(b + (uint32_t)kSortTile, se), sb, se};
+}
+
 // Number of set bits of `mask` in lanes below the calling lane.
 __device__ __forceinline__ uint32_t popc_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

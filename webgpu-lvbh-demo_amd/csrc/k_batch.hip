@@ -52,12 +52,11 @@ __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
 __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
     const float4 *__restrict__ tri, SortSegs segs, uint32_t *seg_ord) {
   __shared__ float s[kSortThreads / 64][6];
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
-  const uint32_t base = segs.seg_off[sg] + (tile - segs.seg_tile0[sg]) * kSortTile;
-  const uint32_t end = min(base + (uint32_t)kSortTile, segs.seg_off[sg + 1]);
+  const TileRange r = sort_tile_range(tile, 0u, segs);
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t i = base + threadIdx.x; i < end; i += kSortThreads) {
+  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads) {
     const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
                  c = tri[3 * (size_t)i + 2];
     v[0] = fminf(v[0], fminf(a.x, fminf(b.x, c.x)));
@@ -80,25 +79,28 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
   __syncthreads();
   if (threadIdx.x < 6) {
     const int c = threadIdx.x;
-    float r = s[0][c];
+    float m = s[0][c];
     for (int w = 1; w < kSortThreads / 64; ++w)
-      r = c < 3 ? fminf(r, s[w][c]) : fmaxf(r, s[w][c]);
-    if (c < 3) atomicMin(&seg_ord[8 * sg + c], f2ord(r));
-    else atomicMax(&seg_ord[8 * sg + c], f2ord(r));
+      m = c < 3 ? fminf(m, s[w][c]) : fmaxf(m, s[w][c]);
+    if (c < 3) atomicMin(&seg_ord[8 * sg + c], f2ord(m));
+    else atomicMax(&seg_ord[8 * sg + c], f2ord(m));
   }
 }
 
-// Per-mesh leaf AABB centre -> Morton key, plus the mesh's digit histograms.
+// Per-mesh leaf AABB centre -> Morton key, plus the mesh's digit histograms
+// and the first sort pass's per-tile digit counts (cnt0[d * tiles + tile]).
 // Bounds: host-provided fp32 (6 per mesh) or the GPU accumulators.
 __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     const float4 *__restrict__ tri, SortSegs segs,
     const float *__restrict__ seg_bounds, const uint32_t *__restrict__ seg_ord,
-    int div_mode, uint32_t *__restrict__ keys, uint32_t *seg_hist) {
+    int div_mode, uint32_t *__restrict__ keys, uint32_t *seg_hist,
+    uint32_t *__restrict__ cnt0) {
+  static_assert(kSortThreads == kRadix, "one thread per digit for cnt0");
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
-  const uint32_t base = segs.seg_off[sg] + (tile - segs.seg_tile0[sg]) * kSortTile;
-  const uint32_t end = min(base + (uint32_t)kSortTile, segs.seg_off[sg + 1]);
+  const TileRange r = sort_tile_range(tile, 0u, segs);
+  const uint32_t base = r.begin, end = r.end;
   for (int k = threadIdx.x; k < kSortPasses * kRadix; k += kSortThreads)
     (&s_hist[0][0])[k] = 0u;
   float bmin[3], ext[3], rcp[3];
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     const uint32_t v = (&s_hist[0][0])[k];
     if (v) atomicAdd(&h[k], v);
   }
+  cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[0][threadIdx.x];
 }
 
 void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
@@ -165,9 +168,10 @@ void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
 void launch_batch_morton(const float4 *tri, const SortSegs &segs,
                          uint32_t tiles, const float *seg_bounds,
                          const uint32_t *seg_ord, int div_mode, uint32_t *keys,
-                         uint32_t *seg_hist, hipStream_t s) {
+                         uint32_t *seg_hist, uint32_t *cnt0, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_morton, dim3(tiles), dim3(kSortThreads), 0, s,
-                     tri, segs, seg_bounds, seg_ord, div_mode, keys, seg_hist);
+                     tri, segs, seg_bounds, seg_ord, div_mode, keys, seg_hist,
+                     cnt0);
 }
 
 }  // namespace lbvh

@@ -116,12 +116,16 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
 //   rel  = (cen - bmin) / (bmax - bmin)               (:192, per div mode)
 //   key  = morton(u32(rel * 1023.99f))                (:194-208)
 // Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
-// fused build, the four digit histograms of the keys (the sort's upfront
-// histogram, saving a second read of the keys).
+// fused build, the four global digit histograms of the keys plus the first
+// sort pass's per-tile digit counts (saving the sort a read of the keys).
+// With cnt0 the grid is one block per sort tile (XCD-aware ids, as the
+// sort's), without it a grid-stride loop.
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
-    uint32_t *__restrict__ keys, Ctl *ctl, int histogram, int bump_epoch) {
+    uint32_t *__restrict__ keys, Ctl *ctl, int histogram, int bump_epoch,
+    uint32_t *__restrict__ cnt0) {
+  static_assert(kMapThreads == kRadix, "one thread per digit for cnt0");
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   if (histogram) {
@@ -146,8 +150,15 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     }
   }
   const bool ieee = (p.div_mode == 1);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  uint32_t first = blockIdx.x * blockDim.x, end = n,
+           stride = gridDim.x * blockDim.x, tile = 0;
+  if (cnt0) {
+    tile = xcd_tile(blockIdx.x, gridDim.x);
+    first = tile * (uint32_t)kSortTile;
+    end = min(first + (uint32_t)kSortTile, n);
+    stride = blockDim.x;
+  }
+  for (uint32_t i = first + threadIdx.x; i < end; i += stride) {
     const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
                  c = tri[3 * (size_t)i + 2];
     const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
@@ -176,6 +187,7 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
       const uint32_t v = (&s_hist[0][0])[k];
       if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
     }
+    if (cnt0) cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[0][threadIdx.x];
   }
 }
 
@@ -199,13 +211,17 @@ void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
                         Ctl *ctl, bool histogram, bool bump_epoch,
-                        hipStream_t s) {
+                        hipStream_t s, uint32_t *cnt0) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
+  if (cnt0) {
+    histogram = true;
+    blocks = sort_tiles(n);
+  }
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_aabb_morton, dim3(blocks), dim3(kMapThreads), 0, s,
                      tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
-                     histogram ? 1 : 0, bump_epoch ? 1 : 0);
+                     histogram ? 1 : 0, bump_epoch ? 1 : 0, cnt0);
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

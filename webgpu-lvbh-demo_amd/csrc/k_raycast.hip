@@ -47,7 +47,9 @@ __device__ __forceinline__ float4 tri_intersect(V3 o, V3 d, V3 v0, V3 v1,
   const float v = __fmul_rn(dot(d, qvec), idet);
   if (v < 0.f || __fadd_rn(u, v) > 1.f) return make_float4(-1.f, -1.f, -1.f, -1.f);
   const V3 nrm = cross(e1, e2);
-  const float len = __fsqrt_rn(dot(nrm, nrm));
+  // __builtin_sqrtf is correctly rounded (the default fp32 mode); HIP's
+  // __fsqrt_rn maps to the hardware approximation, which is not.
+  const float len = __builtin_sqrtf(dot(nrm, nrm));
   return make_float4(__fdiv_rn(nrm.x, len), __fdiv_rn(nrm.y, len),
                      __fdiv_rn(nrm.z, len), __fmul_rn(dot(e2, qvec), idet));
 }

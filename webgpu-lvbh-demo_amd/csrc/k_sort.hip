@@ -4,39 +4,36 @@
 // passes of 2-bit digits, each as 4 dispatches and a host round trip
 // (:386-488), with a 3-level 256-ary scan capped at 2^24 keys (:189-202).
 // Its output is the unique stable-sort permutation of the keys; this file
-// computes the same permutation with 4 passes of 8-bit digits:
+// computes the same permutation with 4 passes of 8-bit digits.
 //
-//   * one upfront histogram of all four digits (fused into the Morton kernel
-//     in the full build; k_histogram here for the stand-alone sort),
-//   * one kernel per digit ("onesweep", Adinets & Merrill 2022): each block
-//     draws a tile id from an atomic counter (CDNA does not promise block
-//     issue order), ranks its 4096 keys with wave64 ballot matching into
-//     per-wave LDS histograms, publishes its digit counts, and resolves its
-//     global digit offsets by single-pass decoupled look-back (Merrill &
-//     Garland 2016) with one thread per digit.  Keys and payloads are then
-//     reordered in LDS and written out digit-run by digit-run.
+// Each pass is reduce-then-scan over tiles of kSortTile keys:
+//   count    per-tile digit counts (pass 0's are written by the Morton
+//            kernel as it produces the keys);
+//   scan     one block per digit scans that digit's column of tile counts
+//            and adds the digit's base -> the first output position of every
+//            (digit, tile) run;
+//   scatter  each tile ranks its keys with wave64 ballot matching into
+//            per-wave LDS histograms (rank order = input order: stable),
+//            reorders them in LDS and writes each digit run contiguously.
+// No block ever waits for another block.  A decoupled look-back chains every
+// tile to its predecessors, and on MI355X each link of that chain is a
+// coherent cross-XCD load; with thousands of tiles the chain, not HBM, set
+// the pass time.  Kernel boundaries are the only global synchronisation.
 //
-// Look-back words are 64-bit: hi = (stamp << 2) | flag, lo = count, with
-// stamp = epoch*4 + pass.  A word from an earlier pass or build carries an
-// older stamp and reads as "not yet published", so the status array is never
-// cleared between passes.  Loads and stores of those words are agent-scope
-// atomics, so no fences are needed (count and flag travel together).
+// Counts and offsets are digit-major: entry (d, t) at d * tiles + t, so the
+// scan reads and writes columns contiguously.  Tile ids are XCD-aware
+// (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
+// neighbouring tiles share -- count columns and the boundaries of their digit
+// runs in the output -- are merged in one L2.
 #include "device_util.h"
 
 namespace lbvh {
 
 constexpr int kWaves = kSortThreads / 64;
-static_assert(kSortThreads == kRadix, "one thread per digit in the look-back");
-constexpr uint32_t kFlagAggregate = 1u;
-constexpr uint32_t kFlagInclusive = 2u;
-constexpr int kLookback = 8;  // predecessor tiles probed per look-back step
+static_assert(kSortThreads == kRadix, "one thread per digit");
+constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 
-__device__ __forceinline__ uint64_t lb_pack(uint32_t stamp, uint32_t flag,
-                                            uint32_t count) {
-  return ((uint64_t)((stamp << 2) | flag) << 32) | count;
-}
-
-// Upfront digit histograms of the stand-alone sort (bumps the epoch).
+// Global digit histograms of the stand-alone sort (bumps the epoch).
 __global__ __launch_bounds__(kMapThreads) void k_histogram(
     const uint32_t *__restrict__ keys, uint32_t n, Ctl *ctl) {
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
@@ -54,57 +51,151 @@ __global__ __launch_bounds__(kMapThreads) void k_histogram(
   }
 }
 
-// One 8-bit digit pass.  kFirst: the payload is the input position (the
+// Per-tile digit counts of one pass -> cnt[d * tiles + tile].
+__global__ __launch_bounds__(kSortThreads) void k_count(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
+    uint32_t *__restrict__ cnt, SortSegs segs) {
+  __shared__ uint32_t s_c[kRadix];
+  const uint32_t tiles = gridDim.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, tiles);
+  s_c[threadIdx.x] = 0u;
+  __syncthreads();
+  const TileRange r = sort_tile_range(tile, n, segs);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads) {
+    const uint32_t d = (keys[i] >> shift) & (kRadix - 1);
+    // a wave whose active lanes share one digit adds once
+    const uint64_t active = __ballot(1);
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const uint32_t d0 = __shfl(d, leader, 64);
+    if (__ballot(d == d0) == active) {
+      if (lane == leader) atomicAdd(&s_c[d0], (uint32_t)__popcll(active));
+    } else {
+      atomicAdd(&s_c[d], 1u);
+    }
+  }
+  __syncthreads();
+  cnt[(size_t)threadIdx.x * tiles + tile] = s_c[threadIdx.x];
+}
+
+// Column scan: block d turns column d of the tile counts into output
+// positions.  P(t) = sum of cnt[d][t'] over t' < t.
+//   one mesh:  off[d][t] = base(d) + P(t), base(d) = keys with a smaller
+//              digit (exclusive scan of the global histogram);
+//   segments:  off[d][t] = base(s, d) + P(t) - P(t0(s)) for the segment s of
+//              tile t with first tile t0(s); seg_base[s][d] holds base(s, d)
+//              (the segment's first key + its keys with a smaller digit;
+//              stride kSortPasses * kRadix per segment) and
+//              is rewritten to base(s, d) - P(t0(s)) (mod 2^32) by the thread
+//              owning the segment's first tile.  Only block d touches column
+//              d of seg_base, so the rewrite needs just this block's barrier.
+__global__ __launch_bounds__(kSortThreads) void k_scan_columns(
+    const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
+    uint32_t tiles, const uint32_t *__restrict__ hist, SortSegs segs,
+    uint32_t *seg_base) {
+  __shared__ uint32_t s_wsum[16];
+  __shared__ uint32_t s_carry;
+  const uint32_t d = blockIdx.x;
+  const bool segmented = segs.tile_seg != nullptr;
+  if (!segmented) {
+    const uint32_t below = block_exclusive_scan(hist[threadIdx.x], s_wsum);
+    if (threadIdx.x == d) s_carry = below;
+  } else if (threadIdx.x == 0) {
+    s_carry = 0u;
+  }
+  __syncthreads();
+  uint32_t carry = s_carry;
+  const uint32_t *col = cnt + (size_t)d * tiles;
+  uint32_t *out = off + (size_t)d * tiles;
+  for (uint32_t c0 = 0; c0 < tiles; c0 += kSortThreads * kScanPer) {
+    const uint32_t t0 = c0 + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+      v[j] = (t0 + j < tiles) ? col[t0 + j] : 0u;
+      sum += v[j];
+    }
+    const uint32_t ex = block_exclusive_scan(sum, s_wsum);
+    if (threadIdx.x == kSortThreads - 1) s_carry = ex + sum;
+    uint32_t p[kScanPer];
+    uint32_t run = carry + ex;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+      p[j] = run;
+      run += v[j];
+    }
+    if (segmented) {
+#pragma unroll
+      for (int j = 0; j < kScanPer; ++j) {
+        const uint32_t t = t0 + j;
+        if (t < tiles) {
+          const uint32_t sg = segs.tile_seg[t];
+          if (t == segs.seg_tile0[sg]) seg_base[(size_t)sg * kSortPasses * kRadix + d] -= p[j];
+        }
+      }
+    }
+    __syncthreads();  // s_carry and the rewritten bases are visible
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+      const uint32_t t = t0 + j;
+      if (t < tiles)
+        out[t] = segmented
+                     ? p[j] + seg_base[(size_t)segs.tile_seg[t] * kSortPasses * kRadix + d]
+                     : p[j];
+    }
+    carry += s_carry;
+    __syncthreads();  // s_carry is rewritten by the next chunk
+  }
+}
+
+// Segment digit bases, all passes: seg_hist[s][p][d] (counts) becomes
+// seg_off[s] + (keys of s whose digit p is below d).  One block per segment.
+__global__ __launch_bounds__(kSortThreads) void k_seg_bases(
+    uint32_t *seg_hist, const uint32_t *__restrict__ seg_off) {
+  __shared__ uint32_t s_wsum[16];
+  const uint32_t sg = blockIdx.x;
+  const uint32_t first = seg_off[sg];
+#pragma unroll
+  for (int p = 0; p < kSortPasses; ++p) {
+    uint32_t *h = seg_hist + ((size_t)sg * kSortPasses + p) * kRadix;
+    const uint32_t below = block_exclusive_scan(h[threadIdx.x], s_wsum);
+    h[threadIdx.x] = first + below;
+  }
+}
+
+// One digit pass.  kFirst: the payload is the position in the segment (the
 // reference's init_idx, radix-sort.js:527-534, folded into the first pass).
-// With segments (batched builds), every segment is sorted on its own: tiles
-// never straddle a segment, the look-back stops at the segment's first tile,
-// digit bases come from the segment's histograms, and payloads are
-// positions within the segment.
+// clear_hist: the last pass of a single-mesh sort zeroes the global digit
+// histograms for the next operation (every scan has read them).
 template <bool kFirst>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+__global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
-    uint32_t n, uint32_t pass, uint32_t num_tiles, Ctl *ctl,
-    uint64_t *status, int last_pass, SortSegs segs) {
+    uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
+    SortSegs segs, Ctl *ctl, int clear_hist) {
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[kWaves][kRadix];
   __shared__ uint32_t s_start[kRadix];  // tile-local start of each digit run
-  __shared__ uint32_t s_gbase[kRadix];  // global position minus local start
+  __shared__ uint32_t s_gbase[kRadix];  // output position minus local start
   __shared__ uint32_t s_wsum[16];
-  __shared__ uint32_t s_tile;
-  __shared__ int s_last;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t shift = pass * kRadixBits;
-
-  if (tid == 0) {
-    const uint32_t t = atomicAdd(&ctl->tile_ctr, 1u);
-    if (t == num_tiles - 1)
-      __hip_atomic_store(&ctl->tile_ctr, 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    s_tile = t;
-  }
+  const uint32_t tiles = gridDim.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, tiles);
+  if (clear_hist && tile == 0)
+    for (int k = tid; k < kSortPasses * kRadix; k += kSortThreads)
+      (&ctl->hist[0][0])[k] = 0u;
   for (int k = tid; k < kWaves * kRadix; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
-  const uint32_t stamp = (ctl->epoch * 4u + pass) & 0x3FFFFFFFu;
+  const uint32_t goff = off[(size_t)tid * tiles + tile];
+  const TileRange r = sort_tile_range(tile, n, segs);
   __syncthreads();
-  const uint32_t tile = s_tile;
-  uint32_t seg_base = 0, seg_end = n, tile0 = 0;
-  const uint32_t *hist = ctl->hist[pass];
-  if (segs.tile_seg) {
-    const uint32_t sg = segs.tile_seg[tile];
-    seg_base = segs.seg_off[sg];
-    seg_end = segs.seg_off[sg + 1];
-    tile0 = segs.seg_tile0[sg];
-    hist = segs.seg_hist + (size_t)sg * kSortPasses * kRadix + pass * kRadix;
-  }
-  const uint32_t gcount = hist[tid];
-  const uint32_t tile_base = seg_base + (tile - tile0) * kSortTile;
 
   // Warp-striped load: wave w owns keys [wbase, wbase + 64*kSortItems), item
   // j of lane l sits at wbase + 64*j + l, so (j, l) order is input order.
-  const uint32_t wbase = tile_base + wave * 64 * kSortItems;
+  const uint32_t wbase = r.begin + wave * 64 * kSortItems;
   uint32_t key[kSortItems];
   int32_t val[kSortItems];
 #pragma unroll
@@ -112,9 +203,9 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     const uint32_t pos = wbase + 64 * j + lane;
     key[j] = 0u;
     val[j] = 0;
-    if (pos < seg_end) {
+    if (pos < r.end) {
       key[j] = keys_in[pos];
-      val[j] = kFirst ? (int32_t)(pos - seg_base) : vals_in[pos];
+      val[j] = kFirst ? (int32_t)(pos - r.seg_begin) : vals_in[pos];
     }
   }
 
@@ -124,7 +215,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   uint32_t rank[kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    const bool valid = (wbase + 64 * j + lane) < seg_end;
+    const bool valid = (wbase + 64 * j + lane) < r.end;
     const uint32_t d = (key[j] >> shift) & (kRadix - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -143,7 +234,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   }
   __syncthreads();
 
-  // Thread d: this tile's count of digit d, per-wave exclusive offsets.
+  // Thread d: this tile's count of digit d and per-wave exclusive offsets.
   const uint32_t d = tid;
   uint32_t total = 0;
 #pragma unroll
@@ -152,67 +243,15 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     s_whist[w][d] = total;
     total += c;
   }
-  uint64_t *my_word = status + (size_t)tile * kRadix + d;
-  __hip_atomic_store(my_word,
-                     lb_pack(stamp, tile == tile0 ? kFlagInclusive : kFlagAggregate,
-                             total),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
   const uint32_t start = block_exclusive_scan(total, s_wsum);
-  const uint32_t gdigit = block_exclusive_scan(gcount, s_wsum);
   s_start[d] = start;
-
-  // Decoupled look-back for digit d, kLookback predecessor words per probe:
-  // the loads of one probe are independent, so a walk over tiles that have
-  // only published aggregates advances kLookback tiles per memory round
-  // trip.  Every earlier tile drew its id before this one and publishes its
-  // aggregate without waiting, so the walk ends; the spin cap only bounds the
-  // loop should that invariant ever break (the result is then wrong and
-  // ctl->fault says so, but the kernel still exits).
-  uint32_t excl = 0;
-  if (tile > tile0) {
-    int64_t k = (int64_t)tile - 1;
-    uint32_t spins = 0;
-    while (k >= (int64_t)tile0) {
-      uint64_t w[kLookback];
-#pragma unroll
-      for (int j = 0; j < kLookback; ++j)
-        w[j] = (k - j >= (int64_t)tile0)
-                   ? __hip_atomic_load(status + (size_t)(k - j) * kRadix + d,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                   : 0ull;
-      int used = 0;
-      bool done = false;
-#pragma unroll
-      for (int j = 0; j < kLookback; ++j) {
-        if (done || used < j) continue;      // stopped earlier in this probe
-        if (k - j < (int64_t)tile0) { done = true; continue; }
-        const uint32_t hi = (uint32_t)(w[j] >> 32);
-        if ((hi >> 2) != stamp) continue;  // not published yet
-        excl += (uint32_t)w[j];
-        used = j + 1;
-        if ((hi & 3u) == kFlagInclusive) done = true;
-      }
-      if (done) break;
-      k -= used;
-      if (used == 0) {
-        if (++spins > kSpinCap) {
-          atomicOr(&ctl->fault, 1u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __hip_atomic_store(my_word, lb_pack(stamp, kFlagInclusive, excl + total),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  s_gbase[d] = seg_base + gdigit + excl - start;
+  s_gbase[d] = goff - start;
   __syncthreads();
 
   // Reorder the tile in LDS by (digit, input order).
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    if ((wbase + 64 * j + lane) < seg_end) {
+    if ((wbase + 64 * j + lane) < r.end) {
       const uint32_t dj = (key[j] >> shift) & (kRadix - 1);
       const uint32_t lp = s_start[dj] + s_whist[wave][dj] + rank[j];
       s_keys[lp] = key[j];
@@ -222,34 +261,16 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   __syncthreads();
 
   // Write out: consecutive threads take consecutive tile positions, so each
-  // digit run goes to consecutive global addresses.
-  const uint32_t tile_n = (seg_end - tile_base < (uint32_t)kSortTile)
-                              ? seg_end - tile_base : (uint32_t)kSortTile;
+  // digit run goes to consecutive addresses.  Positions outside the segment
+  // cannot occur when the counts are right; the guard keeps a corrupted
+  // count from writing outside it.
+  const uint32_t tile_n = r.end - r.begin;
   for (uint32_t p = tid; p < tile_n; p += kSortThreads) {
     const uint32_t k = s_keys[p];
     const uint32_t g = s_gbase[(k >> shift) & (kRadix - 1)] + p;
-    if (g >= seg_base && g < seg_end) {
+    if (g >= r.seg_begin && g < r.seg_end) {
       keys_out[g] = k;
       vals_out[g] = s_vals[p];
-    }
-  }
-
-  if (last_pass && !segs.tile_seg) {
-    // The last block to finish clears the digit histograms for the next
-    // operation: every block read its counts before arriving here.  (Batched
-    // builds clear their per-segment histograms up front.)
-    if (tid == 0) {
-      const uint32_t done = __hip_atomic_fetch_add(
-          &ctl->done_ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_last = (done == num_tiles - 1);
-    }
-    __syncthreads();
-    if (s_last) {
-      for (int k = tid; k < kSortPasses * kRadix; k += kSortThreads)
-        (&ctl->hist[0][0])[k] = 0u;
-      if (tid == 0)
-        __hip_atomic_store(&ctl->done_ctr, 0u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -263,23 +284,38 @@ void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
                      n, ctl);
 }
 
-void launch_onesweep_pass(int pass, const uint32_t *keys_in,
-                          const int32_t *vals_in, uint32_t *keys_out,
-                          int32_t *vals_out, uint32_t n, Ctl *ctl,
-                          uint64_t *status, hipStream_t s,
-                          const SortSegs *segs, uint32_t seg_tiles) {
+void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
+                      uint32_t num_segs, hipStream_t s) {
+  if (num_segs == 0) return;
+  hipLaunchKernelGGL(k_seg_bases, dim3(num_segs), dim3(kSortThreads), 0, s,
+                     seg_hist, seg_off);
+}
+
+void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
+                      uint32_t *keys_out, int32_t *vals_out, uint32_t n,
+                      uint32_t tiles, uint32_t *cnt, uint32_t *off,
+                      bool counted, Ctl *ctl, hipStream_t s,
+                      const SortSegs *segs) {
+  if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
-  const uint32_t tiles = segs ? seg_tiles : sort_tiles(n);
-  const int last = (pass == kSortPasses - 1) ? 1 : 0;
+  const uint32_t shift = (uint32_t)pass * kRadixBits;
+  if (!counted)
+    hipLaunchKernelGGL(k_count, dim3(tiles), dim3(kSortThreads), 0, s, keys_in,
+                       n, shift, cnt, sg);
+  uint32_t *seg_base = segs ? const_cast<uint32_t *>(sg.seg_hist) + pass * kRadix
+                            : nullptr;
+  hipLaunchKernelGGL(k_scan_columns, dim3(kRadix), dim3(kSortThreads), 0, s,
+                     cnt, off, tiles, ctl->hist[pass], sg, seg_base);
+  const int clear = (!segs && pass == kSortPasses - 1) ? 1 : 0;
   if (pass == 0)
-    hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kSortThreads), 0, s,
+    hipLaunchKernelGGL(k_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, s,
                        keys_in, (const int32_t *)nullptr, keys_out, vals_out,
-                       n, 0u, tiles, ctl, status, last, sg);
+                       n, shift, off, sg, ctl, clear);
   else
-    hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kSortThreads), 0,
-                       s, keys_in, vals_in, keys_out, vals_out, n,
-                       (uint32_t)pass, tiles, ctl, status, last, sg);
+    hipLaunchKernelGGL(k_scatter<false>, dim3(tiles), dim3(kSortThreads), 0, s,
+                       keys_in, vals_in, keys_out, vals_out, n, shift, off, sg,
+                       ctl, clear);
 }
 
 }  // namespace lbvh

@@ -268,7 +268,9 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // of the position-augmented keys has at most 64 nodes, so a tile has at most
 // kFrontierMax of them.  The frontier is written, in leaf order, as "items"
 // (leaf range + box) to a compact array whose offsets come from a
-// single-value decoupled look-back over the tiles.  Level L >= 1 runs the
+// single-value decoupled look-back over the tiles, probing 64 predecessors
+// per round trip (a one-tile-per-hop walk made the chain of cross-XCD loads,
+// not the tree work, set the level's time).  Level L >= 1 runs the
 // same agglomeration over the items of level L-1, until one tile holds all
 // remaining items and completes the root.  No block waits on another
 // block's tree work; nodes are written exactly once, by the thread that
@@ -491,40 +493,62 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   if (tid == T - 1) s_count = rank + (out ? 1u : 0u);
   __syncthreads();
   const uint32_t count = s_count;
-  if (tid == 0) {
+  if (tid < 64) {
+    // Look-back by the first wave: lane i probes tile j - i, so one round
+    // trip covers 64 predecessors.  Lanes [0, take) are summed, where take
+    // ends at the first unpublished word or just after the first inclusive
+    // one.  The segment's first tile publishes an inclusive word without
+    // waiting, so the walk ends; the spin cap only bounds it should that
+    // invariant break (ctl->fault reports it, and the kernel still exits).
+    const int lane = (int)tid;
     uint64_t *mine = lstatus + gtile;
-    __hip_atomic_store(mine,
-                       ((uint64_t)((stamp << 2) | (tile == 0 ? kLbInclusive
-                                                             : kLbAggregate))
-                        << 32) | count,
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0)
+      __hip_atomic_store(mine,
+                         ((uint64_t)((stamp << 2) | (tile == 0 ? kLbInclusive
+                                                               : kLbAggregate))
+                          << 32) | count,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t excl = 0;
     int64_t j = gtile - 1;
     uint32_t spins = 0;
     while (j >= tile0) {
-      const uint64_t w = __hip_atomic_load(lstatus + j, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+      const int64_t q = j - lane;
+      const uint64_t w = (q >= tile0)
+          ? __hip_atomic_load(lstatus + q, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT)
+          : 0ull;
       const uint32_t hi = (uint32_t)(w >> 32);
-      if ((hi >> 2) != stamp) {
+      const bool pub = (q >= tile0) && (hi >> 2) == stamp;
+      const uint64_t stop = __ballot(!pub);
+      const uint64_t inc = __ballot(pub && (hi & 3u) == kLbInclusive);
+      const int u = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
+      const uint64_t below_u = (u == 64) ? ~0ull : ((1ull << u) - 1ull);
+      const uint64_t inc_u = inc & below_u;
+      const int take = inc_u ? __ffsll((unsigned long long)inc_u) : u;
+      uint32_t part = (lane < take) ? (uint32_t)w : 0u;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+      excl += part;
+      if (inc_u) break;
+      j -= take;
+      if (take == 0) {
         if (++spins > kSpinCap) {
-          atomicOr(&ctl->fault, 4u);
+          if (lane == 0) atomicOr(&ctl->fault, 4u);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      excl += (uint32_t)w;
-      if ((hi & 3u) == kLbInclusive) break;
-      --j;
     }
-    if (tile > 0)
-      __hip_atomic_store(mine,
-                         ((uint64_t)((stamp << 2) | kLbInclusive) << 32) |
-                             (excl + count),
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (te == M) *count_slot = excl + count;
-    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
-    L.base = excl;
+    if (lane == 0) {
+      if (tile > 0)
+        __hip_atomic_store(mine,
+                           ((uint64_t)((stamp << 2) | kLbInclusive) << 32) |
+                               (excl + count),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (te == M) *count_slot = excl + count;
+      if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
+      L.base = excl;
+    }
   }
   __syncthreads();
   if (out) {

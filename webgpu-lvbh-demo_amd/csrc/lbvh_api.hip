@@ -47,7 +47,7 @@ struct lbvh_ctx {
   Ctl *ctl = nullptr;
   uint64_t epoch_ops = 0;  // epoch bumps issued since the last rewind
   DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
-  DevBuf status;                        // look-back words (zeroed)
+  DevBuf digits;                        // sort tile counts + offsets
   DevBuf gside;                         // per-stage up-pass exchange
   DevBuf items_a, items_b;              // tree agglomeration levels
   DevBuf lstatus;                       // their look-back words (zeroed)
@@ -55,7 +55,7 @@ struct lbvh_ctx {
   DevBuf partials;                      // bounds partials
   // batched builds: scratch, and the plan (tile tables + captured graph) of
   // the last batch layout
-  DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_status, b_hist, b_ord,
+  DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_hist, b_ord,
       b_bounds, b_items_a, b_items_b, b_lstatus, b_table;
   struct BatchPlan {
     std::vector<uint32_t> offsets;
@@ -170,8 +170,7 @@ constexpr uint64_t kEpochRewind = 1ull << 26;
 int begin_op(lbvh_ctx *ctx) {
   if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->status, &ctx->lstatus, &ctx->ctag, &ctx->b_status,
-                    &ctx->b_lstatus})
+  for (DevBuf *b : {&ctx->lstatus, &ctx->ctag, &ctx->b_lstatus})
     if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
   HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
                               ctx->stream));
@@ -214,9 +213,9 @@ struct StageScope {
 
 std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
   return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
-          &ctx->status,  &ctx->gside,    &ctx->items_a,  &ctx->items_b,
+          &ctx->digits,  &ctx->gside,    &ctx->items_a,  &ctx->items_b,
           &ctx->lstatus, &ctx->ctag,     &ctx->partials, &ctx->b_keys_a,
-          &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_status,
+          &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
           &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_items_a,
           &ctx->b_items_b, &ctx->b_lstatus, &ctx->b_table};
 }
@@ -248,7 +247,7 @@ int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
   if ((st = ensure(ctx, ctx->idx_a, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
   const uint64_t words = (uint64_t)sort_tiles(n) * kRadix;
-  return ensure(ctx, ctx->status, 8ull * words, true);
+  return ensure(ctx, ctx->digits, 8ull * words, false);
 }
 
 int bucket_of(uint32_t n) {
@@ -281,19 +280,23 @@ int ensure_partials(lbvh_ctx *ctx) {
   return ensure(ctx, ctx->partials, 32ull * kBoundsBlocks, false);
 }
 
-// The four onesweep passes: in -> tmp -> out -> tmp -> out.  `in` may equal
-// `out` (the stand-alone sort is in place, like the reference's).
+// The four digit passes: in -> tmp -> out -> tmp -> out.  `in` may equal
+// `out` (the stand-alone sort is in place, like the reference's).  counted0:
+// the Morton kernel already wrote pass 0's per-tile counts into the digit
+// buffer.
 void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-              int32_t *vtmp, int32_t *vout, uint32_t n) {
-  uint64_t *status = static_cast<uint64_t *>(ctx->status.ptr);
+              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted0) {
+  const uint32_t tiles = sort_tiles(n);
+  uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
+  uint32_t *off = cnt + (size_t)kRadix * tiles;
   const uint32_t *src[kSortPasses] = {in, tmp, out, tmp};
   uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
   const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
   int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
   for (int p = 0; p < kSortPasses; ++p) {
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
-    launch_onesweep_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, ctx->ctl,
-                         status, ctx->stream);
+    launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, tiles, cnt, off,
+                     p == 0 && counted0, ctx->ctl, ctx->stream);
   }
 }
 
@@ -417,15 +420,19 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   launch_batch_morton(t4, ss, P.sort_tiles,
                       have_bounds ? static_cast<const float *>(ctx->b_bounds.ptr)
                                   : nullptr,
-                      ord, ctx->div_mode, keys, hist, ctx->stream);
-  uint64_t *status = static_cast<uint64_t *>(ctx->b_status.ptr);
+                      ord, ctx->div_mode, keys, hist,
+                      static_cast<uint32_t *>(ctx->b_digits.ptr), ctx->stream);
+  launch_seg_bases(hist, T + P.o_seg_off, P.S, ctx->stream);
+  uint32_t *cnt = static_cast<uint32_t *>(ctx->b_digits.ptr);
+  uint32_t *off = cnt + (size_t)kRadix * P.sort_tiles;
   const uint32_t *src[kSortPasses] = {keys, tmp, out, tmp};
   uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
   const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
   int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
   for (int p = 0; p < kSortPasses; ++p)
-    launch_onesweep_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
-                         ctx->ctl, status, ctx->stream, &ss, P.sort_tiles);
+    launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
+                     P.sort_tiles, cnt, off, p == 0, ctx->ctl, ctx->stream,
+                     &ss);
   int4 *ia = static_cast<int4 *>(ctx->b_items_a.ptr);
   int4 *ib = static_cast<int4 *>(ctx->b_items_b.ptr);
   uint64_t *ls = static_cast<uint64_t *>(ctx->b_lstatus.ptr);
@@ -774,7 +781,7 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
   if ((st = begin_op(ctx))) return st;
   launch_histogram(keys, n, ctx->ctl, ctx->stream);
   run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
-           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n);
+           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false);
   return check_launch(ctx, "lbvh_sort");
 }
 
@@ -856,11 +863,13 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   {
     StageScope sc(ctx, LBVH_STAGE_MORTON);
     launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                       ctx->ctl, sort, bounds_host != nullptr, ctx->stream);
+                       ctx->ctl, sort, bounds_host != nullptr, ctx->stream,
+                       sort ? static_cast<uint32_t *>(ctx->digits.ptr)
+                            : nullptr);
   }
   if (sort) {
     run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), skeys,
-             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n);
+             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n, true);
   } else {
     HIP_TRY(ctx, hipMemsetAsync(perm, 0, 4, ctx->stream));
     if (skeys != keys)
@@ -930,7 +939,7 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if ((st = ensure(ctx, ctx->b_keys_b, 4ull * total, false))) return st;
   if ((st = ensure(ctx, ctx->b_idx_a, 4ull * total, false))) return st;
   if ((st = ensure(ctx, ctx->b_idx_b, 4ull * total, false))) return st;
-  if ((st = ensure(ctx, ctx->b_status, 8ull * kRadix * P.sort_tiles, true)))
+  if ((st = ensure(ctx, ctx->b_digits, 8ull * kRadix * P.sort_tiles, false)))
     return st;
   if ((st = ensure(ctx, ctx->b_hist, 4ull * kSortPasses * kRadix * num_meshes,
                    false)))

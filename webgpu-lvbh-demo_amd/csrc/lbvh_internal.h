@@ -30,23 +30,19 @@ constexpr uint32_t kSpinCap = 1u << 22;  // look-back polls before giving up
 // operations (and graph replays) need no host-side reset:
 //   epoch       bumped once per operation by its first kernel; stamps the
 //               look-back words and arrival tags of that operation
-//   tile_ctr    dynamic tile id of a sort pass; the block that draws the last
-//               id puts it back to 0
-//   done_ctr    completion count of the last sort pass; the last block zeroes
-//               the digit histograms and the counter
 //   bounds_done completion count of the bounds reduction (reset by the last
 //               block)
 //   fault       sticky flag: a bounded wait gave up or a frontier overflowed
 //               (never set in a correct run; read by lbvh_synchronize)
 //   agg_tile_ctr / level_count: tile ids (self-resetting) and item counts of
 //               the tree's agglomeration levels
+//   hist        global digit counts of a single-mesh sort (zeroed by the
+//               last pass's scatter)
 struct Ctl {
   uint32_t epoch;
-  uint32_t tile_ctr;
-  uint32_t done_ctr;
   uint32_t bounds_done;
   uint32_t fault;
-  uint32_t pad0[3];
+  uint32_t pad0[5];
   uint32_t agg_tile_ctr[kMaxTreeLevels];  // per-level dynamic tile ids
   uint32_t level_count[kMaxTreeLevels];   // items produced by each level
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
@@ -73,36 +69,43 @@ struct MortonParams {
 // k_bounds_morton.hip
 void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
                    bool bump_epoch, hipStream_t s);
+// cnt0 (kRadix x sort_tiles(n) words, or null): also write the first sort
+// pass's per-tile digit counts (implies histogram).
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
                         Ctl *ctl, bool histogram, bool bump_epoch,
-                        hipStream_t s);
+                        hipStream_t s, uint32_t *cnt0 = nullptr);
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
 void launch_bump_epoch(Ctl *ctl, hipStream_t s);
 
 // k_sort.hip
 void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
                       hipStream_t s);
-// One onesweep digit pass (pass 0 takes the input position as payload and
-// ignores vals_in; pass kSortPasses-1 clears the histograms when done).
-// The four passes sort keys_in into keys_out through keys_tmp:
-//   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
 // Segments of a batched sort (all null: one segment [0, n)).  Tiles of
 // kSortTile keys never straddle segments; tile_seg maps each tile to its
 // segment, seg_tile0 gives a segment's first tile, seg_off its first key
-// (S+1 entries), seg_hist its kSortPasses x kRadix digit counts.
+// (S+1 entries), seg_hist its kSortPasses x kRadix digit counts, turned
+// into digit bases by launch_seg_bases before the first pass.
 struct SortSegs {
   const uint32_t *tile_seg;
   const uint32_t *seg_tile0;
   const uint32_t *seg_off;
-  const uint32_t *seg_hist;
+  uint32_t *seg_hist;
 };
-void launch_onesweep_pass(int pass, const uint32_t *keys_in,
-                          const int32_t *vals_in, uint32_t *keys_out,
-                          int32_t *vals_out, uint32_t n, Ctl *ctl,
-                          uint64_t *status, hipStream_t s,
-                          const SortSegs *segs = nullptr,
-                          uint32_t seg_tiles = 0);
+void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
+                      uint32_t num_segs, hipStream_t s);
+// One digit pass over `tiles` sort tiles (pass 0 takes the position in the
+// segment as payload and ignores vals_in; the last pass of a single-mesh
+// sort clears the global histograms).  cnt and off hold kRadix x tiles words
+// each; counted: cnt already holds this pass's per-tile digit counts (the
+// Morton kernels write pass 0's).  The four passes sort keys_in into
+// keys_out through keys_tmp:
+//   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
+void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
+                      uint32_t *keys_out, int32_t *vals_out, uint32_t n,
+                      uint32_t tiles, uint32_t *cnt, uint32_t *off,
+                      bool counted, Ctl *ctl, hipStream_t s,
+                      const SortSegs *segs = nullptr);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
@@ -155,7 +158,7 @@ void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
 void launch_batch_morton(const float4 *tri, const SortSegs &segs,
                          uint32_t tiles, const float *seg_bounds,
                          const uint32_t *seg_ord, int div_mode, uint32_t *keys,
-                         uint32_t *seg_hist, hipStream_t s);
+                         uint32_t *seg_hist, uint32_t *cnt0, hipStream_t s);
 
 // k_raycast.hip
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,
