@@ -263,21 +263,27 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // box and climbs, merging in LDS whenever both children of a split lie in
 // the tile.  A climber stops when its parent split is outside the tile, or
 // when it arrives first at a split whose other child never completes in the
-// tile.  Those stopped subtrees are the tile's frontier: each has its parent
-// on one of the two root paths through the tile's edge gaps, and a root path
-// of the position-augmented keys has at most 64 nodes, so a tile has at most
-// kFrontierMax of them.  The frontier is written, in leaf order, as "items"
-// (leaf range + box) to a compact array whose offsets come from a
-// single-value decoupled look-back over the tiles, probing 64 predecessors
-// per round trip (a one-tile-per-hop walk made the chain of cross-XCD loads,
-// not the tree work, set the level's time).  Level L >= 1 runs the
-// same agglomeration over the items of level L-1, until one tile holds all
-// remaining items and completes the root.  No block waits on another
-// block's tree work; nodes are written exactly once, by the thread that
+// tile.  Those stopped subtrees are the tile's frontier and partition its
+// leaves in order.  Each has its parent on one of the two root paths through
+// the tile's edge gaps, and a root path of the position-augmented keys has
+// at most 64 nodes, so a tile has at most kFrontierMax of them.  They are
+// written, in leaf order, as "items" (leaf range + box) into the tile's own
+// kFrontierMax slots, with their count.
+//
+// Level L >= 1 runs the same agglomeration over items: one block of
+// kItemTile threads takes the frontiers of kFanIn consecutive tiles of level
+// L-1 (at most kItemTile items), until a single tile holds all of a mesh's
+// remaining items and completes its root.  The level plan depends only on
+// the leaf count, so no block waits for another and no counter is shared.
+// (A dynamic tile ticket -- one same-address device-scope atomic per tile --
+// and a look-back across tiles both serialise on MI355X, where those
+// operations cross XCDs: at ~0.1 us per tile they, not the tree work, set
+// the stage time.)  Nodes are written exactly once, by the thread that
 // merges their children, as soon as their Karras id is known.
 
-constexpr int kFrontierMax = 128;
-constexpr uint32_t kLbAggregate = 1u, kLbInclusive = 2u;
+constexpr int kFrontierMax = kTreeFrontier;
+constexpr int kItemTile = 1024;
+constexpr int kFanIn = kItemTile / kFrontierMax;  // level L-1 tiles per tile
 
 __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
@@ -291,49 +297,49 @@ struct AggLds {
   int far[2][T];       // far item of the side that arrived
   uint32_t cnt[T];     // arrivals per gap slot
   uint32_t wsum[16];
-  uint32_t tile, base;
+  uint32_t pre[kFanIn + 1];  // item levels: first item of each source tile
 };
 
 // A climbing subtree: items [ia, ib] covering leaves [l, r], its box, and the
-// two halves of its node while that node's id is still open.
+// two halves of its node while that node's id is still open.  Positions are
+// below 2^30, so 32-bit indices suffice.
 struct Agg {
-  int64_t ia, ib, l, r;
+  int ia, ib, l, r;
   float mn[3], mx[3];
   bool have_node;
   int4 h[4];
   int stop;  // 0 root / inactive, 1 pending local arrival, 2 frontier
-  int64_t pgap;
+  int pgap;
 };
 
 template <int T>
-__device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
-                                            int64_t ts, int64_t te,
-                                            int4 *nodes) {
+__device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
+                                            int ts, int te, int4 *nodes) {
   while (true) {
     const int dl = L.dgap[c.ia - ts];      // gap before the first item
     const int dr = L.dgap[c.ib - ts + 1];  // gap after the last item
     const bool root = (c.l == 0 && c.r == n - 1);
     const bool is_left = dr > dl;
     if (c.have_node) {
-      const int64_t K = root ? 0 : (is_left ? c.r : c.l);
-      int4 *node = nodes + 4 * K;
+      const int K = root ? 0 : (is_left ? c.r : c.l);
+      int4 *node = nodes + 4 * (size_t)K;
       node[0] = c.h[0]; node[1] = c.h[1]; node[2] = c.h[2]; node[3] = c.h[3];
       c.have_node = false;
     }
     if (root) { c.stop = 0; return; }
-    const int64_t gap = is_left ? c.ib : c.ia - 1;  // item gap of the split
-    if (gap < ts || gap + 2 > te) {                 // other child off-tile
+    const int gap = is_left ? c.ib : c.ia - 1;  // item gap of the split
+    if (gap < ts || gap + 2 > te) {             // other child off-tile
       c.stop = 2;
       return;
     }
     const int side = is_left ? 0 : 1;
-    const int64_t slot = gap - ts;
+    const int slot = gap - ts;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       L.box[side][k][slot] = c.mn[k];
       L.box[side][3 + k][slot] = c.mx[k];
     }
-    L.far[side][slot] = (int)(is_left ? c.ia : c.ib);
+    L.far[side][slot] = is_left ? c.ia : c.ib;
     const uint32_t old = __hip_atomic_fetch_add(
         &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) {
@@ -347,10 +353,10 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
       smn[k] = L.box[1 - side][k][slot];
       smx[k] = L.box[1 - side][3 + k][slot];
     }
-    const int64_t sfar = L.far[1 - side][slot];
+    const int sfar = L.far[1 - side][slot];
     // Merge into the node split at leaf s: children [Lr, s] and [s+1, Rr].
-    const int64_t s = is_left ? c.r : c.l - 1;
-    int64_t Lr, Rr;
+    const int s = is_left ? c.r : c.l - 1;
+    int Lr, Rr;
     float lmn[3], lmx[3], rmn[3], rmx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -368,8 +374,8 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
       Rr = c.r;
       c.ia = sfar;
     }
-    const int lid = (Lr == s) ? -(int)(s + 1) : (int)s;
-    const int rid = (Rr == s + 1) ? -(int)(s + 2) : (int)(s + 1);
+    const int lid = (Lr == s) ? -(s + 1) : s;
+    const int rid = (Rr == s + 1) ? -(s + 2) : s + 1;
     c.h[0] = pack3(lmn, lid);
     c.h[1] = pack3(lmx, 2);
     c.h[2] = pack3(rmn, rid);
@@ -386,55 +392,68 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int64_t n,
 }
 
 // One agglomeration level.  kLeaves: level 0 over the sorted leaves (gathers
-// O_TRI and the leaf boxes); otherwise the items of the level below.  With
-// segments (batched builds) every segment is an independent mesh: its tiles
-// are consecutive, positions, node ids and the permutation are relative to
-// its first leaf, and its frontier items go to its own output region.
+// O_TRI and the leaf boxes), tile = T leaf positions; otherwise the items of
+// the level below, tile = the frontiers of kFanIn of its tiles.  Tiles are
+// numbered level by level; level L-1's tile g keeps its items in
+// items_in[g * kFrontierMax ...] and their number in count_in[g].  With
+// segments (batched builds) every segment is an independent mesh: positions,
+// node ids and the permutation are relative to its first leaf, and its
+// tiles at each level are consecutive.
 template <int T, bool kLeaves>
 __global__ __launch_bounds__(T) void k_agglomerate(
     const int4 *__restrict__ tri_all, const uint32_t *__restrict__ skeys_all,
     const int32_t *__restrict__ perm_all, uint32_t n_all,
     int4 *__restrict__ otri_all, const int4 *__restrict__ items_in,
-    int4 *__restrict__ items_out, uint32_t out_cap_all, uint32_t level,
-    uint64_t *lstatus, int4 *nodes_all, Ctl *ctl, TreeSegs segs) {
+    const uint32_t *__restrict__ count_in, uint32_t prev_tiles_all,
+    int4 *__restrict__ items_out, uint32_t *__restrict__ count_out,
+    int4 *nodes_all, Ctl *ctl, TreeSegs segs) {
+  static_assert(kLeaves || T == kItemTile, "item levels use kItemTile");
   __shared__ AggLds<T> L;
-  const int64_t tid = threadIdx.x;
-  if (tid == 0) {
-    const uint32_t t = atomicAdd(&ctl->agg_tile_ctr[level], 1u);
-    if (t == gridDim.x - 1)
-      __hip_atomic_store(&ctl->agg_tile_ctr[level], 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    L.tile = t;
-  }
-  const uint32_t stamp = (ctl->epoch * 8u + level) & 0x3FFFFFFFu;
-  __syncthreads();
-  const int64_t gtile = L.tile;
-  int64_t tile0 = 0, leaf_base = 0, N = n_all, M, in_base = 0, out_base = 0;
-  uint32_t out_cap = out_cap_all;
-  uint32_t *count_slot = &ctl->level_count[level];
+  const int tid = threadIdx.x;
+  const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
+  uint32_t tile0 = 0, leaf_base = 0, N = n_all, prev0 = 0,
+           prev_tiles = prev_tiles_all;
   if (segs.tile_seg) {
     const uint32_t sg = segs.tile_seg[gtile];
     tile0 = segs.seg_tile0[sg];
     leaf_base = segs.seg_off[sg];
-    N = (int64_t)segs.seg_off[sg + 1] - leaf_base;
-    M = kLeaves ? N : (int64_t)segs.count_in[sg];
-    if (!kLeaves) in_base = segs.in_off[sg];
-    out_base = segs.out_off[sg];
-    out_cap = segs.out_cap[sg];
-    count_slot = &segs.count_out[sg];
-  } else {
-    M = kLeaves ? N : (int64_t)ctl->level_count[level - 1];
+    N = segs.seg_off[sg + 1] - leaf_base;
+    if (!kLeaves) {
+      prev0 = segs.prev_tile0[sg];
+      prev_tiles = segs.prev_tiles[sg];
+    }
   }
-  const int64_t tile = gtile - tile0;  // tile within the mesh
-  const int64_t ts = tile * T;
-  if (ts >= M) {
-    if (tile == 0) *count_slot = 0u;  // nothing at this level
-    return;
-  }
+  const uint32_t tile = gtile - tile0;  // tile within the mesh
   const uint32_t *skeys = skeys_all + leaf_base;
-  int4 *nodes = nodes_all + 4 * leaf_base;
-  const int64_t te = (ts + T < M) ? ts + T : M;
-  const int64_t k = ts + tid;
+  int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
+  const uint32_t src0 = prev0 + tile * kFanIn;  // item levels: first source
+  int ts, te;
+  if (kLeaves) {
+    ts = (int)(tile * T);
+    te = (int)min(tile * T + T, N);
+  } else {
+    if (tid == 0) {
+      const uint32_t first = tile * kFanIn;
+      const uint32_t nsrc = prev_tiles > first ? min((uint32_t)kFanIn,
+                                                     prev_tiles - first) : 0u;
+      uint32_t c[kFanIn];
+#pragma unroll
+      for (int s = 0; s < kFanIn; ++s)
+        c[s] = (uint32_t)s < nsrc ? min(count_in[src0 + s], (uint32_t)kFrontierMax)
+                                  : 0u;
+      uint32_t run = 0;
+#pragma unroll
+      for (int s = 0; s < kFanIn; ++s) {
+        L.pre[s] = run;
+        run += c[s];
+      }
+      L.pre[kFanIn] = run;
+    }
+    __syncthreads();
+    ts = 0;
+    te = (int)L.pre[kFanIn];
+  }
+  const int k = ts + tid;
   const bool active = k < te;
 
   Agg c;
@@ -446,25 +465,28 @@ __global__ __launch_bounds__(T) void k_agglomerate(
     if (kLeaves) {
       c.l = c.r = k;
       const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
-      const size_t src = 3 * (size_t)(leaf_base + (pi < (uint32_t)N ? pi : 0u));
+      const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
       const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
-      int4 *o = otri_all + 3 * (leaf_base + k);
+      int4 *o = otri_all + 3 * (size_t)(leaf_base + k);
       o[0] = a;
       o[1] = b;
       o[2] = d;
       tri_box(a, b, d, c.mn, c.mx);
       perturb(c.mn, c.mx);
     } else {
-      const int4 a = items_in[2 * (in_base + k)],
-                 b = items_in[2 * (in_base + k) + 1];
+      int s = 0;
+      while (s + 1 < kFanIn && (int)L.pre[s + 1] <= k) ++s;
+      const size_t it =
+          (size_t)(src0 + s) * kFrontierMax + (uint32_t)(k - (int)L.pre[s]);
+      const int4 a = items_in[2 * it], b = items_in[2 * it + 1];
       c.l = a.x; c.r = a.y;
       c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
       c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
       c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
     }
-    L.il[tid] = (int)c.l;
-    L.ir[tid] = (int)c.r;
-    L.dgap[tid + 1] = (c.r + 1 < N)
+    L.il[tid] = c.l;
+    L.ir[tid] = c.r;
+    L.dgap[tid + 1] = ((uint32_t)c.r + 1 < N)
         ? delta_next(skeys[c.r], skeys[c.r + 1], (uint32_t)c.r) : -1;
     if (tid == 0)
       L.dgap[0] = (c.l > 0)
@@ -473,97 +495,36 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   __syncthreads();
   if (kLeaves) {
     if (N == 1) {
-      if (k == 0) write_single_leaf_node(nodes, c.mn, c.mx);
+      if (k == 0) {
+        write_single_leaf_node(nodes, c.mn, c.mx);
+        count_out[gtile] = 0u;
+      }
       return;
     }
-    if (k == N - 1) {
+    if ((uint32_t)k == N - 1) {
       const int4 z = make_int4(0, 0, 0, 0);
-      int4 *last = nodes + 4 * (N - 1);
+      int4 *last = nodes + 4 * (size_t)(N - 1);
       last[0] = z; last[1] = z; last[2] = z; last[3] = z;
     }
   }
-  if (active) agglomerate<T>(c, L, N, ts, te, nodes);
+  if (active) agglomerate<T>(c, L, (int)N, ts, te, nodes);
   __syncthreads();
 
   // Frontier: off-tile parents, and first arrivals whose sibling never came.
   const bool out = active && (c.stop == 2 ||
                               (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
   const uint32_t rank = block_exclusive_scan(out ? 1u : 0u, L.wsum);
-  __shared__ uint32_t s_count;
-  if (tid == T - 1) s_count = rank + (out ? 1u : 0u);
-  __syncthreads();
-  const uint32_t count = s_count;
-  if (tid < 64) {
-    // Look-back by the first wave: lane i probes tile j - i, so one round
-    // trip covers 64 predecessors.  Lanes [0, take) are summed, where take
-    // ends at the first unpublished word or just after the first inclusive
-    // one.  The segment's first tile publishes an inclusive word without
-    // waiting, so the walk ends; the spin cap only bounds it should that
-    // invariant break (ctl->fault reports it, and the kernel still exits).
-    const int lane = (int)tid;
-    uint64_t *mine = lstatus + gtile;
-    if (lane == 0)
-      __hip_atomic_store(mine,
-                         ((uint64_t)((stamp << 2) | (tile == 0 ? kLbInclusive
-                                                               : kLbAggregate))
-                          << 32) | count,
-                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t excl = 0;
-    int64_t j = gtile - 1;
-    uint32_t spins = 0;
-    while (j >= tile0) {
-      const int64_t q = j - lane;
-      const uint64_t w = (q >= tile0)
-          ? __hip_atomic_load(lstatus + q, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT)
-          : 0ull;
-      const uint32_t hi = (uint32_t)(w >> 32);
-      const bool pub = (q >= tile0) && (hi >> 2) == stamp;
-      const uint64_t stop = __ballot(!pub);
-      const uint64_t inc = __ballot(pub && (hi & 3u) == kLbInclusive);
-      const int u = stop ? __ffsll((unsigned long long)stop) - 1 : 64;
-      const uint64_t below_u = (u == 64) ? ~0ull : ((1ull << u) - 1ull);
-      const uint64_t inc_u = inc & below_u;
-      const int take = inc_u ? __ffsll((unsigned long long)inc_u) : u;
-      uint32_t part = (lane < take) ? (uint32_t)w : 0u;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-      excl += part;
-      if (inc_u) break;
-      j -= take;
-      if (take == 0) {
-        if (++spins > kSpinCap) {
-          if (lane == 0) atomicOr(&ctl->fault, 4u);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    if (lane == 0) {
-      if (tile > 0)
-        __hip_atomic_store(mine,
-                           ((uint64_t)((stamp << 2) | kLbInclusive) << 32) |
-                               (excl + count),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (te == M) *count_slot = excl + count;
-      if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
-      L.base = excl;
-    }
+  if (tid == T - 1) {
+    const uint32_t count = rank + (out ? 1u : 0u);
+    count_out[gtile] = count;
+    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
   }
-  __syncthreads();
-  if (out) {
-    const uint32_t dst = L.base + rank;
-    if (dst < out_cap) {
-      int4 *io = items_out + 2 * (out_base + dst);
-      io[0] =
-          make_int4((int)c.l, (int)c.r, __float_as_int(c.mn[0]),
-                    __float_as_int(c.mn[1]));
-      io[1] =
-          make_int4(__float_as_int(c.mn[2]), __float_as_int(c.mx[0]),
-                    __float_as_int(c.mx[1]), __float_as_int(c.mx[2]));
-    } else {
-      atomicOr(&ctl->fault, 8u);
-    }
+  if (out && rank < (uint32_t)kFrontierMax) {
+    int4 *io = items_out + 2 * ((size_t)gtile * kFrontierMax + rank);
+    io[0] = make_int4(c.l, c.r, __float_as_int(c.mn[0]),
+                      __float_as_int(c.mn[1]));
+    io[1] = make_int4(__float_as_int(c.mn[2]), __float_as_int(c.mx[0]),
+                      __float_as_int(c.mx[1]), __float_as_int(c.mx[2]));
   }
 }
 
@@ -600,110 +561,85 @@ void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                      reinterpret_cast<int4 *>(out));
 }
 
-// Level plan: level 0 over n leaves in ceil(n/T) tiles; level L+1 over at
-// most min(items of L, kFrontierMax * tiles of L) items, until one tile.
-int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap) {
-  uint64_t m = n;
-  uint64_t tiles = (m + tile - 1) / tile;
+// Level plan: ceil(n / tile) leaf tiles, then ceil(previous / kFanIn) item
+// tiles per level until one tile remains.
+int tree_levels(uint32_t n, int tile, uint32_t *grid) {
+  uint64_t tiles = ((uint64_t)n + tile - 1) / tile;
   int levels = 0;
-  grid[levels] = (uint32_t)(tiles ? tiles : 1);
-  cap[levels] = 0;
-  ++levels;
+  grid[levels++] = (uint32_t)(tiles ? tiles : 1);
   while (tiles > 1 && levels < kMaxTreeLevels) {
-    const uint64_t bound = m < tiles * kFrontierMax ? m : tiles * kFrontierMax;
-    cap[levels - 1] = (uint32_t)bound;  // items written by the level below
-    m = bound;
-    tiles = (m + tile - 1) / tile;
-    grid[levels] = (uint32_t)tiles;
-    cap[levels] = 0;
-    ++levels;
+    tiles = (tiles + kFanIn - 1) / kFanIn;
+    grid[levels++] = (uint32_t)tiles;
   }
   return levels;
 }
 
-template <int T>
-static void launch_levels(const int4 *t4, const uint32_t *skeys,
-                          const int32_t *perm, uint32_t n, int4 *o4, int4 *n4,
-                          int4 *items_a, int4 *items_b, uint64_t *lstatus,
-                          Ctl *ctl, hipStream_t s) {
-  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
-  const int levels = tree_levels(n, T, grid, cap);
-  uint64_t *st = lstatus;
-  hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid[0]), dim3(T), 0, s,
-                     t4, skeys, perm, n, o4, (const int4 *)nullptr, items_a,
-                     cap[0], 0u, st, n4, ctl, TreeSegs{});
-  st += grid[0];
-  for (int L = 1; L < levels; ++L) {
-    const int4 *in = (L & 1) ? items_a : items_b;
-    int4 *out = (L & 1) ? items_b : items_a;
-    hipLaunchKernelGGL((k_agglomerate<T, false>), dim3(grid[L]), dim3(T), 0,
-                       s, (const int4 *)nullptr, skeys, perm, n,
-                       (int4 *)nullptr, in, out, cap[L], (uint32_t)L, st, n4,
-                       ctl, TreeSegs{});
-    st += grid[L];
-  }
+uint64_t tree_tiles_total(uint32_t n, int tile) {
+  uint32_t grid[kMaxTreeLevels];
+  const int levels = tree_levels(n, tile, grid);
+  uint64_t t = 0;
+  for (int L = 0; L < levels; ++L) t += grid[L];
+  return t;
 }
 
 template <int T>
-static void launch_level_t(int level, uint32_t grid, const int4 *t4,
-                           const uint32_t *skeys, const int32_t *perm,
-                           uint32_t n, int4 *o4, const int4 *in, int4 *out,
-                           uint32_t cap, uint64_t *st, int4 *n4, Ctl *ctl,
-                           const TreeSegs &segs, hipStream_t s) {
-  if (level == 0)
-    hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid), dim3(T), 0, s, t4,
-                       skeys, perm, n, o4, in, out, cap, 0u, st, n4, ctl,
-                       segs);
-  else
-    hipLaunchKernelGGL((k_agglomerate<T, false>), dim3(grid), dim3(T), 0, s,
-                       t4, skeys, perm, n, o4, in, out, cap, (uint32_t)level,
-                       st, n4, ctl, segs);
+static void launch_leaf_level(uint32_t grid, const int4 *t4,
+                              const uint32_t *skeys, const int32_t *perm,
+                              uint32_t n, int4 *o4, int4 *items_out,
+                              uint32_t *count_out, int4 *n4, Ctl *ctl,
+                              const TreeSegs &segs, hipStream_t s) {
+  hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid), dim3(T), 0, s, t4,
+                     skeys, perm, n, o4, (const int4 *)nullptr,
+                     (const uint32_t *)nullptr, 0u, items_out, count_out, n4,
+                     ctl, segs);
 }
 
 void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
                        const uint32_t *skeys, const int32_t *perm, uint32_t n,
-                       float4 *otri, const int4 *items_in, int4 *items_out,
-                       uint32_t out_cap, uint64_t *lstatus, float4 *nodes,
+                       float4 *otri, const int4 *items_in,
+                       const uint32_t *count_in, uint32_t prev_tiles,
+                       int4 *items_out, uint32_t *count_out, float4 *nodes,
                        Ctl *ctl, const TreeSegs &segs, hipStream_t s) {
   const int4 *t4 = reinterpret_cast<const int4 *>(tri);
   int4 *o4 = reinterpret_cast<int4 *>(otri);
   int4 *n4 = reinterpret_cast<int4 *>(nodes);
+  if (level > 0) {
+    hipLaunchKernelGGL((k_agglomerate<kItemTile, false>), dim3(grid),
+                       dim3(kItemTile), 0, s, t4, skeys, perm, n, o4, items_in,
+                       count_in, prev_tiles, items_out, count_out, n4, ctl,
+                       segs);
+    return;
+  }
   switch (tile) {
     case 256:
-      launch_level_t<256>(level, grid, t4, skeys, perm, n, o4, items_in,
-                          items_out, out_cap, lstatus, n4, ctl, segs, s);
+      launch_leaf_level<256>(grid, t4, skeys, perm, n, o4, items_out,
+                             count_out, n4, ctl, segs, s);
       break;
     case 1024:
-      launch_level_t<1024>(level, grid, t4, skeys, perm, n, o4, items_in,
-                           items_out, out_cap, lstatus, n4, ctl, segs, s);
+      launch_leaf_level<1024>(grid, t4, skeys, perm, n, o4, items_out,
+                              count_out, n4, ctl, segs, s);
       break;
     default:
-      launch_level_t<512>(level, grid, t4, skeys, perm, n, o4, items_in,
-                          items_out, out_cap, lstatus, n4, ctl, segs, s);
+      launch_leaf_level<512>(grid, t4, skeys, perm, n, o4, items_out,
+                             count_out, n4, ctl, segs, s);
       break;
   }
 }
 
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, int4 *items_a, int4 *items_b,
-                       uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s) {
-  const int4 *t4 = reinterpret_cast<const int4 *>(tri);
-  int4 *o4 = reinterpret_cast<int4 *>(otri);
-  int4 *n4 = reinterpret_cast<int4 *>(nodes);
-  switch (tile) {
-    case 256:
-      launch_levels<256>(t4, skeys, perm, n, o4, n4, items_a, items_b,
-                         lstatus, ctl, s);
-      break;
-    case 1024:
-      launch_levels<1024>(t4, skeys, perm, n, o4, n4, items_a, items_b,
-                          lstatus, ctl, s);
-      break;
-    default:
-      launch_levels<512>(t4, skeys, perm, n, o4, n4, items_a, items_b,
-                         lstatus, ctl, s);
-      break;
+                       float4 *nodes, int4 *items, uint32_t *counts, Ctl *ctl,
+                       int tile, hipStream_t s) {
+  uint32_t grid[kMaxTreeLevels];
+  const int levels = tree_levels(n, tile, grid);
+  uint64_t base = 0, prev = 0;
+  for (int L = 0; L < levels; ++L) {
+    launch_tree_level(L, tile, grid[L], tri, skeys, perm, n, otri,
+                      items + 2 * kFrontierMax * prev, counts + prev,
+                      L ? grid[L - 1] : 0u, items + 2 * kFrontierMax * base,
+                      counts + base, nodes, ctl, TreeSegs{}, s);
+    prev = base;
+    base += grid[L];
   }
 }
 

@@ -49,25 +49,23 @@ struct lbvh_ctx {
   DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
   DevBuf digits;                        // sort tile counts + offsets
   DevBuf gside;                         // per-stage up-pass exchange
-  DevBuf items_a, items_b;              // tree agglomeration levels
-  DevBuf lstatus;                       // their look-back words (zeroed)
+  DevBuf tree_items, tree_counts;       // agglomeration item slots, counts
   DevBuf ctag;                          // per-stage up-pass tags (zeroed)
   DevBuf partials;                      // bounds partials
   // batched builds: scratch, and the plan (tile tables + captured graph) of
   // the last batch layout
   DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_hist, b_ord,
-      b_bounds, b_items_a, b_items_b, b_lstatus, b_table;
+      b_bounds, b_tree_items, b_tree_counts, b_table;
   struct BatchPlan {
     std::vector<uint32_t> offsets;
     int tile = 0;
     uint32_t S = 0, total = 0, sort_tiles = 0;
     int levels = 0;
-    std::vector<uint32_t> level_tiles, level_cap;
+    std::vector<uint32_t> level_tiles;
     // u32 offsets of the tables inside b_table
     uint64_t o_seg_off = 0, o_tile_seg = 0, o_seg_tile0 = 0;
-    std::vector<uint64_t> o_ltile_seg, o_lseg_tile0, o_lin, o_lout, o_lcap,
-        o_lcount;
-    uint64_t items_cap = 0, lstatus_words = 0;
+    std::vector<uint64_t> o_ltile_seg, o_lseg_tile0, o_lprev0, o_lprev_n;
+    uint64_t tree_tiles = 0;  // all levels
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<const void *> graph_key;
@@ -163,14 +161,14 @@ int ensure(lbvh_ctx *ctx, DevBuf &b, uint64_t bytes, bool zero) {
   return LBVH_OK;
 }
 
-// Epoch stamps are 28 bits wide in the look-back words.  Long before they
-// wrap, every stamped buffer is cleared and the epoch restarts.
+// The per-stage up pass stamps its exchange words with the epoch.  Long
+// before it wraps, the stamped buffers are cleared and the epoch restarts.
 constexpr uint64_t kEpochRewind = 1ull << 26;
 
 int begin_op(lbvh_ctx *ctx) {
   if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->lstatus, &ctx->ctag, &ctx->b_lstatus})
+  for (DevBuf *b : {&ctx->ctag})
     if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
   HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
                               ctx->stream));
@@ -213,11 +211,11 @@ struct StageScope {
 
 std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
   return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
-          &ctx->digits,  &ctx->gside,    &ctx->items_a,  &ctx->items_b,
-          &ctx->lstatus, &ctx->ctag,     &ctx->partials, &ctx->b_keys_a,
+          &ctx->digits,  &ctx->gside,    &ctx->tree_items,
+          &ctx->tree_counts, &ctx->ctag, &ctx->partials, &ctx->b_keys_a,
           &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
-          &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_items_a,
-          &ctx->b_items_b, &ctx->b_lstatus, &ctx->b_table};
+          &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_tree_items,
+          &ctx->b_tree_counts, &ctx->b_table};
 }
 
 void drop_plan(lbvh_ctx *ctx) {
@@ -270,10 +268,10 @@ int tile_for(const lbvh_ctx *ctx, uint32_t n) {
 
 int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n, int tile) {
   int st;
-  const uint64_t items = 32ull * tree_item_capacity(n, tile);
-  if ((st = ensure(ctx, ctx->items_a, items, false))) return st;
-  if ((st = ensure(ctx, ctx->items_b, items, false))) return st;
-  return ensure(ctx, ctx->lstatus, 8ull * tree_status_words(n, tile), true);
+  const uint64_t tiles = tree_tiles_total(n, tile);
+  if ((st = ensure(ctx, ctx->tree_items, 32ull * kTreeFrontier * tiles, false)))
+    return st;
+  return ensure(ctx, ctx->tree_counts, 4ull * tiles, false);
 }
 
 int ensure_partials(lbvh_ctx *ctx) {
@@ -349,46 +347,40 @@ int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
   P.o_seg_off = put(P.offsets);
   std::vector<uint32_t> tile_seg, seg_tile0(S, 0);
   std::vector<int> levels_k(S, 0);
-  std::vector<std::vector<uint32_t>> grid_k(S), cap_k(S);
+  std::vector<std::vector<uint32_t>> grid_k(S);
   for (uint32_t k = 0; k < S; ++k) {
     const uint32_t n = offsets[k + 1] - offsets[k];
     seg_tile0[k] = (uint32_t)tile_seg.size();
     for (uint32_t j = 0; j < sort_tiles(n); ++j) tile_seg.push_back(k);
     if (n) {
       grid_k[k].resize(kMaxTreeLevels);
-      cap_k[k].resize(kMaxTreeLevels);
-      levels_k[k] = tree_levels(n, P.tile, grid_k[k].data(), cap_k[k].data());
+      levels_k[k] = tree_levels(n, P.tile, grid_k[k].data());
       P.levels = std::max(P.levels, levels_k[k]);
     }
   }
   P.sort_tiles = (uint32_t)tile_seg.size();
   P.o_tile_seg = put(tile_seg);
   P.o_seg_tile0 = put(seg_tile0);
-  std::vector<uint32_t> prev_out(S, 0);
+  // Tree levels: each mesh's tiles at level L are consecutive; a tile of
+  // level L >= 1 reads the frontiers of its mesh's level L-1 tiles.
+  std::vector<uint32_t> prev_tile0(S, 0), prev_tiles(S, 0);
   for (int L = 0; L < P.levels; ++L) {
-    std::vector<uint32_t> ltile_seg, lseg_tile0(S, 0), lout(S, 0), lcap(S, 0);
-    uint32_t out_total = 0;
+    std::vector<uint32_t> ltile_seg, lseg_tile0(S, 0), lprev_n(S, 0);
     for (uint32_t k = 0; k < S; ++k) {
       if (L >= levels_k[k]) continue;
       lseg_tile0[k] = (uint32_t)ltile_seg.size();
       for (uint32_t j = 0; j < grid_k[k][L]; ++j) ltile_seg.push_back(k);
-      lout[k] = out_total;
-      lcap[k] = cap_k[k][L];
-      out_total += cap_k[k][L];
+      lprev_n[k] = L ? grid_k[k][L - 1] : 0u;
     }
     P.level_tiles.push_back((uint32_t)ltile_seg.size());
-    P.level_cap.push_back(out_total);
     P.o_ltile_seg.push_back(put(ltile_seg));
     P.o_lseg_tile0.push_back(put(lseg_tile0));
-    P.o_lin.push_back(put(prev_out));
-    P.o_lout.push_back(put(lout));
-    P.o_lcap.push_back(put(lcap));
-    P.o_lcount.push_back(put(std::vector<uint32_t>(S, 0)));
-    prev_out = lout;
-    P.items_cap = std::max<uint64_t>(P.items_cap, out_total);
-    P.lstatus_words += ltile_seg.size();
+    P.o_lprev0.push_back(put(prev_tile0));
+    P.o_lprev_n.push_back(put(lprev_n));
+    prev_tile0 = lseg_tile0;
+    P.tree_tiles += ltile_seg.size();
   }
-  P.items_cap = std::max<uint64_t>(P.items_cap, 1);
+  P.tree_tiles = std::max<uint64_t>(P.tree_tiles, 1);
   int st;
   if ((st = ensure(ctx, ctx->b_table, 4ull * tab.size(), false))) return st;
   HIP_TRY(ctx, hipMemcpyAsync(ctx->b_table.ptr, tab.data(), 4ull * tab.size(),
@@ -403,7 +395,6 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
                    int32_t *perm) {
   const auto &P = ctx->plan;
   const uint32_t *T = static_cast<const uint32_t *>(ctx->b_table.ptr);
-  uint32_t *TW = static_cast<uint32_t *>(ctx->b_table.ptr);
   uint32_t *hist = static_cast<uint32_t *>(ctx->b_hist.ptr);
   uint32_t *ord = static_cast<uint32_t *>(ctx->b_ord.ptr);
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
@@ -433,23 +424,21 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
     launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
                      P.sort_tiles, cnt, off, p == 0, ctx->ctl, ctx->stream,
                      &ss);
-  int4 *ia = static_cast<int4 *>(ctx->b_items_a.ptr);
-  int4 *ib = static_cast<int4 *>(ctx->b_items_b.ptr);
-  uint64_t *ls = static_cast<uint64_t *>(ctx->b_lstatus.ptr);
+  int4 *items = static_cast<int4 *>(ctx->b_tree_items.ptr);
+  uint32_t *counts = static_cast<uint32_t *>(ctx->b_tree_counts.ptr);
+  uint64_t base = 0, prev = 0;
   for (int L = 0; L < P.levels; ++L) {
     if (P.level_tiles[L] == 0) break;
     const TreeSegs ts{T + P.o_ltile_seg[L], T + P.o_lseg_tile0[L],
-                      T + P.o_seg_off,      L ? T + P.o_lin[L] : nullptr,
-                      T + P.o_lout[L],      T + P.o_lcap[L],
-                      L ? T + P.o_lcount[L - 1] : nullptr,
-                      TW + P.o_lcount[L]};
-    const int4 *in = (L & 1) ? ia : ib;
-    int4 *outi = (L & 1) ? ib : ia;
+                      T + P.o_seg_off, T + P.o_lprev0[L], T + P.o_lprev_n[L]};
     launch_tree_level(L, P.tile, P.level_tiles[L], t4, out, vout, P.total,
-                      reinterpret_cast<float4 *>(tri_out), in, outi, 0u, ls,
+                      reinterpret_cast<float4 *>(tri_out),
+                      items + 2 * kTreeFrontier * prev, counts + prev, 0u,
+                      items + 2 * kTreeFrontier * base, counts + base,
                       static_cast<float4 *>(bvh_out), ctx->ctl, ts,
                       ctx->stream);
-    ls += P.level_tiles[L];
+    prev = base;
+    base += P.level_tiles[L];
   }
 }
 
@@ -880,9 +869,8 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
     StageScope sc(ctx, LBVH_STAGE_TREE);
     launch_tree_fused(t4, skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
                       static_cast<float4 *>(bvh_out),
-                      static_cast<int4 *>(ctx->items_a.ptr),
-                      static_cast<int4 *>(ctx->items_b.ptr),
-                      static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
+                      static_cast<int4 *>(ctx->tree_items.ptr),
+                      static_cast<uint32_t *>(ctx->tree_counts.ptr), ctx->ctl,
                       tile, ctx->stream);
   }
   return check_launch(ctx, "lbvh_build");
@@ -899,15 +887,12 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
-  if ((st = begin_op(ctx))) return st;
-  launch_bump_epoch(ctx->ctl, ctx->stream);
   StageScope sc(ctx, LBVH_STAGE_TREE);
   launch_tree_fused(reinterpret_cast<const float4 *>(tri), sorted_keys, perm, n,
                     reinterpret_cast<float4 *>(tri_out),
                     static_cast<float4 *>(bvh_out),
-                    static_cast<int4 *>(ctx->items_a.ptr),
-                    static_cast<int4 *>(ctx->items_b.ptr),
-                    static_cast<uint64_t *>(ctx->lstatus.ptr), ctx->ctl,
+                    static_cast<int4 *>(ctx->tree_items.ptr),
+                    static_cast<uint32_t *>(ctx->tree_counts.ptr), ctx->ctl,
                     tile, ctx->stream);
   return check_launch(ctx, "lbvh_refit");
 }
@@ -945,10 +930,10 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                    false)))
     return st;
   if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;
-  if ((st = ensure(ctx, ctx->b_items_a, 32ull * P.items_cap, false))) return st;
-  if ((st = ensure(ctx, ctx->b_items_b, 32ull * P.items_cap, false))) return st;
-  if ((st = ensure(ctx, ctx->b_lstatus, 8ull * std::max<uint64_t>(P.lstatus_words, 1),
-                   true)))
+  if ((st = ensure(ctx, ctx->b_tree_items, 32ull * kTreeFrontier * P.tree_tiles,
+                   false)))
+    return st;
+  if ((st = ensure(ctx, ctx->b_tree_counts, 4ull * P.tree_tiles, false)))
     return st;
   if (bounds_host) {
     if ((st = ensure(ctx, ctx->b_bounds, 24ull * num_meshes, false))) return st;

@@ -17,6 +17,7 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 
 constexpr int kTreeTileDefault = 1024;  // leaves per tile of the fused up pass
 constexpr int kMaxTreeLevels = 16;      // agglomeration levels (6 at 2^24, T=1024)
+constexpr int kTreeFrontier = 128;      // item slots per agglomeration tile
 
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
 constexpr int kBoundsBlocks = 1024;  // grid of the bounds reduction
@@ -29,13 +30,11 @@ constexpr uint32_t kSpinCap = 1u << 22;  // look-back polls before giving up
 // initial value by the end of the operation that uses it, so back-to-back
 // operations (and graph replays) need no host-side reset:
 //   epoch       bumped once per operation by its first kernel; stamps the
-//               look-back words and arrival tags of that operation
+//               arrival tags of the per-stage up pass
 //   bounds_done completion count of the bounds reduction (reset by the last
 //               block)
 //   fault       sticky flag: a bounded wait gave up or a frontier overflowed
 //               (never set in a correct run; read by lbvh_synchronize)
-//   agg_tile_ctr / level_count: tile ids (self-resetting) and item counts of
-//               the tree's agglomeration levels
 //   hist        global digit counts of a single-mesh sort (zeroed by the
 //               last pass's scatter)
 struct Ctl {
@@ -43,8 +42,6 @@ struct Ctl {
   uint32_t bounds_done;
   uint32_t fault;
   uint32_t pad0[5];
-  uint32_t agg_tile_ctr[kMaxTreeLevels];  // per-level dynamic tile ids
-  uint32_t level_count[kMaxTreeLevels];   // items produced by each level
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
   uint32_t hist[kSortPasses][kRadix];    // global digit counts per pass
 };
@@ -119,34 +116,36 @@ void launch_up_pass(const int32_t *idx, const float4 *aabb,
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                       float4 *out, hipStream_t s);
 // Segments of one agglomeration level of a batched build (all null: one
-// mesh).  Per tile: segment; per segment: first tile at this level, first
-// leaf (seg_off, S+1 entries), input item offset and count (levels >= 1),
-// output item offset, capacity and count.
+// mesh).  Per tile of the level: its segment; per segment: its first tile at
+// this level, its first leaf (seg_off, S+1 entries), and (levels >= 1) its
+// first tile and tile count at the level below.
 struct TreeSegs {
   const uint32_t *tile_seg;
   const uint32_t *seg_tile0;
   const uint32_t *seg_off;
-  const uint32_t *in_off;
-  const uint32_t *out_off;
-  const uint32_t *out_cap;
-  const uint32_t *count_in;
-  uint32_t *count_out;
+  const uint32_t *prev_tile0;
+  const uint32_t *prev_tiles;
 };
 
 // Fused leaf gather + radix tree + node boxes by multi-level agglomeration.
-// tile: 256, 512 or 1024 leaves per block.  items_a/items_b hold
-// tree_item_capacity(n, tile) 32-byte items each; lstatus holds
-// tree_status_words(n, tile) look-back words (zeroed once).
+// tile: 256, 512 or 1024 leaves per level-0 block.  Tiles are numbered level
+// after level; items holds kTreeFrontier 32-byte item slots per tile and
+// counts one word per tile (tree_tiles_total(n, tile) tiles).
 void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
                        const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, int4 *items_a, int4 *items_b,
-                       uint64_t *lstatus, Ctl *ctl, int tile, hipStream_t s);
-int tree_levels(uint32_t n, int tile, uint32_t *grid, uint32_t *cap);
-// One agglomeration level with explicit segments (batched builds).
+                       float4 *nodes, int4 *items, uint32_t *counts, Ctl *ctl,
+                       int tile, hipStream_t s);
+// Tiles of each level of an n-leaf build (returns the number of levels).
+int tree_levels(uint32_t n, int tile, uint32_t *grid);
+uint64_t tree_tiles_total(uint32_t n, int tile);
+// One agglomeration level (batched builds pass their segments).  items_in /
+// count_in: the level below's slots and counts, indexed by its tile number;
+// prev_tiles: its tile count (one mesh).
 void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
                        const uint32_t *skeys, const int32_t *perm, uint32_t n,
-                       float4 *otri, const int4 *items_in, int4 *items_out,
-                       uint32_t out_cap, uint64_t *lstatus, float4 *nodes,
+                       float4 *otri, const int4 *items_in,
+                       const uint32_t *count_in, uint32_t prev_tiles,
+                       int4 *items_out, uint32_t *count_out, float4 *nodes,
                        Ctl *ctl, const TreeSegs &segs, hipStream_t s);
 
 // k_batch.hip
@@ -164,19 +163,5 @@ void launch_batch_morton(const float4 *tri, const SortSegs &segs,
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,
                     const float *rays, uint32_t count, float *hits,
                     hipStream_t s);
-inline uint64_t tree_item_capacity(uint32_t n, int tile) {
-  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
-  const int levels = tree_levels(n, tile, grid, cap);
-  uint64_t m = 1;
-  for (int L = 0; L < levels; ++L) m = cap[L] > m ? cap[L] : m;
-  return m;
-}
-inline uint64_t tree_status_words(uint32_t n, int tile) {
-  uint32_t grid[kMaxTreeLevels], cap[kMaxTreeLevels];
-  const int levels = tree_levels(n, tile, grid, cap);
-  uint64_t w = 0;
-  for (int L = 0; L < levels; ++L) w += grid[L];
-  return w;
-}
 
 }  // namespace lbvh

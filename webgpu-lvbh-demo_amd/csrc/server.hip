@@ -38,10 +38,11 @@ double now_s() {
 // Bytes a resident mesh holds: triangles 48, BVH 64, O_TRI 48, sorted keys
 // 4, permutation 4 per triangle.
 constexpr uint64_t kMeshBytesPerTri = 168;
-// Scratch a slot may need to grow to for a build / refit (keys, indices,
-// look-back words, agglomeration items, with the arena's 25% headroom).
-constexpr uint64_t kBuildScratchPerTri = 40;
-constexpr uint64_t kRefitScratchPerTri = 16;
+// Scratch a slot may need to grow to for a build / refit, with the arena's
+// 25% headroom: keys and indices 16, sort tile counts 0.5, agglomeration
+// item slots up to 18.3 (128 x 32 B per 256-leaf tile, all levels).
+constexpr uint64_t kBuildScratchPerTri = 48;
+constexpr uint64_t kRefitScratchPerTri = 24;
 
 struct Mesh {
   uint32_t n = 0;
