@@ -155,7 +155,9 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
     [1024, 4194304]; 70% refits of a recently built resident mesh (vertices
     + N(0, 0.001)), 30% rebuilds of a new uniform mesh.  Meshes are
     synthesised on the device from their seeds.  The arrival rate is 0.7x
-    the capacity measured on a back-to-back burst."""
+    the capacity measured by running the same request plan back to back
+    (closed loop, so the budget fills and evictions happen as in the timed
+    run)."""
     rng = np.random.default_rng(5)
     sizes = lb.wl_log_uniform_sizes(6, requests, 1024, 4194304)
     is_refit = rng.random(requests) < 0.7
@@ -170,15 +172,14 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
             built.append((next_mesh, n))
             next_mesh += 1
     srv = lb.Server(0, budget, slots)
-    # capacity from a back-to-back burst (its own meshes, then evicted)
-    burst = 400
+    # capacity: the whole plan back to back on its own mesh ids (evicted
+    # afterwards)
     t0 = time.perf_counter()
-    for k in range(burst):
-        kind, m, src, n, seed, p = plan[k]
+    for kind, m, src, n, seed, p in plan:
         srv.submit(kind, 10 ** 9 + m, src, n, seed=seed, param=p)
     srv.drain()
     srv.poll()
-    capacity = burst / (time.perf_counter() - t0)
+    capacity = requests / (time.perf_counter() - t0)
     rate = 0.7 * capacity
     for m, _ in built:
         try:
@@ -207,7 +208,11 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
     st = srv.stats()
     srv.close()
     lat = np.array([c["done_s"] - c["submit_s"] for c in done]) * 1e3
+    kinds = np.array([c["kind"] for c in done])
     ok = np.array([c["status"] == 0 for c in done])
+
+    def pct(sel, q):
+        return round(float(np.percentile(lat[sel], q)), 3) if sel.any() else None
     tris = sum(c["n"] for c in done if c["status"] == 0)
     refits = sum(1 for c in done if c["kind"] == lb.REQ_REFIT)
     return {"requests": len(done), "ok": int(ok.sum()), "refits": refits,
@@ -216,6 +221,10 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
             "offered_req_s": round(rate, 1), "capacity_req_s": round(capacity, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "refit_p50_ms": pct(kinds == lb.REQ_REFIT, 50),
+            "refit_p99_ms": pct(kinds == lb.REQ_REFIT, 99),
+            "build_p50_ms": pct(kinds == lb.REQ_BUILD, 50),
+            "build_p99_ms": pct(kinds == lb.REQ_BUILD, 99),
             "throughput_req_s": round(len(done) / wall, 1),
             "throughput_Mtri_s": round(tris / wall / 1e6, 1),
             "evictions": int(st["evictions"]),

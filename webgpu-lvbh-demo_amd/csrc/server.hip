@@ -88,6 +88,13 @@ struct lbvh_server {
   uint64_t budget = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
   std::vector<Slot> slots;
+  // Mesh memory: a pool of its own that keeps freed memory for reuse (a
+  // pool that hands it back to the driver at every synchronisation makes
+  // each eviction + build map hundreds of MB anew), and an otherwise idle
+  // stream to free on, so a freed mesh is reusable at once by any slot
+  // instead of after the work queued on some busy slot's stream.
+  hipMemPool_t pool = nullptr;
+  hipStream_t free_stream = nullptr;
   std::mutex mu;
   std::condition_variable cv;       // wakes the dispatcher
   std::condition_variable idle_cv;  // wakes lbvh_server_drain
@@ -111,12 +118,13 @@ uint64_t slots_bytes(lbvh_server *s) {
 
 uint64_t in_use(lbvh_server *s) { return s->resident_bytes + slots_bytes(s); }
 
-// Mesh memory is stream-ordered (hipMallocAsync / hipFreeAsync): a plain
+// Mesh memory is stream-ordered (pool allocations, hipFreeAsync): a plain
 // hipFree would synchronise the whole device and stall every other slot.
-// A mesh is only freed while idle (its last request completed).
+// A mesh is only freed while idle (its last request completed), so freeing
+// it on the idle stream needs no ordering against the slots.
 void free_mesh(lbvh_server *s, Mesh &m) {
   if (m.buf) {
-    (void)hipFreeAsync(m.buf, static_cast<hipStream_t>(lbvh_stream(s->slots[0].ctx)));
+    (void)hipFreeAsync(m.buf, s->free_stream);
     s->resident_bytes -= kMeshBytesPerTri * m.n;
   }
   m.buf = nullptr;
@@ -194,9 +202,16 @@ int launch(lbvh_server *s, uint32_t k, Request &r) {
   if (r.kind == LBVH_REQ_BUILD && (m.buf == nullptr || m.n != r.n)) {
     free_mesh(s, m);
     void *p = nullptr;
-    if (hipMallocAsync(&p, kMeshBytesPerTri * r.n, stream) != hipSuccess) {
+    const uint64_t bytes = kMeshBytesPerTri * r.n;
+    if (hipMallocFromPoolAsync(&p, bytes, s->pool, stream) != hipSuccess) {
+      // the pool's reserve may be fragmented: return it and try once more
       (void)hipGetLastError();
-      return LBVH_ERR_OUT_OF_MEMORY;
+      (void)hipStreamSynchronize(s->free_stream);
+      (void)hipMemPoolTrimTo(s->pool, 0);
+      if (hipMallocFromPoolAsync(&p, bytes, s->pool, stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return LBVH_ERR_OUT_OF_MEMORY;
+      }
     }
     m.buf = p;
     m.n = r.n;
@@ -342,19 +357,38 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
   s->device = device_ordinal;
   s->budget = hbm_budget_bytes;
   s->slots.resize(slots);
+  int st = LBVH_OK;
   for (auto &sl : s->slots) {
-    int st = lbvh_create(device_ordinal, 0, &sl.ctx);
+    st = lbvh_create(device_ordinal, 0, &sl.ctx);
     if (st == LBVH_OK &&
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       st = LBVH_ERR_DEVICE;
-    if (st != LBVH_OK) {
-      for (auto &x : s->slots) {
-        if (x.done) (void)hipEventDestroy(x.done);
-        if (x.ctx) lbvh_destroy(x.ctx);
-      }
-      delete s;
-      return st;
+    if (st != LBVH_OK) break;
+  }
+  if (st == LBVH_OK) {
+    // lbvh_create made the device current on this thread
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device_ordinal;
+    uint64_t keep = UINT64_MAX;
+    if (hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipMemPoolCreate(&s->pool, &props) != hipSuccess ||
+        hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold,
+                               &keep) != hipSuccess)
+      st = LBVH_ERR_DEVICE;
+  }
+  if (st != LBVH_OK) {
+    (void)hipGetLastError();
+    if (s->pool) (void)hipMemPoolDestroy(s->pool);
+    if (s->free_stream) (void)hipStreamDestroy(s->free_stream);
+    for (auto &x : s->slots) {
+      if (x.done) (void)hipEventDestroy(x.done);
+      if (x.ctx) lbvh_destroy(x.ctx);
     }
+    delete s;
+    return st;
   }
   s->thr = std::thread(dispatcher, s);
   *out = s;
@@ -373,7 +407,11 @@ int lbvh_server_destroy(lbvh_server *s) {
     lbvh_synchronize(sl.ctx);
     (void)hipEventDestroy(sl.done);
   }
+  (void)hipSetDevice(s->device);
   for (auto &kv : s->meshes) free_mesh(s, kv.second);
+  (void)hipStreamSynchronize(s->free_stream);
+  (void)hipMemPoolDestroy(s->pool);
+  (void)hipStreamDestroy(s->free_stream);
   for (auto &sl : s->slots) lbvh_destroy(sl.ctx);
   delete s;
   return LBVH_OK;
