@@ -61,9 +61,11 @@ __device__ void block_minmax(float v[6]) {
 }
 
 // Scene bounds: fp32 min/max over all vertex coordinates.  Each block
-// reduces a grid-stride share into partials[2*b, 2*b+1]; the last block to
-// finish (arrival counter, agent-scope acq_rel) folds the partials into
-// ctl->bounds and resets the counter for the next operation.
+// reduces a grid-stride share into partials[2*b, 2*b+1]; k_bounds_final (one
+// block, launched next) folds the partials into ctl->bounds.  No atomics and
+// no fences: a last-block-finishes counter costs one same-address
+// device-scope atomic (with a cache writeback for its release) per block,
+// which on MI355X serialises across the XCDs.
 __global__ __launch_bounds__(kMapThreads) void k_bounds(
     const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
     Ctl *ctl, int bump_epoch) {
@@ -81,20 +83,16 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
     v[5] = fmaxf(v[5], fmaxf(a.z, fmaxf(b.z, c.z)));
   }
   block_minmax(v);
-  __shared__ int s_last;
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = make_float4(v[0], v[1], v[2], 0.0f);
     partials[2 * blockIdx.x + 1] = make_float4(v[3], v[4], v[5], 0.0f);
-    uint32_t done = __hip_atomic_fetch_add(&ctl->bounds_done, 1u,
-                                           __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (done == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__global__ __launch_bounds__(kMapThreads) void k_bounds_final(
+    const float4 *__restrict__ partials, uint32_t blocks, Ctl *ctl) {
   float w[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
     const float4 lo = partials[2 * b], hi = partials[2 * b + 1];
     w[0] = fminf(w[0], lo.x); w[1] = fminf(w[1], lo.y); w[2] = fminf(w[2], lo.z);
     w[3] = fmaxf(w[3], hi.x); w[4] = fmaxf(w[4], hi.y); w[5] = fmaxf(w[5], hi.z);
@@ -105,8 +103,6 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
     ctl->bounds[3] = 0.0f;
     ctl->bounds[4] = w[3]; ctl->bounds[5] = w[4]; ctl->bounds[6] = w[5];
     ctl->bounds[7] = 0.0f;
-    __hip_atomic_store(&ctl->bounds_done, 0u, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -116,19 +112,20 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
 //   rel  = (cen - bmin) / (bmax - bmin)               (:192, per div mode)
 //   key  = morton(u32(rel * 1023.99f))                (:194-208)
 // Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
-// fused build, the four global digit histograms of the keys plus the first
-// sort pass's per-tile digit counts (saving the sort a read of the keys).
-// With cnt0 the grid is one block per sort tile (XCD-aware ids, as the
-// sort's), without it a grid-stride loop.
+// fused build, the sort's per-tile digit counts of all four passes,
+// cnt[(p * kRadix + d) * tiles + tile] (saving the sort a read of the keys;
+// pass 0 uses them as they are, the others only through their column sums,
+// the global histograms).  With cnt the grid is one block per sort tile
+// (XCD-aware ids, as the sort's), without it a grid-stride loop.
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
-    uint32_t *__restrict__ keys, Ctl *ctl, int histogram, int bump_epoch,
-    uint32_t *__restrict__ cnt0) {
-  static_assert(kMapThreads == kRadix, "one thread per digit for cnt0");
+    uint32_t *__restrict__ keys, Ctl *ctl, int bump_epoch,
+    uint32_t *__restrict__ cnt) {
+  static_assert(kMapThreads == kRadix, "one thread per digit for cnt");
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
-  if (histogram) {
+  if (cnt) {
     for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
       (&s_hist[0][0])[k] = 0u;
     __syncthreads();
@@ -152,7 +149,7 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
   const bool ieee = (p.div_mode == 1);
   uint32_t first = blockIdx.x * blockDim.x, end = n,
            stride = gridDim.x * blockDim.x, tile = 0;
-  if (cnt0) {
+  if (cnt) {
     tile = xcd_tile(blockIdx.x, gridDim.x);
     first = tile * (uint32_t)kSortTile;
     end = min(first + (uint32_t)kSortTile, n);
@@ -179,15 +176,14 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
       aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
       aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
     }
-    if (histogram) hist_add(s_hist, key);
+    if (cnt) hist_add(s_hist, key);
   }
-  if (histogram) {
+  if (cnt) {
     __syncthreads();
-    for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x) {
-      const uint32_t v = (&s_hist[0][0])[k];
-      if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
-    }
-    if (cnt0) cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[0][threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < kSortPasses; ++q)
+      cnt[((size_t)q * kRadix + threadIdx.x) * gridDim.x + tile] =
+          s_hist[q][threadIdx.x];
   }
 }
 
@@ -206,22 +202,21 @@ void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
                      partials, ctl, bump_epoch ? 1 : 0);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kMapThreads), 0, s,
+                     partials, blocks, ctl);
 }
 
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
-                        Ctl *ctl, bool histogram, bool bump_epoch,
-                        hipStream_t s, uint32_t *cnt0) {
+                        Ctl *ctl, bool bump_epoch, hipStream_t s,
+                        uint32_t *cnt) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
-  if (cnt0) {
-    histogram = true;
-    blocks = sort_tiles(n);
-  }
+  if (cnt) blocks = sort_tiles(n);
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_aabb_morton, dim3(blocks), dim3(kMapThreads), 0, s,
                      tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
-                     histogram ? 1 : 0, bump_epoch ? 1 : 0, cnt0);
+                     bump_epoch ? 1 : 0, cnt);
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

@@ -20,8 +20,11 @@
 // coherent cross-XCD load; with thousands of tiles the chain, not HBM, set
 // the pass time.  Kernel boundaries are the only global synchronisation.
 //
-// Counts and offsets are digit-major: entry (d, t) at d * tiles + t, so the
-// scan reads and writes columns contiguously.  Tile ids are XCD-aware
+// The global digit histograms (the digit bases of a single mesh) are the
+// column sums of per-tile counts of all four passes, which the Morton kernel
+// writes as it produces the keys; no device-scope atomics anywhere.  Counts
+// and offsets are digit-major: entry (d, t) at d * tiles + t, so the scan
+// reads and writes columns contiguously.  Tile ids are XCD-aware
 // (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
 // neighbouring tiles share -- count columns and the boundaries of their digit
 // runs in the output -- are merged in one L2.
@@ -33,22 +36,38 @@ constexpr int kWaves = kSortThreads / 64;
 static_assert(kSortThreads == kRadix, "one thread per digit");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 
-// Global digit histograms of the stand-alone sort (bumps the epoch).
-__global__ __launch_bounds__(kMapThreads) void k_histogram(
-    const uint32_t *__restrict__ keys, uint32_t n, Ctl *ctl) {
+// Per-tile digit counts of every pass (the stand-alone sort; the build's
+// Morton kernel writes the same counts): cnt[(p * kRadix + d) * tiles + t].
+__global__ __launch_bounds__(kSortThreads) void k_tile_hist(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ cnt) {
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
+  const uint32_t tiles = gridDim.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, tiles);
   for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
     (&s_hist[0][0])[k] = 0u;
   __syncthreads();
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+  const uint32_t end = min((tile + 1) * (uint32_t)kSortTile, n);
+  for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += kSortThreads)
     hist_add(s_hist, keys[i]);
   __syncthreads();
-  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x) {
-    const uint32_t v = (&s_hist[0][0])[k];
-    if (v) atomicAdd(&(&ctl->hist[0][0])[k], v);
-  }
+#pragma unroll
+  for (int q = 0; q < kSortPasses; ++q)
+    cnt[((size_t)q * kRadix + threadIdx.x) * tiles + tile] =
+        s_hist[q][threadIdx.x];
+}
+
+// Global digit histograms as column sums of the per-tile counts: block
+// (p, d) sums column p * kRadix + d.  Plain stores: ctl->hist is rewritten by
+// every single-mesh sort, and no device-scope atomics are involved.
+__global__ __launch_bounds__(kSortThreads) void k_digit_totals(
+    const uint32_t *__restrict__ cnt, uint32_t tiles, Ctl *ctl) {
+  __shared__ uint32_t s_wsum[16];
+  const uint32_t *col = cnt + (size_t)blockIdx.x * tiles;
+  uint32_t sum = 0;
+  for (uint32_t t = threadIdx.x; t < tiles; t += kSortThreads) sum += col[t];
+  const uint32_t ex = block_exclusive_scan(sum, s_wsum);
+  if (threadIdx.x == kSortThreads - 1)
+    (&ctl->hist[0][0])[blockIdx.x] = ex + sum;
 }
 
 // Per-tile digit counts of one pass -> cnt[d * tiles + tile].
@@ -166,14 +185,12 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_bases(
 
 // One digit pass.  kFirst: the payload is the position in the segment (the
 // reference's init_idx, radix-sort.js:527-534, folded into the first pass).
-// clear_hist: the last pass of a single-mesh sort zeroes the global digit
-// histograms for the next operation (every scan has read them).
 template <bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
-    SortSegs segs, Ctl *ctl, int clear_hist) {
+    SortSegs segs) {
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[kWaves][kRadix];
@@ -184,9 +201,6 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  if (clear_hist && tile == 0)
-    for (int k = tid; k < kSortPasses * kRadix; k += kSortThreads)
-      (&ctl->hist[0][0])[k] = 0u;
   for (int k = tid; k < kWaves * kRadix; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
   const uint32_t goff = off[(size_t)tid * tiles + tile];
@@ -275,13 +289,19 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 }
 
-void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
+void launch_tile_hist(const uint32_t *keys, uint32_t n, uint32_t *cnt,
                       hipStream_t s) {
-  uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
-  if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_histogram, dim3(blocks), dim3(kMapThreads), 0, s, keys,
-                     n, ctl);
+  const uint32_t tiles = sort_tiles(n);
+  if (tiles == 0) return;
+  hipLaunchKernelGGL(k_tile_hist, dim3(tiles), dim3(kSortThreads), 0, s, keys,
+                     n, cnt);
+}
+
+void launch_digit_totals(const uint32_t *cnt, uint32_t tiles, Ctl *ctl,
+                         hipStream_t s) {
+  if (tiles == 0) return;
+  hipLaunchKernelGGL(k_digit_totals, dim3(kSortPasses * kRadix),
+                     dim3(kSortThreads), 0, s, cnt, tiles, ctl);
 }
 
 void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
@@ -307,15 +327,14 @@ void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                             : nullptr;
   hipLaunchKernelGGL(k_scan_columns, dim3(kRadix), dim3(kSortThreads), 0, s,
                      cnt, off, tiles, ctl->hist[pass], sg, seg_base);
-  const int clear = (!segs && pass == kSortPasses - 1) ? 1 : 0;
   if (pass == 0)
     hipLaunchKernelGGL(k_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, s,
                        keys_in, (const int32_t *)nullptr, keys_out, vals_out,
-                       n, shift, off, sg, ctl, clear);
+                       n, shift, off, sg);
   else
     hipLaunchKernelGGL(k_scatter<false>, dim3(tiles), dim3(kSortThreads), 0, s,
-                       keys_in, vals_in, keys_out, vals_out, n, shift, off, sg,
-                       ctl, clear);
+                       keys_in, vals_in, keys_out, vals_out, n, shift, off,
+                       sg);
 }
 
 }  // namespace lbvh

@@ -244,8 +244,9 @@ int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
   if ((st = ensure(ctx, ctx->keys_b, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_a, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
-  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix;
-  return ensure(ctx, ctx->digits, 8ull * words, false);
+  // per-tile counts of all passes, then the offsets of one pass
+  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix * (kSortPasses + 1);
+  return ensure(ctx, ctx->digits, 4ull * words, false);
 }
 
 int bucket_of(uint32_t n) {
@@ -279,14 +280,16 @@ int ensure_partials(lbvh_ctx *ctx) {
 }
 
 // The four digit passes: in -> tmp -> out -> tmp -> out.  `in` may equal
-// `out` (the stand-alone sort is in place, like the reference's).  counted0:
-// the Morton kernel already wrote pass 0's per-tile counts into the digit
-// buffer.
+// `out` (the stand-alone sort is in place, like the reference's).  counted:
+// the Morton kernel already wrote the per-tile counts of every pass into the
+// digit buffer.  Passes 1-3 recount into pass 0's columns.
 void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted0) {
+              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted) {
   const uint32_t tiles = sort_tiles(n);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
-  uint32_t *off = cnt + (size_t)kRadix * tiles;
+  uint32_t *off = cnt + (size_t)kRadix * kSortPasses * tiles;
+  if (!counted) launch_tile_hist(in, n, cnt, ctx->stream);
+  launch_digit_totals(cnt, tiles, ctx->ctl, ctx->stream);
   const uint32_t *src[kSortPasses] = {in, tmp, out, tmp};
   uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
   const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
@@ -294,7 +297,7 @@ void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
   for (int p = 0; p < kSortPasses; ++p) {
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
     launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, tiles, cnt, off,
-                     p == 0 && counted0, ctx->ctl, ctx->stream);
+                     p == 0, ctx->ctl, ctx->stream);
   }
 }
 
@@ -751,7 +754,7 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
   launch_aabb_morton(reinterpret_cast<const float4 *>(tri), n, p,
                      bounds_host == nullptr,
                      reinterpret_cast<float4 *>(aabb_out), keys_out, ctx->ctl,
-                     false, bounds_host != nullptr, ctx->stream);
+                     bounds_host != nullptr, ctx->stream);
   return check_launch(ctx, "lbvh_aabb_morton");
 }
 
@@ -768,7 +771,6 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
   }
   if ((st = ensure_sort_scratch(ctx, n))) return st;
   if ((st = begin_op(ctx))) return st;
-  launch_histogram(keys, n, ctx->ctl, ctx->stream);
   run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
            static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false);
   return check_launch(ctx, "lbvh_sort");
@@ -852,7 +854,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   {
     StageScope sc(ctx, LBVH_STAGE_MORTON);
     launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                       ctx->ctl, sort, bounds_host != nullptr, ctx->stream,
+                       ctx->ctl, bounds_host != nullptr, ctx->stream,
                        sort ? static_cast<uint32_t *>(ctx->digits.ptr)
                             : nullptr);
   }

@@ -20,28 +20,24 @@ constexpr int kMaxTreeLevels = 16;      // agglomeration levels (6 at 2^24, T=10
 constexpr int kTreeFrontier = 128;      // item slots per agglomeration tile
 
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
-constexpr int kBoundsBlocks = 1024;  // grid of the bounds reduction
-constexpr int kMortonBlocksMax = 2048;  // grid-stride Morton + histogram
-constexpr uint32_t kSpinCap = 1u << 22;  // look-back polls before giving up
+constexpr int kBoundsBlocks = 2048;  // grid of the bounds reduction
+constexpr int kMortonBlocksMax = 2048;  // grid-stride Morton (per-stage entry)
+constexpr uint32_t kSpinCap = 1u << 22;  // polls of a bounded wait (per-stage up pass)
 
 // ---- device control block ------------------------------------------------
 //
-// One per context, zeroed at allocation.  Every counter in it returns to its
-// initial value by the end of the operation that uses it, so back-to-back
-// operations (and graph replays) need no host-side reset:
+// One per context, zeroed at allocation.  Nothing in it needs a reset
+// between operations (or graph replays):
 //   epoch       bumped once per operation by its first kernel; stamps the
 //               arrival tags of the per-stage up pass
-//   bounds_done completion count of the bounds reduction (reset by the last
-//               block)
 //   fault       sticky flag: a bounded wait gave up or a frontier overflowed
 //               (never set in a correct run; read by lbvh_synchronize)
-//   hist        global digit counts of a single-mesh sort (zeroed by the
-//               last pass's scatter)
+//   bounds      scene bounds of the last reduction (k_bounds_final)
+//   hist        global digit counts of the current single-mesh sort
 struct Ctl {
   uint32_t epoch;
-  uint32_t bounds_done;
   uint32_t fault;
-  uint32_t pad0[5];
+  uint32_t pad0[6];
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
   uint32_t hist[kSortPasses][kRadix];    // global digit counts per pass
 };
@@ -66,18 +62,23 @@ struct MortonParams {
 // k_bounds_morton.hip
 void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
                    bool bump_epoch, hipStream_t s);
-// cnt0 (kRadix x sort_tiles(n) words, or null): also write the first sort
-// pass's per-tile digit counts (implies histogram).
+// cnt (kSortPasses x kRadix x sort_tiles(n) words, or null): also write the
+// sort's per-tile digit counts of every pass (see launch_tile_hist).
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
-                        Ctl *ctl, bool histogram, bool bump_epoch,
-                        hipStream_t s, uint32_t *cnt0 = nullptr);
+                        Ctl *ctl, bool bump_epoch, hipStream_t s,
+                        uint32_t *cnt = nullptr);
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
 void launch_bump_epoch(Ctl *ctl, hipStream_t s);
 
 // k_sort.hip
-void launch_histogram(const uint32_t *keys, uint32_t n, Ctl *ctl,
+// Per-tile digit counts of every pass, cnt[(p * kRadix + d) * tiles + t]
+// (sort_tiles(n) tiles), and their column sums, the global digit histograms
+// ctl->hist (rewritten by each single-mesh sort).
+void launch_tile_hist(const uint32_t *keys, uint32_t n, uint32_t *cnt,
                       hipStream_t s);
+void launch_digit_totals(const uint32_t *cnt, uint32_t tiles, Ctl *ctl,
+                         hipStream_t s);
 // Segments of a batched sort (all null: one segment [0, n)).  Tiles of
 // kSortTile keys never straddle segments; tile_seg maps each tile to its
 // segment, seg_tile0 gives a segment's first tile, seg_off its first key
@@ -92,11 +93,11 @@ struct SortSegs {
 void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
                       uint32_t num_segs, hipStream_t s);
 // One digit pass over `tiles` sort tiles (pass 0 takes the position in the
-// segment as payload and ignores vals_in; the last pass of a single-mesh
-// sort clears the global histograms).  cnt and off hold kRadix x tiles words
-// each; counted: cnt already holds this pass's per-tile digit counts (the
-// Morton kernels write pass 0's).  The four passes sort keys_in into
-// keys_out through keys_tmp:
+// segment as payload and ignores vals_in).  cnt and off hold kRadix x tiles
+// words each; counted: cnt already holds this pass's per-tile digit counts
+// (the Morton kernels write pass 0's).  A single mesh takes its digit bases
+// from ctl->hist.  The four passes sort keys_in into keys_out through
+// keys_tmp:
 //   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
 void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                       uint32_t *keys_out, int32_t *vals_out, uint32_t n,

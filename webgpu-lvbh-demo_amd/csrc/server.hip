@@ -6,8 +6,11 @@
 // requests in arrival order as slots and HBM budget free up (requests on a
 // mesh that is busy, or that do not fit yet, wait without blocking refits
 // behind them), watches completion events, and evicts least recently used
-// idle meshes when a new build needs room.  All device work of a request is
-// enqueued on its slot's stream; the dispatcher never waits on the GPU.
+// idle meshes when a new build needs room.  Each slot has a worker thread
+// that enqueues its requests' device work on the slot's stream outside the
+// server lock, up to kSlotDepth requests ahead, so host-side launch costs
+// overlap across slots and with the GPU.  Nothing here waits on the GPU
+// except lbvh_server_drain.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -72,13 +75,28 @@ struct Request {
   std::vector<float> host;
 };
 
+// A request handed to a slot.  The dispatcher fills in the mesh buffers;
+// the slot's worker enqueues the device work and records `ev`.
+struct Work {
+  Request req;
+  Mesh mesh;          // copy of the mesh's buffer layout for the worker
+  hipEvent_t ev = nullptr;
+  double start_s = 0.0;
+  bool launched = false;
+};
+
+constexpr int kSlotDepth = 2;  // requests in flight per slot
+
 struct Slot {
   lbvh_ctx *ctx = nullptr;
-  hipEvent_t done = nullptr;
-  bool busy = false;
-  Request req;
-  double start_s = 0.0;
-  int status = LBVH_OK;
+  hipEvent_t events[kSlotDepth] = {};
+  uint32_t next_event = 0;
+  std::deque<Work> mailbox;   // assigned, not yet taken by the worker
+  std::deque<Work> inflight;  // taken by the worker, in stream order
+  std::condition_variable wcv;
+  std::thread worker;
+  bool idle() const { return mailbox.empty() && inflight.empty(); }
+  size_t depth() const { return mailbox.size() + inflight.size(); }
 };
 
 }  // namespace
@@ -87,7 +105,7 @@ struct lbvh_server {
   int device = 0;
   uint64_t budget = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
-  std::vector<Slot> slots;
+  std::deque<Slot> slots;  // stable addresses: workers hold references
   // Mesh memory: a pool of its own that keeps freed memory for reuse (a
   // pool that hands it back to the driver at every synchronisation makes
   // each eviction + build map hundreds of MB anew), and an otherwise idle
@@ -117,6 +135,12 @@ uint64_t slots_bytes(lbvh_server *s) {
 }
 
 uint64_t in_use(lbvh_server *s) { return s->resident_bytes + slots_bytes(s); }
+
+bool all_idle(lbvh_server *s) {
+  for (auto &sl : s->slots)
+    if (!sl.idle()) return false;
+  return true;
+}
 
 // Mesh memory is stream-ordered (pool allocations, hipFreeAsync): a plain
 // hipFree would synchronise the whole device and stall every other slot.
@@ -168,7 +192,7 @@ bool make_room(lbvh_server *s, uint64_t need, uint64_t keep) {
     }
     bool trimmed = false;
     for (auto &sl : s->slots)
-      if (!sl.busy && lbvh_bytes_in_use(sl.ctx) > sizeof(lbvh::Ctl)) {
+      if (sl.idle() && lbvh_bytes_in_use(sl.ctx) > sizeof(lbvh::Ctl)) {
         lbvh_trim(sl.ctx);
         trimmed = true;
       }
@@ -194,13 +218,24 @@ void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
   if (status != LBVH_OK) s->st.failed += 1;
 }
 
-// Enqueues request r on slot k.  Returns the synchronous status.
-int launch(lbvh_server *s, uint32_t k, Request &r) {
-  Slot &sl = s->slots[k];
-  hipStream_t stream = static_cast<hipStream_t>(lbvh_stream(sl.ctx));
+// Marks a finished (or failed) request's mesh idle and reports it.
+void finish(lbvh_server *s, const Work &w, int status, uint32_t slot) {
+  auto it = s->meshes.find(w.req.mesh_id);
+  if (it != s->meshes.end()) {
+    it->second.busy = false;
+    if (w.req.kind == LBVH_REQ_BUILD) it->second.valid = (status == LBVH_OK);
+  }
+  complete(s, w.req, status, slot, w.start_s);
+}
+
+// Under the lock: gives request r its mesh buffers (allocating on the slot's
+// stream when a build changes the size) and marks the mesh busy.
+int prepare(lbvh_server *s, Slot &sl, Work &w) {
+  Request &r = w.req;
   Mesh &m = s->meshes[r.mesh_id];
   if (r.kind == LBVH_REQ_BUILD && (m.buf == nullptr || m.n != r.n)) {
     free_mesh(s, m);
+    hipStream_t stream = static_cast<hipStream_t>(lbvh_stream(sl.ctx));
     void *p = nullptr;
     const uint64_t bytes = kMeshBytesPerTri * r.n;
     if (hipMallocFromPoolAsync(&p, bytes, s->pool, stream) != hipSuccess) {
@@ -215,10 +250,21 @@ int launch(lbvh_server *s, uint32_t k, Request &r) {
     }
     m.buf = p;
     m.n = r.n;
-    s->resident_bytes += kMeshBytesPerTri * r.n;
+    s->resident_bytes += bytes;
   }
+  if (r.kind == LBVH_REQ_BUILD) m.valid = false;
   m.busy = true;
   m.last_use = ++s->tick;
+  w.mesh = m;
+  return LBVH_OK;
+}
+
+// Without the lock (slot worker): the request's device work on the slot's
+// stream, then its completion event.
+int enqueue(Slot &sl, Work &w) {
+  const Request &r = w.req;
+  const Mesh &m = w.mesh;
+  hipStream_t stream = static_cast<hipStream_t>(lbvh_stream(sl.ctx));
   switch (r.source) {
     case LBVH_SRC_HOST:
       if (hipMemcpyAsync(m.tri(), r.host.data(), 48ull * r.n,
@@ -238,17 +284,43 @@ int launch(lbvh_server *s, uint32_t k, Request &r) {
       break;
   }
   int st;
-  if (r.kind == LBVH_REQ_BUILD) {
-    m.valid = false;
+  if (r.kind == LBVH_REQ_BUILD)
     st = lbvh_build(sl.ctx, m.tri(), r.n, nullptr, m.bvh(), m.otri(),
                     m.keys(), m.perm());
-  } else {
+  else
     st = lbvh_refit(sl.ctx, m.tri(), r.n, m.keys(), m.perm(), m.bvh(),
                     m.otri());
-  }
   if (st != LBVH_OK) return st;
-  if (hipEventRecord(sl.done, stream) != hipSuccess) return LBVH_ERR_DEVICE;
+  if (hipEventRecord(w.ev, stream) != hipSuccess) return LBVH_ERR_DEVICE;
   return LBVH_OK;
+}
+
+// Slot worker: takes assigned requests in order and enqueues their device
+// work without holding the server lock, so the slots' host-side launch
+// costs overlap each other and the dispatcher.
+void worker(lbvh_server *s, uint32_t k) {
+  (void)hipSetDevice(s->device);
+  Slot &sl = s->slots[k];
+  std::unique_lock<std::mutex> lock(s->mu);
+  while (true) {
+    sl.wcv.wait(lock, [&] { return s->stop || !sl.mailbox.empty(); });
+    if (s->stop) break;
+    sl.inflight.push_back(std::move(sl.mailbox.front()));
+    sl.mailbox.pop_front();
+    Work &w = sl.inflight.back();  // deque: stays valid while others change
+    lock.unlock();
+    const int st = enqueue(sl, w);
+    lock.lock();
+    if (st == LBVH_OK) {
+      w.launched = true;
+    } else {
+      // Work queued behind it on this stream was enqueued after it, so the
+      // failed entry is the last one in flight.
+      finish(s, w, st, k);
+      sl.inflight.pop_back();
+    }
+    s->cv.notify_all();
+  }
 }
 
 // Validates a queued request against the resident state.  LBVH_OK if it can
@@ -266,28 +338,38 @@ int check_request(lbvh_server *s, const Request &r) {
   return LBVH_OK;
 }
 
+// Picks the slot for the next admission: the least loaded one with room.
+uint32_t pick_slot(lbvh_server *s) {
+  uint32_t best = UINT32_MAX;
+  size_t best_depth = kSlotDepth;
+  for (uint32_t k = 0; k < s->slots.size(); ++k) {
+    const size_t d = s->slots[k].depth();
+    if (d < best_depth) {
+      best_depth = d;
+      best = k;
+    }
+  }
+  return best;
+}
+
 void dispatcher(lbvh_server *s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lock(s->mu);
   while (!s->stop) {
     bool progress = false;
-    // Completions.
+    // Completions, in each slot's stream order.
     for (uint32_t k = 0; k < s->slots.size(); ++k) {
       Slot &sl = s->slots[k];
-      if (!sl.busy) continue;
-      const hipError_t q = hipEventQuery(sl.done);
-      if (q == hipErrorNotReady) continue;
-      // (Kernel-side bounded-wait faults are sticky per slot and reported
-      // by lbvh_server_drain, which keeps this path free of device reads.)
-      const int status = (q == hipSuccess) ? LBVH_OK : LBVH_ERR_DEVICE;
-      auto it = s->meshes.find(sl.req.mesh_id);
-      if (it != s->meshes.end()) {
-        it->second.busy = false;
-        if (sl.req.kind == LBVH_REQ_BUILD) it->second.valid = (status == LBVH_OK);
+      while (!sl.inflight.empty() && sl.inflight.front().launched) {
+        const hipError_t q = hipEventQuery(sl.inflight.front().ev);
+        if (q == hipErrorNotReady) break;
+        // (Kernel-side bounded-wait faults are sticky per slot and reported
+        // by lbvh_server_drain, which keeps this path free of device reads.)
+        finish(s, sl.inflight.front(), q == hipSuccess ? LBVH_OK
+                                                       : LBVH_ERR_DEVICE, k);
+        sl.inflight.pop_front();
+        progress = true;
       }
-      complete(s, sl.req, status, k, sl.start_s);
-      sl.busy = false;
-      progress = true;
     }
     // Admissions, in arrival order.  A mesh already in a slot holds back
     // later requests on it; a request that does not fit yet holds back
@@ -295,10 +377,8 @@ void dispatcher(lbvh_server *s) {
     bool memory_blocked = false;
     std::vector<uint64_t> held;  // meshes with an earlier request still queued
     for (auto qi = s->queue.begin(); qi != s->queue.end();) {
-      uint32_t free_slot = UINT32_MAX;
-      for (uint32_t k = 0; k < s->slots.size(); ++k)
-        if (!s->slots[k].busy) { free_slot = k; break; }
-      if (free_slot == UINT32_MAX) break;
+      const uint32_t k = pick_slot(s);
+      if (k == UINT32_MAX) break;
       auto mi = s->meshes.find(qi->mesh_id);
       if ((mi != s->meshes.end() && mi->second.busy) ||
           std::find(held.begin(), held.end(), qi->mesh_id) != held.end()) {
@@ -308,25 +388,27 @@ void dispatcher(lbvh_server *s) {
       }
       int st = check_request(s, *qi);
       if (st == LBVH_OK) {
-        const uint64_t need = request_need(s, *qi, s->slots[free_slot]);
+        Slot &sl = s->slots[k];
+        const uint64_t need = request_need(s, *qi, sl);
         if (need > 0 && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
           memory_blocked = true;
           held.push_back(qi->mesh_id);
           ++qi;
           continue;
         }
-        Slot &sl = s->slots[free_slot];
-        sl.req = std::move(*qi);
+        Work w;
+        w.req = std::move(*qi);
         qi = s->queue.erase(qi);
-        sl.start_s = now_s();
-        st = launch(s, free_slot, sl.req);
+        w.start_s = now_s();
+        w.ev = sl.events[sl.next_event];
+        sl.next_event = (sl.next_event + 1) % kSlotDepth;
+        st = prepare(s, sl, w);
         if (st == LBVH_OK) {
-          sl.busy = true;
+          sl.mailbox.push_back(std::move(w));
+          sl.wcv.notify_one();
           s->peak = std::max(s->peak, in_use(s));
         } else {
-          auto m = s->meshes.find(sl.req.mesh_id);
-          if (m != s->meshes.end()) m->second.busy = false;
-          complete(s, sl.req, st, free_slot, sl.start_s);
+          finish(s, w, st, k);
         }
       } else {
         complete(s, *qi, st, UINT32_MAX, now_s());
@@ -334,11 +416,10 @@ void dispatcher(lbvh_server *s) {
       }
       progress = true;
     }
-    bool any_busy = false;
-    for (auto &sl : s->slots) any_busy |= sl.busy;
-    if (!any_busy && s->queue.empty()) s->idle_cv.notify_all();
+    const bool idle = all_idle(s);
+    if (idle && s->queue.empty()) s->idle_cv.notify_all();
     if (progress) continue;
-    if (any_busy)
+    if (!idle)
       s->cv.wait_for(lock, std::chrono::microseconds(20));
     else
       s->cv.wait(lock);
@@ -356,13 +437,14 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
   lbvh_server *s = new lbvh_server();
   s->device = device_ordinal;
   s->budget = hbm_budget_bytes;
-  s->slots.resize(slots);
+  for (uint32_t k = 0; k < slots; ++k) s->slots.emplace_back();
   int st = LBVH_OK;
   for (auto &sl : s->slots) {
     st = lbvh_create(device_ordinal, 0, &sl.ctx);
-    if (st == LBVH_OK &&
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
-      st = LBVH_ERR_DEVICE;
+    for (int e = 0; e < kSlotDepth && st == LBVH_OK; ++e)
+      if (hipEventCreateWithFlags(&sl.events[e], hipEventDisableTiming) !=
+          hipSuccess)
+        st = LBVH_ERR_DEVICE;
     if (st != LBVH_OK) break;
   }
   if (st == LBVH_OK) {
@@ -384,12 +466,15 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
     if (s->pool) (void)hipMemPoolDestroy(s->pool);
     if (s->free_stream) (void)hipStreamDestroy(s->free_stream);
     for (auto &x : s->slots) {
-      if (x.done) (void)hipEventDestroy(x.done);
+      for (hipEvent_t e : x.events)
+        if (e) (void)hipEventDestroy(e);
       if (x.ctx) lbvh_destroy(x.ctx);
     }
     delete s;
     return st;
   }
+  for (uint32_t k = 0; k < slots; ++k)
+    s->slots[k].worker = std::thread(worker, s, k);
   s->thr = std::thread(dispatcher, s);
   *out = s;
   return LBVH_OK;
@@ -400,12 +485,14 @@ int lbvh_server_destroy(lbvh_server *s) {
   {
     std::lock_guard<std::mutex> lock(s->mu);
     s->stop = true;
+    for (auto &sl : s->slots) sl.wcv.notify_all();
   }
   s->cv.notify_all();
   s->thr.join();
+  for (auto &sl : s->slots) sl.worker.join();
   for (auto &sl : s->slots) {
     lbvh_synchronize(sl.ctx);
-    (void)hipEventDestroy(sl.done);
+    for (hipEvent_t e : sl.events) (void)hipEventDestroy(e);
   }
   (void)hipSetDevice(s->device);
   for (auto &kv : s->meshes) free_mesh(s, kv.second);
@@ -479,12 +566,7 @@ int lbvh_server_drain(lbvh_server *s) {
   if (!s) return LBVH_ERR_INVALID_ARGUMENT;
   std::unique_lock<std::mutex> lock(s->mu);
   s->cv.notify_all();
-  s->idle_cv.wait(lock, [s] {
-    if (!s->queue.empty()) return false;
-    for (auto &sl : s->slots)
-      if (sl.busy) return false;
-    return true;
-  });
+  s->idle_cv.wait(lock, [s] { return s->queue.empty() && all_idle(s); });
   for (auto &sl : s->slots) {
     const int st = lbvh_synchronize(sl.ctx);  // also reports kernel faults
     if (st != LBVH_OK) {
@@ -533,7 +615,7 @@ int lbvh_server_stats_get(lbvh_server *s, lbvh_server_stats *out) {
   out->peak_bytes = s->peak;
   out->queued = (uint32_t)s->queue.size();
   uint32_t busy = 0;
-  for (auto &sl : s->slots) busy += sl.busy ? 1 : 0;
+  for (auto &sl : s->slots) busy += (uint32_t)sl.depth();
   out->in_flight = busy;
   return LBVH_OK;
 }
