@@ -391,23 +391,215 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
   }
 }
 
-// One agglomeration level.  kLeaves: level 0 over the sorted leaves (gathers
-// O_TRI and the leaf boxes), tile = T leaf positions; otherwise the items of
-// the level below, tile = the frontiers of kFanIn of its tiles.  Tiles are
-// numbered level by level; level L-1's tile g keeps its items in
-// items_in[g * kFrontierMax ...] and their number in count_in[g].  With
-// segments (batched builds) every segment is an independent mesh: positions,
-// node ids and the permutation are relative to its first leaf, and its
-// tiles at each level are consecutive.
-template <int T, bool kLeaves>
-__global__ __launch_bounds__(T) void k_agglomerate(
+// Frontier output shared by both level kinds: the stopped subtrees that
+// leave the tile, in item order, into the tile's kFrontierMax slots.
+__device__ __forceinline__ void emit_frontier(bool out, int l, int r,
+                                              const float mn[3],
+                                              const float mx[3],
+                                              uint32_t *wsum, uint32_t gtile,
+                                              int4 *items_out,
+                                              uint32_t *count_out, Ctl *ctl,
+                                              int tid, int T) {
+  const uint32_t rank = block_exclusive_scan(out ? 1u : 0u, wsum);
+  if (tid == T - 1) {
+    const uint32_t count = rank + (out ? 1u : 0u);
+    count_out[gtile] = count;
+    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
+  }
+  if (out && rank < (uint32_t)kFrontierMax) {
+    int4 *io = items_out + 2 * ((size_t)gtile * kFrontierMax + rank);
+    io[0] = make_int4(l, r, __float_as_int(mn[0]), __float_as_int(mn[1]));
+    io[1] = make_int4(__float_as_int(mn[2]), __float_as_int(mx[0]),
+                      __float_as_int(mx[1]), __float_as_int(mx[2]));
+  }
+}
+
+// ---- level 0: leaves ------------------------------------------------------
+//
+// At level 0 an item is a leaf, so a climber is just its leaf range.  The
+// exchange slot of split g (the gap after leaf ts + g) is the node of that
+// split in its final layout: each child writes its half (min.xyz | child
+// id, max.xyz | far leaf of its range) before the arrival count, and the
+// second arrival reads the sibling's half, so the node is complete in LDS
+// when the merge happens.  Its Karras id (the merged range's end that faces
+// the parent's split) is known one step later and goes to gap_of; after the
+// climb the tile writes its nodes in id order, 64 contiguous bytes per four
+// lanes (scattered 16-byte stores of whole nodes from the climbing lanes
+// cost more than the rest of the level together).
+template <int T>
+struct LeafLds {
+  int dgap[T + 1];    // delta at the gap after leaf ts - 1 + k
+  int4 node[T][4];    // split slot -> node being assembled
+  uint32_t cnt[T];    // arrivals per split slot
+  int gap_of[T];      // node id - ts -> its split slot (-1: not this tile's)
+  uint32_t wsum[16];
+};
+
+struct Climb {
+  int l, r;  // leaf range
+  float mn[3], mx[3];
+  int open;  // split slot whose node id is not known yet (-1: none)
+  int stop;  // 0 root / inactive, 1 pending local arrival, 2 frontier
+  int pgap;
+};
+
+template <int T>
+__device__ __forceinline__ void climb_leaves(Climb &c, LeafLds<T> &L, int n,
+                                             int ts, int te) {
+  // stop / pgap are set once after the loop: assigned on separate exits, the
+  // struct fields were kept in scratch memory
+  int stop = 0, pgap = 0;
+  while (true) {
+    const int dl = L.dgap[c.l - ts];      // gap before the range
+    const int dr = L.dgap[c.r - ts + 1];  // gap after the range
+    const bool root = (c.l == 0 && c.r == n - 1);
+    const bool is_left = dr > dl;
+    if (c.open >= 0) {
+      const int K = root ? 0 : (is_left ? c.r : c.l);
+      L.gap_of[K - ts] = c.open;
+      c.open = -1;
+    }
+    if (root) break;
+    const int gap = is_left ? c.r : c.l - 1;  // the parent's split
+    if (gap < ts || gap + 2 > te) {           // other child off-tile
+      stop = 2;
+      break;
+    }
+    const int slot = gap - ts;
+    const int q = is_left ? 0 : 2;
+    const int id = (c.l == c.r) ? -(c.l + 1) : (is_left ? c.r : c.l);
+    L.node[slot][q] = pack3(c.mn, id);
+    L.node[slot][q + 1] = pack3(c.mx, is_left ? c.l : c.r);
+    const uint32_t old = __hip_atomic_fetch_add(
+        &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) {
+      stop = 1;
+      pgap = gap;
+      break;
+    }
+    const int4 smn = L.node[slot][2 - q], smx = L.node[slot][3 - q];
+    const float sm[3] = {__int_as_float(smn.x), __int_as_float(smn.y),
+                         __int_as_float(smn.z)};
+    const float sx[3] = {__int_as_float(smx.x), __int_as_float(smx.y),
+                         __int_as_float(smx.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      c.mn[k] = is_left ? fminf(c.mn[k], sm[k]) : fminf(sm[k], c.mn[k]);
+      c.mx[k] = is_left ? fmaxf(c.mx[k], sx[k]) : fmaxf(sx[k], c.mx[k]);
+    }
+    if (is_left) c.r = smx.w;
+    else c.l = smx.w;
+    c.open = slot;
+  }
+  c.stop = stop;
+  c.pgap = pgap;
+}
+
+// Level 0: tile = T sorted leaf positions.  Gathers O_TRI and the leaf
+// boxes, climbs, writes the tile's nodes and its frontier.  With segments
+// (batched builds) every segment is an independent mesh: positions, node ids
+// and the permutation are relative to its first leaf.
+template <int T>
+__global__ __launch_bounds__(T) void k_leaves(
     const int4 *__restrict__ tri_all, const uint32_t *__restrict__ skeys_all,
     const int32_t *__restrict__ perm_all, uint32_t n_all,
-    int4 *__restrict__ otri_all, const int4 *__restrict__ items_in,
-    const uint32_t *__restrict__ count_in, uint32_t prev_tiles_all,
-    int4 *__restrict__ items_out, uint32_t *__restrict__ count_out,
-    int4 *nodes_all, Ctl *ctl, TreeSegs segs) {
-  static_assert(kLeaves || T == kItemTile, "item levels use kItemTile");
+    int4 *__restrict__ otri_all, int4 *__restrict__ items_out,
+    uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
+    TreeSegs segs) {
+  __shared__ LeafLds<T> L;
+  const int tid = threadIdx.x;
+  const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
+  uint32_t tile0 = 0, leaf_base = 0, N = n_all;
+  if (segs.tile_seg) {
+    const uint32_t sg = segs.tile_seg[gtile];
+    tile0 = segs.seg_tile0[sg];
+    leaf_base = segs.seg_off[sg];
+    N = segs.seg_off[sg + 1] - leaf_base;
+  }
+  const uint32_t tile = gtile - tile0;  // tile within the mesh
+  const uint32_t *skeys = skeys_all + leaf_base;
+  int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
+  const int ts = (int)(tile * T);
+  const int te = (int)min(tile * T + T, N);
+  const int k = ts + tid;
+  const bool active = k < te;
+
+  Climb c;
+  c.open = -1;
+  c.stop = 0;
+  c.l = c.r = k;
+  L.cnt[tid] = 0u;
+  L.gap_of[tid] = -1;
+  if (active) {
+    const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
+    const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
+    const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
+    int4 *o = otri_all + 3 * (size_t)(leaf_base + k);
+    o[0] = a;
+    o[1] = b;
+    o[2] = d;
+    tri_box(a, b, d, c.mn, c.mx);
+    perturb(c.mn, c.mx);
+    L.dgap[tid + 1] = ((uint32_t)k + 1 < N)
+        ? delta_next(skeys[k], skeys[k + 1], (uint32_t)k) : -1;
+    if (tid == 0)
+      L.dgap[0] = (k > 0) ? delta_next(skeys[k - 1], skeys[k], (uint32_t)(k - 1))
+                          : -1;
+  }
+  __syncthreads();
+  if (N == 1) {
+    if (k == 0) {
+      write_single_leaf_node(nodes, c.mn, c.mx);
+      count_out[gtile] = 0u;
+    }
+    return;
+  }
+  if ((uint32_t)k == N - 1) {
+    const int4 z = make_int4(0, 0, 0, 0);
+    int4 *last = nodes + 4 * (size_t)(N - 1);
+    last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+  }
+  if (active) climb_leaves<T>(c, L, (int)N, ts, te);
+  __syncthreads();
+
+  // The tile's nodes in id order: int4 e of the tile's node range is
+  // quarter e % 4 of node ts + e / 4.  The max halves carried far leaves in
+  // .w; the node layout has the constant flags 2 (l_max) and 0 (r_max).
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = j * T + tid;
+    const int kk = e >> 2, q = e & 3;
+    if (ts + kk < te) {
+      const int g = L.gap_of[kk];
+      if (g >= 0) {
+        int4 v = L.node[g][q];
+        if (q == 1) v.w = 2;
+        if (q == 3) v.w = 0;
+        nodes[4 * (size_t)(ts + kk) + q] = v;
+      }
+    }
+  }
+
+  const bool out = active && (c.stop == 2 ||
+                              (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
+  emit_frontier(out, c.l, c.r, c.mn, c.mx, L.wsum, gtile, items_out,
+                count_out, ctl, tid, T);
+}
+
+// ---- levels >= 1: items ---------------------------------------------------
+
+// One item level: tile = the frontiers of kFanIn consecutive tiles of the
+// level below.  Level L-1's tile g keeps its items in
+// items_in[g * kFrontierMax ...] and their number in count_in[g].  Items
+// cover many leaves, so nodes are written straight from the climb (they are
+// few: at most kFrontierMax per tile of the level below).
+__global__ __launch_bounds__(kItemTile) void k_items(
+    const uint32_t *__restrict__ skeys_all, uint32_t n_all,
+    const int4 *__restrict__ items_in, const uint32_t *__restrict__ count_in,
+    uint32_t prev_tiles_all, int4 *__restrict__ items_out,
+    uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
+    TreeSegs segs) {
+  constexpr int T = kItemTile;
   __shared__ AggLds<T> L;
   const int tid = threadIdx.x;
   const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
@@ -418,42 +610,33 @@ __global__ __launch_bounds__(T) void k_agglomerate(
     tile0 = segs.seg_tile0[sg];
     leaf_base = segs.seg_off[sg];
     N = segs.seg_off[sg + 1] - leaf_base;
-    if (!kLeaves) {
-      prev0 = segs.prev_tile0[sg];
-      prev_tiles = segs.prev_tiles[sg];
-    }
+    prev0 = segs.prev_tile0[sg];
+    prev_tiles = segs.prev_tiles[sg];
   }
   const uint32_t tile = gtile - tile0;  // tile within the mesh
   const uint32_t *skeys = skeys_all + leaf_base;
   int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
-  const uint32_t src0 = prev0 + tile * kFanIn;  // item levels: first source
-  int ts, te;
-  if (kLeaves) {
-    ts = (int)(tile * T);
-    te = (int)min(tile * T + T, N);
-  } else {
-    if (tid == 0) {
-      const uint32_t first = tile * kFanIn;
-      const uint32_t nsrc = prev_tiles > first ? min((uint32_t)kFanIn,
-                                                     prev_tiles - first) : 0u;
-      uint32_t c[kFanIn];
+  const uint32_t src0 = prev0 + tile * kFanIn;  // first source tile
+  if (tid == 0) {
+    const uint32_t first = tile * kFanIn;
+    const uint32_t nsrc =
+        prev_tiles > first ? min((uint32_t)kFanIn, prev_tiles - first) : 0u;
+    uint32_t cs[kFanIn];
 #pragma unroll
-      for (int s = 0; s < kFanIn; ++s)
-        c[s] = (uint32_t)s < nsrc ? min(count_in[src0 + s], (uint32_t)kFrontierMax)
-                                  : 0u;
-      uint32_t run = 0;
+    for (int s = 0; s < kFanIn; ++s)
+      cs[s] = (uint32_t)s < nsrc
+                  ? min(count_in[src0 + s], (uint32_t)kFrontierMax) : 0u;
+    uint32_t run = 0;
 #pragma unroll
-      for (int s = 0; s < kFanIn; ++s) {
-        L.pre[s] = run;
-        run += c[s];
-      }
-      L.pre[kFanIn] = run;
+    for (int s = 0; s < kFanIn; ++s) {
+      L.pre[s] = run;
+      run += cs[s];
     }
-    __syncthreads();
-    ts = 0;
-    te = (int)L.pre[kFanIn];
+    L.pre[kFanIn] = run;
   }
-  const int k = ts + tid;
+  __syncthreads();
+  const int ts = 0, te = (int)L.pre[kFanIn];
+  const int k = tid;
   const bool active = k < te;
 
   Agg c;
@@ -462,28 +645,15 @@ __global__ __launch_bounds__(T) void k_agglomerate(
   c.ia = c.ib = k;
   L.cnt[tid] = 0u;
   if (active) {
-    if (kLeaves) {
-      c.l = c.r = k;
-      const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
-      const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
-      const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
-      int4 *o = otri_all + 3 * (size_t)(leaf_base + k);
-      o[0] = a;
-      o[1] = b;
-      o[2] = d;
-      tri_box(a, b, d, c.mn, c.mx);
-      perturb(c.mn, c.mx);
-    } else {
-      int s = 0;
-      while (s + 1 < kFanIn && (int)L.pre[s + 1] <= k) ++s;
-      const size_t it =
-          (size_t)(src0 + s) * kFrontierMax + (uint32_t)(k - (int)L.pre[s]);
-      const int4 a = items_in[2 * it], b = items_in[2 * it + 1];
-      c.l = a.x; c.r = a.y;
-      c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
-      c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
-      c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
-    }
+    int s = 0;
+    while (s + 1 < kFanIn && (int)L.pre[s + 1] <= k) ++s;
+    const size_t it =
+        (size_t)(src0 + s) * kFrontierMax + (uint32_t)(k - (int)L.pre[s]);
+    const int4 a = items_in[2 * it], b = items_in[2 * it + 1];
+    c.l = a.x; c.r = a.y;
+    c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
+    c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
+    c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
     L.il[tid] = c.l;
     L.ir[tid] = c.r;
     L.dgap[tid + 1] = ((uint32_t)c.r + 1 < N)
@@ -493,39 +663,12 @@ __global__ __launch_bounds__(T) void k_agglomerate(
           ? delta_next(skeys[c.l - 1], skeys[c.l], (uint32_t)(c.l - 1)) : -1;
   }
   __syncthreads();
-  if (kLeaves) {
-    if (N == 1) {
-      if (k == 0) {
-        write_single_leaf_node(nodes, c.mn, c.mx);
-        count_out[gtile] = 0u;
-      }
-      return;
-    }
-    if ((uint32_t)k == N - 1) {
-      const int4 z = make_int4(0, 0, 0, 0);
-      int4 *last = nodes + 4 * (size_t)(N - 1);
-      last[0] = z; last[1] = z; last[2] = z; last[3] = z;
-    }
-  }
   if (active) agglomerate<T>(c, L, (int)N, ts, te, nodes);
   __syncthreads();
-
-  // Frontier: off-tile parents, and first arrivals whose sibling never came.
   const bool out = active && (c.stop == 2 ||
                               (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-  const uint32_t rank = block_exclusive_scan(out ? 1u : 0u, L.wsum);
-  if (tid == T - 1) {
-    const uint32_t count = rank + (out ? 1u : 0u);
-    count_out[gtile] = count;
-    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
-  }
-  if (out && rank < (uint32_t)kFrontierMax) {
-    int4 *io = items_out + 2 * ((size_t)gtile * kFrontierMax + rank);
-    io[0] = make_int4(c.l, c.r, __float_as_int(c.mn[0]),
-                      __float_as_int(c.mn[1]));
-    io[1] = make_int4(__float_as_int(c.mn[2]), __float_as_int(c.mx[0]),
-                      __float_as_int(c.mx[1]), __float_as_int(c.mx[2]));
-  }
+  emit_frontier(out, c.l, c.r, c.mn, c.mx, L.wsum, gtile, items_out,
+                count_out, ctl, tid, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -588,10 +731,8 @@ static void launch_leaf_level(uint32_t grid, const int4 *t4,
                               uint32_t n, int4 *o4, int4 *items_out,
                               uint32_t *count_out, int4 *n4, Ctl *ctl,
                               const TreeSegs &segs, hipStream_t s) {
-  hipLaunchKernelGGL((k_agglomerate<T, true>), dim3(grid), dim3(T), 0, s, t4,
-                     skeys, perm, n, o4, (const int4 *)nullptr,
-                     (const uint32_t *)nullptr, 0u, items_out, count_out, n4,
-                     ctl, segs);
+  hipLaunchKernelGGL(k_leaves<T>, dim3(grid), dim3(T), 0, s, t4, skeys, perm,
+                     n, o4, items_out, count_out, n4, ctl, segs);
 }
 
 void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
@@ -604,10 +745,9 @@ void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
   int4 *o4 = reinterpret_cast<int4 *>(otri);
   int4 *n4 = reinterpret_cast<int4 *>(nodes);
   if (level > 0) {
-    hipLaunchKernelGGL((k_agglomerate<kItemTile, false>), dim3(grid),
-                       dim3(kItemTile), 0, s, t4, skeys, perm, n, o4, items_in,
-                       count_in, prev_tiles, items_out, count_out, n4, ctl,
-                       segs);
+    hipLaunchKernelGGL(k_items, dim3(grid), dim3(kItemTile), 0, s, skeys, n,
+                       items_in, count_in, prev_tiles, items_out, count_out,
+                       n4, ctl, segs);
     return;
   }
   switch (tile) {

@@ -89,6 +89,10 @@ constexpr int kSlotDepth = 2;  // requests in flight per slot
 
 struct Slot {
   lbvh_ctx *ctx = nullptr;
+  // lbvh_bytes_in_use(ctx) as of the worker's last enqueue (or a trim): the
+  // dispatcher reads this instead of the context, whose lock the worker
+  // holds while it enqueues
+  uint64_t scratch_bytes = 0;
   hipEvent_t events[kSlotDepth] = {};
   uint32_t next_event = 0;
   std::deque<Work> mailbox;   // assigned, not yet taken by the worker
@@ -130,7 +134,7 @@ namespace {
 
 uint64_t slots_bytes(lbvh_server *s) {
   uint64_t b = 0;
-  for (auto &sl : s->slots) b += lbvh_bytes_in_use(sl.ctx);
+  for (auto &sl : s->slots) b += sl.scratch_bytes;
   return b;
 }
 
@@ -167,7 +171,7 @@ uint64_t request_need(lbvh_server *s, const Request &r, const Slot &sl) {
   const uint64_t scratch =
       (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
       r.n + sizeof(lbvh::Ctl);
-  const uint64_t have = lbvh_bytes_in_use(sl.ctx);
+  const uint64_t have = sl.scratch_bytes;
   if (scratch > have) need += scratch - have;
   return need;
 }
@@ -192,8 +196,9 @@ bool make_room(lbvh_server *s, uint64_t need, uint64_t keep) {
     }
     bool trimmed = false;
     for (auto &sl : s->slots)
-      if (sl.idle() && lbvh_bytes_in_use(sl.ctx) > sizeof(lbvh::Ctl)) {
-        lbvh_trim(sl.ctx);
+      if (sl.idle() && sl.scratch_bytes > sizeof(lbvh::Ctl)) {
+        lbvh_trim(sl.ctx);  // an idle slot's worker does not hold its lock
+        sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
         trimmed = true;
       }
     if (!trimmed) return false;
@@ -310,7 +315,9 @@ void worker(lbvh_server *s, uint32_t k) {
     Work &w = sl.inflight.back();  // deque: stays valid while others change
     lock.unlock();
     const int st = enqueue(sl, w);
+    const uint64_t bytes = lbvh_bytes_in_use(sl.ctx);
     lock.lock();
+    sl.scratch_bytes = bytes;
     if (st == LBVH_OK) {
       w.launched = true;
     } else {
@@ -418,7 +425,13 @@ void dispatcher(lbvh_server *s) {
     }
     const bool idle = all_idle(s);
     if (idle && s->queue.empty()) s->idle_cv.notify_all();
-    if (progress) continue;
+    if (progress) {
+      // let submit / poll / the workers in before the next round
+      lock.unlock();
+      std::this_thread::yield();
+      lock.lock();
+      continue;
+    }
     if (!idle)
       s->cv.wait_for(lock, std::chrono::microseconds(20));
     else
@@ -441,6 +454,7 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
   int st = LBVH_OK;
   for (auto &sl : s->slots) {
     st = lbvh_create(device_ordinal, 0, &sl.ctx);
+    if (st == LBVH_OK) sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
     for (int e = 0; e < kSlotDepth && st == LBVH_OK; ++e)
       if (hipEventCreateWithFlags(&sl.events[e], hipEventDisableTiming) !=
           hipSuccess)
