@@ -298,13 +298,15 @@ int lbvh_raycast(lbvh_ctx *ctx, const void *bvh, const float *tri, uint32_t n,
  * for them and returns the accumulated device time per stage. */
 typedef enum {
   LBVH_STAGE_BOUNDS = 0,
-  LBVH_STAGE_MORTON = 1, /* leaf AABB + Morton key (+ digit histograms) */
-  LBVH_STAGE_SORT0 = 2,  /* onesweep digit passes 0..3 */
+  LBVH_STAGE_MORTON = 1,   /* leaf AABB + Morton key (+ per-tile digit counts) */
+  LBVH_STAGE_SORT0 = 2,    /* radix sort digit passes 0..3 */
   LBVH_STAGE_SORT1 = 3,
   LBVH_STAGE_SORT2 = 4,
   LBVH_STAGE_SORT3 = 5,
-  LBVH_STAGE_TREE = 6,   /* fused leaf gather + radix tree + node boxes */
-  LBVH_STAGE_COUNT = 7
+  LBVH_STAGE_TREE = 6,     /* leaf tiles: O_TRI gather, leaf boxes, the radix
+                              tree and node boxes inside each tile */
+  LBVH_STAGE_TREE_TOP = 7, /* the levels above the leaf tiles */
+  LBVH_STAGE_COUNT = 8
 } lbvh_stage;
 
 typedef struct {

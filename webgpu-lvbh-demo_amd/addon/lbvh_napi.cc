@@ -497,7 +497,8 @@ napi_value ProfileRead(napi_env env, napi_callback_info info) {
                    "lbvh_profile_read"))
     return nullptr;
   static const char *names[LBVH_STAGE_COUNT] = {
-      "bounds", "morton", "sort0", "sort1", "sort2", "sort3", "tree"};
+      "bounds", "morton", "sort0", "sort1", "sort2", "sort3", "tree",
+      "treeTop"};
   napi_value obj;
   napi_create_object(env, &obj);
   for (int s = 0; s < LBVH_STAGE_COUNT; ++s) {

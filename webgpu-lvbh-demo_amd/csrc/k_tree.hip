@@ -766,21 +766,4 @@ void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
   }
 }
 
-void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
-                       const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, int4 *items, uint32_t *counts, Ctl *ctl,
-                       int tile, hipStream_t s) {
-  uint32_t grid[kMaxTreeLevels];
-  const int levels = tree_levels(n, tile, grid);
-  uint64_t base = 0, prev = 0;
-  for (int L = 0; L < levels; ++L) {
-    launch_tree_level(L, tile, grid[L], tri, skeys, perm, n, otri,
-                      items + 2 * kFrontierMax * prev, counts + prev,
-                      L ? grid[L - 1] : 0u, items + 2 * kFrontierMax * base,
-                      counts + base, nodes, ctl, TreeSegs{}, s);
-    prev = base;
-    base += grid[L];
-  }
-}
-
 }  // namespace lbvh

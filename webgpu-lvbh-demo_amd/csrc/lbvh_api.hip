@@ -301,6 +301,29 @@ void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
   }
 }
 
+// The tree levels of one mesh (ensure_tree_scratch done): the leaf tiles,
+// then the item levels above them, each level its own profiling stage.
+void run_tree(lbvh_ctx *ctx, const float *tri, const uint32_t *skeys,
+              const int32_t *perm, uint32_t n, float *tri_out, void *bvh_out,
+              int tile) {
+  uint32_t grid[kMaxTreeLevels];
+  const int levels = tree_levels(n, tile, grid);
+  int4 *items = static_cast<int4 *>(ctx->tree_items.ptr);
+  uint32_t *counts = static_cast<uint32_t *>(ctx->tree_counts.ptr);
+  uint64_t base = 0, prev = 0;
+  for (int L = 0; L < levels; ++L) {
+    StageScope sc(ctx, L ? LBVH_STAGE_TREE_TOP : LBVH_STAGE_TREE);
+    launch_tree_level(L, tile, grid[L], reinterpret_cast<const float4 *>(tri),
+                      skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
+                      items + 2 * kTreeFrontier * prev, counts + prev,
+                      L ? grid[L - 1] : 0u, items + 2 * kTreeFrontier * base,
+                      counts + base, static_cast<float4 *>(bvh_out), ctx->ctl,
+                      TreeSegs{}, ctx->stream);
+    prev = base;
+    base += grid[L];
+  }
+}
+
 MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
   MortonParams p{};
   for (int c = 0; c < 3; ++c) {
@@ -867,14 +890,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
       HIP_TRY(ctx, hipMemcpyAsync(skeys, keys, 4, hipMemcpyDeviceToDevice,
                                   ctx->stream));
   }
-  {
-    StageScope sc(ctx, LBVH_STAGE_TREE);
-    launch_tree_fused(t4, skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
-                      static_cast<float4 *>(bvh_out),
-                      static_cast<int4 *>(ctx->tree_items.ptr),
-                      static_cast<uint32_t *>(ctx->tree_counts.ptr), ctx->ctl,
-                      tile, ctx->stream);
-  }
+  run_tree(ctx, tri, skeys, perm, n, tri_out, bvh_out, tile);
   return check_launch(ctx, "lbvh_build");
 }
 
@@ -889,13 +905,7 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
-  StageScope sc(ctx, LBVH_STAGE_TREE);
-  launch_tree_fused(reinterpret_cast<const float4 *>(tri), sorted_keys, perm, n,
-                    reinterpret_cast<float4 *>(tri_out),
-                    static_cast<float4 *>(bvh_out),
-                    static_cast<int4 *>(ctx->tree_items.ptr),
-                    static_cast<uint32_t *>(ctx->tree_counts.ptr), ctx->ctl,
-                    tile, ctx->stream);
+  run_tree(ctx, tri, sorted_keys, perm, n, tri_out, bvh_out, tile);
   return check_launch(ctx, "lbvh_refit");
 }
 

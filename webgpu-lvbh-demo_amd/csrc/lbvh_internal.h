@@ -128,14 +128,12 @@ struct TreeSegs {
   const uint32_t *prev_tiles;
 };
 
-// Fused leaf gather + radix tree + node boxes by multi-level agglomeration.
-// tile: 256, 512 or 1024 leaves per level-0 block.  Tiles are numbered level
-// after level; items holds kTreeFrontier 32-byte item slots per tile and
-// counts one word per tile (tree_tiles_total(n, tile) tiles).
-void launch_tree_fused(const float4 *tri, const uint32_t *skeys,
-                       const int32_t *perm, uint32_t n, float4 *otri,
-                       float4 *nodes, int4 *items, uint32_t *counts, Ctl *ctl,
-                       int tile, hipStream_t s);
+// Fused leaf gather + radix tree + node boxes by multi-level agglomeration:
+// level 0 over tiles of `tile` (256, 512 or 1024) sorted leaves, then item
+// levels until one tile remains.  Tiles are numbered level after level; the
+// item slots hold kTreeFrontier 32-byte items per tile and the counts one
+// word per tile (tree_tiles_total(n, tile) tiles).
+//
 // Tiles of each level of an n-leaf build (returns the number of levels).
 int tree_levels(uint32_t n, int tile, uint32_t *grid);
 uint64_t tree_tiles_total(uint32_t n, int tile);

@@ -87,7 +87,8 @@ SIGNATURES = {
     "lbvh_profile_read": (_i32, [_vp, _vp, _i32]),
 }
 
-STAGES = ["bounds", "morton", "sort0", "sort1", "sort2", "sort3", "tree"]
+STAGES = ["bounds", "morton", "sort0", "sort1", "sort2", "sort3", "tree",
+          "tree_top"]
 
 
 class Profile(C.Structure):
