@@ -48,6 +48,12 @@ void oracle_canonicalize_bvh(void *bvh, uint32_t n);
 void oracle_build(const float *tri, uint32_t n, const float *bounds_or_null,
                   int div_mode, void *bvh_out, float *tri_out,
                   uint32_t *sorted_keys_out, int32_t *perm_out);
+
+/* The same build, multi-threaded (lbvh_cpu_mt.c, OpenMP): byte-identical
+ * outputs; returns the number of threads it used.  The CPU baseline. */
+int oracle_build_mt(const float *tri, uint32_t n, const float *bounds_or_null,
+                    int div_mode, void *bvh_out, float *tri_out,
+                    uint32_t *sorted_keys_out, int32_t *perm_out);
 void oracle_refit(const float *tri, uint32_t n, const uint32_t *sorted_keys,
                   const int32_t *perm, void *bvh_out, float *tri_out);
 

@@ -23,6 +23,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   rearrange              numpy gather
   raycast                against raycast_bruteforce (every triangle, no BVH)
                          on hits/misses and closest distances
+  build_mt               the multi-threaded CPU build (the CPU baseline):
+                         byte-identical to build() on every output
 """
 from __future__ import annotations
 
@@ -67,6 +69,7 @@ def lib():
             "oracle_rearrange": (None, [vp, vp, u32, vp]),
             "oracle_canonicalize_bvh": (None, [vp, u32]),
             "oracle_build": (None, [vp, u32, vp, i32, vp, vp, vp, vp]),
+            "oracle_build_mt": (i32, [vp, u32, vp, i32, vp, vp, vp, vp]),
             "oracle_refit": (None, [vp, u32, vp, vp, vp, vp]),
             "oracle_raycast": (None, [vp, vp, u32, vp, u32, vp]),
             "oracle_raycast_bruteforce": (None, [vp, u32, vp, u32, vp]),
@@ -169,6 +172,20 @@ def build(tri, bnds=None, div_mode=DIV_MUL_RCP):
     pm = np.empty(n, np.int32)
     lib().oracle_build(_p(tri), n, _p(b), div_mode, _p(bvh), _p(otri), _p(sk), _p(pm))
     return bvh, otri, sk, pm
+
+
+def build_mt(tri, bnds=None, div_mode=DIV_MUL_RCP):
+    """The multi-threaded CPU build (same outputs as build()); returns
+    (bvh, otri, sorted keys, permutation, threads used)."""
+    tri = _tri(tri)
+    n = tri.shape[0]
+    b = None if bnds is None else np.ascontiguousarray(bnds, dtype=np.float32)
+    bvh = np.zeros(64 * n, np.uint8)
+    otri = np.empty((n, 12), np.float32)
+    sk = np.empty(n, np.uint32)
+    pm = np.empty(n, np.int32)
+    t = lib().oracle_build_mt(_p(tri), n, _p(b), div_mode, _p(bvh), _p(otri), _p(sk), _p(pm))
+    return bvh, otri, sk, pm, int(t)
 
 
 def refit(tri, skeys, perm):

@@ -318,3 +318,26 @@ def test_raycast_traversal_equals_bruteforce(ora, lb, n):
     assert np.array_equal(h1, h2)
     if n >= 300:
         assert (h1[:, 3] > 0).sum() >= (50 if n >= 20000 else 5)
+
+
+# ---------------------------------------------------------------- multi-threaded CPU build
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 255, 4097, 30000])
+def test_build_mt_matches_build(ora, n):
+    """The multi-threaded CPU build (the CPU baseline) gives the oracle's
+    bytes: sorted keys, permutation, canonical BVH and O_TRI."""
+    rng = np.random.default_rng(n)
+    c = rng.random((n, 3)).astype(np.float32)
+    tri = np.zeros((n, 12), np.float32)
+    for v in range(3):
+        tri[:, 4 * v:4 * v + 3] = c + rng.uniform(-0.01, 0.01, (n, 3)).astype(np.float32)
+        tri[:, 4 * v + 3] = 3.1415
+    if n > 100:
+        tri[n // 2:n // 2 + 50] = tri[:50]      # duplicate keys
+        tri[10, 0:3] = 0.0                      # exact zeros in a leaf box
+    for div in (ora.DIV_MUL_RCP, ora.DIV_IEEE):
+        want = ora.build(tri, None, div)
+        got = ora.build_mt(tri, None, div)
+        for w, g in zip(want, got[:4]):
+            assert np.array_equal(np.asarray(w).view(np.uint8), np.asarray(g).view(np.uint8))
+        assert got[4] >= 1

@@ -48,9 +48,12 @@ __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
       seg_ord[i] = ((i & 7) < 3) ? 0xFFFFFFFFu : 0u;  // min slots, max slots
 }
 
-// Per-mesh fp32 min/max of the vertices, one sort tile per block.
+// Per-mesh fp32 min/max of the vertices, one sort tile per block, and each
+// triangle's box centre (3 x total floats: x, then y, then z) for
+// k_batch_morton.
 __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
-    const float4 *__restrict__ tri, SortSegs segs, uint32_t *seg_ord) {
+    const float4 *__restrict__ tri, SortSegs segs, uint32_t *seg_ord,
+    float *__restrict__ cen, uint32_t total) {
   __shared__ float s[kSortThreads / 64][6];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
@@ -59,12 +62,16 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
   for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads) {
     const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
                  c = tri[3 * (size_t)i + 2];
-    v[0] = fminf(v[0], fminf(a.x, fminf(b.x, c.x)));
-    v[1] = fminf(v[1], fminf(a.y, fminf(b.y, c.y)));
-    v[2] = fminf(v[2], fminf(a.z, fminf(b.z, c.z)));
-    v[3] = fmaxf(v[3], fmaxf(a.x, fmaxf(b.x, c.x)));
-    v[4] = fmaxf(v[4], fmaxf(a.y, fmaxf(b.y, c.y)));
-    v[5] = fmaxf(v[5], fmaxf(a.z, fmaxf(b.z, c.z)));
+    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
+                         fminf(a.z, fminf(b.z, c.z))};
+    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
+                         fmaxf(a.z, fmaxf(b.z, c.z))};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[k] = fminf(v[k], mn[k]);
+      v[3 + k] = fmaxf(v[3 + k], mx[k]);
+      cen[(size_t)k * total + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
@@ -89,11 +96,13 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
 
 // Per-mesh leaf AABB centre -> Morton key, plus the mesh's digit histograms
 // and the first sort pass's per-tile digit counts (cnt0[d * tiles + tile]).
-// Bounds: host-provided fp32 (6 per mesh) or the GPU accumulators.
+// Bounds: host-provided fp32 (6 per mesh; centres from the triangles) or the
+// GPU accumulators (centres from k_batch_bounds, cen non-null).
 __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
-    const float4 *__restrict__ tri, SortSegs segs,
-    const float *__restrict__ seg_bounds, const uint32_t *__restrict__ seg_ord,
-    int div_mode, uint32_t *__restrict__ keys, uint32_t *seg_hist,
+    const float4 *__restrict__ tri, const float *__restrict__ cen,
+    uint32_t total, SortSegs segs, const float *__restrict__ seg_bounds,
+    const uint32_t *__restrict__ seg_ord, int div_mode,
+    uint32_t *__restrict__ keys, uint32_t *seg_hist,
     uint32_t *__restrict__ cnt0) {
   static_assert(kSortThreads == kRadix, "one thread per digit for cnt0");
   __shared__ uint32_t s_hist[kSortPasses][kRadix];
@@ -121,17 +130,27 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
   __syncthreads();
   const bool ieee = (div_mode == 1);
   for (uint32_t i = base + threadIdx.x; i < end; i += kSortThreads) {
-    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
-                 c = tri[3 * (size_t)i + 2];
-    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
-                         fminf(a.z, fminf(b.z, c.z))};
-    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
-                         fmaxf(a.z, fmaxf(b.z, c.z))};
+    float ce[3];
+    if (cen) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) ce[k] = cen[(size_t)k * total + i];
+    } else {
+      const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
+                   c = tri[3 * (size_t)i + 2];
+      const float mn[3] = {fminf(a.x, fminf(b.x, c.x)),
+                           fminf(a.y, fminf(b.y, c.y)),
+                           fminf(a.z, fminf(b.z, c.z))};
+      const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)),
+                           fmaxf(a.y, fmaxf(b.y, c.y)),
+                           fmaxf(a.z, fmaxf(b.z, c.z))};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        ce[k] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+    }
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
-      const float d = __fsub_rn(cen, bmin[k]);
+      const float d = __fsub_rn(ce[k], bmin[k]);
       const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
       q[k] = to_u32_sat(__fmul_rn(rel, 1023.99f));
     }
@@ -160,18 +179,20 @@ void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
 }
 
 void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
-                         uint32_t tiles, uint32_t *seg_ord, hipStream_t s) {
+                         uint32_t tiles, uint32_t *seg_ord, float *cen,
+                         uint32_t total, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_bounds, dim3(tiles), dim3(kSortThreads), 0, s,
-                     tri, segs, seg_ord);
+                     tri, segs, seg_ord, cen, total);
 }
 
-void launch_batch_morton(const float4 *tri, const SortSegs &segs,
-                         uint32_t tiles, const float *seg_bounds,
-                         const uint32_t *seg_ord, int div_mode, uint32_t *keys,
-                         uint32_t *seg_hist, uint32_t *cnt0, hipStream_t s) {
+void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
+                         const SortSegs &segs, uint32_t tiles,
+                         const float *seg_bounds, const uint32_t *seg_ord,
+                         int div_mode, uint32_t *keys, uint32_t *seg_hist,
+                         uint32_t *cnt0, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_morton, dim3(tiles), dim3(kSortThreads), 0, s,
-                     tri, segs, seg_bounds, seg_ord, div_mode, keys, seg_hist,
-                     cnt0);
+                     tri, cen, total, segs, seg_bounds, seg_ord, div_mode,
+                     keys, seg_hist, cnt0);
 }
 
 }  // namespace lbvh

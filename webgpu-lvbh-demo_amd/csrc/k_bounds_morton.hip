@@ -66,21 +66,32 @@ __device__ void block_minmax(float v[6]) {
 // no fences: a last-block-finishes counter costs one same-address
 // device-scope atomic (with a cache writeback for its release) per block,
 // which on MI355X serialises across the XCDs.
+// cen (3 x n floats, x then y then z, or null): also the centre of each
+// triangle's box, (max + min) * .5f as aabb-z-idx.js:191 computes it, so the
+// Morton pass reads 12 bytes per triangle instead of the 48-byte triangle.
 __global__ __launch_bounds__(kMapThreads) void k_bounds(
     const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
-    Ctl *ctl, int bump_epoch) {
+    Ctl *ctl, int bump_epoch, float *__restrict__ cen) {
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
                  c = tri[3 * (size_t)i + 2];
-    v[0] = fminf(v[0], fminf(a.x, fminf(b.x, c.x)));
-    v[1] = fminf(v[1], fminf(a.y, fminf(b.y, c.y)));
-    v[2] = fminf(v[2], fminf(a.z, fminf(b.z, c.z)));
-    v[3] = fmaxf(v[3], fmaxf(a.x, fmaxf(b.x, c.x)));
-    v[4] = fmaxf(v[4], fmaxf(a.y, fmaxf(b.y, c.y)));
-    v[5] = fmaxf(v[5], fmaxf(a.z, fmaxf(b.z, c.z)));
+    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
+                         fminf(a.z, fminf(b.z, c.z))};
+    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
+                         fmaxf(a.z, fmaxf(b.z, c.z))};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[k] = fminf(v[k], mn[k]);
+      v[3 + k] = fmaxf(v[3 + k], mx[k]);
+    }
+    if (cen) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        cen[(size_t)k * n + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+    }
   }
   block_minmax(v);
   if (threadIdx.x == 0) {
@@ -187,6 +198,46 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
   }
 }
 
+// compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
+// :192-195 onward; the centre of :191 is already rounded the same way).  One
+// block per sort tile, as k_aabb_morton with cnt; bounds from ctl.
+__global__ __launch_bounds__(kMapThreads) void k_morton_cen(
+    const float *__restrict__ cen, uint32_t n, int div_mode,
+    uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t s_hist[kSortPasses][kRadix];
+  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
+    (&s_hist[0][0])[k] = 0u;
+  float bmin[3], ext[3], rcp[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    bmin[c] = ctl->bounds[c];
+    ext[c] = __fsub_rn(ctl->bounds[4 + c], bmin[c]);
+    rcp[c] = __fdiv_rn(1.0f, ext[c]);
+  }
+  __syncthreads();
+  const bool ieee = (div_mode == 1);
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t first = tile * (uint32_t)kSortTile;
+  const uint32_t end = min(first + (uint32_t)kSortTile, n);
+  for (uint32_t i = first + threadIdx.x; i < end; i += blockDim.x) {
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float d = __fsub_rn(cen[(size_t)k * n + i], bmin[k]);
+      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
+      q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
+    }
+    const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+    keys[i] = key;
+    hist_add(s_hist, key);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < kSortPasses; ++p)
+    cnt[((size_t)p * kRadix + threadIdx.x) * gridDim.x + tile] =
+        s_hist[p][threadIdx.x];
+}
+
 __global__ void k_copy_bounds(const Ctl *ctl, float *out) {
   const int t = threadIdx.x;
   if (t < 3) out[t] = ctl->bounds[t];
@@ -196,12 +247,12 @@ __global__ void k_copy_bounds(const Ctl *ctl, float *out) {
 __global__ void k_bump_epoch(Ctl *ctl) { ctl->epoch += 1; }
 
 void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
-                   bool bump_epoch, hipStream_t s) {
+                   bool bump_epoch, hipStream_t s, float *cen) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kBoundsBlocks) blocks = kBoundsBlocks;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
-                     partials, ctl, bump_epoch ? 1 : 0);
+                     partials, ctl, bump_epoch ? 1 : 0, cen);
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kMapThreads), 0, s,
                      partials, blocks, ctl);
 }
@@ -217,6 +268,15 @@ void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
   hipLaunchKernelGGL(k_aabb_morton, dim3(blocks), dim3(kMapThreads), 0, s,
                      tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
                      bump_epoch ? 1 : 0, cnt);
+}
+
+void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
+                       uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
+                       hipStream_t s) {
+  const uint32_t tiles = sort_tiles(n);
+  if (tiles == 0) return;
+  hipLaunchKernelGGL(k_morton_cen, dim3(tiles), dim3(kMapThreads), 0, s, cen,
+                     n, div_mode, keys, ctl, cnt);
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

@@ -108,10 +108,15 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
 //              is rewritten to base(s, d) - P(t0(s)) (mod 2^32) by the thread
 //              owning the segment's first tile.  Only block d touches column
 //              d of seg_base, so the rewrite needs just this block's barrier.
+// Chunks of kSortThreads * kScanPer column entries go through LDS, so global
+// loads and stores are coalesced while each thread scans kScanPer consecutive
+// entries (one pad word per kScanPer keeps the per-thread reads conflict-free).
 __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
     uint32_t tiles, const uint32_t *__restrict__ hist, SortSegs segs,
     uint32_t *seg_base) {
+  constexpr int kChunk = kSortThreads * kScanPer;
+  __shared__ uint32_t s_col[kChunk + kChunk / kScanPer];
   __shared__ uint32_t s_wsum[16];
   __shared__ uint32_t s_carry;
   const uint32_t d = blockIdx.x;
@@ -126,13 +131,20 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
   uint32_t carry = s_carry;
   const uint32_t *col = cnt + (size_t)d * tiles;
   uint32_t *out = off + (size_t)d * tiles;
-  for (uint32_t c0 = 0; c0 < tiles; c0 += kSortThreads * kScanPer) {
+  const auto pad = [](uint32_t e) { return e + e / kScanPer; };
+  for (uint32_t c0 = 0; c0 < tiles; c0 += kChunk) {
     const uint32_t t0 = c0 + threadIdx.x * kScanPer;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+      const uint32_t e = j * kSortThreads + threadIdx.x;
+      s_col[pad(e)] = (c0 + e < tiles) ? col[c0 + e] : 0u;
+    }
+    __syncthreads();
     uint32_t v[kScanPer];
     uint32_t sum = 0;
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
-      v[j] = (t0 + j < tiles) ? col[t0 + j] : 0u;
+      v[j] = s_col[pad(threadIdx.x * kScanPer + j)];
       sum += v[j];
     }
     const uint32_t ex = block_exclusive_scan(sum, s_wsum);
@@ -150,7 +162,8 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
         const uint32_t t = t0 + j;
         if (t < tiles) {
           const uint32_t sg = segs.tile_seg[t];
-          if (t == segs.seg_tile0[sg]) seg_base[(size_t)sg * kSortPasses * kRadix + d] -= p[j];
+          if (t == segs.seg_tile0[sg])
+            seg_base[(size_t)sg * kSortPasses * kRadix + d] -= p[j];
         }
       }
     }
@@ -158,13 +171,19 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
       const uint32_t t = t0 + j;
-      if (t < tiles)
-        out[t] = segmented
-                     ? p[j] + seg_base[(size_t)segs.tile_seg[t] * kSortPasses * kRadix + d]
-                     : p[j];
+      uint32_t o = p[j];
+      if (segmented && t < tiles)
+        o += seg_base[(size_t)segs.tile_seg[t] * kSortPasses * kRadix + d];
+      s_col[pad(threadIdx.x * kScanPer + j)] = o;
     }
     carry += s_carry;
-    __syncthreads();  // s_carry is rewritten by the next chunk
+    __syncthreads();  // offsets staged; s_carry is rewritten by the next chunk
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+      const uint32_t e = j * kSortThreads + threadIdx.x;
+      if (c0 + e < tiles) out[c0 + e] = s_col[pad(e)];
+    }
+    __syncthreads();  // s_col is refilled by the next chunk
   }
 }
 

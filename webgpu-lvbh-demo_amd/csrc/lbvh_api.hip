@@ -52,10 +52,11 @@ struct lbvh_ctx {
   DevBuf tree_items, tree_counts;       // agglomeration item slots, counts
   DevBuf ctag;                          // per-stage up-pass tags (zeroed)
   DevBuf partials;                      // bounds partials
+  DevBuf cen;                           // triangle box centres (GPU bounds)
   // batched builds: scratch, and the plan (tile tables + captured graph) of
   // the last batch layout
   DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_hist, b_ord,
-      b_bounds, b_tree_items, b_tree_counts, b_table;
+      b_bounds, b_tree_items, b_tree_counts, b_table, b_cen;
   struct BatchPlan {
     std::vector<uint32_t> offsets;
     int tile = 0;
@@ -212,10 +213,11 @@ struct StageScope {
 std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
   return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
           &ctx->digits,  &ctx->gside,    &ctx->tree_items,
-          &ctx->tree_counts, &ctx->ctag, &ctx->partials, &ctx->b_keys_a,
+          &ctx->tree_counts, &ctx->ctag, &ctx->partials, &ctx->cen,
+          &ctx->b_keys_a,
           &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
           &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_tree_items,
-          &ctx->b_tree_counts, &ctx->b_table};
+          &ctx->b_tree_counts, &ctx->b_table, &ctx->b_cen};
 }
 
 void drop_plan(lbvh_ctx *ctx) {
@@ -427,14 +429,15 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   const SortSegs ss{T + P.o_tile_seg, T + P.o_seg_tile0, T + P.o_seg_off,
                     hist};
   launch_batch_prologue(hist, ord, P.S, !have_bounds, ctx->ctl, ctx->stream);
+  float *cen = have_bounds ? nullptr : static_cast<float *>(ctx->b_cen.ptr);
   if (!have_bounds)
-    launch_batch_bounds(t4, ss, P.sort_tiles, ord, ctx->stream);
+    launch_batch_bounds(t4, ss, P.sort_tiles, ord, cen, P.total, ctx->stream);
   uint32_t *keys = static_cast<uint32_t *>(ctx->b_keys_a.ptr);
   uint32_t *tmp = static_cast<uint32_t *>(ctx->b_keys_b.ptr);
   uint32_t *out = skeys ? skeys : keys;
   int32_t *vtmp = static_cast<int32_t *>(ctx->b_idx_b.ptr);
   int32_t *vout = perm ? perm : static_cast<int32_t *>(ctx->b_idx_a.ptr);
-  launch_batch_morton(t4, ss, P.sort_tiles,
+  launch_batch_morton(t4, cen, P.total, ss, P.sort_tiles,
                       have_bounds ? static_cast<const float *>(ctx->b_bounds.ptr)
                                   : nullptr,
                       ord, ctx->div_mode, keys, hist,
@@ -857,10 +860,16 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if ((st = ensure_sort_scratch(ctx, n))) return st;
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
+  const bool sort = n > 1;
+  // GPU bounds: the bounds pass also writes the box centres, and the Morton
+  // pass reads those (12 bytes per triangle) instead of the triangles
+  const bool use_cen = !bounds_host && sort;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
+  if (use_cen && (st = ensure(ctx, ctx->cen, 12ull * n, false))) return st;
   if ((st = begin_op(ctx))) return st;
 
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
+  float *cen = use_cen ? static_cast<float *>(ctx->cen.ptr) : nullptr;
   uint32_t *keys = static_cast<uint32_t *>(ctx->keys_a.ptr);
   uint32_t *skeys = sorted_keys_out ? sorted_keys_out : keys;
   int32_t *perm = perm_out ? perm_out : static_cast<int32_t *>(ctx->idx_a.ptr);
@@ -871,15 +880,17 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
     p.div_mode = ctx->div_mode;
     StageScope sc(ctx, LBVH_STAGE_BOUNDS);
     launch_bounds(t4, n, static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
-                  true, ctx->stream);
+                  true, ctx->stream, cen);
   }
-  const bool sort = n > 1;
   {
     StageScope sc(ctx, LBVH_STAGE_MORTON);
-    launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                       ctx->ctl, bounds_host != nullptr, ctx->stream,
-                       sort ? static_cast<uint32_t *>(ctx->digits.ptr)
-                            : nullptr);
+    uint32_t *cnt = sort ? static_cast<uint32_t *>(ctx->digits.ptr) : nullptr;
+    if (use_cen)
+      launch_morton_cen(cen, n, ctx->div_mode, keys, ctx->ctl, cnt,
+                        ctx->stream);
+    else
+      launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
+                         ctx->ctl, bounds_host != nullptr, ctx->stream, cnt);
   }
   if (sort) {
     run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), skeys,
@@ -946,6 +957,9 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                    false)))
     return st;
   if ((st = ensure(ctx, ctx->b_tree_counts, 4ull * P.tree_tiles, false)))
+    return st;
+  if (!bounds_host &&
+      (st = ensure(ctx, ctx->b_cen, 12ull * total, false)))
     return st;
   if (bounds_host) {
     if ((st = ensure(ctx, ctx->b_bounds, 24ull * num_meshes, false))) return st;

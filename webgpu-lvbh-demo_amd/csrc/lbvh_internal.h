@@ -60,8 +60,15 @@ struct MortonParams {
 // ---- kernel launchers (each .hip file owns its kernels) --------------------
 
 // k_bounds_morton.hip
+// Scene bounds into ctl->bounds; cen (3 x n floats, or null): also each
+// triangle's box centre, for launch_morton_cen.
 void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
-                   bool bump_epoch, hipStream_t s);
+                   bool bump_epoch, hipStream_t s, float *cen = nullptr);
+// Morton keys + per-tile digit counts (as launch_aabb_morton with cnt) from
+// the centres launch_bounds wrote and the bounds in ctl.
+void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
+                       uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
+                       hipStream_t s);
 // cnt (kSortPasses x kRadix x sort_tiles(n) words, or null): also write the
 // sort's per-tile digit counts of every pass (see launch_tile_hist).
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
@@ -151,12 +158,16 @@ void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
 void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
                            uint32_t num_segs, bool init_bounds, Ctl *ctl,
                            hipStream_t s);
+// Per-mesh bounds (into seg_ord) and box centres (3 x total floats).
 void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
-                         uint32_t tiles, uint32_t *seg_ord, hipStream_t s);
-void launch_batch_morton(const float4 *tri, const SortSegs &segs,
-                         uint32_t tiles, const float *seg_bounds,
-                         const uint32_t *seg_ord, int div_mode, uint32_t *keys,
-                         uint32_t *seg_hist, uint32_t *cnt0, hipStream_t s);
+                         uint32_t tiles, uint32_t *seg_ord, float *cen,
+                         uint32_t total, hipStream_t s);
+// Per-mesh Morton keys; centres from cen when non-null, else from tri.
+void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
+                         const SortSegs &segs, uint32_t tiles,
+                         const float *seg_bounds, const uint32_t *seg_ord,
+                         int div_mode, uint32_t *keys, uint32_t *seg_hist,
+                         uint32_t *cnt0, hipStream_t s);
 
 // k_raycast.hip
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,
