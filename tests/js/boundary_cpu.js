@@ -8,8 +8,13 @@ const path = require('path')
 const bvh = require(path.join(__dirname, '..', '..', 'webgpu-lvbh-demo_amd', 'bvh'))
 
 for (const name of ['initBVHBuild', 'initAABB_ZidxKernel', 'initRadixSortKernel',
-  'initRadixTreeKernel', 'initBVHUpPassKernel', 'initRearrangeKernel', 'createDevice']) {
+  'initRadixTreeKernel', 'initBVHUpPassKernel', 'initRearrangeKernel', 'createDevice',
+  'initBVHBatchBuild', 'raycast', 'createServer']) {
   assert.strictEqual(typeof bvh[name], 'function', name)
+}
+for (const name of ['buildBatched', 'autotune', 'raycast', 'createServer', 'serverSubmit',
+  'serverPoll', 'serverDrain', 'serverMesh', 'serverEvict', 'serverStats']) {
+  assert.strictEqual(typeof bvh.addon[name], 'function', 'addon.' + name)
 }
 
 const g = bvh.installGlobals({})
@@ -37,4 +42,12 @@ try {
   assert.strictEqual(e.code, 6)
 }
 assert.ok(threw, 'createDevice on a missing device must throw')
+threw = false
+try {
+  bvh.createServer({ ordinal: 4096 })
+} catch (e) {
+  threw = true
+  assert.strictEqual(e.code, 6)
+}
+assert.ok(threw, 'createServer on a missing device must throw')
 console.log('boundary_cpu ok')

@@ -54,3 +54,75 @@ def test_js_build_without_bounds(tmp_path, lb, ora):
     prefix = run_cli(tmp_path, tri, None, "build")
     wb, wo, _, _ = ora.build(tri, None, ora.DIV_MUL_RCP)
     assert np.array_equal(np.fromfile(prefix + ".bvh", np.uint8), wb)
+
+
+EXTRAS = os.path.join(ROOT, "tests", "js", "extras_cli.js")
+
+
+def run_extras(*args):
+    node = shutil.which("node")
+    assert node, "node is required for the JS boundary test"
+    print(f"node extras_cli {args[0]}", flush=True)
+    r = subprocess.run([node, EXTRAS, *map(str, args)], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_js_batched_matches_oracle(tmp_path, lb, ora):
+    """initBVHBatchBuild: every mesh of the batch equals its own oracle build."""
+    sizes = [1, 300, 4097, 0, 20000, 2]
+    meshes = [lb.wl_uniform(70 + k, n, 0.01) if n else np.zeros((0, 12), np.float32)
+              for k, n in enumerate(sizes)]
+    off = np.zeros(len(sizes) + 1, np.uint32)
+    off[1:] = np.cumsum(sizes)
+    np.concatenate(meshes).astype(np.float32).tofile(tmp_path / "tri.f32")
+    off.tofile(tmp_path / "off.u32")
+    run_extras("batched", tmp_path / "tri.f32", tmp_path / "off.u32", tmp_path / "b")
+    got_bvh = np.fromfile(str(tmp_path / "b.bvh"), np.uint8)
+    got_otri = np.fromfile(str(tmp_path / "b.otri"), np.uint32)
+    for k, m in enumerate(meshes):
+        a, b = int(off[k]), int(off[k + 1])
+        if a == b:
+            continue
+        wb, wo, _, _ = ora.build(m, None, ora.DIV_MUL_RCP)
+        assert np.array_equal(got_bvh[64 * a:64 * b], wb), k
+        assert np.array_equal(got_otri[12 * a:12 * b], wo.view(np.uint32).ravel()), k
+
+
+def test_js_raycast_matches_oracle(tmp_path, lb, ora):
+    n = 5000
+    tri = lb.wl_uniform(71, n, 0.02)
+    rng = np.random.default_rng(9)
+    rays = np.zeros((2048, 8), np.float32)
+    o = rng.uniform(-0.5, 1.5, (2048, 3)).astype(np.float32)
+    t = rng.uniform(0, 1, (2048, 3)).astype(np.float32)
+    rays[:, 0:3], rays[:, 4:7] = o, t - o
+    tri.tofile(tmp_path / "tri.f32")
+    rays.tofile(tmp_path / "rays.f32")
+    run_extras("raycast", tmp_path / "tri.f32", n, tmp_path / "rays.f32", tmp_path / "r")
+    bvh = np.fromfile(str(tmp_path / "r.bvh"), np.uint8)
+    otri = np.fromfile(str(tmp_path / "r.otri"), np.float32).reshape(-1, 12)
+    hits = np.fromfile(str(tmp_path / "r.hits"), np.float32).reshape(-1, 4)
+    assert np.array_equal(hits.view(np.uint32), ora.raycast(bvh, otri, rays).view(np.uint32))
+    assert np.array_equal(hits.view(np.uint32),
+                          ora.raycast_bruteforce(otri, rays).view(np.uint32))
+
+
+def test_js_server_build_and_refit(tmp_path, lb, ora):
+    """createServer: a host-sourced build equals the oracle build; a jitter
+    refit equals the oracle refit of the jittered vertices it produced."""
+    n = 30000
+    tri = lb.wl_clustered(72, n, 0.05)
+    tri.tofile(tmp_path / "tri.f32")
+    run_extras("serve", tmp_path / "tri.f32", n, tmp_path / "s")
+    wb, wo, wsk, wpm = ora.build(tri, None, ora.DIV_MUL_RCP)
+    assert np.array_equal(np.fromfile(str(tmp_path / "s.build.bvh"), np.uint8), wb)
+    assert np.array_equal(np.fromfile(str(tmp_path / "s.build.otri"), np.uint32),
+                          wo.view(np.uint32).ravel())
+    # the refit's O_TRI holds the jittered vertices in leaf order
+    jit_otri = np.fromfile(str(tmp_path / "s.refit.otri"), np.float32).reshape(-1, 12)
+    jit = np.empty_like(jit_otri)
+    jit[wpm] = jit_otri
+    rb, ro = ora.refit(jit, wsk, wpm)
+    assert np.array_equal(np.fromfile(str(tmp_path / "s.refit.bvh"), np.uint8), rb)
+    assert not np.array_equal(jit, tri)

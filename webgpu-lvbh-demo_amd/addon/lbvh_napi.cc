@@ -424,6 +424,339 @@ napi_value Refit(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+// ---- batched builds, autotune, ray casting --------------------------------------
+
+// buildBatched(ctx, tri, offsets: Uint32Array(S+1), bounds: Float32Array(6S) |
+// null, bvh, otri, keys | null, perm | null)
+napi_value BuildBatched(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  lbvh_ctx *ctx;
+  void *tri, *bvh, *otri, *sk, *pm, *off, *bnd = nullptr;
+  size_t off_len = 0, bnd_len = 0;
+  if (!get_args(env, info, 8, argv) || !get_ctx(env, argv[0], &ctx) ||
+      !get_ptr(env, argv[1], &tri) || !get_bytes(env, argv[2], &off, &off_len) ||
+      !get_ptr(env, argv[4], &bvh) || !get_ptr(env, argv[5], &otri) ||
+      !get_ptr(env, argv[6], &sk) || !get_ptr(env, argv[7], &pm))
+    return nullptr;
+  if (off_len < 4 || off_len % 4) {
+    napi_throw_type_error(env, "EARGS", "offsets must be a Uint32Array(S+1)");
+    return nullptr;
+  }
+  const uint32_t S = (uint32_t)(off_len / 4) - 1;
+  if (!is_nullish(env, argv[3])) {
+    if (!get_bytes(env, argv[3], &bnd, &bnd_len)) return nullptr;
+    if (bnd_len != 24ull * S) {
+      napi_throw_type_error(env, "EARGS", "bounds must be a Float32Array(6 * S)");
+      return nullptr;
+    }
+  }
+  if (throw_status(env, ctx,
+                   lbvh_build_batched(ctx, (const float *)tri,
+                                      (const uint32_t *)off, S,
+                                      (const float *)bnd, bvh, (float *)otri,
+                                      (uint32_t *)sk, (int32_t *)pm),
+                   "lbvh_build_batched"))
+    return nullptr;
+  return undefined(env);
+}
+
+// autotune(ctx, nMin, nMax, reps) -> Float32Array(buckets x 3 tree tiles) ms
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  lbvh_ctx *ctx;
+  uint32_t nmin, nmax, reps;
+  if (!get_args(env, info, 4, argv) || !get_ctx(env, argv[0], &ctx) ||
+      !get_u32(env, argv[1], &nmin) || !get_u32(env, argv[2], &nmax) ||
+      !get_u32(env, argv[3], &reps))
+    return nullptr;
+  int b0 = 0, b1 = 0;
+  while (nmin > 1 && (2ull << b0) <= nmin) ++b0;
+  while (nmax > 1 && (2ull << b1) <= nmax) ++b1;
+  const size_t count = nmax >= nmin ? (size_t)(b1 - b0 + 1) * 3 : 0;
+  void *data = nullptr;
+  napi_value ab, arr;
+  NAPI_OK(env, napi_create_arraybuffer(env, 4 * (count ? count : 1), &data, &ab));
+  if (throw_status(env, ctx,
+                   lbvh_autotune(ctx, nmin, nmax, (int)reps, (float *)data),
+                   "lbvh_autotune"))
+    return nullptr;
+  NAPI_OK(env, napi_create_typedarray(env, napi_float32_array, count, ab, 0, &arr));
+  return arr;
+}
+
+// raycast(ctx, bvh, tri, n, rays, count, hits): device pointers
+napi_value Raycast(napi_env env, napi_callback_info info) {
+  napi_value argv[7];
+  lbvh_ctx *ctx;
+  void *bvh, *tri, *rays, *hits;
+  uint32_t n, count;
+  if (!get_args(env, info, 7, argv) || !get_ctx(env, argv[0], &ctx) ||
+      !get_ptr(env, argv[1], &bvh) || !get_ptr(env, argv[2], &tri) ||
+      !get_u32(env, argv[3], &n) || !get_ptr(env, argv[4], &rays) ||
+      !get_u32(env, argv[5], &count) || !get_ptr(env, argv[6], &hits))
+    return nullptr;
+  if (throw_status(env, ctx,
+                   lbvh_raycast(ctx, bvh, (const float *)tri, n,
+                                (const float *)rays, count, (float *)hits),
+                   "lbvh_raycast"))
+    return nullptr;
+  return undefined(env);
+}
+
+// ---- serving ----------------------------------------------------------------------
+
+void finalize_server(napi_env, void *data, void *) {
+  lbvh_server_destroy(static_cast<lbvh_server *>(data));
+}
+
+bool get_server(napi_env env, napi_value v, lbvh_server **out) {
+  void *p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, "EARGS", "expected an LBVH server");
+    return false;
+  }
+  *out = static_cast<lbvh_server *>(p);
+  return true;
+}
+
+bool throw_server(napi_env env, lbvh_server *srv, int st, const char *what) {
+  if (st == LBVH_OK) return false;
+  std::string msg = std::string(what) + ": " + lbvh_status_string(st) + " (" +
+                    lbvh_server_last_error(srv) + ")";
+  napi_throw(env, make_error(env, st, msg));
+  return true;
+}
+
+napi_value u64_value(napi_env env, uint64_t x) {
+  napi_value r;
+  napi_create_bigint_uint64(env, x, &r);
+  return r;
+}
+
+napi_value num_value(napi_env env, double x) {
+  napi_value r;
+  napi_create_double(env, x, &r);
+  return r;
+}
+
+// createServer(ordinal, hbmBudget, slots) -> external
+napi_value CreateServer(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  int32_t ordinal = 0;
+  uint64_t budget = 0;
+  uint32_t slots = 0;
+  napi_get_value_int32(env, argv[0], &ordinal);
+  if (!get_u64(env, argv[1], &budget) || !get_u32(env, argv[2], &slots))
+    return nullptr;
+  lbvh_server *srv = nullptr;
+  const int st = lbvh_server_create(ordinal, budget, slots, &srv);
+  if (st != LBVH_OK) {
+    napi_throw(env, make_error(env, st, std::string("lbvh_server_create: ") +
+                                            lbvh_status_string(st)));
+    return nullptr;
+  }
+  napi_value ext;
+  NAPI_OK(env, napi_create_external(env, srv, finalize_server, nullptr, &ext));
+  return ext;
+}
+
+napi_value ServerSetDivMode(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  int32_t mode = 0;
+  napi_get_value_int32(env, argv[1], &mode);
+  if (throw_server(env, srv, lbvh_server_set_div_mode(srv, mode),
+                   "lbvh_server_set_div_mode"))
+    return nullptr;
+  return undefined(env);
+}
+
+// serverSubmit(srv, kind, meshId, source, tri, n, seed, param) -> request id.
+// tri: a TypedArray (host source, copied at submit), a BigInt device pointer
+// (device source) or null (synthesised sources).
+napi_value ServerSubmit(napi_env env, napi_callback_info info) {
+  napi_value argv[8];
+  lbvh_server *srv;
+  int32_t kind = 0, source = 0;
+  uint64_t mesh = 0, seed = 0;
+  uint32_t n = 0;
+  double param = 0.0;
+  if (!get_args(env, info, 8, argv) || !get_server(env, argv[0], &srv) ||
+      !get_u64(env, argv[2], &mesh) || !get_u32(env, argv[5], &n) ||
+      !get_u64(env, argv[6], &seed))
+    return nullptr;
+  napi_get_value_int32(env, argv[1], &kind);
+  napi_get_value_int32(env, argv[3], &source);
+  napi_get_value_double(env, argv[7], &param);
+  const float *tri = nullptr;
+  if (!is_nullish(env, argv[4])) {
+    napi_valuetype t;
+    napi_typeof(env, argv[4], &t);
+    void *p = nullptr;
+    size_t len = 0;
+    if (t == napi_bigint) {
+      if (!get_ptr(env, argv[4], &p)) return nullptr;
+    } else {
+      if (!get_bytes(env, argv[4], &p, &len)) return nullptr;
+      if (len < 48ull * n) {
+        napi_throw_type_error(env, "EARGS", "triangle array shorter than 48 * n bytes");
+        return nullptr;
+      }
+    }
+    tri = static_cast<const float *>(p);
+  }
+  uint64_t id = 0;
+  if (throw_server(env, srv,
+                   lbvh_server_submit(srv, kind, mesh, source, tri, n, seed,
+                                      param, &id),
+                   "lbvh_server_submit"))
+    return nullptr;
+  return u64_value(env, id);
+}
+
+// serverPoll(srv) -> [{requestId, meshId, kind, status, n, slot, submitS,
+// startS, doneS}]
+napi_value ServerPoll(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  lbvh_server *srv;
+  if (!get_args(env, info, 1, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  napi_value arr;
+  NAPI_OK(env, napi_create_array(env, &arr));
+  lbvh_completion c[256];
+  uint32_t got = 0, at = 0;
+  do {
+    if (throw_server(env, srv, lbvh_server_poll(srv, c, 256, &got),
+                     "lbvh_server_poll"))
+      return nullptr;
+    for (uint32_t i = 0; i < got; ++i) {
+      napi_value o;
+      napi_create_object(env, &o);
+      napi_set_named_property(env, o, "requestId", u64_value(env, c[i].request_id));
+      napi_set_named_property(env, o, "meshId", u64_value(env, c[i].mesh_id));
+      napi_set_named_property(env, o, "kind", num_value(env, c[i].kind));
+      napi_set_named_property(env, o, "status", num_value(env, c[i].status));
+      napi_set_named_property(env, o, "n", num_value(env, c[i].n));
+      napi_set_named_property(env, o, "slot", num_value(env, c[i].slot));
+      napi_set_named_property(env, o, "submitS", num_value(env, c[i].submit_s));
+      napi_set_named_property(env, o, "startS", num_value(env, c[i].start_s));
+      napi_set_named_property(env, o, "doneS", num_value(env, c[i].done_s));
+      napi_set_element(env, arr, at++, o);
+    }
+  } while (got == 256);
+  return arr;
+}
+
+struct DrainWork {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref srv_ref = nullptr;
+  lbvh_server *srv = nullptr;
+  int status = LBVH_OK;
+  std::string error;
+};
+
+void drain_execute(napi_env, void *data) {
+  DrainWork *w = static_cast<DrainWork *>(data);
+  w->status = lbvh_server_drain(w->srv);
+  if (w->status != LBVH_OK) w->error = lbvh_server_last_error(w->srv);
+}
+
+void drain_complete(napi_env env, napi_status, void *data) {
+  DrainWork *w = static_cast<DrainWork *>(data);
+  if (w->status == LBVH_OK) {
+    napi_resolve_deferred(env, w->deferred, undefined(env));
+  } else {
+    napi_reject_deferred(
+        env, w->deferred,
+        make_error(env, w->status,
+                   std::string("lbvh_server_drain: ") +
+                       lbvh_status_string(w->status) + " (" + w->error + ")"));
+  }
+  napi_delete_reference(env, w->srv_ref);
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+// serverDrain(srv) -> Promise, settled off the JS thread
+napi_value ServerDrain(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  lbvh_server *srv;
+  if (!get_args(env, info, 1, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  DrainWork *w = new DrainWork();
+  w->srv = srv;
+  napi_value promise, name;
+  NAPI_OK(env, napi_create_promise(env, &w->deferred, &promise));
+  NAPI_OK(env, napi_create_reference(env, argv[0], 1, &w->srv_ref));
+  napi_create_string_utf8(env, "lbvh_server_drain", NAPI_AUTO_LENGTH, &name);
+  NAPI_OK(env, napi_create_async_work(env, nullptr, name, drain_execute,
+                                      drain_complete, w, &w->work));
+  NAPI_OK(env, napi_queue_async_work(env, w->work));
+  return promise;
+}
+
+// serverMesh(srv, meshId) -> {bvh: BigInt, tri: BigInt, n}
+napi_value ServerMesh(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  uint64_t mesh = 0;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv) ||
+      !get_u64(env, argv[1], &mesh))
+    return nullptr;
+  void *bvh = nullptr;
+  float *tri = nullptr;
+  uint32_t n = 0;
+  if (throw_server(env, srv, lbvh_server_mesh(srv, mesh, &bvh, &tri, &n),
+                   "lbvh_server_mesh"))
+    return nullptr;
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "bvh", ptr_value(env, bvh));
+  napi_set_named_property(env, o, "tri", ptr_value(env, tri));
+  napi_set_named_property(env, o, "n", num_value(env, n));
+  return o;
+}
+
+napi_value ServerEvict(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  uint64_t mesh = 0;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv) ||
+      !get_u64(env, argv[1], &mesh))
+    return nullptr;
+  if (throw_server(env, srv, lbvh_server_evict(srv, mesh), "lbvh_server_evict"))
+    return nullptr;
+  return undefined(env);
+}
+
+napi_value ServerStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  lbvh_server *srv;
+  if (!get_args(env, info, 1, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  lbvh_server_stats st;
+  if (throw_server(env, srv, lbvh_server_stats_get(srv, &st),
+                   "lbvh_server_stats_get"))
+    return nullptr;
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "submitted", num_value(env, (double)st.submitted));
+  napi_set_named_property(env, o, "completed", num_value(env, (double)st.completed));
+  napi_set_named_property(env, o, "failed", num_value(env, (double)st.failed));
+  napi_set_named_property(env, o, "evictions", num_value(env, (double)st.evictions));
+  napi_set_named_property(env, o, "residentMeshes", num_value(env, (double)st.resident_meshes));
+  napi_set_named_property(env, o, "residentBytes", num_value(env, (double)st.resident_bytes));
+  napi_set_named_property(env, o, "bytesInUse", num_value(env, (double)st.bytes_in_use));
+  napi_set_named_property(env, o, "peakBytes", num_value(env, (double)st.peak_bytes));
+  napi_set_named_property(env, o, "queued", num_value(env, st.queued));
+  napi_set_named_property(env, o, "inFlight", num_value(env, st.in_flight));
+  return o;
+}
+
 // ---- synchronize: Promise settled after the stream drains ---------------------
 
 struct SyncWork {
@@ -523,6 +856,12 @@ napi_value Init(napi_env env, napi_value exports) {
       {"rearrange", Rearrange},         {"build", Build},
       {"refit", Refit},                 {"synchronize", Synchronize},
       {"profileEnable", ProfileEnable}, {"profileRead", ProfileRead},
+      {"buildBatched", BuildBatched},   {"autotune", Autotune},
+      {"raycast", Raycast},             {"createServer", CreateServer},
+      {"serverSetDivMode", ServerSetDivMode},
+      {"serverSubmit", ServerSubmit},   {"serverPoll", ServerPoll},
+      {"serverDrain", ServerDrain},     {"serverMesh", ServerMesh},
+      {"serverEvict", ServerEvict},     {"serverStats", ServerStats},
   };
   for (const Fn &f : fns) {
     napi_value v;

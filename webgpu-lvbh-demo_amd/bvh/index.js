@@ -73,6 +73,20 @@ class LBVHDevice {
   }
 
   bytesInUse () { return addon.bytesInUse(this.ctx) }
+
+  // Times the tree tile variants per power-of-two size bucket in
+  // [nMin, nMax] on device-synthesised meshes and keeps the fastest for
+  // later builds.  -> { bucketSize: { 256: ms, 512: ms, 1024: ms } }
+  autotune (nMin, nMax, reps) {
+    const ms = addon.autotune(this.ctx, nMin, nMax, reps || 3)
+    const out = {}
+    let bucket = 1
+    while (2 * bucket <= nMin) bucket *= 2
+    for (let i = 0; i < ms.length / 3; i++, bucket *= 2) {
+      out[bucket] = { 256: ms[3 * i], 512: ms[3 * i + 1], 1024: ms[3 * i + 2] }
+    }
+    return out
+  }
 }
 
 function createDevice (options) {
@@ -119,6 +133,127 @@ function initBVHBuild (device) {
     I_TRIANGLE_BUFFER.destroy()
     return { BVH_BUFFER, O_TRIANGLE_BUFFER }
   }
+}
+
+// ---- batched builds (no counterpart in the reference) ------------------------
+
+// initBVHBatchBuild(device).build(MESHES, BOUNDS) builds many independent
+// meshes in one graph-captured device pipeline.  MESHES: array of
+// { TRI_ARRAY, NUM_TRIS } (or Float32Arrays of 12 * n floats); BOUNDS: null
+// (per-mesh GPU bounds) or an array of MODEL_BOUNDS, one per mesh.  Mesh k's
+// nodes are BVH_BUFFER bytes [64 * OFFSETS[k], 64 * OFFSETS[k + 1]) and its
+// triangles O_TRIANGLE_BUFFER bytes [48 * OFFSETS[k], ...), each exactly the
+// single-mesh build's output.
+function initBVHBatchBuild (device) {
+  return { build }
+
+  async function build (MESHES, BOUNDS) {
+    const S = MESHES.length
+    const OFFSETS = new Uint32Array(S + 1)
+    const parts = MESHES.map(function (m) {
+      if (m instanceof Float32Array) return m
+      return toFloat32(m.TRI_ARRAY, m.NUM_TRIS)
+    })
+    for (let k = 0; k < S; k++) OFFSETS[k + 1] = OFFSETS[k] + parts[k].length / 12
+    const total = OFFSETS[S]
+    const tris = new Float32Array(12 * total)
+    for (let k = 0; k < S; k++) tris.set(parts[k], 12 * OFFSETS[k])
+    let bounds = null
+    if (BOUNDS !== undefined && BOUNDS !== null) {
+      bounds = new Float32Array(6 * S)
+      for (let k = 0; k < S; k++) bounds.set(toBounds(BOUNDS[k]), 6 * k)
+    }
+    const I_TRIANGLE_BUFFER = device.createBuffer({ size: total * 48 }, tris)
+    const BVH_BUFFER = device.createBuffer({ size: total * 64 })
+    const O_TRIANGLE_BUFFER = device.createBuffer({ size: total * 48 })
+    if (total > 0) {
+      addon.buildBatched(device.ctx, I_TRIANGLE_BUFFER.ptr, OFFSETS, bounds,
+        BVH_BUFFER.ptr, O_TRIANGLE_BUFFER.ptr, null, null)
+    }
+    await device.queue.onSubmittedWorkDone()
+    I_TRIANGLE_BUFFER.destroy()
+    return { BVH_BUFFER, O_TRIANGLE_BUFFER, OFFSETS }
+  }
+}
+
+// ---- consumer check: the path tracer's traversal (pathtracer.js:495-628) ----
+
+// Closest hits of RAYS (Float32Array, 8 floats per ray: origin.xyz, pad,
+// dir.xyz, pad) through a built BVH; resolves to a Float32Array of 4 floats
+// per ray (normal.xyz facing the ray, distance; -1 for a miss).
+async function raycast (device, BVH_BUFFER, O_TRIANGLE_BUFFER, NUM_TRIS, RAYS) {
+  const count = RAYS.length / 8
+  const R = device.createBuffer({ size: RAYS.length * 4 }, RAYS)
+  const H = device.createBuffer({ size: count * 16 })
+  if (count > 0) {
+    addon.raycast(device.ctx, BVH_BUFFER.ptr, O_TRIANGLE_BUFFER.ptr, NUM_TRIS,
+      R.ptr, count, H.ptr)
+  }
+  const hits = new Float32Array(await H.read())
+  R.destroy()
+  H.destroy()
+  return hits
+}
+
+// ---- serving (no counterpart in the reference) ---------------------------------
+
+const REQUEST_KINDS = { build: 0, refit: 1 }
+const SOURCES = { host: 0, device: 1, uniform: 2, jitter: 3 }
+
+// A device-resident build/refit service: requests overlap on the GPU in
+// execution slots, are admitted by HBM budget (least recently used idle
+// meshes are evicted), and built trees stay resident.
+class BVHServer {
+  constructor (options) {
+    const o = options || {}
+    this.ordinal = o.ordinal || 0
+    this.srv = addon.createServer(this.ordinal, o.hbmBudget || 0, o.slots || 4)
+    if (o.divMode) addon.serverSetDivMode(this.srv, DIV_MODES[o.divMode])
+    this.reader = null
+  }
+
+  // request: { kind: 'build' | 'refit', meshId, source: 'host' | 'device' |
+  // 'uniform' | 'jitter', tri (Float32Array for 'host', LBVHBuffer for
+  // 'device'), n, seed, param } -> request id (BigInt)
+  submit (request) {
+    const r = request
+    const kind = REQUEST_KINDS[r.kind]
+    const source = SOURCES[r.source || 'host']
+    if (kind === undefined || source === undefined) {
+      throw new TypeError('unknown request kind or source')
+    }
+    let tri = null
+    if (r.source === 'device') tri = r.tri.ptr
+    else if (r.tri !== undefined && r.tri !== null) tri = toFloat32(r.tri, r.n)
+    return addon.serverSubmit(this.srv, kind, BigInt(r.meshId), source, tri, r.n,
+      BigInt(r.seed || 0), r.param || 0)
+  }
+
+  // Completions not yet returned: [{ requestId, meshId, kind, status, n,
+  // slot, submitS, startS, doneS }]; times in seconds (steady clock).
+  poll () { return addon.serverPoll(this.srv) }
+
+  drain () { return addon.serverDrain(this.srv) }
+
+  evict (meshId) { addon.serverEvict(this.srv, BigInt(meshId)) }
+
+  stats () { return addon.serverStats(this.srv) }
+
+  // The resident mesh's BVH_BUFFER and O_TRIANGLE_BUFFER bytes.
+  async readMesh (meshId) {
+    const m = addon.serverMesh(this.srv, BigInt(meshId))
+    if (this.reader === null) this.reader = createDevice({ ordinal: this.ordinal })
+    const ctx = this.reader.ctx
+    return {
+      n: m.n,
+      bvh: addon.download(ctx, m.bvh, 64 * m.n),
+      otri: addon.download(ctx, m.tri, 48 * m.n)
+    }
+  }
+}
+
+function createServer (options) {
+  return new BVHServer(options)
 }
 
 // ---- per-stage shims (the reference's execute() functions) -----------------
@@ -231,6 +366,10 @@ function installGlobals (target) {
 module.exports = {
   createDevice,
   initBVHBuild,
+  initBVHBatchBuild,
+  raycast,
+  createServer,
+  BVHServer,
   initAABB_ZidxKernel,
   initRadixSortKernel,
   initRadixTreeKernel,
