@@ -267,8 +267,9 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // leaves in order.  Each has its parent on one of the two root paths through
 // the tile's edge gaps, and a root path of the position-augmented keys has
 // at most 64 nodes, so a tile has at most kFrontierMax of them.  They are
-// written, in leaf order, as "items" (leaf range + box) into the tile's own
-// kFrontierMax slots, with their count.
+// written, in leaf order, as "items" (leaf range, box, the deltas at the
+// range's two boundary gaps) into the tile's own kFrontierMax slots, with
+// their count.
 //
 // Level L >= 1 runs the same agglomeration over items: one block of
 // kItemTile threads takes the frontiers of kFanIn consecutive tiles of level
@@ -392,9 +393,11 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
 }
 
 // Frontier output shared by both level kinds: the stopped subtrees that
-// leave the tile, in item order, into the tile's kFrontierMax slots.
-__device__ __forceinline__ void emit_frontier(bool out, int l, int r,
-                                              const float mn[3],
+// leave the tile, in item order, into the tile's kFrontierMax slots.  An item
+// is its leaf range, its box and the deltas at the gaps on either side of
+// the range, so the level above needs no sorted keys.
+__device__ __forceinline__ void emit_frontier(bool out, int l, int r, int dl,
+                                              int dr, const float mn[3],
                                               const float mx[3],
                                               uint32_t *wsum, uint32_t gtile,
                                               int4 *items_out,
@@ -407,10 +410,11 @@ __device__ __forceinline__ void emit_frontier(bool out, int l, int r,
     if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
   }
   if (out && rank < (uint32_t)kFrontierMax) {
-    int4 *io = items_out + 2 * ((size_t)gtile * kFrontierMax + rank);
+    int4 *io = items_out + kItemInt4 * ((size_t)gtile * kFrontierMax + rank);
     io[0] = make_int4(l, r, __float_as_int(mn[0]), __float_as_int(mn[1]));
     io[1] = make_int4(__float_as_int(mn[2]), __float_as_int(mx[0]),
                       __float_as_int(mx[1]), __float_as_int(mx[2]));
+    io[2] = make_int4(dl, dr, 0, 0);
   }
 }
 
@@ -582,8 +586,9 @@ __global__ __launch_bounds__(T) void k_leaves(
 
   const bool out = active && (c.stop == 2 ||
                               (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-  emit_frontier(out, c.l, c.r, c.mn, c.mx, L.wsum, gtile, items_out,
-                count_out, ctl, tid, T);
+  emit_frontier(out, c.l, c.r, out ? L.dgap[c.l - ts] : 0,
+                out ? L.dgap[c.r - ts + 1] : 0, c.mn, c.mx, L.wsum, gtile,
+                items_out, count_out, ctl, tid, T);
 }
 
 // ---- levels >= 1: items ---------------------------------------------------
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(T) void k_leaves(
 // cover many leaves, so nodes are written straight from the climb (they are
 // few: at most kFrontierMax per tile of the level below).
 __global__ __launch_bounds__(kItemTile) void k_items(
-    const uint32_t *__restrict__ skeys_all, uint32_t n_all,
+    uint32_t n_all,
     const int4 *__restrict__ items_in, const uint32_t *__restrict__ count_in,
     uint32_t prev_tiles_all, int4 *__restrict__ items_out,
     uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
@@ -614,7 +619,6 @@ __global__ __launch_bounds__(kItemTile) void k_items(
     prev_tiles = segs.prev_tiles[sg];
   }
   const uint32_t tile = gtile - tile0;  // tile within the mesh
-  const uint32_t *skeys = skeys_all + leaf_base;
   int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
   const uint32_t src0 = prev0 + tile * kFanIn;  // first source tile
   if (tid == 0) {
@@ -649,26 +653,25 @@ __global__ __launch_bounds__(kItemTile) void k_items(
     while (s + 1 < kFanIn && (int)L.pre[s + 1] <= k) ++s;
     const size_t it =
         (size_t)(src0 + s) * kFrontierMax + (uint32_t)(k - (int)L.pre[s]);
-    const int4 a = items_in[2 * it], b = items_in[2 * it + 1];
+    const int4 a = items_in[kItemInt4 * it], b = items_in[kItemInt4 * it + 1],
+               g = items_in[kItemInt4 * it + 2];
     c.l = a.x; c.r = a.y;
     c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
     c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
     c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
     L.il[tid] = c.l;
     L.ir[tid] = c.r;
-    L.dgap[tid + 1] = ((uint32_t)c.r + 1 < N)
-        ? delta_next(skeys[c.r], skeys[c.r + 1], (uint32_t)c.r) : -1;
-    if (tid == 0)
-      L.dgap[0] = (c.l > 0)
-          ? delta_next(skeys[c.l - 1], skeys[c.l], (uint32_t)(c.l - 1)) : -1;
+    L.dgap[tid + 1] = g.y;        // delta at the gap after the item
+    if (tid == 0) L.dgap[0] = g.x;  // and before the first one
   }
   __syncthreads();
   if (active) agglomerate<T>(c, L, (int)N, ts, te, nodes);
   __syncthreads();
   const bool out = active && (c.stop == 2 ||
                               (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-  emit_frontier(out, c.l, c.r, c.mn, c.mx, L.wsum, gtile, items_out,
-                count_out, ctl, tid, T);
+  emit_frontier(out, c.l, c.r, out ? L.dgap[c.ia - ts] : 0,
+                out ? L.dgap[c.ib - ts + 1] : 0, c.mn, c.mx, L.wsum, gtile,
+                items_out, count_out, ctl, tid, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -745,7 +748,7 @@ void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
   int4 *o4 = reinterpret_cast<int4 *>(otri);
   int4 *n4 = reinterpret_cast<int4 *>(nodes);
   if (level > 0) {
-    hipLaunchKernelGGL(k_items, dim3(grid), dim3(kItemTile), 0, s, skeys, n,
+    hipLaunchKernelGGL(k_items, dim3(grid), dim3(kItemTile), 0, s, n,
                        items_in, count_in, prev_tiles, items_out, count_out,
                        n4, ctl, segs);
     return;

@@ -272,7 +272,8 @@ int tile_for(const lbvh_ctx *ctx, uint32_t n) {
 int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n, int tile) {
   int st;
   const uint64_t tiles = tree_tiles_total(n, tile);
-  if ((st = ensure(ctx, ctx->tree_items, 32ull * kTreeFrontier * tiles, false)))
+  if ((st = ensure(ctx, ctx->tree_items, 16ull * kItemInt4 * kTreeFrontier * tiles,
+                   false)))
     return st;
   return ensure(ctx, ctx->tree_counts, 4ull * tiles, false);
 }
@@ -317,8 +318,8 @@ void run_tree(lbvh_ctx *ctx, const float *tri, const uint32_t *skeys,
     StageScope sc(ctx, L ? LBVH_STAGE_TREE_TOP : LBVH_STAGE_TREE);
     launch_tree_level(L, tile, grid[L], reinterpret_cast<const float4 *>(tri),
                       skeys, perm, n, reinterpret_cast<float4 *>(tri_out),
-                      items + 2 * kTreeFrontier * prev, counts + prev,
-                      L ? grid[L - 1] : 0u, items + 2 * kTreeFrontier * base,
+                      items + kItemInt4 * kTreeFrontier * prev, counts + prev,
+                      L ? grid[L - 1] : 0u, items + kItemInt4 * kTreeFrontier * base,
                       counts + base, static_cast<float4 *>(bvh_out), ctx->ctl,
                       TreeSegs{}, ctx->stream);
     prev = base;
@@ -462,8 +463,8 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
                       T + P.o_seg_off, T + P.o_lprev0[L], T + P.o_lprev_n[L]};
     launch_tree_level(L, P.tile, P.level_tiles[L], t4, out, vout, P.total,
                       reinterpret_cast<float4 *>(tri_out),
-                      items + 2 * kTreeFrontier * prev, counts + prev, 0u,
-                      items + 2 * kTreeFrontier * base, counts + base,
+                      items + kItemInt4 * kTreeFrontier * prev, counts + prev, 0u,
+                      items + kItemInt4 * kTreeFrontier * base, counts + base,
                       static_cast<float4 *>(bvh_out), ctx->ctl, ts,
                       ctx->stream);
     prev = base;
@@ -953,8 +954,8 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                    false)))
     return st;
   if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;
-  if ((st = ensure(ctx, ctx->b_tree_items, 32ull * kTreeFrontier * P.tree_tiles,
-                   false)))
+  if ((st = ensure(ctx, ctx->b_tree_items,
+                   16ull * kItemInt4 * kTreeFrontier * P.tree_tiles, false)))
     return st;
   if ((st = ensure(ctx, ctx->b_tree_counts, 4ull * P.tree_tiles, false)))
     return st;

@@ -18,6 +18,7 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
 constexpr int kTreeTileDefault = 1024;  // leaves per tile of the fused up pass
 constexpr int kMaxTreeLevels = 16;      // agglomeration levels (6 at 2^24, T=1024)
 constexpr int kTreeFrontier = 128;      // item slots per agglomeration tile
+constexpr int kItemInt4 = 3;            // 48-byte items: range, box, gap deltas
 
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
 constexpr int kBoundsBlocks = 2048;  // grid of the bounds reduction
@@ -138,8 +139,8 @@ struct TreeSegs {
 // Fused leaf gather + radix tree + node boxes by multi-level agglomeration:
 // level 0 over tiles of `tile` (256, 512 or 1024) sorted leaves, then item
 // levels until one tile remains.  Tiles are numbered level after level; the
-// item slots hold kTreeFrontier 32-byte items per tile and the counts one
-// word per tile (tree_tiles_total(n, tile) tiles).
+// item slots hold kTreeFrontier items of kItemInt4 int4 per tile and the
+// counts one word per tile (tree_tiles_total(n, tile) tiles).
 //
 // Tiles of each level of an n-leaf build (returns the number of levels).
 int tree_levels(uint32_t n, int tile, uint32_t *grid);

@@ -43,10 +43,10 @@ double now_s() {
 constexpr uint64_t kMeshBytesPerTri = 168;
 // Scratch a slot may need to grow to for a build / refit, with the arena's
 // 25% headroom: keys and indices 16, box centres 12, sort tile counts and
-// offsets 1.25, agglomeration item slots up to 18.3 (128 x 32 B per 256-leaf
+// offsets 1.25, agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf
 // tile, all levels).
-constexpr uint64_t kBuildScratchPerTri = 64;
-constexpr uint64_t kRefitScratchPerTri = 24;
+constexpr uint64_t kBuildScratchPerTri = 72;
+constexpr uint64_t kRefitScratchPerTri = 36;
 
 struct Mesh {
   uint32_t n = 0;
