@@ -53,6 +53,20 @@ __device__ __forceinline__ TileRange sort_tile_range(uint32_t tile, uint32_t n,
   return {b, min(b + (uint32_t)kSortTile, se), sb, se};
 }
 
+// Adds one digit to a block's LDS histogram; a wave whose active lanes all
+// hold the same digit adds its lane count once.
+__device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
+  const uint64_t active = __ballot(1);
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  const uint32_t d0 = __shfl(d, leader, 64);
+  if (__ballot(d == d0) == active) {
+    if ((int)(threadIdx.x & 63) == leader)
+      atomicAdd(&s_hist[d0], (uint32_t)__popcll(active));
+  } else {
+    atomicAdd(&s_hist[d], 1u);
+  }
+}
+
 // Number of set bits of `mask` in lanes below the calling lane.
 __device__ __forceinline__ uint32_t popc_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

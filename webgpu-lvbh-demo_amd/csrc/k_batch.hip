@@ -33,16 +33,13 @@ __device__ __forceinline__ uint32_t to_u32_sat(float v) {
   return (uint32_t)v;
 }
 
-// Bumps the epoch and resets the per-mesh digit histograms and (when the
-// GPU computes them) the per-mesh bounds accumulators.
+// Bumps the epoch and (when the GPU computes the bounds) resets the
+// per-mesh bounds accumulators.
 __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
-    uint32_t *seg_hist, uint32_t *seg_ord, uint32_t num_segs, int init_bounds,
-    Ctl *ctl) {
+    uint32_t *seg_ord, uint32_t num_segs, int init_bounds, Ctl *ctl) {
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t hwords = (uint64_t)num_segs * kSortPasses * kRadix;
-  for (uint64_t i = first; i < hwords; i += stride) seg_hist[i] = 0u;
   if (init_bounds)
     for (uint64_t i = first; i < 8ull * num_segs; i += stride)
       seg_ord[i] = ((i & 7) < 3) ? 0xFFFFFFFFu : 0u;  // min slots, max slots
@@ -94,24 +91,22 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
   }
 }
 
-// Per-mesh leaf AABB centre -> Morton key, plus the mesh's digit histograms
-// and the first sort pass's per-tile digit counts (cnt0[d * tiles + tile]).
-// Bounds: host-provided fp32 (6 per mesh; centres from the triangles) or the
-// GPU accumulators (centres from k_batch_bounds, cen non-null).
+// Per-mesh leaf AABB centre -> Morton key, plus the first sort pass's
+// per-tile digit counts (cnt0[d * tiles + tile]).  Bounds: host-provided fp32
+// (6 per mesh; centres from the triangles) or the GPU accumulators (centres
+// from k_batch_bounds, cen non-null).
 __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     const float4 *__restrict__ tri, const float *__restrict__ cen,
     uint32_t total, SortSegs segs, const float *__restrict__ seg_bounds,
     const uint32_t *__restrict__ seg_ord, int div_mode,
-    uint32_t *__restrict__ keys, uint32_t *seg_hist,
-    uint32_t *__restrict__ cnt0) {
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ cnt0) {
   static_assert(kSortThreads == kRadix, "one thread per digit for cnt0");
-  __shared__ uint32_t s_hist[kSortPasses][kRadix];
+  __shared__ uint32_t s_hist[kRadix];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
   const TileRange r = sort_tile_range(tile, 0u, segs);
   const uint32_t base = r.begin, end = r.end;
-  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += kSortThreads)
-    (&s_hist[0][0])[k] = 0u;
+  s_hist[threadIdx.x] = 0u;
   float bmin[3], ext[3], rcp[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -156,26 +151,20 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     }
     const uint32_t key = split3(q[0]) | (split3(q[1]) << 1) | (split3(q[2]) << 2);
     keys[i] = key;
-    hist_add(s_hist, key);
+    digit_add(s_hist, key & (kRadix - 1));
   }
   __syncthreads();
-  uint32_t *h = seg_hist + (size_t)sg * kSortPasses * kRadix;
-  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += kSortThreads) {
-    const uint32_t v = (&s_hist[0][0])[k];
-    if (v) atomicAdd(&h[k], v);
-  }
-  cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[0][threadIdx.x];
+  cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
 }
 
-void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
-                           uint32_t num_segs, bool init_bounds, Ctl *ctl,
-                           hipStream_t s) {
-  uint64_t words = (uint64_t)num_segs * kSortPasses * kRadix;
+void launch_batch_prologue(uint32_t *seg_ord, uint32_t num_segs,
+                           bool init_bounds, Ctl *ctl, hipStream_t s) {
+  const uint64_t words = 8ull * num_segs;
   uint32_t blocks = (uint32_t)((words + kMapThreads - 1) / kMapThreads);
   if (blocks > 1024) blocks = 1024;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_batch_prologue, dim3(blocks), dim3(kMapThreads), 0, s,
-                     seg_hist, seg_ord, num_segs, init_bounds ? 1 : 0, ctl);
+                     seg_ord, num_segs, init_bounds ? 1 : 0, ctl);
 }
 
 void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
@@ -188,11 +177,11 @@ void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
 void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
                          const SortSegs &segs, uint32_t tiles,
                          const float *seg_bounds, const uint32_t *seg_ord,
-                         int div_mode, uint32_t *keys, uint32_t *seg_hist,
-                         uint32_t *cnt0, hipStream_t s) {
+                         int div_mode, uint32_t *keys, uint32_t *cnt0,
+                         hipStream_t s) {
   hipLaunchKernelGGL(k_batch_morton, dim3(tiles), dim3(kSortThreads), 0, s,
                      tri, cen, total, segs, seg_bounds, seg_ord, div_mode,
-                     keys, seg_hist, cnt0);
+                     keys, cnt0);
 }
 
 }  // namespace lbvh

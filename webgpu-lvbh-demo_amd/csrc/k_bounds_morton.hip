@@ -123,22 +123,20 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds_final(
 //   rel  = (cen - bmin) / (bmax - bmin)               (:192, per div mode)
 //   key  = morton(u32(rel * 1023.99f))                (:194-208)
 // Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
-// fused build, the sort's per-tile digit counts of all four passes,
-// cnt[(p * kRadix + d) * tiles + tile] (saving the sort a read of the keys;
-// pass 0 uses them as they are, the others only through their column sums,
-// the global histograms).  With cnt the grid is one block per sort tile
-// (XCD-aware ids, as the sort's), without it a grid-stride loop.
+// fused build, the first sort pass's per-tile digit counts,
+// cnt[d * tiles + tile] (saving the sort a read of the keys).  With cnt the
+// grid is one block per sort tile (XCD-aware ids, as the sort's), without it
+// a grid-stride loop.
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
     uint32_t *__restrict__ keys, Ctl *ctl, int bump_epoch,
     uint32_t *__restrict__ cnt) {
   static_assert(kMapThreads == kRadix, "one thread per digit for cnt");
-  __shared__ uint32_t s_hist[kSortPasses][kRadix];
+  __shared__ uint32_t s_hist[kRadix];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   if (cnt) {
-    for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
-      (&s_hist[0][0])[k] = 0u;
+    s_hist[threadIdx.x] = 0u;
     __syncthreads();
   }
   float bmin[3], ext[3], rcp[3];
@@ -187,26 +185,23 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
       aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
       aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
     }
-    if (cnt) hist_add(s_hist, key);
+    if (cnt) digit_add(s_hist, key & (kRadix - 1));
   }
   if (cnt) {
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kSortPasses; ++q)
-      cnt[((size_t)q * kRadix + threadIdx.x) * gridDim.x + tile] =
-          s_hist[q][threadIdx.x];
+    cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
   }
 }
 
 // compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
-// :192-195 onward; the centre of :191 is already rounded the same way).  One
-// block per sort tile, as k_aabb_morton with cnt; bounds from ctl.
+// :192-195 onward; the centre of :191 is already rounded the same way), plus
+// the first sort pass's per-tile digit counts.  One block per sort tile, as
+// k_aabb_morton with cnt; bounds from ctl.
 __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     const float *__restrict__ cen, uint32_t n, int div_mode,
     uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt) {
-  __shared__ uint32_t s_hist[kSortPasses][kRadix];
-  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
-    (&s_hist[0][0])[k] = 0u;
+  __shared__ uint32_t s_hist[kRadix];
+  s_hist[threadIdx.x] = 0u;
   float bmin[3], ext[3], rcp[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -229,13 +224,10 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     }
     const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
     keys[i] = key;
-    hist_add(s_hist, key);
+    digit_add(s_hist, key & (kRadix - 1));
   }
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < kSortPasses; ++p)
-    cnt[((size_t)p * kRadix + threadIdx.x) * gridDim.x + tile] =
-        s_hist[p][threadIdx.x];
+  cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
 }
 
 __global__ void k_copy_bounds(const Ctl *ctl, float *out) {

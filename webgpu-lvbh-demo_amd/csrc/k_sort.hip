@@ -8,7 +8,7 @@
 //
 // Each pass is reduce-then-scan over tiles of kSortTile keys:
 //   count    per-tile digit counts (pass 0's are written by the Morton
-//            kernel as it produces the keys);
+//            kernel as it produces the keys), and their column sums;
 //   scan     one block per digit scans that digit's column of tile counts
 //            and adds the digit's base -> the first output position of every
 //            (digit, tile) run;
@@ -20,11 +20,11 @@
 // coherent cross-XCD load; with thousands of tiles the chain, not HBM, set
 // the pass time.  Kernel boundaries are the only global synchronisation.
 //
-// The global digit histograms (the digit bases of a single mesh) are the
-// column sums of per-tile counts of all four passes, which the Morton kernel
-// writes as it produces the keys; no device-scope atomics anywhere.  Counts
-// and offsets are digit-major: entry (d, t) at d * tiles + t, so the scan
-// reads and writes columns contiguously.  Tile ids are XCD-aware
+// A pass's global digit histogram (the digit bases of a single mesh) is the
+// column sums of its per-tile counts, and a batched mesh's bases come from
+// its own tiles' counts: no device-scope atomics anywhere.  Counts and
+// offsets are digit-major: entry (d, t) at d * tiles + t, so the scan reads
+// and writes columns contiguously.  Tile ids are XCD-aware
 // (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
 // neighbouring tiles share -- count columns and the boundaries of their digit
 // runs in the output -- are merged in one L2.
@@ -36,38 +36,17 @@ constexpr int kWaves = kSortThreads / 64;
 static_assert(kSortThreads == kRadix, "one thread per digit");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 
-// Per-tile digit counts of every pass (the stand-alone sort; the build's
-// Morton kernel writes the same counts): cnt[(p * kRadix + d) * tiles + t].
-__global__ __launch_bounds__(kSortThreads) void k_tile_hist(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ cnt) {
-  __shared__ uint32_t s_hist[kSortPasses][kRadix];
-  const uint32_t tiles = gridDim.x;
-  const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  for (int k = threadIdx.x; k < kSortPasses * kRadix; k += blockDim.x)
-    (&s_hist[0][0])[k] = 0u;
-  __syncthreads();
-  const uint32_t end = min((tile + 1) * (uint32_t)kSortTile, n);
-  for (uint32_t i = tile * kSortTile + threadIdx.x; i < end; i += kSortThreads)
-    hist_add(s_hist, keys[i]);
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kSortPasses; ++q)
-    cnt[((size_t)q * kRadix + threadIdx.x) * tiles + tile] =
-        s_hist[q][threadIdx.x];
-}
-
-// Global digit histograms as column sums of the per-tile counts: block
-// (p, d) sums column p * kRadix + d.  Plain stores: ctl->hist is rewritten by
-// every single-mesh sort, and no device-scope atomics are involved.
+// The global histogram of one pass's digit: column sums of its per-tile
+// counts (block d sums column d).  Plain stores into ctl->hist[pass]: no
+// device-scope atomics, and each sort rewrites it.
 __global__ __launch_bounds__(kSortThreads) void k_digit_totals(
-    const uint32_t *__restrict__ cnt, uint32_t tiles, Ctl *ctl) {
+    const uint32_t *__restrict__ cnt, uint32_t tiles, uint32_t *hist) {
   __shared__ uint32_t s_wsum[16];
   const uint32_t *col = cnt + (size_t)blockIdx.x * tiles;
   uint32_t sum = 0;
   for (uint32_t t = threadIdx.x; t < tiles; t += kSortThreads) sum += col[t];
   const uint32_t ex = block_exclusive_scan(sum, s_wsum);
-  if (threadIdx.x == kSortThreads - 1)
-    (&ctl->hist[0][0])[blockIdx.x] = ex + sum;
+  if (threadIdx.x == kSortThreads - 1) hist[blockIdx.x] = ex + sum;
 }
 
 // Per-tile digit counts of one pass -> cnt[d * tiles + tile].
@@ -80,19 +59,8 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
   s_c[threadIdx.x] = 0u;
   __syncthreads();
   const TileRange r = sort_tile_range(tile, n, segs);
-  const int lane = threadIdx.x & 63;
-  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads) {
-    const uint32_t d = (keys[i] >> shift) & (kRadix - 1);
-    // a wave whose active lanes share one digit adds once
-    const uint64_t active = __ballot(1);
-    const int leader = __ffsll((unsigned long long)active) - 1;
-    const uint32_t d0 = __shfl(d, leader, 64);
-    if (__ballot(d == d0) == active) {
-      if (lane == leader) atomicAdd(&s_c[d0], (uint32_t)__popcll(active));
-    } else {
-      atomicAdd(&s_c[d], 1u);
-    }
-  }
+  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads)
+    digit_add(s_c, (keys[i] >> shift) & (kRadix - 1));
   __syncthreads();
   cnt[(size_t)threadIdx.x * tiles + tile] = s_c[threadIdx.x];
 }
@@ -103,8 +71,7 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
 //              digit (exclusive scan of the global histogram);
 //   segments:  off[d][t] = base(s, d) + P(t) - P(t0(s)) for the segment s of
 //              tile t with first tile t0(s); seg_base[s][d] holds base(s, d)
-//              (the segment's first key + its keys with a smaller digit;
-//              stride kSortPasses * kRadix per segment) and
+//              (the segment's first key + its keys with a smaller digit) and
 //              is rewritten to base(s, d) - P(t0(s)) (mod 2^32) by the thread
 //              owning the segment's first tile.  Only block d touches column
 //              d of seg_base, so the rewrite needs just this block's barrier.
@@ -163,7 +130,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
         if (t < tiles) {
           const uint32_t sg = segs.tile_seg[t];
           if (t == segs.seg_tile0[sg])
-            seg_base[(size_t)sg * kSortPasses * kRadix + d] -= p[j];
+            seg_base[(size_t)sg * kRadix + d] -= p[j];
         }
       }
     }
@@ -173,7 +140,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
       const uint32_t t = t0 + j;
       uint32_t o = p[j];
       if (segmented && t < tiles)
-        o += seg_base[(size_t)segs.tile_seg[t] * kSortPasses * kRadix + d];
+        o += seg_base[(size_t)segs.tile_seg[t] * kRadix + d];
       s_col[pad(threadIdx.x * kScanPer + j)] = o;
     }
     carry += s_carry;
@@ -187,19 +154,21 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
   }
 }
 
-// Segment digit bases, all passes: seg_hist[s][p][d] (counts) becomes
-// seg_off[s] + (keys of s whose digit p is below d).  One block per segment.
+// Segment digit bases of one pass from its per-tile counts: block s sums
+// column d over the segment's tiles (thread d), then seg_base[s][d] =
+// seg_off[s] + (keys of s whose digit is below d).
 __global__ __launch_bounds__(kSortThreads) void k_seg_bases(
-    uint32_t *seg_hist, const uint32_t *__restrict__ seg_off) {
+    const uint32_t *__restrict__ cnt, uint32_t tiles, SortSegs segs,
+    uint32_t num_segs, uint32_t *__restrict__ seg_base) {
   __shared__ uint32_t s_wsum[16];
   const uint32_t sg = blockIdx.x;
-  const uint32_t first = seg_off[sg];
-#pragma unroll
-  for (int p = 0; p < kSortPasses; ++p) {
-    uint32_t *h = seg_hist + ((size_t)sg * kSortPasses + p) * kRadix;
-    const uint32_t below = block_exclusive_scan(h[threadIdx.x], s_wsum);
-    h[threadIdx.x] = first + below;
-  }
+  const uint32_t t0 = segs.seg_tile0[sg];
+  const uint32_t t1 = (sg + 1 < num_segs) ? segs.seg_tile0[sg + 1] : tiles;
+  const uint32_t *col = cnt + (size_t)threadIdx.x * tiles;
+  uint32_t total = 0;
+  for (uint32_t t = t0; t < t1; ++t) total += col[t];
+  const uint32_t below = block_exclusive_scan(total, s_wsum);
+  seg_base[(size_t)sg * kRadix + threadIdx.x] = segs.seg_off[sg] + below;
 }
 
 // One digit pass.  kFirst: the payload is the position in the segment (the
@@ -308,33 +277,12 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 }
 
-void launch_tile_hist(const uint32_t *keys, uint32_t n, uint32_t *cnt,
-                      hipStream_t s) {
-  const uint32_t tiles = sort_tiles(n);
-  if (tiles == 0) return;
-  hipLaunchKernelGGL(k_tile_hist, dim3(tiles), dim3(kSortThreads), 0, s, keys,
-                     n, cnt);
-}
-
-void launch_digit_totals(const uint32_t *cnt, uint32_t tiles, Ctl *ctl,
-                         hipStream_t s) {
-  if (tiles == 0) return;
-  hipLaunchKernelGGL(k_digit_totals, dim3(kSortPasses * kRadix),
-                     dim3(kSortThreads), 0, s, cnt, tiles, ctl);
-}
-
-void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
-                      uint32_t num_segs, hipStream_t s) {
-  if (num_segs == 0) return;
-  hipLaunchKernelGGL(k_seg_bases, dim3(num_segs), dim3(kSortThreads), 0, s,
-                     seg_hist, seg_off);
-}
-
 void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                       uint32_t *keys_out, int32_t *vals_out, uint32_t n,
                       uint32_t tiles, uint32_t *cnt, uint32_t *off,
                       bool counted, Ctl *ctl, hipStream_t s,
-                      const SortSegs *segs) {
+                      const SortSegs *segs, uint32_t num_segs,
+                      uint32_t *seg_base) {
   if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
@@ -342,8 +290,12 @@ void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   if (!counted)
     hipLaunchKernelGGL(k_count, dim3(tiles), dim3(kSortThreads), 0, s, keys_in,
                        n, shift, cnt, sg);
-  uint32_t *seg_base = segs ? const_cast<uint32_t *>(sg.seg_hist) + pass * kRadix
-                            : nullptr;
+  if (segs)
+    hipLaunchKernelGGL(k_seg_bases, dim3(num_segs), dim3(kSortThreads), 0, s,
+                       cnt, tiles, sg, num_segs, seg_base);
+  else
+    hipLaunchKernelGGL(k_digit_totals, dim3(kRadix), dim3(kSortThreads), 0, s,
+                       cnt, tiles, ctl->hist[pass]);
   hipLaunchKernelGGL(k_scan_columns, dim3(kRadix), dim3(kSortThreads), 0, s,
                      cnt, off, tiles, ctl->hist[pass], sg, seg_base);
   if (pass == 0)

@@ -499,165 +499,96 @@ __device__ __forceinline__ void climb_leaves(Climb &c, LeafLds<T> &L, int n,
   c.pgap = pgap;
 }
 
-// Where a level-0 tile lies: its mesh's first leaf and size, and its leaf
-// range [ts, te) within the mesh.
-struct LeafTile {
-  uint32_t base, n;
-  int ts, te;
-};
+// Level 0: tile = T sorted leaf positions.  Gathers O_TRI and the leaf
+// boxes, climbs, writes the tile's nodes and its frontier.  With segments
+// (batched builds) every segment is an independent mesh: positions, node ids
+// and the permutation are relative to its first leaf.
 template <int T>
-__device__ __forceinline__ LeafTile leaf_tile(uint32_t g, uint32_t n_all,
-                                              const TreeSegs &segs) {
-  uint32_t tile0 = 0, base = 0, n = n_all;
-  if (segs.tile_seg) {
-    const uint32_t sg = segs.tile_seg[g];
-    tile0 = segs.seg_tile0[sg];
-    base = segs.seg_off[sg];
-    n = segs.seg_off[sg + 1] - base;
-  }
-  const uint32_t ts = (g - tile0) * T;
-  // block-uniform: keep them in scalar registers
-  return {(uint32_t)__builtin_amdgcn_readfirstlane(base),
-          (uint32_t)__builtin_amdgcn_readfirstlane(n),
-          __builtin_amdgcn_readfirstlane((int)ts),
-          __builtin_amdgcn_readfirstlane((int)min(ts + T, n))};
-}
-
-// Level 0: tiles of T sorted leaf positions.  For each tile: gather O_TRI
-// and the leaf boxes, climb, write the tile's nodes and its frontier.  With
-// segments (batched builds) every segment is an independent mesh: positions,
-// node ids and the permutation are relative to its first leaf.
-//
-// The grid is persistent (as many blocks as fit the device): a block walks
-// tiles g0 + j, g0 + j + nb, ... of one contiguous range per XCD-sized group
-// (so the blocks of one XCD work on neighbouring tiles and share L2 lines).
-// While it climbs one tile, the triangles of its next tile and the
-// permutation entries of the one after are already in flight, so the
-// gather's latency is hidden behind the LDS-bound climb.  No block waits for
-// another.
-template <int T>
-__global__ __launch_bounds__(T)
-__attribute__((amdgpu_waves_per_eu(8, 8))) void k_leaves(
+__global__ __launch_bounds__(T) void k_leaves(
     const int4 *__restrict__ tri_all, const uint32_t *__restrict__ skeys_all,
-    const int32_t *__restrict__ perm_all, uint32_t n_all, uint32_t tiles,
+    const int32_t *__restrict__ perm_all, uint32_t n_all,
     int4 *__restrict__ otri_all, int4 *__restrict__ items_out,
     uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
     TreeSegs segs) {
   __shared__ LeafLds<T> L;
   const int tid = threadIdx.x;
-  // tile schedule: X groups (the XCDs when the grid has >= 8 blocks), each
-  // owning a contiguous range of tiles, walked by its nb blocks
-  const uint32_t B = gridDim.x, X = B < kXcds ? B : kXcds;
-  const uint32_t x = blockIdx.x % X, j = blockIdx.x / X;
-  const uint32_t nb = (B - x + X - 1) / X;
-  const uint32_t q = tiles / X, rem = tiles % X;
-  const uint32_t g_end = x * q + (x < rem ? x : rem) + q + (x < rem ? 1u : 0u);
-  uint32_t g = x * q + (x < rem ? x : rem) + j;
-
-  // prologue: this tile's triangles, the next tile's permutation entry
-  LeafTile cur{}, nxt{};
-  int4 a = make_int4(0, 0, 0, 0), b = a, d = a;
-  int32_t p_next = 0;
-  if (g < g_end) {
-    cur = leaf_tile<T>(g, n_all, segs);
-    if (cur.ts + tid < cur.te) {
-      const uint32_t pi = (uint32_t)perm_all[cur.base + cur.ts + tid];
-      const size_t src = 3 * (size_t)(cur.base + (pi < cur.n ? pi : 0u));
-      a = tri_all[src]; b = tri_all[src + 1]; d = tri_all[src + 2];
-    }
-    if (g + nb < g_end) {
-      nxt = leaf_tile<T>(g + nb, n_all, segs);
-      if (nxt.ts + tid < nxt.te) p_next = perm_all[nxt.base + nxt.ts + tid];
-    }
+  const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
+  uint32_t tile0 = 0, leaf_base = 0, N = n_all;
+  if (segs.tile_seg) {
+    const uint32_t sg = segs.tile_seg[gtile];
+    tile0 = segs.seg_tile0[sg];
+    leaf_base = segs.seg_off[sg];
+    N = segs.seg_off[sg + 1] - leaf_base;
   }
+  const uint32_t tile = gtile - tile0;  // tile within the mesh
+  const uint32_t *skeys = skeys_all + leaf_base;
+  int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
+  const int ts = (int)(tile * T);
+  const int te = (int)min(tile * T + T, N);
+  const int k = ts + tid;
+  const bool active = k < te;
 
-  for (; g < g_end; g += nb) {
-    const uint32_t N = cur.n;
-    const uint32_t *skeys = skeys_all + cur.base;
-    int4 *nodes = nodes_all + 4 * (size_t)cur.base;
-    const int ts = cur.ts, te = cur.te;
-    const int k = ts + tid;
-    const bool active = k < te;
-
-    __syncthreads();  // the previous tile is done with the LDS
-    Climb c;
-    c.open = -1;
-    c.stop = 0;
-    c.l = c.r = k;
-    L.cnt[tid] = 0u;
-    L.gap_of[tid] = -1;
-    if (active) {
-      int4 *o = otri_all + 3 * (size_t)(cur.base + k);
-      o[0] = a;
-      o[1] = b;
-      o[2] = d;
-      tri_box(a, b, d, c.mn, c.mx);
-      perturb(c.mn, c.mx);
-      L.dgap[tid + 1] = ((uint32_t)k + 1 < N)
-          ? delta_next(skeys[k], skeys[k + 1], (uint32_t)k) : -1;
-      if (tid == 0)
-        L.dgap[0] = (k > 0)
-            ? delta_next(skeys[k - 1], skeys[k], (uint32_t)(k - 1)) : -1;
+  Climb c;
+  c.open = -1;
+  c.stop = 0;
+  c.l = c.r = k;
+  L.cnt[tid] = 0u;
+  L.gap_of[tid] = -1;
+  if (active) {
+    const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
+    const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
+    const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
+    int4 *o = otri_all + 3 * (size_t)(leaf_base + k);
+    o[0] = a;
+    o[1] = b;
+    o[2] = d;
+    tri_box(a, b, d, c.mn, c.mx);
+    perturb(c.mn, c.mx);
+    L.dgap[tid + 1] = ((uint32_t)k + 1 < N)
+        ? delta_next(skeys[k], skeys[k + 1], (uint32_t)k) : -1;
+    if (tid == 0)
+      L.dgap[0] = (k > 0) ? delta_next(skeys[k - 1], skeys[k], (uint32_t)(k - 1))
+                          : -1;
+  }
+  __syncthreads();
+  if (N == 1) {
+    if (k == 0) {
+      write_single_leaf_node(nodes, c.mn, c.mx);
+      count_out[gtile] = 0u;
     }
-    __syncthreads();
+    return;
+  }
+  if ((uint32_t)k == N - 1) {
+    const int4 z = make_int4(0, 0, 0, 0);
+    int4 *last = nodes + 4 * (size_t)(N - 1);
+    last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+  }
+  if (active) climb_leaves<T>(c, L, (int)N, ts, te);
+  __syncthreads();
 
-    // prefetch: the next tile's triangles, the permutation of the one after
-    LeafTile nn{};
-    int32_t p_after = 0;
-    if (g + nb < g_end) {
-      if (nxt.ts + tid < nxt.te) {
-        const uint32_t pi = (uint32_t)p_next;
-        const size_t src = 3 * (size_t)(nxt.base + (pi < nxt.n ? pi : 0u));
-        a = tri_all[src]; b = tri_all[src + 1]; d = tri_all[src + 2];
-      }
-      if (g + 2 * nb < g_end) {
-        nn = leaf_tile<T>(g + 2 * nb, n_all, segs);
-        if (nn.ts + tid < nn.te) p_after = perm_all[nn.base + nn.ts + tid];
-      }
-    }
-
-    if (N == 1) {  // no internal node: the lone leaf fills node 0's l_* half
-      if (k == 0) {
-        write_single_leaf_node(nodes, c.mn, c.mx);
-        count_out[g] = 0u;
-      }
-    } else {
-      if ((uint32_t)k == N - 1) {
-        const int4 z = make_int4(0, 0, 0, 0);
-        int4 *last = nodes + 4 * (size_t)(N - 1);
-        last[0] = z; last[1] = z; last[2] = z; last[3] = z;
-      }
-      if (active) climb_leaves<T>(c, L, (int)N, ts, te);
-      __syncthreads();
-
-      // The tile's nodes in id order: int4 e of the tile's node range is
-      // quarter e % 4 of node ts + e / 4.  The max halves carried far leaves
-      // in .w; the node layout has the constant flags 2 (l_max), 0 (r_max).
+  // The tile's nodes in id order: int4 e of the tile's node range is
+  // quarter e % 4 of node ts + e / 4.  The max halves carried far leaves in
+  // .w; the node layout has the constant flags 2 (l_max) and 0 (r_max).
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj) {
-        const int e = jj * T + tid;
-        const int kk = e >> 2, qq = e & 3;
-        if (ts + kk < te) {
-          const int gg = L.gap_of[kk];
-          if (gg >= 0) {
-            int4 v = L.node[gg][qq];
-            if (qq == 1) v.w = 2;
-            if (qq == 3) v.w = 0;
-            nodes[4 * (size_t)(ts + kk) + qq] = v;
-          }
-        }
+  for (int j = 0; j < 4; ++j) {
+    const int e = j * T + tid;
+    const int kk = e >> 2, q = e & 3;
+    if (ts + kk < te) {
+      const int g = L.gap_of[kk];
+      if (g >= 0) {
+        int4 v = L.node[g][q];
+        if (q == 1) v.w = 2;
+        if (q == 3) v.w = 0;
+        nodes[4 * (size_t)(ts + kk) + q] = v;
       }
-
-      const bool out = active && (c.stop == 2 ||
-                                  (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-      emit_frontier(out, c.l, c.r, out ? L.dgap[c.l - ts] : 0,
-                    out ? L.dgap[c.r - ts + 1] : 0, c.mn, c.mx, L.wsum, g,
-                    items_out, count_out, ctl, tid, T);
     }
-    cur = nxt;
-    nxt = nn;
-    p_next = p_after;
   }
+
+  const bool out = active && (c.stop == 2 ||
+                              (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
+  emit_frontier(out, c.l, c.r, out ? L.dgap[c.l - ts] : 0,
+                out ? L.dgap[c.r - ts + 1] : 0, c.mn, c.mx, L.wsum, gtile,
+                items_out, count_out, ctl, tid, T);
 }
 
 // ---- levels >= 1: items ---------------------------------------------------
@@ -797,20 +728,14 @@ uint64_t tree_tiles_total(uint32_t n, int tile) {
   return t;
 }
 
-// Level 0 runs a persistent grid: the blocks that fit at once (LDS allows
-// 2 blocks of 1024 leaves per CU, 4 of 512, 8 of 256; 256 CUs), at most one
-// per tile.
 template <int T>
-static void launch_leaf_level(uint32_t tiles, const int4 *t4,
+static void launch_leaf_level(uint32_t grid, const int4 *t4,
                               const uint32_t *skeys, const int32_t *perm,
                               uint32_t n, int4 *o4, int4 *items_out,
                               uint32_t *count_out, int4 *n4, Ctl *ctl,
                               const TreeSegs &segs, hipStream_t s) {
-  constexpr uint32_t kResident = 256u * (2048u / T);
-  const uint32_t grid = tiles < kResident ? tiles : kResident;
-  hipLaunchKernelGGL(k_leaves<T>, dim3(grid ? grid : 1), dim3(T), 0, s, t4,
-                     skeys, perm, n, tiles, o4, items_out, count_out, n4, ctl,
-                     segs);
+  hipLaunchKernelGGL(k_leaves<T>, dim3(grid), dim3(T), 0, s, t4, skeys, perm,
+                     n, o4, items_out, count_out, n4, ctl, segs);
 }
 
 void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,

@@ -55,6 +55,7 @@ struct lbvh_ctx {
   DevBuf cen;                           // triangle box centres (GPU bounds)
   // batched builds: scratch, and the plan (tile tables + captured graph) of
   // the last batch layout
+  // b_hist: the segments' digit bases of the current sort pass
   DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_hist, b_ord,
       b_bounds, b_tree_items, b_tree_counts, b_table, b_cen;
   struct BatchPlan {
@@ -246,8 +247,8 @@ int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
   if ((st = ensure(ctx, ctx->keys_b, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_a, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
-  // per-tile counts of all passes, then the offsets of one pass
-  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix * (kSortPasses + 1);
+  // per-tile digit counts and offsets of one pass
+  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix * 2;
   return ensure(ctx, ctx->digits, 4ull * words, false);
 }
 
@@ -283,16 +284,14 @@ int ensure_partials(lbvh_ctx *ctx) {
 }
 
 // The four digit passes: in -> tmp -> out -> tmp -> out.  `in` may equal
-// `out` (the stand-alone sort is in place, like the reference's).  counted:
-// the Morton kernel already wrote the per-tile counts of every pass into the
-// digit buffer.  Passes 1-3 recount into pass 0's columns.
+// `out` (the stand-alone sort is in place, like the reference's).  counted0:
+// the Morton kernel already wrote pass 0's per-tile counts into the digit
+// buffer; the other passes count their own.
 void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted) {
+              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted0) {
   const uint32_t tiles = sort_tiles(n);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
-  uint32_t *off = cnt + (size_t)kRadix * kSortPasses * tiles;
-  if (!counted) launch_tile_hist(in, n, cnt, ctx->stream);
-  launch_digit_totals(cnt, tiles, ctx->ctl, ctx->stream);
+  uint32_t *off = cnt + (size_t)kRadix * tiles;
   const uint32_t *src[kSortPasses] = {in, tmp, out, tmp};
   uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
   const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
@@ -300,7 +299,7 @@ void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
   for (int p = 0; p < kSortPasses; ++p) {
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
     launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, tiles, cnt, off,
-                     p == 0, ctx->ctl, ctx->stream);
+                     p == 0 && counted0, ctx->ctl, ctx->stream);
   }
 }
 
@@ -424,12 +423,11 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
                    int32_t *perm) {
   const auto &P = ctx->plan;
   const uint32_t *T = static_cast<const uint32_t *>(ctx->b_table.ptr);
-  uint32_t *hist = static_cast<uint32_t *>(ctx->b_hist.ptr);
+  uint32_t *seg_base = static_cast<uint32_t *>(ctx->b_hist.ptr);
   uint32_t *ord = static_cast<uint32_t *>(ctx->b_ord.ptr);
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
-  const SortSegs ss{T + P.o_tile_seg, T + P.o_seg_tile0, T + P.o_seg_off,
-                    hist};
-  launch_batch_prologue(hist, ord, P.S, !have_bounds, ctx->ctl, ctx->stream);
+  const SortSegs ss{T + P.o_tile_seg, T + P.o_seg_tile0, T + P.o_seg_off};
+  launch_batch_prologue(ord, P.S, !have_bounds, ctx->ctl, ctx->stream);
   float *cen = have_bounds ? nullptr : static_cast<float *>(ctx->b_cen.ptr);
   if (!have_bounds)
     launch_batch_bounds(t4, ss, P.sort_tiles, ord, cen, P.total, ctx->stream);
@@ -441,9 +439,8 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   launch_batch_morton(t4, cen, P.total, ss, P.sort_tiles,
                       have_bounds ? static_cast<const float *>(ctx->b_bounds.ptr)
                                   : nullptr,
-                      ord, ctx->div_mode, keys, hist,
+                      ord, ctx->div_mode, keys,
                       static_cast<uint32_t *>(ctx->b_digits.ptr), ctx->stream);
-  launch_seg_bases(hist, T + P.o_seg_off, P.S, ctx->stream);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->b_digits.ptr);
   uint32_t *off = cnt + (size_t)kRadix * P.sort_tiles;
   const uint32_t *src[kSortPasses] = {keys, tmp, out, tmp};
@@ -453,7 +450,7 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   for (int p = 0; p < kSortPasses; ++p)
     launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
                      P.sort_tiles, cnt, off, p == 0, ctx->ctl, ctx->stream,
-                     &ss);
+                     &ss, P.S, seg_base);
   int4 *items = static_cast<int4 *>(ctx->b_tree_items.ptr);
   uint32_t *counts = static_cast<uint32_t *>(ctx->b_tree_counts.ptr);
   uint64_t base = 0, prev = 0;
@@ -950,7 +947,7 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if ((st = ensure(ctx, ctx->b_idx_b, 4ull * total, false))) return st;
   if ((st = ensure(ctx, ctx->b_digits, 8ull * kRadix * P.sort_tiles, false)))
     return st;
-  if ((st = ensure(ctx, ctx->b_hist, 4ull * kSortPasses * kRadix * num_meshes,
+  if ((st = ensure(ctx, ctx->b_hist, 4ull * kRadix * num_meshes,
                    false)))
     return st;
   if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;

@@ -34,7 +34,7 @@ constexpr uint32_t kSpinCap = 1u << 22;  // polls of a bounded wait (per-stage u
 //   fault       sticky flag: a bounded wait gave up or a frontier overflowed
 //               (never set in a correct run; read by lbvh_synchronize)
 //   bounds      scene bounds of the last reduction (k_bounds_final)
-//   hist        global digit counts of the current single-mesh sort
+//   hist        global digit counts of each pass of the current sort
 struct Ctl {
   uint32_t epoch;
   uint32_t fault;
@@ -70,8 +70,8 @@ void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
                        hipStream_t s);
-// cnt (kSortPasses x kRadix x sort_tiles(n) words, or null): also write the
-// sort's per-tile digit counts of every pass (see launch_tile_hist).
+// cnt (kRadix x sort_tiles(n) words, or null): also write the first sort
+// pass's per-tile digit counts.
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
                         Ctl *ctl, bool bump_epoch, hipStream_t s,
@@ -80,38 +80,29 @@ void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
 void launch_bump_epoch(Ctl *ctl, hipStream_t s);
 
 // k_sort.hip
-// Per-tile digit counts of every pass, cnt[(p * kRadix + d) * tiles + t]
-// (sort_tiles(n) tiles), and their column sums, the global digit histograms
-// ctl->hist (rewritten by each single-mesh sort).
-void launch_tile_hist(const uint32_t *keys, uint32_t n, uint32_t *cnt,
-                      hipStream_t s);
-void launch_digit_totals(const uint32_t *cnt, uint32_t tiles, Ctl *ctl,
-                         hipStream_t s);
 // Segments of a batched sort (all null: one segment [0, n)).  Tiles of
 // kSortTile keys never straddle segments; tile_seg maps each tile to its
 // segment, seg_tile0 gives a segment's first tile, seg_off its first key
-// (S+1 entries), seg_hist its kSortPasses x kRadix digit counts, turned
-// into digit bases by launch_seg_bases before the first pass.
+// (S+1 entries).
 struct SortSegs {
   const uint32_t *tile_seg;
   const uint32_t *seg_tile0;
   const uint32_t *seg_off;
-  uint32_t *seg_hist;
 };
-void launch_seg_bases(uint32_t *seg_hist, const uint32_t *seg_off,
-                      uint32_t num_segs, hipStream_t s);
 // One digit pass over `tiles` sort tiles (pass 0 takes the position in the
 // segment as payload and ignores vals_in).  cnt and off hold kRadix x tiles
 // words each; counted: cnt already holds this pass's per-tile digit counts
 // (the Morton kernels write pass 0's).  A single mesh takes its digit bases
-// from ctl->hist.  The four passes sort keys_in into keys_out through
-// keys_tmp:
+// from the column sums of cnt (into ctl->hist[pass]); segments from their
+// own tiles' sums (into seg_base, kRadix words per segment).  The four
+// passes sort keys_in into keys_out through keys_tmp:
 //   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
 void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                       uint32_t *keys_out, int32_t *vals_out, uint32_t n,
                       uint32_t tiles, uint32_t *cnt, uint32_t *off,
                       bool counted, Ctl *ctl, hipStream_t s,
-                      const SortSegs *segs = nullptr);
+                      const SortSegs *segs = nullptr, uint32_t num_segs = 0,
+                      uint32_t *seg_base = nullptr);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
@@ -156,19 +147,19 @@ void launch_tree_level(int level, int tile, uint32_t grid, const float4 *tri,
                        Ctl *ctl, const TreeSegs &segs, hipStream_t s);
 
 // k_batch.hip
-void launch_batch_prologue(uint32_t *seg_hist, uint32_t *seg_ord,
-                           uint32_t num_segs, bool init_bounds, Ctl *ctl,
-                           hipStream_t s);
+void launch_batch_prologue(uint32_t *seg_ord, uint32_t num_segs,
+                           bool init_bounds, Ctl *ctl, hipStream_t s);
 // Per-mesh bounds (into seg_ord) and box centres (3 x total floats).
 void launch_batch_bounds(const float4 *tri, const SortSegs &segs,
                          uint32_t tiles, uint32_t *seg_ord, float *cen,
                          uint32_t total, hipStream_t s);
-// Per-mesh Morton keys; centres from cen when non-null, else from tri.
+// Per-mesh Morton keys and the first sort pass's per-tile digit counts;
+// centres from cen when non-null, else from tri.
 void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
                          const SortSegs &segs, uint32_t tiles,
                          const float *seg_bounds, const uint32_t *seg_ord,
-                         int div_mode, uint32_t *keys, uint32_t *seg_hist,
-                         uint32_t *cnt0, hipStream_t s);
+                         int div_mode, uint32_t *keys, uint32_t *cnt0,
+                         hipStream_t s);
 
 // k_raycast.hip
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,

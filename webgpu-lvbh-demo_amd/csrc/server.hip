@@ -12,6 +12,7 @@
 // overlap across slots and with the GPU.  Nothing here waits on the GPU
 // except lbvh_server_drain.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -122,6 +123,12 @@ struct lbvh_server {
   std::condition_variable cv;       // wakes the dispatcher
   std::condition_variable idle_cv;  // wakes lbvh_server_drain
   std::deque<Request> queue;
+  // Submissions and completions cross their own small locks, so submit and
+  // poll never wait for the dispatcher's admission work under `mu`.
+  std::mutex in_mu;
+  std::deque<Request> inbox;
+  std::atomic<uint32_t> inbox_n{0};
+  std::mutex done_mu;
   std::deque<lbvh_completion> done;
   std::unordered_map<uint64_t, Mesh> meshes;
   uint64_t next_id = 1, tick = 0, resident_bytes = 0, peak = 0;
@@ -219,7 +226,10 @@ void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
   c.submit_s = r.submit_s;
   c.start_s = start;
   c.done_s = now_s();
-  s->done.push_back(c);
+  {
+    std::lock_guard<std::mutex> g(s->done_mu);
+    s->done.push_back(c);
+  }
   s->st.completed += 1;
   if (status != LBVH_OK) s->st.failed += 1;
 }
@@ -365,6 +375,14 @@ void dispatcher(lbvh_server *s) {
   std::unique_lock<std::mutex> lock(s->mu);
   while (!s->stop) {
     bool progress = false;
+    {
+      std::lock_guard<std::mutex> g(s->in_mu);
+      const uint32_t moved = (uint32_t)s->inbox.size();
+      for (auto &r : s->inbox) s->queue.push_back(std::move(r));
+      s->inbox.clear();
+      s->inbox_n -= moved;
+      s->st.submitted += moved;
+    }
     // Completions, in each slot's stream order.
     for (uint32_t k = 0; k < s->slots.size(); ++k) {
       Slot &sl = s->slots[k];
@@ -425,18 +443,18 @@ void dispatcher(lbvh_server *s) {
       progress = true;
     }
     const bool idle = all_idle(s);
-    if (idle && s->queue.empty()) s->idle_cv.notify_all();
+    if (idle && s->queue.empty() && s->inbox_n == 0) s->idle_cv.notify_all();
     if (progress) {
-      // let submit / poll / the workers in before the next round
+      // let the workers and drain in before the next round
       lock.unlock();
       std::this_thread::yield();
       lock.lock();
       continue;
     }
-    if (!idle)
-      s->cv.wait_for(lock, std::chrono::microseconds(20));
-    else
-      s->cv.wait(lock);
+    // submit notifies without taking `mu`; the timeout bounds a wake-up it
+    // might race past
+    if (s->inbox_n == 0)
+      s->cv.wait_for(lock, std::chrono::microseconds(idle ? 50 : 20));
   }
 }
 
@@ -534,8 +552,8 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
                        const float *tri, uint32_t n, uint64_t seed,
                        double param, uint64_t *out_request_id) {
   if (!s) return LBVH_ERR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lock(s->mu);
   if (kind != LBVH_REQ_BUILD && kind != LBVH_REQ_REFIT) {
+    std::lock_guard<std::mutex> lock(s->mu);
     s->last_error = "unknown request kind";
     return LBVH_ERR_INVALID_ARGUMENT;
   }
@@ -543,11 +561,11 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
       (source == LBVH_SRC_JITTER && kind != LBVH_REQ_REFIT) ||
       ((source == LBVH_SRC_HOST || source == LBVH_SRC_DEVICE) && !tri) ||
       n == 0 || n > LBVH_MAX_TRIS) {
+    std::lock_guard<std::mutex> lock(s->mu);
     s->last_error = "invalid source, pointer or triangle count";
     return LBVH_ERR_INVALID_ARGUMENT;
   }
   Request r;
-  r.id = s->next_id++;
   r.mesh_id = mesh_id;
   r.kind = kind;
   r.source = source;
@@ -557,9 +575,13 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
   r.submit_s = now_s();
   if (source == LBVH_SRC_HOST) r.host.assign(tri, tri + 12ull * n);
   if (source == LBVH_SRC_DEVICE) r.dev = tri;
-  if (out_request_id) *out_request_id = r.id;
-  s->queue.push_back(std::move(r));
-  s->st.submitted += 1;
+  {
+    std::lock_guard<std::mutex> lock(s->in_mu);
+    r.id = s->next_id++;
+    if (out_request_id) *out_request_id = r.id;
+    s->inbox.push_back(std::move(r));
+    s->inbox_n += 1;
+  }
   s->cv.notify_all();
   return LBVH_OK;
 }
@@ -567,7 +589,7 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
 int lbvh_server_poll(lbvh_server *s, lbvh_completion *out, uint32_t max,
                      uint32_t *out_count) {
   if (!s || !out_count || (max && !out)) return LBVH_ERR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lock(s->mu);
+  std::lock_guard<std::mutex> lock(s->done_mu);
   uint32_t k = 0;
   while (k < max && !s->done.empty()) {
     out[k++] = s->done.front();
@@ -581,7 +603,9 @@ int lbvh_server_drain(lbvh_server *s) {
   if (!s) return LBVH_ERR_INVALID_ARGUMENT;
   std::unique_lock<std::mutex> lock(s->mu);
   s->cv.notify_all();
-  s->idle_cv.wait(lock, [s] { return s->queue.empty() && all_idle(s); });
+  s->idle_cv.wait(lock, [s] {
+    return s->queue.empty() && s->inbox_n == 0 && all_idle(s);
+  });
   for (auto &sl : s->slots) {
     const int st = lbvh_synchronize(sl.ctx);  // also reports kernel faults
     if (st != LBVH_OK) {
@@ -623,12 +647,13 @@ int lbvh_server_stats_get(lbvh_server *s, lbvh_server_stats *out) {
   if (!s || !out) return LBVH_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(s->mu);
   *out = s->st;
+  out->submitted += s->inbox_n;  // not yet moved to the queue
   out->resident_meshes = 0;
   for (auto &kv : s->meshes) out->resident_meshes += kv.second.buf ? 1 : 0;
   out->resident_bytes = s->resident_bytes;
   out->bytes_in_use = in_use(s);
   out->peak_bytes = s->peak;
-  out->queued = (uint32_t)s->queue.size();
+  out->queued = (uint32_t)s->queue.size() + s->inbox_n;
   uint32_t busy = 0;
   for (auto &sl : s->slots) busy += (uint32_t)sl.depth();
   out->in_flight = busy;
