@@ -193,7 +193,7 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, requests))
     done = []
     t0 = time.perf_counter()
-    i, last_poll = 0, 0.0
+    i, last_poll, max_queued = 0, 0.0, 0
     while i < requests:
         now = time.perf_counter() - t0
         while i < requests and arrivals[i] <= now:
@@ -203,6 +203,7 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
         if now - last_poll > 1e-3:  # completions are timestamped server-side
             done += srv.poll()
             last_poll = now
+            max_queued = max(max_queued, srv.stats()["queued"])
         if i < requests:
             # sleep (not spin on the server's lock) until the next arrival;
             # arrivals that come due meanwhile are submitted together
@@ -215,6 +216,8 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
     st = srv.stats()
     srv.close()
     lat = np.array([c["done_s"] - c["submit_s"] for c in done]) * 1e3
+    wait = np.array([c["start_s"] - c["submit_s"] for c in done]) * 1e3
+    service = np.array([c["done_s"] - c["start_s"] for c in done]) * 1e3
     kinds = np.array([c["kind"] for c in done])
     ok = np.array([c["status"] == 0 for c in done])
 
@@ -232,6 +235,11 @@ def serving_bench(lb, requests=20000, budget=200 << 30, slots=4, log=print):
             "refit_p99_ms": pct(kinds == lb.REQ_REFIT, 99),
             "build_p50_ms": pct(kinds == lb.REQ_BUILD, 50),
             "build_p99_ms": pct(kinds == lb.REQ_BUILD, 99),
+            "queue_wait_p50_ms": round(float(np.percentile(wait, 50)), 3),
+            "queue_wait_p99_ms": round(float(np.percentile(wait, 99)), 3),
+            "service_p50_ms": round(float(np.percentile(service, 50)), 3),
+            "service_p99_ms": round(float(np.percentile(service, 99)), 3),
+            "max_queued": max_queued,
             "throughput_req_s": round(len(done) / wall, 1),
             "throughput_Mtri_s": round(tris / wall / 1e6, 1),
             "evictions": int(st["evictions"]),

@@ -20,6 +20,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/lbvh.h"
@@ -88,6 +89,11 @@ struct Work {
 };
 
 constexpr int kSlotDepth = 2;  // requests in flight per slot
+// Admission looks at this many requests at the head of the queue per round:
+// enough to find runnable work past held ones, and it keeps a round's cost
+// independent of a backlog (a scan of the whole queue made each round slower
+// as the queue grew, which grew the queue).
+constexpr size_t kAdmitWindow = 64;
 
 struct Slot {
   lbvh_ctx *ctx = nullptr;
@@ -401,14 +407,16 @@ void dispatcher(lbvh_server *s) {
     // later requests on it; a request that does not fit yet holds back
     // later requests that need memory, so large builds are not starved.
     bool memory_blocked = false;
-    std::vector<uint64_t> held;  // meshes with an earlier request still queued
-    for (auto qi = s->queue.begin(); qi != s->queue.end();) {
+    std::unordered_set<uint64_t> held;  // meshes with an earlier request queued
+    size_t scanned = 0;
+    for (auto qi = s->queue.begin();
+         qi != s->queue.end() && scanned < kAdmitWindow; ++scanned) {
       const uint32_t k = pick_slot(s);
       if (k == UINT32_MAX) break;
       auto mi = s->meshes.find(qi->mesh_id);
       if ((mi != s->meshes.end() && mi->second.busy) ||
-          std::find(held.begin(), held.end(), qi->mesh_id) != held.end()) {
-        held.push_back(qi->mesh_id);
+          held.count(qi->mesh_id)) {
+        held.insert(qi->mesh_id);
         ++qi;
         continue;
       }
@@ -418,7 +426,7 @@ void dispatcher(lbvh_server *s) {
         const uint64_t need = request_need(s, *qi, sl);
         if (need > 0 && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
           memory_blocked = true;
-          held.push_back(qi->mesh_id);
+          held.insert(qi->mesh_id);
           ++qi;
           continue;
         }
