@@ -7,15 +7,19 @@
 // mesh that is busy, or that do not fit yet, wait without blocking refits
 // behind them), watches completion events, and evicts least recently used
 // idle meshes when a new build needs room.  Each slot has a worker thread
-// that enqueues its requests' device work on the slot's stream outside the
-// server lock, up to kSlotDepth requests ahead, so host-side launch costs
-// overlap across slots and with the GPU.  Nothing here waits on the GPU
-// except lbvh_server_drain.
+// that enqueues its requests' device work on the slot's stream, up to
+// kSlotDepth requests ahead, so host-side launch costs overlap across slots
+// and with the GPU.  A worker only ever takes its own slot's lock (never the
+// server lock, which the dispatcher holds for most of each round), so it is
+// never queued behind admission work.  Lock order: server `mu`, then a
+// slot's `m`, then `done_mu`.  Nothing here waits on the GPU except
+// lbvh_server_drain.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -25,6 +29,7 @@
 
 #include "../../include/lbvh.h"
 #include "lbvh_internal.h"
+#include "range_arena.h"
 
 namespace lbvh {
 void launch_synth_uniform(uint64_t seed, uint32_t n, double half_extent,
@@ -52,7 +57,9 @@ constexpr uint64_t kRefitScratchPerTri = 36;
 
 struct Mesh {
   uint32_t n = 0;
-  void *buf = nullptr;  // one allocation: tri | bvh | otri | keys | perm
+  void *buf = nullptr;  // one arena range: tri | bvh | otri | keys | perm
+  uint64_t bytes = 0;   // the range's size
+  uint32_t chunk = 0;   // the arena chunk holding it
   bool valid = false;   // holds a completed build
   bool busy = false;    // a request on it is queued in a slot
   uint64_t last_use = 0;
@@ -68,6 +75,26 @@ struct Mesh {
     return reinterpret_cast<int32_t *>(static_cast<char *>(buf) + 164ull * n);
   }
 };
+
+// Mesh memory.  Device chunks, allocated as the resident set grows, are
+// carved by a best-fit range allocator that merges neighbouring free ranges.
+// A mesh is freed only when idle (its last request completed), so its range
+// is reusable at once by any slot, and admission in steady state makes no
+// driver call: a driver pool that keeps freed blocks cannot always reuse one
+// for another size, and once those fill HBM each allocation failure costs a
+// trim (thousands of unmaps) and fresh mappings while admission waits.
+struct HipChunks {
+  static void *alloc(uint64_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    return p;
+  }
+  static void release(void *p) { (void)hipFree(p); }  // waits for the device
+};
+using MeshArena = lbvh::RangeArena<HipChunks>;
 
 struct Request {
   uint64_t id = 0, mesh_id = 0, seed = 0;
@@ -85,7 +112,8 @@ struct Work {
   Mesh mesh;          // copy of the mesh's buffer layout for the worker
   hipEvent_t ev = nullptr;
   double start_s = 0.0;
-  bool launched = false;
+  bool launched = false;  // the worker has enqueued it (or failed to)
+  int status = LBVH_OK;   // the enqueue's status
 };
 
 constexpr int kSlotDepth = 2;  // requests in flight per slot
@@ -100,15 +128,22 @@ struct Slot {
   // lbvh_bytes_in_use(ctx) as of the worker's last enqueue (or a trim): the
   // dispatcher reads this instead of the context, whose lock the worker
   // holds while it enqueues
-  uint64_t scratch_bytes = 0;
+  std::atomic<uint64_t> scratch_bytes{0};
   hipEvent_t events[kSlotDepth] = {};
-  uint32_t next_event = 0;
+  uint32_t next_event = 0;  // dispatcher only
+  std::mutex m;             // guards mailbox, inflight and their Work
   std::deque<Work> mailbox;   // assigned, not yet taken by the worker
   std::deque<Work> inflight;  // taken by the worker, in stream order
   std::condition_variable wcv;
   std::thread worker;
-  bool idle() const { return mailbox.empty() && inflight.empty(); }
-  size_t depth() const { return mailbox.size() + inflight.size(); }
+  bool idle() {
+    std::lock_guard<std::mutex> g(m);
+    return mailbox.empty() && inflight.empty();
+  }
+  size_t depth() {
+    std::lock_guard<std::mutex> g(m);
+    return mailbox.size() + inflight.size();
+  }
 };
 
 }  // namespace
@@ -118,13 +153,7 @@ struct lbvh_server {
   uint64_t budget = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
   std::deque<Slot> slots;  // stable addresses: workers hold references
-  // Mesh memory: a pool of its own that keeps freed memory for reuse (a
-  // pool that hands it back to the driver at every synchronisation makes
-  // each eviction + build map hundreds of MB anew), and an otherwise idle
-  // stream to free on, so a freed mesh is reusable at once by any slot
-  // instead of after the work queued on some busy slot's stream.
-  hipMemPool_t pool = nullptr;
-  hipStream_t free_stream = nullptr;
+  MeshArena arena;
   std::mutex mu;
   std::condition_variable cv;       // wakes the dispatcher
   std::condition_variable idle_cv;  // wakes lbvh_server_drain
@@ -140,7 +169,7 @@ struct lbvh_server {
   uint64_t next_id = 1, tick = 0, resident_bytes = 0, peak = 0;
   lbvh_server_stats st{};
   std::string last_error;
-  bool stop = false;
+  std::atomic<bool> stop{false};
   std::thread thr;
 };
 
@@ -152,7 +181,11 @@ uint64_t slots_bytes(lbvh_server *s) {
   return b;
 }
 
-uint64_t in_use(lbvh_server *s) { return s->resident_bytes + slots_bytes(s); }
+// Device memory held: every arena chunk (free ranges included) and every
+// slot's scratch.  The budget bounds this.
+uint64_t in_use(lbvh_server *s) {
+  return s->arena.reserved() + slots_bytes(s);
+}
 
 bool all_idle(lbvh_server *s) {
   for (auto &sl : s->slots)
@@ -160,64 +193,86 @@ bool all_idle(lbvh_server *s) {
   return true;
 }
 
-// Mesh memory is stream-ordered (pool allocations, hipFreeAsync): a plain
-// hipFree would synchronise the whole device and stall every other slot.
-// A mesh is only freed while idle (its last request completed), so freeing
-// it on the idle stream needs no ordering against the slots.
+// A mesh is only freed while idle (its last request completed), so its
+// range goes back to the arena at once.
 void free_mesh(lbvh_server *s, Mesh &m) {
   if (m.buf) {
-    (void)hipFreeAsync(m.buf, s->free_stream);
-    s->resident_bytes -= kMeshBytesPerTri * m.n;
+    s->arena.free({static_cast<char *>(m.buf), m.chunk, m.bytes});
+    s->resident_bytes -= m.bytes;
   }
   m.buf = nullptr;
+  m.bytes = 0;
   m.n = 0;
   m.valid = false;
 }
 
-// Device memory a request still needs beyond what is allocated.
-uint64_t request_need(lbvh_server *s, const Request &r, const Slot &sl) {
-  uint64_t need = 0;
+// Memory a request still needs: a new mesh range (0 if its mesh already has
+// one of its size) and slot scratch beyond what the slot holds.
+struct Need {
+  uint64_t mesh = 0, scratch = 0;
+  bool any() const { return mesh || scratch; }
+};
+Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
+  Need need;
   if (r.kind == LBVH_REQ_BUILD) {
     auto it = s->meshes.find(r.mesh_id);
-    if (it == s->meshes.end() || it->second.n != r.n)
-      need += kMeshBytesPerTri * r.n;
+    if (it == s->meshes.end() || !it->second.buf || it->second.n != r.n)
+      need.mesh = MeshArena::round(kMeshBytesPerTri * r.n);
   }
   const uint64_t scratch =
       (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
       r.n + sizeof(lbvh::Ctl);
   const uint64_t have = sl.scratch_bytes;
-  if (scratch > have) need += scratch - have;
+  if (scratch > have) need.scratch = scratch - have;
   return need;
 }
 
-// Frees idle memory (LRU idle meshes other than `keep`, then idle slots'
-// scratch) until `need` more bytes fit the budget.  True if they fit.
-bool make_room(lbvh_server *s, uint64_t need, uint64_t keep) {
-  if (s->budget == 0) return true;
-  while (in_use(s) + need > s->budget) {
-    uint64_t victim = 0, oldest = UINT64_MAX;
-    for (auto &kv : s->meshes)
-      if (kv.first != keep && !kv.second.busy && kv.second.buf &&
-          kv.second.last_use < oldest) {
-        oldest = kv.second.last_use;
-        victim = kv.first;
-      }
-    if (oldest != UINT64_MAX) {
-      free_mesh(s, s->meshes[victim]);
-      s->meshes.erase(victim);
-      s->st.evictions += 1;
+bool evict_lru(lbvh_server *s, uint64_t keep) {
+  uint64_t victim = 0, oldest = UINT64_MAX;
+  for (auto &kv : s->meshes)
+    if (kv.first != keep && !kv.second.busy && kv.second.buf &&
+        kv.second.last_use < oldest) {
+      oldest = kv.second.last_use;
+      victim = kv.first;
+    }
+  if (oldest == UINT64_MAX) return false;
+  free_mesh(s, s->meshes[victim]);
+  s->meshes.erase(victim);
+  s->st.evictions += 1;
+  return true;
+}
+
+// Frees idle memory until a free arena range holds need.mesh bytes (or a
+// new chunk for it fits the budget) and need.scratch more slot bytes fit the
+// budget: chunks left empty go first when the budget is short, then least
+// recently used idle meshes other than `keep`, then idle slots' scratch.
+// True if the request fits.
+bool make_room(lbvh_server *s, const Need &need, uint64_t keep) {
+  while (true) {
+    const bool fit = !need.mesh || s->arena.can_fit(need.mesh);
+    const uint64_t grow =
+        fit ? 0 : MeshArena::chunk_for(need.mesh, s->budget);
+    const bool in_budget =
+        !s->budget || in_use(s) + grow + need.scratch <= s->budget;
+    if (in_budget) {
+      if (fit) return true;
+      if (s->arena.add_chunk(grow)) continue;  // the device is full if not
+    } else if (s->arena.release_empty()) {
       continue;
     }
+    if (evict_lru(s, keep)) continue;
+    if (s->arena.release_empty()) continue;
     bool trimmed = false;
     for (auto &sl : s->slots)
       if (sl.idle() && sl.scratch_bytes > sizeof(lbvh::Ctl)) {
-        lbvh_trim(sl.ctx);  // an idle slot's worker does not hold its lock
+        // an idle slot's worker waits for work that only this thread
+        // hands out, so it does not touch the context meanwhile
+        lbvh_trim(sl.ctx);
         sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
         trimmed = true;
       }
     if (!trimmed) return false;
   }
-  return true;
 }
 
 void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
@@ -250,29 +305,20 @@ void finish(lbvh_server *s, const Work &w, int status, uint32_t slot) {
   complete(s, w.req, status, slot, w.start_s);
 }
 
-// Under the lock: gives request r its mesh buffers (allocating on the slot's
-// stream when a build changes the size) and marks the mesh busy.
-int prepare(lbvh_server *s, Slot &sl, Work &w) {
+// Under the lock: gives request r its mesh range (make_room has found room
+// for a new one) and marks the mesh busy.
+int prepare(lbvh_server *s, Work &w) {
   Request &r = w.req;
   Mesh &m = s->meshes[r.mesh_id];
   if (r.kind == LBVH_REQ_BUILD && (m.buf == nullptr || m.n != r.n)) {
     free_mesh(s, m);
-    hipStream_t stream = static_cast<hipStream_t>(lbvh_stream(sl.ctx));
-    void *p = nullptr;
-    const uint64_t bytes = kMeshBytesPerTri * r.n;
-    if (hipMallocFromPoolAsync(&p, bytes, s->pool, stream) != hipSuccess) {
-      // the pool's reserve may be fragmented: return it and try once more
-      (void)hipGetLastError();
-      (void)hipStreamSynchronize(s->free_stream);
-      (void)hipMemPoolTrimTo(s->pool, 0);
-      if (hipMallocFromPoolAsync(&p, bytes, s->pool, stream) != hipSuccess) {
-        (void)hipGetLastError();
-        return LBVH_ERR_OUT_OF_MEMORY;
-      }
-    }
-    m.buf = p;
+    const MeshArena::Range g = s->arena.alloc(kMeshBytesPerTri * r.n);
+    if (!g.p) return LBVH_ERR_OUT_OF_MEMORY;
+    m.buf = g.p;
+    m.chunk = g.chunk;
+    m.bytes = g.bytes;
     m.n = r.n;
-    s->resident_bytes += bytes;
+    s->resident_bytes += g.bytes;
   }
   if (r.kind == LBVH_REQ_BUILD) m.valid = false;
   m.busy = true;
@@ -318,12 +364,13 @@ int enqueue(Slot &sl, Work &w) {
 }
 
 // Slot worker: takes assigned requests in order and enqueues their device
-// work without holding the server lock, so the slots' host-side launch
-// costs overlap each other and the dispatcher.
+// work, so the slots' host-side launch costs overlap each other and the
+// dispatcher.  It reports each enqueue's status in the Work; the dispatcher
+// completes it (a failed one without waiting on its event).
 void worker(lbvh_server *s, uint32_t k) {
   (void)hipSetDevice(s->device);
   Slot &sl = s->slots[k];
-  std::unique_lock<std::mutex> lock(s->mu);
+  std::unique_lock<std::mutex> lock(sl.m);
   while (true) {
     sl.wcv.wait(lock, [&] { return s->stop || !sl.mailbox.empty(); });
     if (s->stop) break;
@@ -332,18 +379,10 @@ void worker(lbvh_server *s, uint32_t k) {
     Work &w = sl.inflight.back();  // deque: stays valid while others change
     lock.unlock();
     const int st = enqueue(sl, w);
-    const uint64_t bytes = lbvh_bytes_in_use(sl.ctx);
+    sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
     lock.lock();
-    sl.scratch_bytes = bytes;
-    if (st == LBVH_OK) {
-      w.launched = true;
-    } else {
-      // Work queued behind it on this stream was enqueued after it, so the
-      // failed entry is the last one in flight.
-      finish(s, w, st, k);
-      sl.inflight.pop_back();
-    }
-    s->cv.notify_all();
+    w.status = st;
+    w.launched = true;
   }
 }
 
@@ -392,13 +431,19 @@ void dispatcher(lbvh_server *s) {
     // Completions, in each slot's stream order.
     for (uint32_t k = 0; k < s->slots.size(); ++k) {
       Slot &sl = s->slots[k];
+      std::lock_guard<std::mutex> g(sl.m);
       while (!sl.inflight.empty() && sl.inflight.front().launched) {
-        const hipError_t q = hipEventQuery(sl.inflight.front().ev);
-        if (q == hipErrorNotReady) break;
-        // (Kernel-side bounded-wait faults are sticky per slot and reported
-        // by lbvh_server_drain, which keeps this path free of device reads.)
-        finish(s, sl.inflight.front(), q == hipSuccess ? LBVH_OK
-                                                       : LBVH_ERR_DEVICE, k);
+        Work &w = sl.inflight.front();
+        int st = w.status;
+        if (st == LBVH_OK) {
+          const hipError_t q = hipEventQuery(w.ev);
+          if (q == hipErrorNotReady) break;
+          // (Kernel-side bounded-wait faults are sticky per slot and
+          // reported by lbvh_server_drain, which keeps this path free of
+          // device reads.)
+          st = q == hipSuccess ? LBVH_OK : LBVH_ERR_DEVICE;
+        }
+        finish(s, w, st, k);
         sl.inflight.pop_front();
         progress = true;
       }
@@ -423,8 +468,8 @@ void dispatcher(lbvh_server *s) {
       int st = check_request(s, *qi);
       if (st == LBVH_OK) {
         Slot &sl = s->slots[k];
-        const uint64_t need = request_need(s, *qi, sl);
-        if (need > 0 && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
+        const Need need = request_need(s, *qi, sl);
+        if (need.any() && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
           memory_blocked = true;
           held.insert(qi->mesh_id);
           ++qi;
@@ -436,9 +481,12 @@ void dispatcher(lbvh_server *s) {
         w.start_s = now_s();
         w.ev = sl.events[sl.next_event];
         sl.next_event = (sl.next_event + 1) % kSlotDepth;
-        st = prepare(s, sl, w);
+        st = prepare(s, w);
         if (st == LBVH_OK) {
-          sl.mailbox.push_back(std::move(w));
+          {
+            std::lock_guard<std::mutex> g(sl.m);
+            sl.mailbox.push_back(std::move(w));
+          }
           sl.wcv.notify_one();
           s->peak = std::max(s->peak, in_use(s));
         } else {
@@ -488,24 +536,8 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
         st = LBVH_ERR_DEVICE;
     if (st != LBVH_OK) break;
   }
-  if (st == LBVH_OK) {
-    // lbvh_create made the device current on this thread
-    hipMemPoolProps props{};
-    props.allocType = hipMemAllocationTypePinned;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = device_ordinal;
-    uint64_t keep = UINT64_MAX;
-    if (hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking) !=
-            hipSuccess ||
-        hipMemPoolCreate(&s->pool, &props) != hipSuccess ||
-        hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold,
-                               &keep) != hipSuccess)
-      st = LBVH_ERR_DEVICE;
-  }
   if (st != LBVH_OK) {
     (void)hipGetLastError();
-    if (s->pool) (void)hipMemPoolDestroy(s->pool);
-    if (s->free_stream) (void)hipStreamDestroy(s->free_stream);
     for (auto &x : s->slots) {
       for (hipEvent_t e : x.events)
         if (e) (void)hipEventDestroy(e);
@@ -526,7 +558,10 @@ int lbvh_server_destroy(lbvh_server *s) {
   {
     std::lock_guard<std::mutex> lock(s->mu);
     s->stop = true;
-    for (auto &sl : s->slots) sl.wcv.notify_all();
+  }
+  for (auto &sl : s->slots) {
+    std::lock_guard<std::mutex> g(sl.m);  // no wake-up lost to the check
+    sl.wcv.notify_all();
   }
   s->cv.notify_all();
   s->thr.join();
@@ -537,9 +572,7 @@ int lbvh_server_destroy(lbvh_server *s) {
   }
   (void)hipSetDevice(s->device);
   for (auto &kv : s->meshes) free_mesh(s, kv.second);
-  (void)hipStreamSynchronize(s->free_stream);
-  (void)hipMemPoolDestroy(s->pool);
-  (void)hipStreamDestroy(s->free_stream);
+  s->arena.release_all();
   for (auto &sl : s->slots) lbvh_destroy(sl.ctx);
   delete s;
   return LBVH_OK;
