@@ -67,6 +67,58 @@ __device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
   }
 }
 
+// Lanes of the wave that are `valid` and hold the calling lane's digit d
+// (kRadixBits bits), found with one ballot per bit.  Every lane of the wave
+// must call it.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t vote = __ballot(bit);
+    peers &= bit ? vote : ~vote;
+  }
+  return peers;
+}
+
+// Triangles in chunks of kMapThreads: a chunk's 3 * kMapThreads float4 are
+// loaded contiguously (each load instruction covers 4 KB, where a triangle
+// per lane would stride 48 bytes between lanes), staged in LDS, and thread t
+// then reads triangle t of the chunk (staged_box).  A caller loads the next
+// chunk into registers before it uses the current one, so those loads are in
+// flight meanwhile.  Block size kMapThreads.
+struct TriChunks {
+  const float4 *tri;
+  size_t q_end;  // 3 x (the first triangle past the range)
+  float4 r[3];
+  // the chunk of triangles [first, first + kMapThreads)
+  __device__ void load(uint32_t first) {
+    const size_t q0 = 3 * (size_t)first + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const size_t q = q0 + (size_t)j * kMapThreads;
+      r[j] = q < q_end ? tri[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  __device__ void stage(float4 *s_tri) const {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s_tri[threadIdx.x + j * kMapThreads] = r[j];
+  }
+};
+
+// Box of triangle t of a staged chunk: min then max of the three vertices in
+// vertex order (aabb-z-idx.js:186-187).
+__device__ __forceinline__ void staged_box(const float4 *s_tri, uint32_t t,
+                                           float mn[3], float mx[3]) {
+  const float4 a = s_tri[3 * t], b = s_tri[3 * t + 1], d = s_tri[3 * t + 2];
+  mn[0] = fminf(a.x, fminf(b.x, d.x));
+  mn[1] = fminf(a.y, fminf(b.y, d.y));
+  mn[2] = fminf(a.z, fminf(b.z, d.z));
+  mx[0] = fmaxf(a.x, fmaxf(b.x, d.x));
+  mx[1] = fmaxf(a.y, fmaxf(b.y, d.y));
+  mx[2] = fmaxf(a.z, fmaxf(b.z, d.z));
+}
+
 // Number of set bits of `mask` in lanes below the calling lane.
 __device__ __forceinline__ uint32_t popc_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),

@@ -51,24 +51,32 @@ __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
 __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
     const float4 *__restrict__ tri, SortSegs segs, uint32_t *seg_ord,
     float *__restrict__ cen, uint32_t total) {
+  static_assert(kSortThreads == kMapThreads, "TriChunks block size");
   __shared__ float s[kSortThreads / 64][6];
+  __shared__ float4 s_tri[3 * kMapThreads];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
   const TileRange r = sort_tile_range(tile, 0u, segs);
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads) {
-    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
-                 c = tri[3 * (size_t)i + 2];
-    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
-                         fminf(a.z, fminf(b.z, c.z))};
-    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
-                         fmaxf(a.z, fmaxf(b.z, c.z))};
+  const uint32_t nch = (r.end - r.begin + kMapThreads - 1) / kMapThreads;
+  TriChunks tc{tri, 3 * (size_t)r.end, {}};
+  if (nch) tc.load(r.begin);
+  for (uint32_t k = 0; k < nch; ++k) {
+    tc.stage(s_tri);
+    if (k + 1 < nch) tc.load(r.begin + (k + 1) * kMapThreads);
+    __syncthreads();
+    const uint32_t i = r.begin + k * kMapThreads + threadIdx.x;
+    if (i < r.end) {
+      float mn[3], mx[3];
+      staged_box(s_tri, threadIdx.x, mn, mx);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      v[k] = fminf(v[k], mn[k]);
-      v[3 + k] = fmaxf(v[3 + k], mx[k]);
-      cen[(size_t)k * total + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+      for (int c = 0; c < 3; ++c) {
+        v[c] = fminf(v[c], mn[c]);
+        v[3 + c] = fmaxf(v[3 + c], mx[c]);
+        cen[(size_t)c * total + i] = __fmul_rn(__fadd_rn(mx[c], mn[c]), 0.5f);
+      }
     }
+    __syncthreads();  // s_tri is refilled for the next chunk
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1)
@@ -124,24 +132,34 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
   }
   __syncthreads();
   const bool ieee = (div_mode == 1);
-  for (uint32_t i = base + threadIdx.x; i < end; i += kSortThreads) {
-    float ce[3];
+  // Centres from k_batch_bounds, or from the triangles in TriChunks chunks
+  // (cen is the same for every block, so the barriers are uniform).
+  __shared__ float4 s_tri[3 * kMapThreads];
+  const uint32_t nch = (end - base + kMapThreads - 1) / kMapThreads;
+  TriChunks tc{tri, 3 * (size_t)end, {}};
+  if (!cen && nch) tc.load(base);
+  for (uint32_t ch = 0; ch < nch; ++ch) {
+    const uint32_t i = base + ch * kMapThreads + threadIdx.x;
+    float ce[3] = {0.f, 0.f, 0.f};
     if (cen) {
+      if (i < end) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) ce[k] = cen[(size_t)k * total + i];
+        for (int c = 0; c < 3; ++c) ce[c] = cen[(size_t)c * total + i];
+      }
     } else {
-      const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
-                   c = tri[3 * (size_t)i + 2];
-      const float mn[3] = {fminf(a.x, fminf(b.x, c.x)),
-                           fminf(a.y, fminf(b.y, c.y)),
-                           fminf(a.z, fminf(b.z, c.z))};
-      const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)),
-                           fmaxf(a.y, fmaxf(b.y, c.y)),
-                           fmaxf(a.z, fmaxf(b.z, c.z))};
+      tc.stage(s_tri);
+      if (ch + 1 < nch) tc.load(base + (ch + 1) * kMapThreads);
+      __syncthreads();
+      if (i < end) {
+        float mn[3], mx[3];
+        staged_box(s_tri, threadIdx.x, mn, mx);
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        ce[k] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+        for (int c = 0; c < 3; ++c)
+          ce[c] = __fmul_rn(__fadd_rn(mx[c], mn[c]), 0.5f);
+      }
+      __syncthreads();  // s_tri is refilled for the next chunk
     }
+    if (i >= end) continue;
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {

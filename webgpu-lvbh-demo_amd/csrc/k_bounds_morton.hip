@@ -61,37 +61,45 @@ __device__ void block_minmax(float v[6]) {
 }
 
 // Scene bounds: fp32 min/max over all vertex coordinates.  Each block
-// reduces a grid-stride share into partials[2*b, 2*b+1]; k_bounds_final (one
-// block, launched next) folds the partials into ctl->bounds.  No atomics and
-// no fences: a last-block-finishes counter costs one same-address
+// reduces a share of the triangles into partials[2*b, 2*b+1]; k_bounds_final
+// (one block, launched next) folds the partials into ctl->bounds.  No atomics
+// and no fences: a last-block-finishes counter costs one same-address
 // device-scope atomic (with a cache writeback for its release) per block,
 // which on MI355X serialises across the XCDs.
 // cen (3 x n floats, x then y then z, or null): also the centre of each
 // triangle's box, (max + min) * .5f as aabb-z-idx.js:191 computes it, so the
 // Morton pass reads 12 bytes per triangle instead of the 48-byte triangle.
+// A block takes chunks of kMapThreads triangles (TriChunks).
 __global__ __launch_bounds__(kMapThreads) void k_bounds(
     const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
     Ctl *ctl, int bump_epoch, float *__restrict__ cen) {
+  __shared__ float4 s_tri[3 * kMapThreads];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
-                 c = tri[3 * (size_t)i + 2];
-    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
-                         fminf(a.z, fminf(b.z, c.z))};
-    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
-                         fmaxf(a.z, fmaxf(b.z, c.z))};
+  const uint32_t t = threadIdx.x;
+  const uint32_t chunks = (n + kMapThreads - 1) / kMapThreads;
+  TriChunks tc{tri, 3 * (size_t)n, {}};
+  if (blockIdx.x < chunks) tc.load(blockIdx.x * kMapThreads);
+  for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+    tc.stage(s_tri);
+    if (c + gridDim.x < chunks) tc.load((c + gridDim.x) * kMapThreads);
+    __syncthreads();
+    const uint32_t i = c * kMapThreads + t;
+    if (i < n) {
+      float mn[3], mx[3];
+      staged_box(s_tri, t, mn, mx);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      v[k] = fminf(v[k], mn[k]);
-      v[3 + k] = fmaxf(v[3 + k], mx[k]);
-    }
-    if (cen) {
+      for (int k = 0; k < 3; ++k) {
+        v[k] = fminf(v[k], mn[k]);
+        v[3 + k] = fmaxf(v[3 + k], mx[k]);
+      }
+      if (cen) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        cen[(size_t)k * n + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+        for (int k = 0; k < 3; ++k)
+          cen[(size_t)k * n + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+      }
     }
+    __syncthreads();  // s_tri is refilled for the next chunk
   }
   block_minmax(v);
   if (threadIdx.x == 0) {
@@ -124,21 +132,20 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds_final(
 //   key  = morton(u32(rel * 1023.99f))                (:194-208)
 // Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
 // fused build, the first sort pass's per-tile digit counts,
-// cnt[d * tiles + tile] (saving the sort a read of the keys).  With cnt the
-// grid is one block per sort tile (XCD-aware ids, as the sort's), without it
-// a grid-stride loop.
+// cnt[d * tiles + tile] (saving the sort a read of the keys).  With cnt a
+// block covers one sort tile (XCD-aware ids, as the sort's), without it a
+// grid-stride share; either way in TriChunks chunks.
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
     uint32_t *__restrict__ keys, Ctl *ctl, int bump_epoch,
     uint32_t *__restrict__ cnt) {
   static_assert(kMapThreads == kRadix, "one thread per digit for cnt");
+  static_assert(kSortTile % kMapThreads == 0, "tiles are whole chunks");
   __shared__ uint32_t s_hist[kRadix];
+  __shared__ float4 s_tri[3 * kMapThreads];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
-  if (cnt) {
-    s_hist[threadIdx.x] = 0u;
-    __syncthreads();
-  }
+  if (cnt) s_hist[threadIdx.x] = 0u;
   float bmin[3], ext[3], rcp[3];
   if (bounds_from_ctl) {
 #pragma unroll
@@ -156,41 +163,46 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     }
   }
   const bool ieee = (p.div_mode == 1);
-  uint32_t first = blockIdx.x * blockDim.x, end = n,
-           stride = gridDim.x * blockDim.x, tile = 0;
+  const uint32_t t = threadIdx.x;
+  const uint32_t chunks = (n + kMapThreads - 1) / kMapThreads;
+  uint32_t c0 = blockIdx.x, c1 = chunks, cstep = gridDim.x, tile = 0;
   if (cnt) {
+    constexpr uint32_t kPerTile = kSortTile / kMapThreads;
     tile = xcd_tile(blockIdx.x, gridDim.x);
-    first = tile * (uint32_t)kSortTile;
-    end = min(first + (uint32_t)kSortTile, n);
-    stride = blockDim.x;
+    c0 = tile * kPerTile;
+    c1 = min(c0 + kPerTile, chunks);
+    cstep = 1;
   }
-  for (uint32_t i = first + threadIdx.x; i < end; i += stride) {
-    const float4 a = tri[3 * (size_t)i], b = tri[3 * (size_t)i + 1],
-                 c = tri[3 * (size_t)i + 2];
-    const float mn[3] = {fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
-                         fminf(a.z, fminf(b.z, c.z))};
-    const float mx[3] = {fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
-                         fmaxf(a.z, fmaxf(b.z, c.z))};
-    uint32_t q[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
-      const float d = __fsub_rn(cen, bmin[k]);
-      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
-      q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
-    }
-    const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
-    keys[i] = key;
-    if (aabb) {
-      aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
-      aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
-    }
-    if (cnt) digit_add(s_hist, key & (kRadix - 1));
-  }
-  if (cnt) {
+  TriChunks tc{tri, 3 * (size_t)n, {}};
+  if (c0 < c1) tc.load(c0 * kMapThreads);
+  for (uint32_t c = c0; c < c1; c += cstep) {
+    tc.stage(s_tri);
+    if (c + cstep < c1) tc.load((c + cstep) * kMapThreads);
     __syncthreads();
-    cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
+    const uint32_t i = c * kMapThreads + t;
+    if (i < n) {
+      float mn[3], mx[3];
+      staged_box(s_tri, t, mn, mx);
+      uint32_t q[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+        const float d = __fsub_rn(cen, bmin[k]);
+        const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
+        q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
+      }
+      const uint32_t key =
+          split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+      keys[i] = key;
+      if (aabb) {
+        aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
+        aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
+      }
+      if (cnt) digit_add(s_hist, key & (kRadix - 1));
+    }
+    __syncthreads();  // s_tri is refilled for the next chunk
   }
+  if (cnt) cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
 }
 
 // compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
@@ -214,11 +226,25 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t first = tile * (uint32_t)kSortTile;
   const uint32_t end = min(first + (uint32_t)kSortTile, n);
-  for (uint32_t i = first + threadIdx.x; i < end; i += blockDim.x) {
+  // All of the thread's centres are loaded before any is used: one
+  // iteration's loads per wave in flight (768 bytes) leave the kernel
+  // latency-bound far below HBM bandwidth.
+  constexpr int kPer = kSortTile / kMapThreads;
+  float c[3][kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = first + threadIdx.x + j * kMapThreads;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k][j] = i < end ? cen[(size_t)k * n + i] : 0.f;
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t i = first + threadIdx.x + j * kMapThreads;
+    if (i >= end) continue;
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float d = __fsub_rn(cen[(size_t)k * n + i], bmin[k]);
+      const float d = __fsub_rn(c[k][j], bmin[k]);
       const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
       q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
     }

@@ -59,8 +59,18 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
   s_c[threadIdx.x] = 0u;
   __syncthreads();
   const TileRange r = sort_tile_range(tile, n, segs);
-  for (uint32_t i = r.begin + threadIdx.x; i < r.end; i += kSortThreads)
-    digit_add(s_c, (keys[i] >> shift) & (kRadix - 1));
+  // All of the thread's keys are loaded before any is counted, so a wave
+  // has its whole share of the tile in flight.
+  uint32_t k[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t i = r.begin + threadIdx.x + j * kSortThreads;
+    k[j] = i < r.end ? keys[i] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j)
+    if (r.begin + threadIdx.x + j * kSortThreads < r.end)
+      digit_add(s_c, (k[j] >> shift) & (kRadix - 1));
   __syncthreads();
   cnt[(size_t)threadIdx.x * tiles + tile] = s_c[threadIdx.x];
 }
@@ -182,6 +192,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[kWaves][kRadix];
+  __shared__ uint32_t s_spare[kWaves][64];  // targets of the no-op adds
   __shared__ uint32_t s_start[kRadix];  // tile-local start of each digit run
   __shared__ uint32_t s_gbase[kRadix];  // output position minus local start
   __shared__ uint32_t s_wsum[16];
@@ -213,26 +224,34 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
 
   // Rank: for each item, the lanes holding the same digit are found with 8
   // ballots; the lowest of them adds the group size to the wave's digit
-  // counter.  rank = counter before the add + position inside the group.
-  uint32_t rank[kSortItems];
+  // counter with an LDS atomic that returns the count before the add, and
+  // the group reads it from that lane.  rank = that count + position inside
+  // the group.  A wave's LDS operations complete in issue order, so all the
+  // items' atomics are issued first and their results read afterwards
+  // (rank[j] holds the position in bits 0-7 and the group's first lane in
+  // bits 8-13 meanwhile).
+  uint32_t rank[kSortItems], prev[kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const bool valid = (wbase + 64 * j + lane) < r.end;
     const uint32_t d = (key[j] >> shift) & (kRadix - 1);
-    uint64_t peers = __ballot(valid);
+    const uint64_t peers = match_digit(d, valid);
+    const uint32_t below = popc_below(peers);
+    // Every lane issues the atomic (no branch, so nothing waits for its
+    // result here): a group's first lane adds the group size to the digit
+    // counter, the others add 0 to a word of their own.
+    const bool lead = valid && below == 0;
+    uint32_t *slot = lead ? &s_whist[wave][d] : &s_spare[wave][lane];
+    prev[j] = atomicAdd(slot, lead ? (uint32_t)__popcll(peers) : 0u);
+    // a valid lane's group holds itself, so peers != 0
+    const uint32_t leader =
+        valid ? (uint32_t)__ffsll((unsigned long long)peers) - 1 : lane;
+    rank[j] = below | (leader << 8);
+  }
 #pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t vote = __ballot(bit);
-      peers &= bit ? vote : ~vote;
-    }
-    rank[j] = 0u;
-    if (valid) {
-      const uint32_t below = popc_below(peers);
-      const uint32_t prev = s_whist[wave][d];
-      if (below == 0) s_whist[wave][d] = prev + (uint32_t)__popcll(peers);
-      rank[j] = prev + below;
-    }
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t p = __shfl(prev[j], (int)(rank[j] >> 8), 64);
+    rank[j] = (wbase + 64 * j + lane) < r.end ? p + (rank[j] & 0xffu) : 0u;
   }
   __syncthreads();
 
