@@ -268,7 +268,10 @@ def main():
     ap.add_argument("--workload", default="heightfield-16m")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tree-tile", type=int, default=0,
-                    help="leaves per block of the fused up pass (256/512/1024)")
+                    help="leaves per block of the fused up pass (256/512/1024); "
+                         "with --sort-bits, replaces the autotuner")
+    ap.add_argument("--sort-bits", type=int, default=0,
+                    help="radix-sort digit width (8 or 10)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=20000)
@@ -288,8 +291,11 @@ def main():
     n = tri.shape[0]
     ctx = lb.Context(0)
     tuning = None
-    if args.tree_tile:
-        ctx.set_tuning(lb.TUNE_TREE_TILE, args.tree_tile)
+    if args.tree_tile or args.sort_bits:
+        if args.tree_tile:
+            ctx.set_tuning(lb.TUNE_TREE_TILE, args.tree_tile)
+        if args.sort_bits:
+            ctx.set_tuning(lb.TUNE_SORT_BITS, args.sort_bits)
     else:
         log("autotune")
         tuning = ctx.autotune(1 << 16, n, reps=3)
@@ -311,6 +317,7 @@ def main():
     ctx.synchronize()
 
     tile = ctx.get_tuning_for(lb.TUNE_TREE_TILE, n)
+    bits = ctx.get_tuning_for(lb.TUNE_SORT_BITS, n)
     log(f"timed x{args.steps}")
     ms, wall = timed_builds(lb, ctx, hip, d_tri, n, None, bvh, otri, args.steps)
     ms_per_step = ms / args.steps
@@ -395,7 +402,7 @@ def main():
         "dtype": "fp32+u32",
         "data": "synthetic",
         "config": {"workload": args.workload, "triangles": n,
-                   "tree_tile": tile,
+                   "tree_tile": tile, "sort_digit_bits": bits,
                    "bounds": "GPU reduction", "div_mode": "mul_rcp",
                    "inputs": "resident in HBM"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
@@ -406,7 +413,8 @@ def main():
                               "frac_of_hbm_peak": round(pipe_gbs / HBM_PEAK_GBS, 4)},
         "stages": stages,
         "autotune_ms": None if tuning is None else
-        {str(k): {str(t): round(v, 4) for t, v in d.items()} for k, d in tuning.items()},
+        {str(k): {f"{t}/{b}": round(v, 4) for (t, b), v in d.items()}
+         for k, d in tuning.items()},
         "host_inclusive": {"ms": round(host_ms, 3),
                            "Mtri_s": round(n / (host_ms * 1e-3) / 1e6, 1),
                            "note": "upload of the triangles from pageable host "

@@ -83,10 +83,16 @@ int lbvh_get_div_mode(const lbvh_ctx *ctx);
  * digit sizes are constants, e.g. WG_SIZE = 64 at aabb-z-idx.js:4).  Values
  * are validated; results do not depend on them. */
 typedef enum {
-  LBVH_TUNE_TREE_TILE = 0 /* leaves per block of the fused up pass: 256|512|1024 */
+  LBVH_TUNE_TREE_TILE = 0, /* leaves per block of the fused up pass: 256|512|1024 */
+  LBVH_TUNE_SORT_BITS = 1  /* radix-sort digit width: 8 (4 passes over the
+                              30-bit keys) or 10 (3 passes); the reference uses
+                              2-bit digits, radix-sort.js:486 */
 } lbvh_tunable;
-/* set_tuning fixes a value for every size; get_tuning returns the value a
- * build of n triangles would use (n = 0: the fixed/default value). */
+/* set_tuning fixes a value for every size (value 0: no fixed value, builds
+ * use the autotuned table, else the size-based default: tree tile 1024, and
+ * 10-bit digits up to 2^21 triangles, 8-bit above); get_tuning_for returns
+ * the value a build of n triangles would use (n = 0: the fixed/default
+ * value). */
 int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value);
 int lbvh_get_tuning(const lbvh_ctx *ctx, int param);
 int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n);
@@ -96,8 +102,9 @@ int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n);
  * after a warm-up) and keeps the fastest per bucket; later builds pick the
  * entry of their size's bucket (or the nearest tuned one).  A fixed value
  * from lbvh_set_tuning is cleared.  ms_out (optional) receives, per bucket
- * and variant in order, the measured milliseconds: buckets x 3 tree tiles
- * (256, 512, 1024).  Synchronous. */
+ * and variant in order, the measured milliseconds: buckets x 6 variants,
+ * tree tile major (256, 512, 1024) and digit width minor (8, 10); NaN for a
+ * variant that did not run.  Synchronous. */
 int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
                   float *ms_out);
 /* The context's hipStream_t, for callers that record events or chain work. */

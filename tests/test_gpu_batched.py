@@ -34,16 +34,21 @@ def run_batch(ctx, tri, offsets, bounds=None):
             b.destroy()
 
 
+@pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("with_bounds", [False, True])
-def test_batch_meshes_match_oracle(ctx, lb, ora, with_bounds):
+def test_batch_meshes_match_oracle(ctx, lb, ora, with_bounds, bits):
     sizes = [1, 2, 0, 3, 255, 256, 257, 4096, 4097, 9000, 1, 33333, 5]
-    say(f"batch {len(sizes)} meshes bounds={with_bounds}")
+    say(f"batch {len(sizes)} meshes bounds={with_bounds} bits={bits}")
     meshes, offsets, tri = make_batch(lb, sizes, 100)
     bounds = None
     if with_bounds:
         bounds = np.concatenate([lb.wl_bounds(m) if len(m) else np.zeros(6, np.float32)
                                  for m in meshes])
-    bvh, otri, sk, pm = run_batch(ctx, tri, offsets, bounds)
+    ctx.set_tuning(lb.TUNE_SORT_BITS, bits)
+    try:
+        bvh, otri, sk, pm = run_batch(ctx, tri, offsets, bounds)
+    finally:
+        ctx.set_tuning(lb.TUNE_SORT_BITS, 0)
     for k, m in enumerate(meshes):
         a, b = int(offsets[k]), int(offsets[k + 1])
         if a == b:

@@ -102,14 +102,29 @@ def test_aabb_morton_caller_bounds_narrower_than_mesh(ctx, lb, ora):
     ctx.set_div_mode(0)
 
 
+class sort_bits:
+    """Fixes the radix-sort digit width of ctx inside a with-block (the
+    size-based default after)."""
+    def __init__(self, ctx, lb, bits):
+        self.ctx, self.lb, self.bits = ctx, lb, bits
+
+    def __enter__(self):
+        self.ctx.set_tuning(self.lb.TUNE_SORT_BITS, self.bits)
+
+    def __exit__(self, *exc):
+        self.ctx.set_tuning(self.lb.TUNE_SORT_BITS, 0)
+
+
+@pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65536, 1000003, 4194304])
-def test_sort_is_stable(ctx, n):
-    say(f"sort n={n}")
+def test_sort_is_stable(ctx, lb, n, bits):
+    say(f"sort n={n} bits={bits}")
     rng = np.random.default_rng(n)
     keys = sort_keys(rng, n)
     dk = ctx.to_device(keys)
     di = ctx.alloc(4 * n)
-    ctx.sort(dk, di, n)
+    with sort_bits(ctx, lb, bits):
+        ctx.sort(dk, di, n)
     gk = dk.read(np.uint32, n)
     gi = di.read(np.int32, n)
     want = np.argsort(keys, kind="stable")
@@ -118,15 +133,18 @@ def test_sort_is_stable(ctx, n):
     dk.destroy(); di.destroy()
 
 
-def test_sort_heavy_duplicates_and_constant_digits(ctx):
+@pytest.mark.parametrize("bits", [8, 10])
+def test_sort_heavy_duplicates_and_constant_digits(ctx, lb, bits):
     for n, keys in [(300000, np.zeros(300000, np.uint32)),
                     (300000, np.full(300000, 0x3FFFFFFF, np.uint32)),
                     (123457, (np.arange(123457, dtype=np.uint32) % 3) << 24),
+                    (123457, (np.arange(123457, dtype=np.uint32) % 5) << 20),
                     (200000, np.arange(200000, dtype=np.uint32)[::-1].copy())]:
-        say(f"sort special n={n}")
+        say(f"sort special n={n} bits={bits}")
         dk = ctx.to_device(keys)
         di = ctx.alloc(4 * n)
-        ctx.sort(dk, di, n)
+        with sort_bits(ctx, lb, bits):
+            ctx.sort(dk, di, n)
         want = np.argsort(keys, kind="stable")
         assert np.array_equal(di.read(np.int32, n), want)
         assert np.array_equal(dk.read(np.uint32, n), keys[want])
@@ -218,20 +236,39 @@ def test_build_independent_of_tree_tile(ctx, lb, ora, tile):
         tri = lb.workload("clustered-1m")
         assert np.array_equal(ctx.build_arrays(tri, None)[0], ora.build(tri, None, 0)[0])
     finally:
-        ctx.set_tuning(lb.TUNE_TREE_TILE, 512)
+        ctx.set_tuning(lb.TUNE_TREE_TILE, 0)
 
 
 def test_autotune_table_never_changes_results(lb, ora):
     with lb.Context(0) as c:
         times = c.autotune(1 << 10, 1 << 16, reps=2)
         assert sorted(times) == [1 << k for k in range(10, 17)]
-        for bucket, per_tile in times.items():
-            assert set(per_tile) == {256, 512, 1024}
-            assert all(t > 0 for t in per_tile.values())
-            assert c.get_tuning_for(lb.TUNE_TREE_TILE, bucket) in (256, 512, 1024)
+        for bucket, per_variant in times.items():
+            assert set(per_variant) == {(t, b) for t in (256, 512, 1024) for b in (8, 10)}
+            assert all(t > 0 for t in per_variant.values())
+            best = min(per_variant, key=per_variant.get)
+            assert c.get_tuning_for(lb.TUNE_TREE_TILE, bucket) == best[0]
+            assert c.get_tuning_for(lb.TUNE_SORT_BITS, bucket) == best[1]
         for n in (1, 700, 5000, 70000):
             tri = lb.wl_clustered(90 + n, n, 0.05)
             assert np.array_equal(c.build_arrays(tri, None)[0], ora.build(tri, None, 0)[0])
+
+
+@pytest.mark.parametrize("keep_topology", [False, True])
+def test_build_wide_digits_match_oracle(ctx, lb, ora, keep_topology):
+    """10-bit digits (3 sort passes): the same bytes.  Without caller key and
+    permutation buffers the odd pass count ends in the context's other key
+    buffer, with them in the caller's."""
+    with sort_bits(ctx, lb, 10):
+        for name, tri in list(mesh_cases(lb).items()) + [("clustered-1m", lb.workload("clustered-1m"))]:
+            say(f"build {name} bits=10 keep_topology={keep_topology}")
+            got = ctx.build_arrays(tri, None, keep_topology=keep_topology)
+            want = ora.build(tri, None, 0)
+            assert np.array_equal(got[0], want[0]), name
+            assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32)), name
+            if keep_topology:
+                assert np.array_equal(got[2], want[2]), name
+                assert np.array_equal(got[3], want[3]), name
 
 
 def test_build_is_deterministic(ctx, lb):
@@ -242,14 +279,19 @@ def test_build_is_deterministic(ctx, lb):
         assert np.array_equal(ctx.build_arrays(tri, None)[0], first)
 
 
-def test_build_heightfield_16m_full_size(ctx, lb, ora):
-    """The 16,777,216-triangle heightfield against the oracle, plus
-    size-independent checks: sorted keys, a permutation, root box = the
-    scene bounds, every child box inside its parent's."""
-    say("build heightfield-16m")
+_HF16 = {}
+
+
+@pytest.mark.parametrize("bits", [8, 10])
+def test_build_heightfield_16m_full_size(ctx, lb, ora, bits):
+    """The 16,777,216-triangle heightfield against the oracle (with either
+    sort digit width), plus size-independent checks: sorted keys, a
+    permutation, root box = the scene bounds."""
+    say(f"build heightfield-16m bits={bits}")
     tri = lb.workload("heightfield-16m")
     n = len(tri)
-    bvh, otri, sk, pm = ctx.build_arrays(tri, None, keep_topology=True)
+    with sort_bits(ctx, lb, bits):
+        bvh, otri, sk, pm = ctx.build_arrays(tri, None, keep_topology=True)
     assert np.all(sk[1:] >= sk[:-1])
     assert np.array_equal(np.sort(pm), np.arange(n, dtype=np.int32))
     f, i = ora.bvh_fields(bvh, n)
@@ -259,7 +301,9 @@ def test_build_heightfield_16m_full_size(ctx, lb, ora):
     assert np.array_equal(root_min, np.where(b[:3] == 0, np.float32(-1e-8), b[:3]))
     assert np.array_equal(root_max, np.where(b[3:] == 0, np.float32(1e-8), b[3:]))
     say("oracle heightfield-16m")
-    wb, wo, wsk, wpm = ora.build(tri, None, 0)
+    if "want" not in _HF16:
+        _HF16["want"] = ora.build(tri, None, 0)
+    wb, wo, wsk, wpm = _HF16["want"]
     assert np.array_equal(sk, wsk)
     assert np.array_equal(pm, wpm)
     assert np.array_equal(bvh, wb)

@@ -472,7 +472,8 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   int b0 = 0, b1 = 0;
   while (nmin > 1 && (2ull << b0) <= nmin) ++b0;
   while (nmax > 1 && (2ull << b1) <= nmax) ++b1;
-  const size_t count = nmax >= nmin ? (size_t)(b1 - b0 + 1) * 3 : 0;
+  // per bucket: tree tiles 256, 512, 1024 x digit widths 8, 10
+  const size_t count = nmax >= nmin ? (size_t)(b1 - b0 + 1) * 6 : 0;
   void *data = nullptr;
   napi_value ab, arr;
   NAPI_OK(env, napi_create_arraybuffer(env, 4 * (count ? count : 1), &data, &ab));

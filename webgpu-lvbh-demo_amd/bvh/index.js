@@ -74,16 +74,20 @@ class LBVHDevice {
 
   bytesInUse () { return addon.bytesInUse(this.ctx) }
 
-  // Times the tree tile variants per power-of-two size bucket in
-  // [nMin, nMax] on device-synthesised meshes and keeps the fastest for
-  // later builds.  -> { bucketSize: { 256: ms, 512: ms, 1024: ms } }
+  // Times every (tree tile, sort digit width) variant per power-of-two size
+  // bucket in [nMin, nMax] on device-synthesised meshes and keeps the
+  // fastest for later builds.
+  // -> { bucketSize: { '256/8': ms, '256/10': ms, ..., '1024/10': ms } }
   autotune (nMin, nMax, reps) {
     const ms = addon.autotune(this.ctx, nMin, nMax, reps || 3)
+    const names = []
+    for (const tile of [256, 512, 1024]) for (const bits of [8, 10]) names.push(tile + '/' + bits)
     const out = {}
     let bucket = 1
     while (2 * bucket <= nMin) bucket *= 2
-    for (let i = 0; i < ms.length / 3; i++, bucket *= 2) {
-      out[bucket] = { 256: ms[3 * i], 512: ms[3 * i + 1], 1024: ms[3 * i + 2] }
+    for (let i = 0; i < ms.length / names.length; i++, bucket *= 2) {
+      out[bucket] = {}
+      names.forEach((name, v) => { out[bucket][name] = ms[names.length * i + v] })
     }
     return out
   }

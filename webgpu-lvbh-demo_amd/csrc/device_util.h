@@ -5,27 +5,6 @@
 
 namespace lbvh {
 
-// Adds one digit per pass to the block's LDS histograms.  A wave whose active
-// lanes all hold the same digit (common for the high digits of coherent
-// meshes) adds its lane count once instead of contending on one LDS word.
-__device__ __forceinline__ void hist_add(uint32_t (*s_hist)[kRadix],
-                                         uint32_t key) {
-  const uint64_t active = __ballot(1);
-  const int leader = __ffsll((unsigned long long)active) - 1;
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int p = 0; p < kSortPasses; ++p) {
-    const uint32_t d = (key >> (kRadixBits * p)) & (kRadix - 1);
-    const uint32_t d0 = __shfl(d, leader, 64);
-    const uint64_t same = __ballot(d == d0);
-    if (same == active) {
-      if (lane == leader) atomicAdd(&s_hist[p][d0], (uint32_t)__popcll(active));
-    } else {
-      atomicAdd(&s_hist[p][d], 1u);
-    }
-  }
-}
-
 // XCD-aware tile id.  MI355X hands workgroups to its 8 XCDs round-robin by
 // id (block b runs on XCD b % 8), and each XCD has its own L2.  This
 // bijection on [0, tiles) gives each XCD one contiguous run of tiles, so
@@ -68,12 +47,13 @@ __device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
 }
 
 // Lanes of the wave that are `valid` and hold the calling lane's digit d
-// (kRadixBits bits), found with one ballot per bit.  Every lane of the wave
-// must call it.
+// (kBits bits), found with one ballot per bit.  Every lane of the wave must
+// call it.
+template <int kBits>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int b = 0; b < kRadixBits; ++b) {
+  for (int b = 0; b < kBits; ++b) {
     const bool bit = (d >> b) & 1u;
     const uint64_t vote = __ballot(bit);
     peers &= bit ? vote : ~vote;

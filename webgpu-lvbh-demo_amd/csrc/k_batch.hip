@@ -103,18 +103,19 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
 // per-tile digit counts (cnt0[d * tiles + tile]).  Bounds: host-provided fp32
 // (6 per mesh; centres from the triangles) or the GPU accumulators (centres
 // from k_batch_bounds, cen non-null).
+template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     const float4 *__restrict__ tri, const float *__restrict__ cen,
     uint32_t total, SortSegs segs, const float *__restrict__ seg_bounds,
     const uint32_t *__restrict__ seg_ord, int div_mode,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ cnt0) {
-  static_assert(kSortThreads == kRadix, "one thread per digit for cnt0");
-  __shared__ uint32_t s_hist[kRadix];
+  constexpr int kR = 1 << kBits;
+  __shared__ uint32_t s_hist[kR];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t sg = segs.tile_seg[tile];
   const TileRange r = sort_tile_range(tile, 0u, segs);
   const uint32_t base = r.begin, end = r.end;
-  s_hist[threadIdx.x] = 0u;
+  for (int d = threadIdx.x; d < kR; d += kSortThreads) s_hist[d] = 0u;
   float bmin[3], ext[3], rcp[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -169,10 +170,11 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     }
     const uint32_t key = split3(q[0]) | (split3(q[1]) << 1) | (split3(q[2]) << 2);
     keys[i] = key;
-    digit_add(s_hist, key & (kRadix - 1));
+    digit_add(s_hist, key & (kR - 1));
   }
   __syncthreads();
-  cnt0[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
+  for (int d = threadIdx.x; d < kR; d += kSortThreads)
+    cnt0[(size_t)d * gridDim.x + tile] = s_hist[d];
 }
 
 void launch_batch_prologue(uint32_t *seg_ord, uint32_t num_segs,
@@ -196,10 +198,15 @@ void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
                          const SortSegs &segs, uint32_t tiles,
                          const float *seg_bounds, const uint32_t *seg_ord,
                          int div_mode, uint32_t *keys, uint32_t *cnt0,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_morton, dim3(tiles), dim3(kSortThreads), 0, s,
-                     tri, cen, total, segs, seg_bounds, seg_ord, div_mode,
-                     keys, cnt0);
+                         int bits, hipStream_t s) {
+  if (bits == kWideBits)
+    hipLaunchKernelGGL(k_batch_morton<kWideBits>, dim3(tiles),
+                       dim3(kSortThreads), 0, s, tri, cen, total, segs,
+                       seg_bounds, seg_ord, div_mode, keys, cnt0);
+  else
+    hipLaunchKernelGGL(k_batch_morton<kRadixBits>, dim3(tiles),
+                       dim3(kSortThreads), 0, s, tri, cen, total, segs,
+                       seg_bounds, seg_ord, div_mode, keys, cnt0);
 }
 
 }  // namespace lbvh

@@ -135,17 +135,19 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds_final(
 // cnt[d * tiles + tile] (saving the sort a read of the keys).  With cnt a
 // block covers one sort tile (XCD-aware ids, as the sort's), without it a
 // grid-stride share; either way in TriChunks chunks.
+template <int kBits>
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
     uint32_t *__restrict__ keys, Ctl *ctl, int bump_epoch,
     uint32_t *__restrict__ cnt) {
-  static_assert(kMapThreads == kRadix, "one thread per digit for cnt");
+  constexpr int kR = 1 << kBits;
   static_assert(kSortTile % kMapThreads == 0, "tiles are whole chunks");
-  __shared__ uint32_t s_hist[kRadix];
+  __shared__ uint32_t s_hist[kR];
   __shared__ float4 s_tri[3 * kMapThreads];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
-  if (cnt) s_hist[threadIdx.x] = 0u;
+  if (cnt)
+    for (int d = threadIdx.x; d < kR; d += kMapThreads) s_hist[d] = 0u;
   float bmin[3], ext[3], rcp[3];
   if (bounds_from_ctl) {
 #pragma unroll
@@ -198,22 +200,28 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
         aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
         aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
       }
-      if (cnt) digit_add(s_hist, key & (kRadix - 1));
+      if (cnt) digit_add(s_hist, key & (kR - 1));
     }
     __syncthreads();  // s_tri is refilled for the next chunk
   }
-  if (cnt) cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
+  if (cnt) {
+    __syncthreads();  // also when the block had no chunk
+    for (int d = threadIdx.x; d < kR; d += kMapThreads)
+      cnt[(size_t)d * gridDim.x + tile] = s_hist[d];
+  }
 }
 
 // compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
 // :192-195 onward; the centre of :191 is already rounded the same way), plus
 // the first sort pass's per-tile digit counts.  One block per sort tile, as
 // k_aabb_morton with cnt; bounds from ctl.
+template <int kBits>
 __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     const float *__restrict__ cen, uint32_t n, int div_mode,
     uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt) {
-  __shared__ uint32_t s_hist[kRadix];
-  s_hist[threadIdx.x] = 0u;
+  constexpr int kR = 1 << kBits;
+  __shared__ uint32_t s_hist[kR];
+  for (int d = threadIdx.x; d < kR; d += kMapThreads) s_hist[d] = 0u;
   float bmin[3], ext[3], rcp[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -250,10 +258,11 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     }
     const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
     keys[i] = key;
-    digit_add(s_hist, key & (kRadix - 1));
+    digit_add(s_hist, key & (kR - 1));
   }
   __syncthreads();
-  cnt[(size_t)threadIdx.x * gridDim.x + tile] = s_hist[threadIdx.x];
+  for (int d = threadIdx.x; d < kR; d += kMapThreads)
+    cnt[(size_t)d * gridDim.x + tile] = s_hist[d];
 }
 
 __global__ void k_copy_bounds(const Ctl *ctl, float *out) {
@@ -278,23 +287,35 @@ void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
                         Ctl *ctl, bool bump_epoch, hipStream_t s,
-                        uint32_t *cnt) {
+                        uint32_t *cnt, int bits) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
   if (cnt) blocks = sort_tiles(n);
   if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(k_aabb_morton, dim3(blocks), dim3(kMapThreads), 0, s,
-                     tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
-                     bump_epoch ? 1 : 0, cnt);
+  if (cnt && bits == kWideBits)
+    hipLaunchKernelGGL(k_aabb_morton<kWideBits>, dim3(blocks),
+                       dim3(kMapThreads), 0, s, tri, n, p,
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
+                       bump_epoch ? 1 : 0, cnt);
+  else
+    hipLaunchKernelGGL(k_aabb_morton<kRadixBits>, dim3(blocks),
+                       dim3(kMapThreads), 0, s, tri, n, p,
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
+                       bump_epoch ? 1 : 0, cnt);
 }
 
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
-                       uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
+                       uint32_t *keys, const Ctl *ctl, uint32_t *cnt, int bits,
                        hipStream_t s) {
   const uint32_t tiles = sort_tiles(n);
   if (tiles == 0) return;
-  hipLaunchKernelGGL(k_morton_cen, dim3(tiles), dim3(kMapThreads), 0, s, cen,
-                     n, div_mode, keys, ctl, cnt);
+  if (bits == kWideBits)
+    hipLaunchKernelGGL(k_morton_cen<kWideBits>, dim3(tiles), dim3(kMapThreads),
+                       0, s, cen, n, div_mode, keys, ctl, cnt);
+  else
+    hipLaunchKernelGGL(k_morton_cen<kRadixBits>, dim3(tiles),
+                       dim3(kMapThreads), 0, s, cen, n, div_mode, keys, ctl,
+                       cnt);
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

@@ -4,7 +4,10 @@
 // passes of 2-bit digits, each as 4 dispatches and a host round trip
 // (:386-488), with a 3-level 256-ary scan capped at 2^24 keys (:189-202).
 // Its output is the unique stable-sort permutation of the keys; this file
-// computes the same permutation with 4 passes of 8-bit digits.
+// computes the same permutation with 4 passes of 8-bit digits, or 3 passes
+// of 10-bit digits (the digit width is a tunable: wider digits take fewer
+// passes and launches, narrower ones less LDS per block and fewer count
+// words per tile).
 //
 // Each pass is reduce-then-scan over tiles of kSortTile keys:
 //   count    per-tile digit counts (pass 0's are written by the Morton
@@ -33,12 +36,20 @@
 namespace lbvh {
 
 constexpr int kWaves = kSortThreads / 64;
-static_assert(kSortThreads == kRadix, "one thread per digit");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 
+// Digits of width kBits: kR values, kDpt of them per thread of a block of
+// kSortThreads (thread t owns digits [t * kDpt, (t + 1) * kDpt)).
+template <int kBits>
+struct Digits {
+  static constexpr int kR = 1 << kBits;
+  static constexpr int kDpt = kR / kSortThreads;
+  static_assert(kDpt >= 1 && kR % kSortThreads == 0, "whole digits per thread");
+};
+
 // The global histogram of one pass's digit: column sums of its per-tile
-// counts (block d sums column d).  Plain stores into ctl->hist[pass]: no
-// device-scope atomics, and each sort rewrites it.
+// counts (block d sums column d).  Plain stores into the pass's histogram:
+// no device-scope atomics, and each sort rewrites it.
 __global__ __launch_bounds__(kSortThreads) void k_digit_totals(
     const uint32_t *__restrict__ cnt, uint32_t tiles, uint32_t *hist) {
   __shared__ uint32_t s_wsum[16];
@@ -50,13 +61,15 @@ __global__ __launch_bounds__(kSortThreads) void k_digit_totals(
 }
 
 // Per-tile digit counts of one pass -> cnt[d * tiles + tile].
+template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_count(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
     uint32_t *__restrict__ cnt, SortSegs segs) {
-  __shared__ uint32_t s_c[kRadix];
+  using D = Digits<kBits>;
+  __shared__ uint32_t s_c[D::kR];
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  s_c[threadIdx.x] = 0u;
+  for (int d = threadIdx.x; d < D::kR; d += kSortThreads) s_c[d] = 0u;
   __syncthreads();
   const TileRange r = sort_tile_range(tile, n, segs);
   // All of the thread's keys are loaded before any is counted, so a wave
@@ -70,9 +83,28 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j)
     if (r.begin + threadIdx.x + j * kSortThreads < r.end)
-      digit_add(s_c, (k[j] >> shift) & (kRadix - 1));
+      digit_add(s_c, (k[j] >> shift) & (D::kR - 1));
   __syncthreads();
-  cnt[(size_t)threadIdx.x * tiles + tile] = s_c[threadIdx.x];
+  for (int d = threadIdx.x; d < D::kR; d += kSortThreads)
+    cnt[(size_t)d * tiles + tile] = s_c[d];
+}
+
+// Exclusive prefix of h over all kR digits at each of the calling thread's
+// kDpt digits (digit t * kDpt + i -> out[i]); sum = the thread's total.
+template <int kBits>
+__device__ __forceinline__ void digit_prefix(const uint32_t (&h)[Digits<kBits>::kDpt],
+                                             uint32_t (&out)[Digits<kBits>::kDpt],
+                                             uint32_t *s_wsum) {
+  using D = Digits<kBits>;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) sum += h[i];
+  uint32_t run = block_exclusive_scan(sum, s_wsum);
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) {
+    out[i] = run;
+    run += h[i];
+  }
 }
 
 // Column scan: block d turns column d of the tile counts into output
@@ -88,10 +120,12 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
 // Chunks of kSortThreads * kScanPer column entries go through LDS, so global
 // loads and stores are coalesced while each thread scans kScanPer consecutive
 // entries (one pad word per kScanPer keeps the per-thread reads conflict-free).
+template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
     uint32_t tiles, const uint32_t *__restrict__ hist, SortSegs segs,
     uint32_t *seg_base) {
+  using D = Digits<kBits>;
   constexpr int kChunk = kSortThreads * kScanPer;
   __shared__ uint32_t s_col[kChunk + kChunk / kScanPer];
   __shared__ uint32_t s_wsum[16];
@@ -99,8 +133,13 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
   const uint32_t d = blockIdx.x;
   const bool segmented = segs.tile_seg != nullptr;
   if (!segmented) {
-    const uint32_t below = block_exclusive_scan(hist[threadIdx.x], s_wsum);
-    if (threadIdx.x == d) s_carry = below;
+    uint32_t h[D::kDpt], below[D::kDpt];
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i) h[i] = hist[threadIdx.x * D::kDpt + i];
+    digit_prefix<kBits>(h, below, s_wsum);
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i)
+      if (threadIdx.x * D::kDpt + i == d) s_carry = below[i];
   } else if (threadIdx.x == 0) {
     s_carry = 0u;
   }
@@ -140,7 +179,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
         if (t < tiles) {
           const uint32_t sg = segs.tile_seg[t];
           if (t == segs.seg_tile0[sg])
-            seg_base[(size_t)sg * kRadix + d] -= p[j];
+            seg_base[(size_t)sg * D::kR + d] -= p[j];
         }
       }
     }
@@ -150,7 +189,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
       const uint32_t t = t0 + j;
       uint32_t o = p[j];
       if (segmented && t < tiles)
-        o += seg_base[(size_t)segs.tile_seg[t] * kRadix + d];
+        o += seg_base[(size_t)segs.tile_seg[t] * D::kR + d];
       s_col[pad(threadIdx.x * kScanPer + j)] = o;
     }
     carry += s_carry;
@@ -165,44 +204,57 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
 }
 
 // Segment digit bases of one pass from its per-tile counts: block s sums
-// column d over the segment's tiles (thread d), then seg_base[s][d] =
+// each digit's column over the segment's tiles, then seg_base[s][d] =
 // seg_off[s] + (keys of s whose digit is below d).
+template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_seg_bases(
     const uint32_t *__restrict__ cnt, uint32_t tiles, SortSegs segs,
     uint32_t num_segs, uint32_t *__restrict__ seg_base) {
+  using D = Digits<kBits>;
   __shared__ uint32_t s_wsum[16];
   const uint32_t sg = blockIdx.x;
   const uint32_t t0 = segs.seg_tile0[sg];
   const uint32_t t1 = (sg + 1 < num_segs) ? segs.seg_tile0[sg + 1] : tiles;
-  const uint32_t *col = cnt + (size_t)threadIdx.x * tiles;
-  uint32_t total = 0;
-  for (uint32_t t = t0; t < t1; ++t) total += col[t];
-  const uint32_t below = block_exclusive_scan(total, s_wsum);
-  seg_base[(size_t)sg * kRadix + threadIdx.x] = segs.seg_off[sg] + below;
+  uint32_t total[D::kDpt], below[D::kDpt];
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) {
+    const uint32_t *col = cnt + (size_t)(threadIdx.x * D::kDpt + i) * tiles;
+    total[i] = 0;
+    for (uint32_t t = t0; t < t1; ++t) total[i] += col[t];
+  }
+  digit_prefix<kBits>(total, below, s_wsum);
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i)
+    seg_base[(size_t)sg * D::kR + threadIdx.x * D::kDpt + i] =
+        segs.seg_off[sg] + below[i];
 }
 
 // One digit pass.  kFirst: the payload is the position in the segment (the
 // reference's init_idx, radix-sort.js:527-534, folded into the first pass).
-template <bool kFirst>
+template <int kBits, bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
     SortSegs segs) {
+  using D = Digits<kBits>;
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
-  __shared__ uint32_t s_whist[kWaves][kRadix];
+  __shared__ uint32_t s_whist[kWaves][D::kR];
   __shared__ uint32_t s_spare[kWaves][64];  // targets of the no-op adds
-  __shared__ uint32_t s_start[kRadix];  // tile-local start of each digit run
-  __shared__ uint32_t s_gbase[kRadix];  // output position minus local start
+  __shared__ uint32_t s_start[D::kR];  // tile-local start of each digit run
+  __shared__ uint32_t s_gbase[D::kR];  // output position minus local start
   __shared__ uint32_t s_wsum[16];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  for (int k = tid; k < kWaves * kRadix; k += kSortThreads)
+  for (int k = tid; k < kWaves * D::kR; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
-  const uint32_t goff = off[(size_t)tid * tiles + tile];
+  uint32_t goff[D::kDpt];
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i)
+    goff[i] = off[(size_t)(tid * D::kDpt + i) * tiles + tile];
   const TileRange r = sort_tile_range(tile, n, segs);
   __syncthreads();
 
@@ -222,20 +274,20 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     }
   }
 
-  // Rank: for each item, the lanes holding the same digit are found with 8
-  // ballots; the lowest of them adds the group size to the wave's digit
-  // counter with an LDS atomic that returns the count before the add, and
-  // the group reads it from that lane.  rank = that count + position inside
-  // the group.  A wave's LDS operations complete in issue order, so all the
-  // items' atomics are issued first and their results read afterwards
-  // (rank[j] holds the position in bits 0-7 and the group's first lane in
-  // bits 8-13 meanwhile).
+  // Rank: for each item, the lanes holding the same digit are found with
+  // kBits ballots; the lowest of them adds the group size to the wave's
+  // digit counter with an LDS atomic that returns the count before the add,
+  // and the group reads it from that lane.  rank = that count + position
+  // inside the group.  A wave's LDS operations complete in issue order, so
+  // all the items' atomics are issued first and their results read
+  // afterwards (rank[j] holds the position in bits 0-7 and the group's first
+  // lane in bits 8-13 meanwhile).
   uint32_t rank[kSortItems], prev[kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const bool valid = (wbase + 64 * j + lane) < r.end;
-    const uint32_t d = (key[j] >> shift) & (kRadix - 1);
-    const uint64_t peers = match_digit(d, valid);
+    const uint32_t d = (key[j] >> shift) & (D::kR - 1);
+    const uint64_t peers = match_digit<kBits>(d, valid);
     const uint32_t below = popc_below(peers);
     // Every lane issues the atomic (no branch, so nothing waits for its
     // result here): a group's first lane adds the group size to the digit
@@ -255,25 +307,33 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
   __syncthreads();
 
-  // Thread d: this tile's count of digit d and per-wave exclusive offsets.
-  const uint32_t d = tid;
-  uint32_t total = 0;
+  // This tile's count of each of the thread's digits and per-wave exclusive
+  // offsets, then the tile-local start of each digit run.
+  uint32_t total[D::kDpt], start[D::kDpt];
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    const uint32_t c = s_whist[w][d];
-    s_whist[w][d] = total;
-    total += c;
+  for (int i = 0; i < D::kDpt; ++i) {
+    const uint32_t d = tid * D::kDpt + i;
+    total[i] = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_whist[w][d];
+      s_whist[w][d] = total[i];
+      total[i] += c;
+    }
   }
-  const uint32_t start = block_exclusive_scan(total, s_wsum);
-  s_start[d] = start;
-  s_gbase[d] = goff - start;
+  digit_prefix<kBits>(total, start, s_wsum);
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) {
+    s_start[tid * D::kDpt + i] = start[i];
+    s_gbase[tid * D::kDpt + i] = goff[i] - start[i];
+  }
   __syncthreads();
 
   // Reorder the tile in LDS by (digit, input order).
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     if ((wbase + 64 * j + lane) < r.end) {
-      const uint32_t dj = (key[j] >> shift) & (kRadix - 1);
+      const uint32_t dj = (key[j] >> shift) & (D::kR - 1);
       const uint32_t lp = s_start[dj] + s_whist[wave][dj] + rank[j];
       s_keys[lp] = key[j];
       s_vals[lp] = val[j];
@@ -288,7 +348,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   const uint32_t tile_n = r.end - r.begin;
   for (uint32_t p = tid; p < tile_n; p += kSortThreads) {
     const uint32_t k = s_keys[p];
-    const uint32_t g = s_gbase[(k >> shift) & (kRadix - 1)] + p;
+    const uint32_t g = s_gbase[(k >> shift) & (D::kR - 1)] + p;
     if (g >= r.seg_begin && g < r.seg_end) {
       keys_out[g] = k;
       vals_out[g] = s_vals[p];
@@ -296,35 +356,54 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 }
 
-void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
-                      uint32_t *keys_out, int32_t *vals_out, uint32_t n,
-                      uint32_t tiles, uint32_t *cnt, uint32_t *off,
-                      bool counted, Ctl *ctl, hipStream_t s,
-                      const SortSegs *segs, uint32_t num_segs,
+template <int kBits>
+void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
+               uint32_t *keys_out, int32_t *vals_out, uint32_t n,
+               uint32_t tiles, uint32_t *cnt, uint32_t *off, bool counted,
+               Ctl *ctl, hipStream_t s, const SortSegs &sg, bool segmented,
+               uint32_t num_segs, uint32_t *seg_base) {
+  constexpr int kR = Digits<kBits>::kR;
+  const uint32_t shift = (uint32_t)pass * kBits;
+  uint32_t *hist = ctl->hist + (size_t)pass * kR;
+  if (!counted)
+    hipLaunchKernelGGL(k_count<kBits>, dim3(tiles), dim3(kSortThreads), 0, s,
+                       keys_in, n, shift, cnt, sg);
+  if (segmented)
+    hipLaunchKernelGGL(k_seg_bases<kBits>, dim3(num_segs), dim3(kSortThreads),
+                       0, s, cnt, tiles, sg, num_segs, seg_base);
+  else
+    hipLaunchKernelGGL(k_digit_totals, dim3(kR), dim3(kSortThreads), 0, s, cnt,
+                       tiles, hist);
+  hipLaunchKernelGGL(k_scan_columns<kBits>, dim3(kR), dim3(kSortThreads), 0, s,
+                     cnt, off, tiles, hist, sg, seg_base);
+  if (pass == 0)
+    hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
+                       dim3(kSortThreads), 0, s, keys_in,
+                       (const int32_t *)nullptr, keys_out, vals_out, n, shift,
+                       off, sg);
+  else
+    hipLaunchKernelGGL((k_scatter<kBits, false>), dim3(tiles),
+                       dim3(kSortThreads), 0, s, keys_in, vals_in, keys_out,
+                       vals_out, n, shift, off, sg);
+}
+
+void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
+                      const int32_t *vals_in, uint32_t *keys_out,
+                      int32_t *vals_out, uint32_t n, uint32_t tiles,
+                      uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
+                      hipStream_t s, const SortSegs *segs, uint32_t num_segs,
                       uint32_t *seg_base) {
   if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
-  const uint32_t shift = (uint32_t)pass * kRadixBits;
-  if (!counted)
-    hipLaunchKernelGGL(k_count, dim3(tiles), dim3(kSortThreads), 0, s, keys_in,
-                       n, shift, cnt, sg);
-  if (segs)
-    hipLaunchKernelGGL(k_seg_bases, dim3(num_segs), dim3(kSortThreads), 0, s,
-                       cnt, tiles, sg, num_segs, seg_base);
+  if (bits == kWideBits)
+    sort_pass<kWideBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
+                         cnt, off, counted, ctl, s, sg, segs != nullptr,
+                         num_segs, seg_base);
   else
-    hipLaunchKernelGGL(k_digit_totals, dim3(kRadix), dim3(kSortThreads), 0, s,
-                       cnt, tiles, ctl->hist[pass]);
-  hipLaunchKernelGGL(k_scan_columns, dim3(kRadix), dim3(kSortThreads), 0, s,
-                     cnt, off, tiles, ctl->hist[pass], sg, seg_base);
-  if (pass == 0)
-    hipLaunchKernelGGL(k_scatter<true>, dim3(tiles), dim3(kSortThreads), 0, s,
-                       keys_in, (const int32_t *)nullptr, keys_out, vals_out,
-                       n, shift, off, sg);
-  else
-    hipLaunchKernelGGL(k_scatter<false>, dim3(tiles), dim3(kSortThreads), 0, s,
-                       keys_in, vals_in, keys_out, vals_out, n, shift, off,
-                       sg);
+    sort_pass<kRadixBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
+                          cnt, off, counted, ctl, s, sg, segs != nullptr,
+                          num_segs, seg_base);
 }
 
 }  // namespace lbvh

@@ -6,6 +6,7 @@
 // their arguments on the host, enqueue kernels and return; nothing here
 // blocks on the GPU except lbvh_upload/lbvh_download/lbvh_synchronize.
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <new>
 #include <string>
@@ -40,6 +41,9 @@ struct lbvh_ctx {
   int tree_tile = kTreeTileDefault;
   bool tree_tile_fixed = false;  // set_tuning overrides the tuned table
   int tune_tile[33] = {0};       // autotuned tree tile per log2 size bucket
+  int sort_bits = kRadixBits;    // sort digit width (8 or 10)
+  bool sort_bits_fixed = false;
+  int tune_bits[33] = {0};       // autotuned digit width per size bucket
   std::string last_error;
   std::unordered_map<void *, uint64_t> user_allocs;
   std::recursive_mutex mu;
@@ -60,7 +64,7 @@ struct lbvh_ctx {
       b_bounds, b_tree_items, b_tree_counts, b_table, b_cen;
   struct BatchPlan {
     std::vector<uint32_t> offsets;
-    int tile = 0;
+    int tile = 0, bits = kRadixBits;
     uint32_t S = 0, total = 0, sort_tiles = 0;
     int levels = 0;
     std::vector<uint32_t> level_tiles;
@@ -241,14 +245,14 @@ int check_n(lbvh_ctx *ctx, uint32_t n) {
   return LBVH_OK;
 }
 
-int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n) {
+int ensure_sort_scratch(lbvh_ctx *ctx, uint32_t n, int bits) {
   int st;
   if ((st = ensure(ctx, ctx->keys_a, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->keys_b, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_a, 4ull * n, false))) return st;
   if ((st = ensure(ctx, ctx->idx_b, 4ull * n, false))) return st;
   // per-tile digit counts and offsets of one pass
-  const uint64_t words = (uint64_t)sort_tiles(n) * kRadix * 2;
+  const uint64_t words = (uint64_t)sort_tiles(n) * sort_radix(bits) * 2;
   return ensure(ctx, ctx->digits, 4ull * words, false);
 }
 
@@ -258,16 +262,74 @@ int bucket_of(uint32_t n) {
   return b;  // floor(log2 n)
 }
 
-// Tree tile for an n-triangle build: a fixed value, else the autotuned entry
-// of n's size bucket (or the nearest tuned bucket), else the default.
-int tile_for(const lbvh_ctx *ctx, uint32_t n) {
-  if (ctx->tree_tile_fixed || n == 0) return ctx->tree_tile;
+// A tunable's value for an n-triangle build: a fixed value, else the
+// autotuned entry of n's size bucket (or the nearest tuned bucket), else
+// `untuned` (0: the context's default value).
+int tuned_for(int value, bool fixed, const int (&table)[33], uint32_t n,
+              int untuned = 0) {
+  if (fixed || n == 0) return value;
   const int b = bucket_of(n);
   for (int d = 0; d <= 32; ++d) {
-    if (b - d >= 0 && ctx->tune_tile[b - d]) return ctx->tune_tile[b - d];
-    if (b + d <= 32 && ctx->tune_tile[b + d]) return ctx->tune_tile[b + d];
+    if (b - d >= 0 && table[b - d]) return table[b - d];
+    if (b + d <= 32 && table[b + d]) return table[b + d];
   }
-  return ctx->tree_tile;
+  return untuned ? untuned : value;
+}
+
+int tile_for(const lbvh_ctx *ctx, uint32_t n) {
+  return tuned_for(ctx->tree_tile, ctx->tree_tile_fixed, ctx->tune_tile, n);
+}
+
+// Untuned, 10-bit digits up to 2^21 keys (one pass and four launches fewer,
+// 5-9% faster builds of uniform meshes of 2^16..2^21 triangles on MI355X)
+// and 8-bit digits above (the 10-bit scatter holds 57 KB of LDS per block,
+// half the 8-bit one's occupancy, and was 2-4% slower at 2^22..2^24).
+constexpr uint32_t kWideDigitsMaxKeys = 1u << 21;
+int bits_for(const lbvh_ctx *ctx, uint32_t n) {
+  return tuned_for(ctx->sort_bits, ctx->sort_bits_fixed, ctx->tune_bits, n,
+                   n <= kWideDigitsMaxKeys ? kWideBits : kRadixBits);
+}
+
+// The key buffers of each pass of a sort from `in` to `out`: each pass reads
+// one buffer and writes another.  An even number of passes runs
+// in -> tmp -> out -> tmp -> out (`in` may equal `out`); an odd one runs
+// in -> out -> tmp -> out when in != out (tmp may equal in), else
+// in -> tmp -> x -> out.  Payloads: generated -> ... ending in vout, through
+// vtmp.
+struct SortPlan {
+  const uint32_t *src[4];
+  uint32_t *dst[4];
+  const int32_t *vsrc[4];
+  int32_t *vdst[4];
+  int passes;
+};
+SortPlan sort_plan(int bits, const uint32_t *in, uint32_t *tmp, uint32_t *out,
+                   uint32_t *x, int32_t *vtmp, int32_t *vout) {
+  SortPlan P{};
+  P.passes = sort_passes(bits);
+  uint32_t *chain[5];
+  int32_t *vchain[5];
+  if (P.passes % 2 == 0) {
+    uint32_t *c[5] = {const_cast<uint32_t *>(in), tmp, out, tmp, out};
+    std::copy(c, c + 5, chain);
+  } else if (in != out) {
+    uint32_t *c[5] = {const_cast<uint32_t *>(in), out, tmp, out, nullptr};
+    std::copy(c, c + 5, chain);
+  } else {
+    uint32_t *c[5] = {const_cast<uint32_t *>(in), tmp, x, out, nullptr};
+    std::copy(c, c + 5, chain);
+  }
+  // payloads end in vout: alternate backwards from the last pass
+  for (int p = P.passes; p >= 1; --p)
+    vchain[p] = ((P.passes - p) % 2 == 0) ? vout : vtmp;
+  vchain[0] = nullptr;
+  for (int p = 0; p < P.passes; ++p) {
+    P.src[p] = chain[p];
+    P.dst[p] = chain[p + 1];
+    P.vsrc[p] = vchain[p];
+    P.vdst[p] = vchain[p + 1];
+  }
+  return P;
 }
 
 int ensure_tree_scratch(lbvh_ctx *ctx, uint32_t n, int tile) {
@@ -283,23 +345,23 @@ int ensure_partials(lbvh_ctx *ctx) {
   return ensure(ctx, ctx->partials, 32ull * kBoundsBlocks, false);
 }
 
-// The four digit passes: in -> tmp -> out -> tmp -> out.  `in` may equal
-// `out` (the stand-alone sort is in place, like the reference's).  counted0:
-// the Morton kernel already wrote pass 0's per-tile counts into the digit
-// buffer; the other passes count their own.
+// The digit passes of one sort (sort_plan: `in` may equal `out`, as the
+// stand-alone sort is in place like the reference's; x is used only then,
+// with an odd pass count).  counted0: the Morton kernel already wrote pass
+// 0's per-tile counts into the digit buffer; the other passes count their
+// own.
 void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-              int32_t *vtmp, int32_t *vout, uint32_t n, bool counted0) {
+              uint32_t *x, int32_t *vtmp, int32_t *vout, uint32_t n,
+              bool counted0, int bits) {
   const uint32_t tiles = sort_tiles(n);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
-  uint32_t *off = cnt + (size_t)kRadix * tiles;
-  const uint32_t *src[kSortPasses] = {in, tmp, out, tmp};
-  uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
-  const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
-  int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
-  for (int p = 0; p < kSortPasses; ++p) {
+  uint32_t *off = cnt + (size_t)sort_radix(bits) * tiles;
+  const SortPlan P = sort_plan(bits, in, tmp, out, x, vtmp, vout);
+  for (int p = 0; p < P.passes; ++p) {
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
-    launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], n, tiles, cnt, off,
-                     p == 0 && counted0, ctx->ctl, ctx->stream);
+    launch_sort_pass(bits, p, P.src[p], P.vsrc[p], P.dst[p], P.vdst[p], n,
+                     tiles, cnt, off, p == 0 && counted0, ctx->ctl,
+                     ctx->stream);
   }
 }
 
@@ -355,6 +417,7 @@ int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
   uint32_t big = 0;
   for (uint32_t k = 0; k < S; ++k) big = std::max(big, offsets[k + 1] - offsets[k]);
   if (P.S == S && P.tile == tile_for(ctx, big) &&
+      P.bits == bits_for(ctx, offsets[S]) &&
       std::equal(P.offsets.begin(), P.offsets.end(), offsets) &&
       P.offsets.size() == (size_t)S + 1)
     return LBVH_OK;
@@ -365,6 +428,9 @@ int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
   for (uint32_t k = 0; k < S; ++k)
     largest = std::max(largest, offsets[k + 1] - offsets[k]);
   P.tile = tile_for(ctx, largest);
+  // one segmented sort runs over all the keys: its digit width is that of
+  // a sort of the total
+  P.bits = bits_for(ctx, offsets[S]);
   P.total = offsets[S];
   std::vector<uint32_t> tab;
   auto put = [&](const std::vector<uint32_t> &v) {
@@ -434,23 +500,22 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   uint32_t *keys = static_cast<uint32_t *>(ctx->b_keys_a.ptr);
   uint32_t *tmp = static_cast<uint32_t *>(ctx->b_keys_b.ptr);
   uint32_t *out = skeys ? skeys : keys;
+  if (sort_passes(P.bits) % 2 && out == keys) std::swap(out, tmp);
   int32_t *vtmp = static_cast<int32_t *>(ctx->b_idx_b.ptr);
   int32_t *vout = perm ? perm : static_cast<int32_t *>(ctx->b_idx_a.ptr);
   launch_batch_morton(t4, cen, P.total, ss, P.sort_tiles,
                       have_bounds ? static_cast<const float *>(ctx->b_bounds.ptr)
                                   : nullptr,
                       ord, ctx->div_mode, keys,
-                      static_cast<uint32_t *>(ctx->b_digits.ptr), ctx->stream);
+                      static_cast<uint32_t *>(ctx->b_digits.ptr), P.bits,
+                      ctx->stream);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->b_digits.ptr);
-  uint32_t *off = cnt + (size_t)kRadix * P.sort_tiles;
-  const uint32_t *src[kSortPasses] = {keys, tmp, out, tmp};
-  uint32_t *dst[kSortPasses] = {tmp, out, tmp, out};
-  const int32_t *vsrc[kSortPasses] = {nullptr, vtmp, vout, vtmp};
-  int32_t *vdst[kSortPasses] = {vtmp, vout, vtmp, vout};
-  for (int p = 0; p < kSortPasses; ++p)
-    launch_sort_pass(p, src[p], vsrc[p], dst[p], vdst[p], P.total,
-                     P.sort_tiles, cnt, off, p == 0, ctx->ctl, ctx->stream,
-                     &ss, P.S, seg_base);
+  uint32_t *off = cnt + (size_t)sort_radix(P.bits) * P.sort_tiles;
+  const SortPlan SP = sort_plan(P.bits, keys, tmp, out, nullptr, vtmp, vout);
+  for (int p = 0; p < SP.passes; ++p)
+    launch_sort_pass(P.bits, p, SP.src[p], SP.vsrc[p], SP.dst[p], SP.vdst[p],
+                     P.total, P.sort_tiles, cnt, off, p == 0, ctx->ctl,
+                     ctx->stream, &ss, P.S, seg_base);
   int4 *items = static_cast<int4 *>(ctx->b_tree_items.ptr);
   uint32_t *counts = static_cast<uint32_t *>(ctx->b_tree_counts.ptr);
   uint64_t base = 0, prev = 0;
@@ -567,11 +632,28 @@ int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value) {
   CTX_GUARD(ctx);
   switch (param) {
     case LBVH_TUNE_TREE_TILE:
+      if (value == 0) {
+        ctx->tree_tile = kTreeTileDefault;
+        ctx->tree_tile_fixed = false;
+        return LBVH_OK;
+      }
       if (value != 256 && value != 512 && value != 1024)
         return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
                     "tree tile must be 256, 512 or 1024");
       ctx->tree_tile = value;
       ctx->tree_tile_fixed = true;
+      return LBVH_OK;
+    case LBVH_TUNE_SORT_BITS:
+      if (value == 0) {
+        ctx->sort_bits = kRadixBits;
+        ctx->sort_bits_fixed = false;
+        return LBVH_OK;
+      }
+      if (value != kRadixBits && value != kWideBits)
+        return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                    "sort digit width must be 8 or 10");
+      ctx->sort_bits = value;
+      ctx->sort_bits_fixed = true;
       return LBVH_OK;
     default:
       return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "unknown tuning parameter");
@@ -586,6 +668,7 @@ int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n) {
   if (!ctx) return -1;
   switch (param) {
     case LBVH_TUNE_TREE_TILE: return tile_for(ctx, n);
+    case LBVH_TUNE_SORT_BITS: return bits_for(ctx, n);
     default: return -1;
   }
 }
@@ -597,6 +680,7 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
   if (n_min == 0 || n_max < n_min || n_max > LBVH_MAX_TRIS || reps < 1)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bad autotune range");
   static const int tiles[3] = {256, 512, 1024};
+  static const int widths[2] = {kRadixBits, kWideBits};
   const int b0 = bucket_of(n_min), b1 = bucket_of(n_max);
   const uint32_t nbig = 1u << b1;
   void *tri = nullptr, *bvh = nullptr, *otri = nullptr;
@@ -613,9 +697,11 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
   hipEvent_t a = nullptr, b = nullptr;
   (void)hipEventCreate(&a);
   (void)hipEventCreate(&b);
-  const bool was_fixed = ctx->tree_tile_fixed;
+  const bool tile_was_fixed = ctx->tree_tile_fixed;
+  const bool bits_were_fixed = ctx->sort_bits_fixed;
   ctx->tree_tile_fixed = true;  // force each variant while timing
-  const int saved = ctx->tree_tile;
+  ctx->sort_bits_fixed = true;
+  const int saved_tile = ctx->tree_tile, saved_bits = ctx->sort_bits;
   int out_i = 0;
   st = LBVH_OK;
   for (int k = b0; k <= b1 && st == LBVH_OK; ++k) {
@@ -623,9 +709,10 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
     launch_synth_uniform(0x5EED0000ull + k, n, 0.01, static_cast<float *>(tri),
                          ctx->stream);
     float best = 1e30f;
-    int best_tile = kTreeTileDefault;
-    for (int v = 0; v < 3 && st == LBVH_OK; ++v) {
-      ctx->tree_tile = tiles[v];
+    int best_tile = kTreeTileDefault, best_bits = kRadixBits;
+    for (int v = 0; v < 6 && st == LBVH_OK; ++v) {
+      ctx->tree_tile = tiles[v / 2];
+      ctx->sort_bits = widths[v % 2];
       std::vector<float> ms;
       for (int r = -1; r < reps && st == LBVH_OK; ++r) {
         (void)hipEventRecord(a, ctx->stream);
@@ -638,17 +725,28 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
           ms.push_back(t);
         }
       }
-      if (ms.empty()) continue;
-      std::sort(ms.begin(), ms.end());
-      const float med = ms[ms.size() / 2];
+      float med = NAN;
+      if (!ms.empty()) {
+        std::sort(ms.begin(), ms.end());
+        med = ms[ms.size() / 2];
+        if (med < best) {
+          best = med;
+          best_tile = tiles[v / 2];
+          best_bits = widths[v % 2];
+        }
+      }
       if (ms_out) ms_out[out_i] = med;
       ++out_i;
-      if (med < best) { best = med; best_tile = tiles[v]; }
     }
-    if (st == LBVH_OK) ctx->tune_tile[k] = best_tile;
+    if (st == LBVH_OK) {
+      ctx->tune_tile[k] = best_tile;
+      ctx->tune_bits[k] = best_bits;
+    }
   }
-  ctx->tree_tile = saved;
-  ctx->tree_tile_fixed = (st == LBVH_OK) ? false : was_fixed;
+  ctx->tree_tile = saved_tile;
+  ctx->sort_bits = saved_bits;
+  ctx->tree_tile_fixed = (st == LBVH_OK) ? false : tile_was_fixed;
+  ctx->sort_bits_fixed = (st == LBVH_OK) ? false : bits_were_fixed;
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   dev_free_raw(ctx, tri, 48ull * nbig);
@@ -793,10 +891,12 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
     HIP_TRY(ctx, hipMemsetAsync(idx_out, 0, 4, ctx->stream));
     return LBVH_OK;
   }
-  if ((st = ensure_sort_scratch(ctx, n))) return st;
+  const int bits = bits_for(ctx, n);
+  if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   if ((st = begin_op(ctx))) return st;
   run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
-           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false);
+           static_cast<uint32_t *>(ctx->keys_a.ptr),
+           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false, bits);
   return check_launch(ctx, "lbvh_sort");
 }
 
@@ -855,7 +955,8 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = ensure_sort_scratch(ctx, n))) return st;
+  const int bits = bits_for(ctx, n);
+  if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
   const bool sort = n > 1;
@@ -869,7 +970,10 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
   float *cen = use_cen ? static_cast<float *>(ctx->cen.ptr) : nullptr;
   uint32_t *keys = static_cast<uint32_t *>(ctx->keys_a.ptr);
+  uint32_t *tmp = static_cast<uint32_t *>(ctx->keys_b.ptr);
   uint32_t *skeys = sorted_keys_out ? sorted_keys_out : keys;
+  // an odd number of passes ends in the other buffer: sort into keys_b
+  if (sort && sort_passes(bits) % 2 && skeys == keys) std::swap(skeys, tmp);
   int32_t *perm = perm_out ? perm_out : static_cast<int32_t *>(ctx->idx_a.ptr);
   MortonParams p{};
   if (bounds_host) {
@@ -884,15 +988,16 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
     StageScope sc(ctx, LBVH_STAGE_MORTON);
     uint32_t *cnt = sort ? static_cast<uint32_t *>(ctx->digits.ptr) : nullptr;
     if (use_cen)
-      launch_morton_cen(cen, n, ctx->div_mode, keys, ctx->ctl, cnt,
+      launch_morton_cen(cen, n, ctx->div_mode, keys, ctx->ctl, cnt, bits,
                         ctx->stream);
     else
       launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                         ctx->ctl, bounds_host != nullptr, ctx->stream, cnt);
+                         ctx->ctl, bounds_host != nullptr, ctx->stream, cnt,
+                         bits);
   }
   if (sort) {
-    run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), skeys,
-             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n, true);
+    run_sort(ctx, keys, tmp, skeys, nullptr,
+             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n, true, bits);
   } else {
     HIP_TRY(ctx, hipMemsetAsync(perm, 0, 4, ctx->stream));
     if (skeys != keys)
@@ -945,10 +1050,10 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if ((st = ensure(ctx, ctx->b_keys_b, 4ull * total, false))) return st;
   if ((st = ensure(ctx, ctx->b_idx_a, 4ull * total, false))) return st;
   if ((st = ensure(ctx, ctx->b_idx_b, 4ull * total, false))) return st;
-  if ((st = ensure(ctx, ctx->b_digits, 8ull * kRadix * P.sort_tiles, false)))
+  const uint64_t radix = sort_radix(P.bits);
+  if ((st = ensure(ctx, ctx->b_digits, 8ull * radix * P.sort_tiles, false)))
     return st;
-  if ((st = ensure(ctx, ctx->b_hist, 4ull * kRadix * num_meshes,
-                   false)))
+  if ((st = ensure(ctx, ctx->b_hist, 4ull * radix * num_meshes, false)))
     return st;
   if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;
   if ((st = ensure(ctx, ctx->b_tree_items,
@@ -971,7 +1076,8 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   std::vector<const void *> key = {
       tri, bvh_out, tri_out, sorted_keys_out, perm_out,
       reinterpret_cast<const void *>((uintptr_t)(bounds_host ? 1 : 0)),
-      reinterpret_cast<const void *>((uintptr_t)ctx->div_mode)};
+      reinterpret_cast<const void *>((uintptr_t)ctx->div_mode),
+      reinterpret_cast<const void *>((uintptr_t)P.bits)};
   for (DevBuf *b : all_scratch(ctx)) key.push_back(b->ptr);
   auto &Pm = ctx->plan;
   if (!Pm.exec || Pm.graph_key != key) {

@@ -8,9 +8,20 @@ namespace lbvh {
 
 // ---- sizes and launch shapes --------------------------------------------
 
+// Sort digits: 8 bits (4 passes over the 30-bit keys: bits 0-7, 8-15,
+// 16-23, 24-29) or, as a tunable, 10 bits (3 passes: 0-9, 10-19, 20-29).
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;  // 256 digit values per pass
-constexpr int kSortPasses = 4;           // 30-bit keys: bits 0-7,8-15,16-23,24-29
+constexpr int kWideBits = 10;
+constexpr int kWideRadix = 1 << kWideBits;
+constexpr int kKeyBits = 30;
+constexpr int kHistWords = 3 * kWideRadix;  // >= passes x digits of either width
+__host__ __device__ constexpr int sort_passes(int bits) {
+  return (kKeyBits + bits - 1) / bits;
+}
+__host__ __device__ constexpr int sort_radix(int bits) { return 1 << bits; }
+static_assert(sort_passes(kRadixBits) * kRadix <= kHistWords, "hist size");
+static_assert(sort_passes(kWideBits) * kWideRadix <= kHistWords, "hist size");
 constexpr int kSortThreads = 256;        // 4 wave64 per tile
 constexpr int kSortItems = 16;           // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
@@ -34,13 +45,14 @@ constexpr uint32_t kSpinCap = 1u << 22;  // polls of a bounded wait (per-stage u
 //   fault       sticky flag: a bounded wait gave up or a frontier overflowed
 //               (never set in a correct run; read by lbvh_synchronize)
 //   bounds      scene bounds of the last reduction (k_bounds_final)
-//   hist        global digit counts of each pass of the current sort
+//   hist        global digit counts of each pass of the current sort (pass
+//               p's at hist + p * radix)
 struct Ctl {
   uint32_t epoch;
   uint32_t fault;
   uint32_t pad0[6];
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
-  uint32_t hist[kSortPasses][kRadix];    // global digit counts per pass
+  uint32_t hist[kHistWords];             // global digit counts per pass
 };
 
 // Exchange record of one side of a split in the fused up pass: the side's box
@@ -69,13 +81,13 @@ void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
 // the centres launch_bounds wrote and the bounds in ctl.
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
-                       hipStream_t s);
-// cnt (kRadix x sort_tiles(n) words, or null): also write the first sort
-// pass's per-tile digit counts.
+                       int bits, hipStream_t s);
+// cnt (sort_radix(bits) x sort_tiles(n) words, or null): also write the
+// first sort pass's per-tile digit counts for digits of `bits` bits.
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
                         Ctl *ctl, bool bump_epoch, hipStream_t s,
-                        uint32_t *cnt = nullptr);
+                        uint32_t *cnt = nullptr, int bits = kRadixBits);
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
 void launch_bump_epoch(Ctl *ctl, hipStream_t s);
 
@@ -89,20 +101,20 @@ struct SortSegs {
   const uint32_t *seg_tile0;
   const uint32_t *seg_off;
 };
-// One digit pass over `tiles` sort tiles (pass 0 takes the position in the
-// segment as payload and ignores vals_in).  cnt and off hold kRadix x tiles
-// words each; counted: cnt already holds this pass's per-tile digit counts
-// (the Morton kernels write pass 0's).  A single mesh takes its digit bases
-// from the column sums of cnt (into ctl->hist[pass]); segments from their
-// own tiles' sums (into seg_base, kRadix words per segment).  The four
-// passes sort keys_in into keys_out through keys_tmp:
-//   0: in -> tmp, 1: tmp -> out, 2: out -> tmp, 3: tmp -> out.
-void launch_sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
-                      uint32_t *keys_out, int32_t *vals_out, uint32_t n,
-                      uint32_t tiles, uint32_t *cnt, uint32_t *off,
-                      bool counted, Ctl *ctl, hipStream_t s,
-                      const SortSegs *segs = nullptr, uint32_t num_segs = 0,
-                      uint32_t *seg_base = nullptr);
+// One digit pass of `bits`-bit digits (kRadixBits or kWideBits) over
+// `tiles` sort tiles (pass 0 takes the position in the segment as payload
+// and ignores vals_in).  cnt and off hold sort_radix(bits) x tiles words
+// each; counted: cnt already holds this pass's per-tile digit counts (the
+// Morton kernels write pass 0's).  A single mesh takes its digit bases from
+// the column sums of cnt (into ctl->hist); segments from their own tiles'
+// sums (into seg_base, sort_radix(bits) words per segment).  keys_in and
+// keys_out must differ.
+void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
+                      const int32_t *vals_in, uint32_t *keys_out,
+                      int32_t *vals_out, uint32_t n, uint32_t tiles,
+                      uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
+                      hipStream_t s, const SortSegs *segs = nullptr,
+                      uint32_t num_segs = 0, uint32_t *seg_base = nullptr);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
@@ -159,7 +171,7 @@ void launch_batch_morton(const float4 *tri, const float *cen, uint32_t total,
                          const SortSegs &segs, uint32_t tiles,
                          const float *seg_bounds, const uint32_t *seg_ord,
                          int div_mode, uint32_t *keys, uint32_t *cnt0,
-                         hipStream_t s);
+                         int bits, hipStream_t s);
 
 // k_raycast.hip
 void launch_raycast(const void *nodes, const float *tri, uint32_t n,

@@ -24,6 +24,7 @@ DIV_MUL_RCP = 0
 DIV_IEEE = 1
 
 TUNE_TREE_TILE = 0
+TUNE_SORT_BITS = 1
 
 STATUS = {
     0: "ok",
@@ -242,14 +243,16 @@ class Context:
         return self.lib.lbvh_get_tuning_for(self.handle, param, n)
 
     TUNE_TILES = (256, 512, 1024)
+    TUNE_BITS = (8, 10)
 
     def autotune(self, n_min: int, n_max: int, reps: int = 3) -> dict:
-        """Runs lbvh_autotune; returns {bucket size: {tile: ms}}."""
+        """Runs lbvh_autotune; returns {bucket size: {(tile, digit bits): ms}}."""
         b0, b1 = n_min.bit_length() - 1, n_max.bit_length() - 1
-        ms = np.zeros((b1 - b0 + 1) * len(self.TUNE_TILES), np.float32)
+        variants = [(t, b) for t in self.TUNE_TILES for b in self.TUNE_BITS]
+        ms = np.zeros((b1 - b0 + 1) * len(variants), np.float32)
         self._check(self.lib.lbvh_autotune(self.handle, n_min, n_max, reps, _ptr(ms)))
-        ms = ms.reshape(-1, len(self.TUNE_TILES))
-        return {1 << (b0 + i): dict(zip(self.TUNE_TILES, map(float, ms[i])))
+        ms = ms.reshape(-1, len(variants))
+        return {1 << (b0 + i): dict(zip(variants, map(float, ms[i])))
                 for i in range(ms.shape[0])}
 
     def profile_enable(self, on: bool = True):
