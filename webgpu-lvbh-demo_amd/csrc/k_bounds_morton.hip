@@ -108,15 +108,24 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
   }
 }
 
-__global__ __launch_bounds__(kMapThreads) void k_bounds_final(
-    const float4 *__restrict__ partials, uint32_t blocks, Ctl *ctl) {
-  float w[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+// Folds the bounds partials of `blocks` blocks (min.xyz, max.xyz in w of
+// thread 0).  Every thread of the block must call it.
+__device__ void fold_partials(const float4 *__restrict__ partials,
+                              uint32_t blocks, float w[6]) {
+  w[0] = w[1] = w[2] = INFINITY;
+  w[3] = w[4] = w[5] = -INFINITY;
   for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
     const float4 lo = partials[2 * b], hi = partials[2 * b + 1];
     w[0] = fminf(w[0], lo.x); w[1] = fminf(w[1], lo.y); w[2] = fminf(w[2], lo.z);
     w[3] = fmaxf(w[3], hi.x); w[4] = fmaxf(w[4], hi.y); w[5] = fmaxf(w[5], hi.z);
   }
   block_minmax(w);
+}
+
+__global__ __launch_bounds__(kMapThreads) void k_bounds_final(
+    const float4 *__restrict__ partials, uint32_t blocks, Ctl *ctl) {
+  float w[6];
+  fold_partials(partials, blocks, w);
   if (threadIdx.x == 0) {
     ctl->bounds[0] = w[0]; ctl->bounds[1] = w[1]; ctl->bounds[2] = w[2];
     ctl->bounds[3] = 0.0f;
@@ -214,22 +223,36 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
 // compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
 // :192-195 onward; the centre of :191 is already rounded the same way), plus
 // the first sort pass's per-tile digit counts.  One block per sort tile, as
-// k_aabb_morton with cnt; bounds from ctl.
+// k_aabb_morton with cnt.  Bounds from ctl, or (parts != 0) folded by every
+// block from the bounds partials as k_bounds_final folds them, which saves
+// small builds that launch.
 template <int kBits>
 __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     const float *__restrict__ cen, uint32_t n, int div_mode,
-    uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt) {
+    uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt,
+    const float4 *__restrict__ partials, uint32_t parts) {
   constexpr int kR = 1 << kBits;
   __shared__ uint32_t s_hist[kR];
+  __shared__ float s_b[8];
   for (int d = threadIdx.x; d < kR; d += kMapThreads) s_hist[d] = 0u;
+  if (parts) {
+    float w[6];
+    fold_partials(partials, parts, w);
+    if (threadIdx.x == 0) {
+      s_b[0] = w[0]; s_b[1] = w[1]; s_b[2] = w[2];
+      s_b[4] = w[3]; s_b[5] = w[4]; s_b[6] = w[5];
+    }
+  } else if (threadIdx.x < 8) {
+    s_b[threadIdx.x] = ctl->bounds[threadIdx.x];
+  }
+  __syncthreads();
   float bmin[3], ext[3], rcp[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    bmin[c] = ctl->bounds[c];
-    ext[c] = __fsub_rn(ctl->bounds[4 + c], bmin[c]);
+    bmin[c] = s_b[c];
+    ext[c] = __fsub_rn(s_b[4 + c], bmin[c]);
     rcp[c] = __fdiv_rn(1.0f, ext[c]);
   }
-  __syncthreads();
   const bool ieee = (div_mode == 1);
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t first = tile * (uint32_t)kSortTile;
@@ -273,15 +296,18 @@ __global__ void k_copy_bounds(const Ctl *ctl, float *out) {
 
 __global__ void k_bump_epoch(Ctl *ctl) { ctl->epoch += 1; }
 
-void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
-                   bool bump_epoch, hipStream_t s, float *cen) {
+uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
+                       Ctl *ctl, bool bump_epoch, hipStream_t s, float *cen,
+                       uint32_t fold_max) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kBoundsBlocks) blocks = kBoundsBlocks;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
                      partials, ctl, bump_epoch ? 1 : 0, cen);
+  if (blocks <= fold_max) return blocks;  // the consumer folds the partials
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kMapThreads), 0, s,
                      partials, blocks, ctl);
+  return 0;
 }
 
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
@@ -306,16 +332,16 @@ void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
 
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        uint32_t *keys, const Ctl *ctl, uint32_t *cnt, int bits,
-                       hipStream_t s) {
+                       hipStream_t s, const float4 *partials, uint32_t parts) {
   const uint32_t tiles = sort_tiles(n);
   if (tiles == 0) return;
   if (bits == kWideBits)
     hipLaunchKernelGGL(k_morton_cen<kWideBits>, dim3(tiles), dim3(kMapThreads),
-                       0, s, cen, n, div_mode, keys, ctl, cnt);
+                       0, s, cen, n, div_mode, keys, ctl, cnt, partials, parts);
   else
     hipLaunchKernelGGL(k_morton_cen<kRadixBits>, dim3(tiles),
                        dim3(kMapThreads), 0, s, cen, n, div_mode, keys, ctl,
-                       cnt);
+                       cnt, partials, parts);
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

@@ -37,6 +37,8 @@ namespace lbvh {
 
 constexpr int kWaves = kSortThreads / 64;
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
+// Single-mesh sorts of at most this many tiles (64K keys) scan in one block.
+constexpr uint32_t kSmallScanTiles = 16;
 
 // Digits of width kBits: kR values, kDpt of them per thread of a block of
 // kSortThreads (thread t owns digits [t * kDpt, (t + 1) * kDpt)).
@@ -200,6 +202,60 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
       if (c0 + e < tiles) out[c0 + e] = s_col[pad(e)];
     }
     __syncthreads();  // s_col is refilled by the next chunk
+  }
+}
+
+// Digit totals and column scan of a single mesh in one block, for sorts of
+// at most kSmallScanTiles tiles: a small build is a chain of short
+// dependent kernels, and this replaces two per pass.  One thread per digit
+// (a block of 2^kBits threads).  The count matrix goes through LDS: loaded
+// with consecutive threads on consecutive words (a thread reading its own
+// column from global memory would put every lane on another cache line),
+// stored with rows padded to kSmallScanTiles + 1 words so that thread d's
+// walk along row d is free of bank conflicts.  The block scans the column
+// totals (the digit bases); each thread writes its row's running offsets
+// back (as k_scan_columns computes them) for a coalesced store.
+template <int kBits>
+__global__ __launch_bounds__(1 << kBits) void k_scan_small(
+    const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
+    uint32_t tiles) {
+  constexpr uint32_t kR = 1u << kBits;
+  constexpr uint32_t kStride = kSmallScanTiles + 1;
+  static_assert(kR <= 1024, "one block of one thread per digit");
+  __shared__ uint32_t s_m[kR * kStride];
+  __shared__ uint32_t s_wsum[16];
+  const uint32_t words = kR * tiles;
+  uint32_t v[kSmallScanTiles];
+#pragma unroll
+  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
+    const uint32_t e = threadIdx.x + j * kR;
+    v[j] = e < words ? cnt[e] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
+    const uint32_t e = threadIdx.x + j * kR;
+    if (e < words) s_m[(e / tiles) * kStride + e % tiles] = v[j];
+  }
+  __syncthreads();
+  uint32_t *row = s_m + threadIdx.x * kStride;
+  uint32_t total = 0;
+#pragma unroll
+  for (uint32_t t = 0; t < kSmallScanTiles; ++t)
+    if (t < tiles) total += row[t];
+  uint32_t run = block_exclusive_scan(total, s_wsum);
+#pragma unroll
+  for (uint32_t t = 0; t < kSmallScanTiles; ++t) {
+    if (t < tiles) {
+      const uint32_t c = row[t];
+      row[t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
+    const uint32_t e = threadIdx.x + j * kR;
+    if (e < words) off[e] = s_m[(e / tiles) * kStride + e % tiles];
   }
 }
 
@@ -368,14 +424,20 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   if (!counted)
     hipLaunchKernelGGL(k_count<kBits>, dim3(tiles), dim3(kSortThreads), 0, s,
                        keys_in, n, shift, cnt, sg);
-  if (segmented)
-    hipLaunchKernelGGL(k_seg_bases<kBits>, dim3(num_segs), dim3(kSortThreads),
-                       0, s, cnt, tiles, sg, num_segs, seg_base);
-  else
-    hipLaunchKernelGGL(k_digit_totals, dim3(kR), dim3(kSortThreads), 0, s, cnt,
-                       tiles, hist);
-  hipLaunchKernelGGL(k_scan_columns<kBits>, dim3(kR), dim3(kSortThreads), 0, s,
-                     cnt, off, tiles, hist, sg, seg_base);
+  if (!segmented && tiles <= kSmallScanTiles) {
+    hipLaunchKernelGGL(k_scan_small<kBits>, dim3(1), dim3(kR), 0, s, cnt, off,
+                       tiles);
+  } else {
+    if (segmented)
+      hipLaunchKernelGGL(k_seg_bases<kBits>, dim3(num_segs),
+                         dim3(kSortThreads), 0, s, cnt, tiles, sg, num_segs,
+                         seg_base);
+    else
+      hipLaunchKernelGGL(k_digit_totals, dim3(kR), dim3(kSortThreads), 0, s,
+                         cnt, tiles, hist);
+    hipLaunchKernelGGL(k_scan_columns<kBits>, dim3(kR), dim3(kSortThreads), 0,
+                       s, cnt, off, tiles, hist, sg, seg_base);
+  }
   if (pass == 0)
     hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in,

@@ -976,20 +976,22 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (sort && sort_passes(bits) % 2 && skeys == keys) std::swap(skeys, tmp);
   int32_t *perm = perm_out ? perm_out : static_cast<int32_t *>(ctx->idx_a.ptr);
   MortonParams p{};
+  float4 *partials = static_cast<float4 *>(ctx->partials.ptr);
+  uint32_t parts = 0;  // bounds partials the Morton pass folds itself
   if (bounds_host) {
     p = morton_params(ctx, bounds_host);
   } else {
     p.div_mode = ctx->div_mode;
     StageScope sc(ctx, LBVH_STAGE_BOUNDS);
-    launch_bounds(t4, n, static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
-                  true, ctx->stream, cen);
+    parts = launch_bounds(t4, n, partials, ctx->ctl, true, ctx->stream, cen,
+                          use_cen ? kFoldPartialsMax : 0);
   }
   {
     StageScope sc(ctx, LBVH_STAGE_MORTON);
     uint32_t *cnt = sort ? static_cast<uint32_t *>(ctx->digits.ptr) : nullptr;
     if (use_cen)
       launch_morton_cen(cen, n, ctx->div_mode, keys, ctx->ctl, cnt, bits,
-                        ctx->stream);
+                        ctx->stream, partials, parts);
     else
       launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
                          ctx->ctl, bounds_host != nullptr, ctx->stream, cnt,

@@ -74,14 +74,21 @@ struct MortonParams {
 
 // k_bounds_morton.hip
 // Scene bounds into ctl->bounds; cen (3 x n floats, or null): also each
-// triangle's box centre, for launch_morton_cen.
-void launch_bounds(const float4 *tri, uint32_t n, float4 *partials, Ctl *ctl,
-                   bool bump_epoch, hipStream_t s, float *cen = nullptr);
+// triangle's box centre, for launch_morton_cen.  When the reduction leaves
+// at most fold_max partials, the folding launch is skipped and the partial
+// count is returned (launch_morton_cen folds them); otherwise 0.
+uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
+                       Ctl *ctl, bool bump_epoch, hipStream_t s,
+                       float *cen = nullptr, uint32_t fold_max = 0);
 // Morton keys + per-tile digit counts (as launch_aabb_morton with cnt) from
-// the centres launch_bounds wrote and the bounds in ctl.
+// the centres launch_bounds wrote and the bounds in ctl, or folded from
+// `parts` bounds partials when parts != 0.
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        uint32_t *keys, const Ctl *ctl, uint32_t *cnt,
-                       int bits, hipStream_t s);
+                       int bits, hipStream_t s,
+                       const float4 *partials = nullptr, uint32_t parts = 0);
+// Bounds partials a Morton block folds itself (beyond this, k_bounds_final).
+constexpr uint32_t kFoldPartialsMax = 512;
 // cnt (sort_radix(bits) x sort_tiles(n) words, or null): also write the
 // first sort pass's per-tile digit counts for digits of `bits` bits.
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
