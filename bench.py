@@ -311,6 +311,20 @@ def main():
         src.destroy()
         dst.destroy()
         log(f"copy calibration: 3 x {nbytes} bytes")
+    # The practical ceiling beside the 8 TB/s nameplate: the runtime's
+    # device-to-device copy of 1 GiB (read + write bytes / time).
+    cbytes = 1 << 30
+    csrc, cdst = ctx.alloc(cbytes), ctx.alloc(cbytes)
+    hip.copy_d2d(cdst.ptr, csrc.ptr, cbytes, ctx.stream)
+    ca, cb = hip.event(), hip.event()
+    hip.record(ca, ctx.stream)
+    for _ in range(5):
+        hip.copy_d2d(cdst.ptr, csrc.ptr, cbytes, ctx.stream)
+    hip.record(cb, ctx.stream)
+    copy_gbs = 5 * 2 * cbytes / (hip.elapsed_ms(ca, cb) * 1e-3) / 1e9
+    csrc.destroy()
+    cdst.destroy()
+    log(f"device copy {copy_gbs:.0f} GB/s")
     log(f"warmup x{args.warmup}")
     for _ in range(args.warmup):
         ctx.build(d_tri, n, None, bvh, otri)
@@ -410,7 +424,13 @@ def main():
                      "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "pipeline_roofline": {"algorithmic_bytes": pipe_bytes,
                               "achieved": round(pipe_gbs, 1), "unit": "GB/s",
-                              "frac_of_hbm_peak": round(pipe_gbs / HBM_PEAK_GBS, 4)},
+                              "frac_of_hbm_peak": round(pipe_gbs / HBM_PEAK_GBS, 4),
+                              "frac_of_device_copy": round(pipe_gbs / copy_gbs, 4)},
+        "device_copy": {"GB/s": round(copy_gbs, 1),
+                        "note": "hipMemcpyAsync device-to-device of 1 GiB, "
+                                "read + write bytes per second: the practical "
+                                "streaming ceiling on this GPU",
+                        "dominant_kernel_frac": round(dom_gbs / copy_gbs, 4)},
         "stages": stages,
         "autotune_ms": None if tuning is None else
         {str(k): {f"{t}/{b}": round(v, 4) for (t, b), v in d.items()}
