@@ -538,10 +538,13 @@ __global__ __launch_bounds__(T) void k_leaves(
     const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
     const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
     const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
-    int4 *o = otri_all + 3 * (size_t)(leaf_base + k);
-    o[0] = a;
-    o[1] = b;
-    o[2] = d;
+    // O_TRI goes out through LDS (the node slots are free until the climb):
+    // a lane's own 48-byte row would put 48 bytes between the lanes of each
+    // store, where the staged rows are stored 16 contiguous bytes per lane
+    int4 *stage = &L.node[0][0];
+    stage[3 * tid] = a;
+    stage[3 * tid + 1] = b;
+    stage[3 * tid + 2] = d;
     tri_box(a, b, d, c.mn, c.mx);
     perturb(c.mn, c.mx);
     L.dgap[tid + 1] = ((uint32_t)k + 1 < N)
@@ -551,6 +554,17 @@ __global__ __launch_bounds__(T) void k_leaves(
                           : -1;
   }
   __syncthreads();
+  {
+    const int4 *stage = &L.node[0][0];
+    int4 *o = otri_all + 3 * (size_t)(leaf_base + ts);
+    const int nq = 3 * (te - ts);  // the tile's O_TRI int4s
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = j * T + tid;
+      if (e < nq) o[e] = stage[e];
+    }
+  }
+  __syncthreads();  // the node slots are the climb's from here
   if (N == 1) {
     if (k == 0) {
       write_single_leaf_node(nodes, c.mn, c.mx);
