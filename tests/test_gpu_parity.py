@@ -133,6 +133,27 @@ def test_sort_is_stable(ctx, lb, n, bits):
     dk.destroy(); di.destroy()
 
 
+@pytest.mark.parametrize("offset", [4, 8, 12])
+def test_sort_at_unaligned_addresses(ctx, lb, offset):
+    """Keys and payloads that do not start on a 16-byte boundary take the
+    sort's 4-byte loads; the result is the same stable sort."""
+    n = 70000
+    say(f"sort at +{offset} bytes")
+    rng = np.random.default_rng(offset)
+    keys = sort_keys(rng, n)
+    pad = offset // 4
+    dk = ctx.to_device(np.concatenate([np.zeros(pad, np.uint32), keys]))
+    di = ctx.alloc(4 * (n + pad))
+    for bits in (8, 10):
+        ctx.upload(dk, np.concatenate([np.zeros(pad, np.uint32), keys]))
+        with sort_bits(ctx, lb, bits):
+            ctx.sort(dk.ptr + offset, di.ptr + offset, n)
+        want = np.argsort(keys, kind="stable")
+        assert np.array_equal(di.read(np.int32, n + pad)[pad:], want), bits
+        assert np.array_equal(dk.read(np.uint32, n + pad)[pad:], keys[want]), bits
+    dk.destroy(); di.destroy()
+
+
 @pytest.mark.parametrize("bits", [8, 10])
 def test_sort_heavy_duplicates_and_constant_digits(ctx, lb, bits):
     for n, keys in [(300000, np.zeros(300000, np.uint32)),

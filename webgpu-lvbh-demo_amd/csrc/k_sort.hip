@@ -36,6 +36,7 @@
 namespace lbvh {
 
 constexpr int kWaves = kSortThreads / 64;
+static_assert(kSortItems % 4 == 0, "16-byte tile loads");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 // Single-mesh sorts of at most this many tiles (64K keys) scan in one block.
 constexpr uint32_t kSmallScanTiles = 16;
@@ -75,17 +76,37 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
   __syncthreads();
   const TileRange r = sort_tile_range(tile, n, segs);
   // All of the thread's keys are loaded before any is counted, so a wave
-  // has its whole share of the tile in flight.
+  // has its whole share of the tile in flight; a whole 16-byte-aligned tile
+  // with 16-byte loads (counting does not depend on which thread holds
+  // which key).
   uint32_t k[kSortItems];
+  const bool vec =
+      r.end - r.begin == (uint32_t)kSortTile &&
+      (reinterpret_cast<uintptr_t>(keys + r.begin) & 15u) == 0;
+  if (vec) {
+    const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + r.begin);
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    const uint32_t i = r.begin + threadIdx.x + j * kSortThreads;
-    k[j] = i < r.end ? keys[i] : 0u;
-  }
+    for (int q = 0; q < kSortItems / 4; ++q) {
+      const uint4 v = k4[threadIdx.x + q * kSortThreads];
+      k[4 * q] = v.x;
+      k[4 * q + 1] = v.y;
+      k[4 * q + 2] = v.z;
+      k[4 * q + 3] = v.w;
+    }
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j)
-    if (r.begin + threadIdx.x + j * kSortThreads < r.end)
+    for (int j = 0; j < kSortItems; ++j)
       digit_add(s_c, (k[j] >> shift) & (D::kR - 1));
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      const uint32_t i = r.begin + threadIdx.x + j * kSortThreads;
+      k[j] = i < r.end ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j)
+      if (r.begin + threadIdx.x + j * kSortThreads < r.end)
+        digit_add(s_c, (k[j] >> shift) & (D::kR - 1));
+  }
   __syncthreads();
   for (int d = threadIdx.x; d < D::kR; d += kSortThreads)
     cnt[(size_t)d * tiles + tile] = s_c[d];
@@ -314,19 +335,52 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   const TileRange r = sort_tile_range(tile, n, segs);
   __syncthreads();
 
-  // Warp-striped load: wave w owns keys [wbase, wbase + 64*kSortItems), item
+  // Warp-striped keys: wave w owns keys [wbase, wbase + 64*kSortItems), item
   // j of lane l sits at wbase + 64*j + l, so (j, l) order is input order.
+  // A whole 16-byte-aligned tile is loaded with 16-byte loads into the LDS
+  // reorder buffer (free until the reorder) and read back striped: 4 loads
+  // per thread instead of 16 (keys) + 16 (payloads).
   const uint32_t wbase = r.begin + wave * 64 * kSortItems;
   uint32_t key[kSortItems];
   int32_t val[kSortItems];
+  const bool vec =
+      r.end - r.begin == (uint32_t)kSortTile &&
+      ((reinterpret_cast<uintptr_t>(keys_in + r.begin) |
+        (kFirst ? 0u : reinterpret_cast<uintptr_t>(vals_in + r.begin))) &
+       15u) == 0;
+  if (vec) {
+    const uint4 *k4 = reinterpret_cast<const uint4 *>(keys_in + r.begin);
+    const int4 *v4 = reinterpret_cast<const int4 *>(vals_in + r.begin);
+    uint4 kq[kSortItems / 4];
+    int4 vq[kSortItems / 4];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    const uint32_t pos = wbase + 64 * j + lane;
-    key[j] = 0u;
-    val[j] = 0;
-    if (pos < r.end) {
-      key[j] = keys_in[pos];
-      val[j] = kFirst ? (int32_t)(pos - r.seg_begin) : vals_in[pos];
+    for (int q = 0; q < kSortItems / 4; ++q) {
+      kq[q] = k4[tid + q * kSortThreads];
+      if (!kFirst) vq[q] = v4[tid + q * kSortThreads];
+    }
+#pragma unroll
+    for (int q = 0; q < kSortItems / 4; ++q) {
+      reinterpret_cast<uint4 *>(s_keys)[tid + q * kSortThreads] = kq[q];
+      if (!kFirst)
+        reinterpret_cast<int4 *>(s_vals)[tid + q * kSortThreads] = vq[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      const uint32_t lp = wave * 64 * kSortItems + 64 * j + lane;
+      key[j] = s_keys[lp];
+      val[j] = kFirst ? (int32_t)(r.begin + lp - r.seg_begin) : s_vals[lp];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      const uint32_t pos = wbase + 64 * j + lane;
+      key[j] = 0u;
+      val[j] = 0;
+      if (pos < r.end) {
+        key[j] = keys_in[pos];
+        val[j] = kFirst ? (int32_t)(pos - r.seg_begin) : vals_in[pos];
+      }
     }
   }
 
