@@ -66,18 +66,23 @@ __device__ void block_minmax(float v[6]) {
 // and no fences: a last-block-finishes counter costs one same-address
 // device-scope atomic (with a cache writeback for its release) per block,
 // which on MI355X serialises across the XCDs.
-// cen (3 x n floats, x then y then z, or null): also the centre of each
-// triangle's box, (max + min) * .5f as aabb-z-idx.js:191 computes it, so the
-// Morton pass reads 12 bytes per triangle instead of the 48-byte triangle.
-// A block takes chunks of kMapThreads triangles (TriChunks).
+// cen (x, y, z arrays of cen_stride(n) floats, or null): also the centre of
+// each triangle's box, (max + min) * .5f as aabb-z-idx.js:191 computes it, so
+// the Morton pass reads 12 bytes per triangle instead of the 48-byte
+// triangle.  A block takes chunks of kMapThreads triangles (TriChunks); a
+// chunk's centres go out through LDS, one component per wave, 16 bytes per
+// lane.
 __global__ __launch_bounds__(kMapThreads) void k_bounds(
     const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
     Ctl *ctl, int bump_epoch, float *__restrict__ cen) {
+  static_assert(kMapThreads == 256, "a wave stores one component of a chunk");
   __shared__ float4 s_tri[3 * kMapThreads];
+  __shared__ float s_cen[3][kMapThreads];
   if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   const uint32_t t = threadIdx.x;
   const uint32_t chunks = (n + kMapThreads - 1) / kMapThreads;
+  const size_t cs = cen_stride(n);
   TriChunks tc{tri, 3 * (size_t)n, {}};
   if (blockIdx.x < chunks) tc.load(blockIdx.x * kMapThreads);
   for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
@@ -92,14 +97,22 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
       for (int k = 0; k < 3; ++k) {
         v[k] = fminf(v[k], mn[k]);
         v[3 + k] = fmaxf(v[3 + k], mx[k]);
-      }
-      if (cen) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          cen[(size_t)k * n + i] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+        s_cen[k][t] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
       }
     }
-    __syncthreads();  // s_tri is refilled for the next chunk
+    __syncthreads();  // s_tri is refilled for the next chunk; s_cen is full
+    if (cen && t < 3 * 64) {
+      const uint32_t k = t >> 6, q = (t & 63) * 4;  // component, chunk slot
+      const uint32_t i0 = c * kMapThreads + q;
+      float *dst = cen + k * cs + i0;
+      if (i0 + 3 < n) {
+        *reinterpret_cast<float4 *>(dst) =
+            make_float4(s_cen[k][q], s_cen[k][q + 1], s_cen[k][q + 2],
+                        s_cen[k][q + 3]);
+      } else {
+        for (uint32_t e = 0; e < 4 && i0 + e < n; ++e) dst[e] = s_cen[k][q + e];
+      }
+    }
   }
   block_minmax(v);
   if (threadIdx.x == 0) {
@@ -257,31 +270,64 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t first = tile * (uint32_t)kSortTile;
   const uint32_t end = min(first + (uint32_t)kSortTile, n);
-  // All of the thread's centres are loaded before any is used: one
-  // iteration's loads per wave in flight (768 bytes) leave the kernel
-  // latency-bound far below HBM bandwidth.
-  constexpr int kPer = kSortTile / kMapThreads;
-  float c[3][kPer];
-#pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const uint32_t i = first + threadIdx.x + j * kMapThreads;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) c[k][j] = i < end ? cen[(size_t)k * n + i] : 0.f;
-  }
-#pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const uint32_t i = first + threadIdx.x + j * kMapThreads;
-    if (i >= end) continue;
+  const size_t cs = cen_stride(n);
+  const auto key_of = [&](float x, float y, float z) {
+    const float ce[3] = {x, y, z};
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float d = __fsub_rn(c[k][j], bmin[k]);
+      const float d = __fsub_rn(ce[k], bmin[k]);
       const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
       q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
     }
-    const uint32_t key = split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
-    keys[i] = key;
-    digit_add(s_hist, key & (kR - 1));
+    return split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+  };
+  // All of the thread's centres are loaded before any is used: one
+  // iteration's loads per wave in flight leave the kernel latency-bound far
+  // below HBM bandwidth.  A whole tile goes 16 bytes per lane: thread t takes
+  // triangles first + 1024 g + 4 t .. + 3 of each group g (counting is
+  // indifferent to which thread holds which key).
+  constexpr int kPer = kSortTile / kMapThreads;
+  static_assert(kPer % 4 == 0, "whole float4 groups per thread");
+  if (end - first == (uint32_t)kSortTile) {
+    float4 c[3][kPer / 4];
+#pragma unroll
+    for (int g = 0; g < kPer / 4; ++g) {
+      const size_t i = first + (size_t)g * 4 * kMapThreads + 4 * threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        c[k][g] = *reinterpret_cast<const float4 *>(cen + k * cs + i);
+    }
+#pragma unroll
+    for (int g = 0; g < kPer / 4; ++g) {
+      const size_t i = first + (size_t)g * 4 * kMapThreads + 4 * threadIdx.x;
+      uint4 kq;
+      kq.x = key_of(c[0][g].x, c[1][g].x, c[2][g].x);
+      kq.y = key_of(c[0][g].y, c[1][g].y, c[2][g].y);
+      kq.z = key_of(c[0][g].z, c[1][g].z, c[2][g].z);
+      kq.w = key_of(c[0][g].w, c[1][g].w, c[2][g].w);
+      *reinterpret_cast<uint4 *>(keys + i) = kq;
+      digit_add(s_hist, kq.x & (kR - 1));
+      digit_add(s_hist, kq.y & (kR - 1));
+      digit_add(s_hist, kq.z & (kR - 1));
+      digit_add(s_hist, kq.w & (kR - 1));
+    }
+  } else {
+    float c[3][kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t i = first + threadIdx.x + j * kMapThreads;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[k][j] = i < end ? cen[k * cs + i] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t i = first + threadIdx.x + j * kMapThreads;
+      if (i >= end) continue;
+      const uint32_t key = key_of(c[0][j], c[1][j], c[2][j]);
+      keys[i] = key;
+      digit_add(s_hist, key & (kR - 1));
+    }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kR; d += kMapThreads)

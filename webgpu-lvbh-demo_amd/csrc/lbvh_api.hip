@@ -964,7 +964,8 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   // pass reads those (12 bytes per triangle) instead of the triangles
   const bool use_cen = !bounds_host && sort;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
-  if (use_cen && (st = ensure(ctx, ctx->cen, 12ull * n, false))) return st;
+  if (use_cen && (st = ensure(ctx, ctx->cen, 12ull * cen_stride(n), false)))
+    return st;
   if ((st = begin_op(ctx))) return st;
 
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
@@ -1064,7 +1065,7 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if ((st = ensure(ctx, ctx->b_tree_counts, 4ull * P.tree_tiles, false)))
     return st;
   if (!bounds_host &&
-      (st = ensure(ctx, ctx->b_cen, 12ull * total, false)))
+      (st = ensure(ctx, ctx->b_cen, 12ull * cen_stride(total), false)))
     return st;
   if (bounds_host) {
     if ((st = ensure(ctx, ctx->b_bounds, 24ull * num_meshes, false))) return st;

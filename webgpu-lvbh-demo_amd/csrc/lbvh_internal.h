@@ -73,8 +73,8 @@ struct MortonParams {
 // ---- kernel launchers (each .hip file owns its kernels) --------------------
 
 // k_bounds_morton.hip
-// Scene bounds into ctl->bounds; cen (3 x n floats, or null): also each
-// triangle's box centre, for launch_morton_cen.  When the reduction leaves
+// Scene bounds into ctl->bounds; cen (3 x cen_stride(n) floats, or null):
+// also each triangle's box centre, for launch_morton_cen.  When the reduction leaves
 // at most fold_max partials, the folding launch is skipped and the partial
 // count is returned (launch_morton_cen folds them); otherwise 0.
 uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
@@ -89,6 +89,12 @@ void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        const float4 *partials = nullptr, uint32_t parts = 0);
 // Bounds partials a Morton block folds itself (beyond this, k_bounds_final).
 constexpr uint32_t kFoldPartialsMax = 512;
+// Box centres: x, y and z arrays of cen_stride(n) floats each (a multiple of
+// 64, so every array and every 256-triangle chunk starts 256-byte aligned
+// and is read and written 16 bytes per lane).
+__host__ __device__ constexpr uint32_t cen_stride(uint32_t n) {
+  return (n + 63u) & ~63u;
+}
 // cnt (sort_radix(bits) x sort_tiles(n) words, or null): also write the
 // first sort pass's per-tile digit counts for digits of `bits` bits.
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
