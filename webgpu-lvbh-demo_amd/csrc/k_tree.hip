@@ -293,19 +293,22 @@ __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
 template <int T>
 struct AggLds {
   int dgap[T + 1];     // delta at the gap after item ts-1+k (leaf space)
-  int il[T], ir[T];    // leaf range of each item of the tile
-  float box[2][6][T];  // [side][min.xyz max.xyz][gap slot]
-  int far[2][T];       // far item of the side that arrived
+  // [side][gap slot]: what the child that arrived first at a split leaves
+  // for its sibling, two int4 read back in one round trip:
+  //   .x-.z of [0] box min, .w: far item | (delta at its outer gap + 1) << 16
+  //   .x-.z of [1] box max, .w: its outer leaf
+  int4 rec[2][T][2];
   uint32_t cnt[T];     // arrivals per gap slot
   uint32_t wsum[16];
   uint32_t pre[kFanIn + 1];  // item levels: first item of each source tile
 };
 
-// A climbing subtree: items [ia, ib] covering leaves [l, r], its box, and the
-// two halves of its node while that node's id is still open.  Positions are
+// A climbing subtree: items [ia, ib] covering leaves [l, r], the deltas at
+// the gaps just outside it (dl before ia, dr after ib), its box, and the two
+// halves of its node while that node's id is still open.  Positions are
 // below 2^30, so 32-bit indices suffice.
 struct Agg {
-  int ia, ib, l, r;
+  int ia, ib, l, r, dl, dr;
   float mn[3], mx[3];
   bool have_node;
   int4 h[4];
@@ -313,14 +316,18 @@ struct Agg {
   int pgap;
 };
 
+// The climb over items.  Each step is one LDS atomic and, for the second
+// child to arrive, one read of the sibling's record: everything the merge
+// needs (box, far item, far leaf, the delta beyond it) travels in it, so the
+// serial chain of merges at the top of the tree pays two LDS round trips
+// per level.
 template <int T>
 __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
                                             int ts, int te, int4 *nodes) {
+  static_assert(T <= 0x10000, "far items pack into 16 bits");
   while (true) {
-    const int dl = L.dgap[c.ia - ts];      // gap before the first item
-    const int dr = L.dgap[c.ib - ts + 1];  // gap after the last item
     const bool root = (c.l == 0 && c.r == n - 1);
-    const bool is_left = dr > dl;
+    const bool is_left = c.dr > c.dl;
     if (c.have_node) {
       const int K = root ? 0 : (is_left ? c.r : c.l);
       int4 *node = nodes + 4 * (size_t)K;
@@ -335,12 +342,16 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
     }
     const int side = is_left ? 0 : 1;
     const int slot = gap - ts;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      L.box[side][k][slot] = c.mn[k];
-      L.box[side][3 + k][slot] = c.mx[k];
-    }
-    L.far[side][slot] = is_left ? c.ia : c.ib;
+    // a left child's outer side is its left end, a right child's its right
+    const int far_item = is_left ? c.ia : c.ib;
+    const int far_delta = is_left ? c.dl : c.dr;
+    L.rec[side][slot][0] =
+        make_int4(__float_as_int(c.mn[0]), __float_as_int(c.mn[1]),
+                  __float_as_int(c.mn[2]),
+                  (far_item - ts) | ((far_delta + 1) << 16));
+    L.rec[side][slot][1] =
+        make_int4(__float_as_int(c.mx[0]), __float_as_int(c.mx[1]),
+                  __float_as_int(c.mx[2]), is_left ? c.l : c.r);
     const uint32_t old = __hip_atomic_fetch_add(
         &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0) {
@@ -348,13 +359,14 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
       c.pgap = gap;
       return;
     }
-    float smn[3], smx[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      smn[k] = L.box[1 - side][k][slot];
-      smx[k] = L.box[1 - side][3 + k][slot];
-    }
-    const int sfar = L.far[1 - side][slot];
+    const int4 r0 = L.rec[1 - side][slot][0], r1 = L.rec[1 - side][slot][1];
+    const float smn[3] = {__int_as_float(r0.x), __int_as_float(r0.y),
+                          __int_as_float(r0.z)};
+    const float smx[3] = {__int_as_float(r1.x), __int_as_float(r1.y),
+                          __int_as_float(r1.z)};
+    const int sfar = ts + (r0.w & 0xffff);
+    const int sdelta = (r0.w >> 16) - 1;
+    const int souter = r1.w;
     // Merge into the node split at leaf s: children [Lr, s] and [s+1, Rr].
     const int s = is_left ? c.r : c.l - 1;
     int Lr, Rr;
@@ -368,12 +380,14 @@ __device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
     }
     if (is_left) {
       Lr = c.l;
-      Rr = L.ir[sfar - ts];
+      Rr = souter;
       c.ib = sfar;
+      c.dr = sdelta;
     } else {
-      Lr = L.il[sfar - ts];
+      Lr = souter;
       Rr = c.r;
       c.ia = sfar;
+      c.dl = sdelta;
     }
     const int lid = (Lr == s) ? -(s + 1) : s;
     const int rid = (Rr == s + 1) ? -(s + 2) : s + 1;
@@ -673,19 +687,21 @@ __global__ __launch_bounds__(kItemTile) void k_items(
     c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
     c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
     c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
-    L.il[tid] = c.l;
-    L.ir[tid] = c.r;
     L.dgap[tid + 1] = g.y;        // delta at the gap after the item
     if (tid == 0) L.dgap[0] = g.x;  // and before the first one
   }
   __syncthreads();
-  if (active) agglomerate<T>(c, L, (int)N, ts, te, nodes);
+  if (active) {
+    // the gap before item k is the one after item k - 1
+    c.dl = L.dgap[k];
+    c.dr = L.dgap[k + 1];
+    agglomerate<T>(c, L, (int)N, ts, te, nodes);
+  }
   __syncthreads();
   const bool out = active && (c.stop == 2 ||
                               (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-  emit_frontier(out, c.l, c.r, out ? L.dgap[c.ia - ts] : 0,
-                out ? L.dgap[c.ib - ts + 1] : 0, c.mn, c.mx, L.wsum, gtile,
-                items_out, count_out, ctl, tid, T);
+  emit_frontier(out, c.l, c.r, out ? c.dl : 0, out ? c.dr : 0, c.mn, c.mx,
+                L.wsum, gtile, items_out, count_out, ctl, tid, T);
 }
 
 // ---------------------------------------------------------------------------
