@@ -271,10 +271,13 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // range's two boundary gaps) into the tile's own kFrontierMax slots, with
 // their count.
 //
-// Level L >= 1 runs the same agglomeration over items: one block of
-// kItemTile threads takes the frontiers of kFanIn consecutive tiles of level
-// L-1 (at most kItemTile items), until a single tile holds all of a mesh's
-// remaining items and completes its root.  The level plan depends only on
+// Level L >= 1 builds the tree over items: one block of kItemTile threads
+// takes the frontiers of kFanIn consecutive tiles of level L-1 (at most
+// kItemTile items), until a single tile holds all of a mesh's remaining
+// items and completes its root.  Frontier items are the subtrees hanging off
+// root paths, so climbing over them would be a serial chain of merges as
+// long as those paths; the item levels compute their nodes in parallel
+// instead (k_items).  The level plan depends only on
 // the leaf count, so no block waits for another and no counter is shared.
 // (A dynamic tile ticket -- one same-address device-scope atomic per tile --
 // and a look-back across tiles both serialise on MI355X, where those
@@ -288,122 +291,6 @@ constexpr int kFanIn = kItemTile / kFrontierMax;  // level L-1 tiles per tile
 
 __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
-}
-
-template <int T>
-struct AggLds {
-  int dgap[T + 1];     // delta at the gap after item ts-1+k (leaf space)
-  // [side][gap slot]: what the child that arrived first at a split leaves
-  // for its sibling, two int4 read back in one round trip:
-  //   .x-.z of [0] box min, .w: far item | (delta at its outer gap + 1) << 16
-  //   .x-.z of [1] box max, .w: its outer leaf
-  int4 rec[2][T][2];
-  uint32_t cnt[T];     // arrivals per gap slot
-  uint32_t wsum[16];
-  uint32_t pre[kFanIn + 1];  // item levels: first item of each source tile
-};
-
-// A climbing subtree: items [ia, ib] covering leaves [l, r], the deltas at
-// the gaps just outside it (dl before ia, dr after ib), its box, and the two
-// halves of its node while that node's id is still open.  Positions are
-// below 2^30, so 32-bit indices suffice.
-struct Agg {
-  int ia, ib, l, r, dl, dr;
-  float mn[3], mx[3];
-  bool have_node;
-  int4 h[4];
-  int stop;  // 0 root / inactive, 1 pending local arrival, 2 frontier
-  int pgap;
-};
-
-// The climb over items.  Each step is one LDS atomic and, for the second
-// child to arrive, one read of the sibling's record: everything the merge
-// needs (box, far item, far leaf, the delta beyond it) travels in it, so the
-// serial chain of merges at the top of the tree pays two LDS round trips
-// per level.
-template <int T>
-__device__ __forceinline__ void agglomerate(Agg &c, AggLds<T> &L, int n,
-                                            int ts, int te, int4 *nodes) {
-  static_assert(T <= 0x10000, "far items pack into 16 bits");
-  while (true) {
-    const bool root = (c.l == 0 && c.r == n - 1);
-    const bool is_left = c.dr > c.dl;
-    if (c.have_node) {
-      const int K = root ? 0 : (is_left ? c.r : c.l);
-      int4 *node = nodes + 4 * (size_t)K;
-      node[0] = c.h[0]; node[1] = c.h[1]; node[2] = c.h[2]; node[3] = c.h[3];
-      c.have_node = false;
-    }
-    if (root) { c.stop = 0; return; }
-    const int gap = is_left ? c.ib : c.ia - 1;  // item gap of the split
-    if (gap < ts || gap + 2 > te) {             // other child off-tile
-      c.stop = 2;
-      return;
-    }
-    const int side = is_left ? 0 : 1;
-    const int slot = gap - ts;
-    // a left child's outer side is its left end, a right child's its right
-    const int far_item = is_left ? c.ia : c.ib;
-    const int far_delta = is_left ? c.dl : c.dr;
-    L.rec[side][slot][0] =
-        make_int4(__float_as_int(c.mn[0]), __float_as_int(c.mn[1]),
-                  __float_as_int(c.mn[2]),
-                  (far_item - ts) | ((far_delta + 1) << 16));
-    L.rec[side][slot][1] =
-        make_int4(__float_as_int(c.mx[0]), __float_as_int(c.mx[1]),
-                  __float_as_int(c.mx[2]), is_left ? c.l : c.r);
-    const uint32_t old = __hip_atomic_fetch_add(
-        &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0) {
-      c.stop = 1;
-      c.pgap = gap;
-      return;
-    }
-    const int4 r0 = L.rec[1 - side][slot][0], r1 = L.rec[1 - side][slot][1];
-    const float smn[3] = {__int_as_float(r0.x), __int_as_float(r0.y),
-                          __int_as_float(r0.z)};
-    const float smx[3] = {__int_as_float(r1.x), __int_as_float(r1.y),
-                          __int_as_float(r1.z)};
-    const int sfar = ts + (r0.w & 0xffff);
-    const int sdelta = (r0.w >> 16) - 1;
-    const int souter = r1.w;
-    // Merge into the node split at leaf s: children [Lr, s] and [s+1, Rr].
-    const int s = is_left ? c.r : c.l - 1;
-    int Lr, Rr;
-    float lmn[3], lmx[3], rmn[3], rmx[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lmn[k] = is_left ? c.mn[k] : smn[k];
-      lmx[k] = is_left ? c.mx[k] : smx[k];
-      rmn[k] = is_left ? smn[k] : c.mn[k];
-      rmx[k] = is_left ? smx[k] : c.mx[k];
-    }
-    if (is_left) {
-      Lr = c.l;
-      Rr = souter;
-      c.ib = sfar;
-      c.dr = sdelta;
-    } else {
-      Lr = souter;
-      Rr = c.r;
-      c.ia = sfar;
-      c.dl = sdelta;
-    }
-    const int lid = (Lr == s) ? -(s + 1) : s;
-    const int rid = (Rr == s + 1) ? -(s + 2) : s + 1;
-    c.h[0] = pack3(lmn, lid);
-    c.h[1] = pack3(lmx, 2);
-    c.h[2] = pack3(rmn, rid);
-    c.h[3] = pack3(rmx, 0);
-    c.have_node = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      c.mn[k] = fminf(lmn[k], rmn[k]);
-      c.mx[k] = fmaxf(lmx[k], rmx[k]);
-    }
-    c.l = Lr;
-    c.r = Rr;
-  }
 }
 
 // Frontier output shared by both level kinds: the stopped subtrees that
@@ -623,9 +510,67 @@ __global__ __launch_bounds__(T) void k_leaves(
 
 // One item level: tile = the frontiers of kFanIn consecutive tiles of the
 // level below.  Level L-1's tile g keeps its items in
-// items_in[g * kFrontierMax ...] and their number in count_in[g].  Items
-// cover many leaves, so nodes are written straight from the climb (they are
-// few: at most kFrontierMax per tile of the level below).
+// items_in[g * kFrontierMax ...] and their number in count_in[g].
+//
+// The tree over the block's m items is the min-Cartesian tree of the deltas
+// at the gaps between them: the node split at gap g spans, on each side, up
+// to the nearest gap with a smaller delta (equal deltas never face each
+// other across a larger-delta stretch, so "smaller" is unambiguous); gap -1
+// before the first item and gap m-1 after the last one carry the deltas to
+// the leaves outside the block.  A node is complete, and written here, when
+// both bounds are found; its parent is the bound with the larger delta.
+// So, with one thread per gap and O(log m) LDS steps each instead of a serial
+// climb:
+//   1. load the items; the gap deltas form a sparse table of range minima
+//      (position p = gap + 1) and the item boxes a segment tree;
+//   2. each gap finds its nearest smaller delta on either side by binary
+//      lifting over the table; a complete node takes its children's boxes
+//      from the segment tree and is written at its Karras id;
+//   3. the roots of the resulting forest -- complete nodes and items whose
+//      parent gap is incomplete or outside the block -- are the frontier,
+//      marked at their first item so the emit keeps leaf order.
+constexpr int kItemLog = 10;
+static_assert(kItemTile == 1 << kItemLog, "sparse table levels");
+
+struct ItemLds {
+  signed char dm[kItemLog + 1][kItemTile + 2];  // range minima of gap deltas
+  float smn[3][2 * kItemTile], smx[3][2 * kItemTile];  // box segment tree
+  int il[kItemTile], ir[kItemTile];  // leaf range of each item
+  short qL[kItemTile], qR[kItemTile];  // per gap: nearest smaller positions
+  unsigned char complete[kItemTile];  // per gap: node complete in the block
+  int root_of[kItemTile];  // per first item: frontier root (gap, T + item)
+  uint32_t wsum[16];
+  uint32_t pre[kFanIn + 1];  // first item of each source tile
+};
+
+// Box of items [a, b] from the segment tree (leaves at kItemTile + item).
+__device__ __forceinline__ void item_range_box(const ItemLds &L, int a, int b,
+                                               float mn[3], float mx[3]) {
+  mn[0] = mn[1] = mn[2] = INFINITY;
+  mx[0] = mx[1] = mx[2] = -INFINITY;
+  int lo = a + kItemTile, hi = b + kItemTile + 1;
+  while (lo < hi) {
+    if (lo & 1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        mn[c] = fminf(mn[c], L.smn[c][lo]);
+        mx[c] = fmaxf(mx[c], L.smx[c][lo]);
+      }
+      ++lo;
+    }
+    if (hi & 1) {
+      --hi;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        mn[c] = fminf(mn[c], L.smn[c][hi]);
+        mx[c] = fmaxf(mx[c], L.smx[c][hi]);
+      }
+    }
+    lo >>= 1;
+    hi >>= 1;
+  }
+}
+
 __global__ __launch_bounds__(kItemTile) void k_items(
     uint32_t n_all,
     const int4 *__restrict__ items_in, const uint32_t *__restrict__ count_in,
@@ -633,7 +578,7 @@ __global__ __launch_bounds__(kItemTile) void k_items(
     uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
     TreeSegs segs) {
   constexpr int T = kItemTile;
-  __shared__ AggLds<T> L;
+  __shared__ ItemLds L;
   const int tid = threadIdx.x;
   const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
   uint32_t tile0 = 0, leaf_base = 0, N = n_all, prev0 = 0,
@@ -653,55 +598,142 @@ __global__ __launch_bounds__(kItemTile) void k_items(
     const uint32_t first = tile * kFanIn;
     const uint32_t nsrc =
         prev_tiles > first ? min((uint32_t)kFanIn, prev_tiles - first) : 0u;
-    uint32_t cs[kFanIn];
-#pragma unroll
-    for (int s = 0; s < kFanIn; ++s)
-      cs[s] = (uint32_t)s < nsrc
-                  ? min(count_in[src0 + s], (uint32_t)kFrontierMax) : 0u;
     uint32_t run = 0;
 #pragma unroll
     for (int s = 0; s < kFanIn; ++s) {
       L.pre[s] = run;
-      run += cs[s];
+      run += (uint32_t)s < nsrc
+                 ? min(count_in[src0 + s], (uint32_t)kFrontierMax) : 0u;
     }
     L.pre[kFanIn] = run;
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    L.smn[c][T + tid] = INFINITY;  // leaves past the items stay neutral
+    L.smx[c][T + tid] = -INFINITY;
+  }
+  L.complete[tid] = 0;
+  L.root_of[tid] = -1;
   __syncthreads();
-  const int ts = 0, te = (int)L.pre[kFanIn];
-  const int k = tid;
-  const bool active = k < te;
+  const int m = (int)L.pre[kFanIn];
 
-  Agg c;
-  c.have_node = false;
-  c.stop = 0;
-  c.ia = c.ib = k;
-  L.cnt[tid] = 0u;
-  if (active) {
+  // 1. items: leaf ranges, boxes (segment-tree leaves), gap deltas
+  if (tid < m) {
     int s = 0;
-    while (s + 1 < kFanIn && (int)L.pre[s + 1] <= k) ++s;
+    while (s + 1 < kFanIn && (int)L.pre[s + 1] <= tid) ++s;
     const size_t it =
-        (size_t)(src0 + s) * kFrontierMax + (uint32_t)(k - (int)L.pre[s]);
+        (size_t)(src0 + s) * kFrontierMax + (uint32_t)(tid - (int)L.pre[s]);
     const int4 a = items_in[kItemInt4 * it], b = items_in[kItemInt4 * it + 1],
                g = items_in[kItemInt4 * it + 2];
-    c.l = a.x; c.r = a.y;
-    c.mn[0] = __int_as_float(a.z); c.mn[1] = __int_as_float(a.w);
-    c.mn[2] = __int_as_float(b.x); c.mx[0] = __int_as_float(b.y);
-    c.mx[1] = __int_as_float(b.z); c.mx[2] = __int_as_float(b.w);
-    L.dgap[tid + 1] = g.y;        // delta at the gap after the item
-    if (tid == 0) L.dgap[0] = g.x;  // and before the first one
+    L.il[tid] = a.x;
+    L.ir[tid] = a.y;
+    L.smn[0][T + tid] = __int_as_float(a.z);
+    L.smn[1][T + tid] = __int_as_float(a.w);
+    L.smn[2][T + tid] = __int_as_float(b.x);
+    L.smx[0][T + tid] = __int_as_float(b.y);
+    L.smx[1][T + tid] = __int_as_float(b.z);
+    L.smx[2][T + tid] = __int_as_float(b.w);
+    L.dm[0][tid + 1] = (signed char)g.y;  // delta at the gap after the item
+    if (tid == 0) L.dm[0][0] = (signed char)g.x;  // and before the first one
   }
   __syncthreads();
-  if (active) {
-    // the gap before item k is the one after item k - 1
-    c.dl = L.dgap[k];
-    c.dr = L.dgap[k + 1];
-    agglomerate<T>(c, L, (int)N, ts, te, nodes);
+  // Table and tree levels for m items: lifting over levels 0..L reaches
+  // 2^(L+1) - 1 positions (>= m - 1 needed), and a range query reads tree
+  // nodes no larger than its m leaves.  m <= 1024 keeps L <= kItemLog.
+  const int levels = min(32 - __clz((uint32_t)m), kItemLog);
+  for (int lv = 1; lv <= levels; ++lv) {
+    const int w = 1 << lv;
+    // range minima over positions [p, p + w) for p + w <= m + 1
+    if (tid + w <= m + 1)
+      L.dm[lv][tid] = min(L.dm[lv - 1][tid], L.dm[lv - 1][tid + (w >> 1)]);
+    if (tid < (T >> lv)) {
+      const int i = (T >> lv) + tid;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        L.smn[c][i] = fminf(L.smn[c][2 * i], L.smn[c][2 * i + 1]);
+        L.smx[c][i] = fmaxf(L.smx[c][2 * i], L.smx[c][2 * i + 1]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // 2. gap tid: its node's extent, and the node when complete
+  if (tid < m - 1) {
+    const int p = tid + 1;
+    const int d = L.dm[0][p];
+    // binary lifting on both sides at once (two independent LDS chains):
+    // [cl, p) and (p, cr) hold no delta below d
+    int cl = p, cr = p + 1;
+    for (int lv = levels; lv >= 0; --lv) {
+      const int w = 1 << lv;
+      const bool gl = cl - w >= 0 && L.dm[lv][cl - w] >= d;
+      const bool gr = cr + w - 1 <= m && L.dm[lv][cr] >= d;
+      if (gl) cl -= w;
+      if (gr) cr += w;
+    }
+    const int qL = cl - 1;  // -1: none inside the block
+    const int qR = cr <= m ? cr : -1;
+    L.qL[tid] = (short)qL;
+    L.qR[tid] = (short)qR;
+    if (qL >= 0 && qR >= 0) {
+      L.complete[tid] = 1;
+      const int a = qL, b = qR - 1;  // the node's first and last item
+      float lmn[3], lmx[3], rmn[3], rmx[3];
+      item_range_box(L, a, tid, lmn, lmx);
+      item_range_box(L, tid + 1, b, rmn, rmx);
+      const int Lr = L.il[a], s = L.ir[tid], Rr = L.ir[b];
+      const int dl = L.dm[0][qL], dr = L.dm[0][qR];
+      const bool root = (Lr == 0 && Rr == (int)N - 1);
+      const int K = root ? 0 : (dr > dl ? Rr : Lr);
+      const int lid = (Lr == s) ? -(s + 1) : s;
+      const int rid = (Rr == s + 1) ? -(s + 2) : s + 1;
+      int4 *node = nodes + 4 * (size_t)K;
+      node[0] = pack3(lmn, lid);
+      node[1] = pack3(lmx, 2);
+      node[2] = pack3(rmn, rid);
+      node[3] = pack3(rmx, 0);
+    }
   }
   __syncthreads();
-  const bool out = active && (c.stop == 2 ||
-                              (c.stop == 1 && L.cnt[c.pgap - ts] == 1u));
-  emit_frontier(out, c.l, c.r, out ? c.dl : 0, out ? c.dr : 0, c.mn, c.mx,
-                L.wsum, gtile, items_out, count_out, ctl, tid, T);
+
+  // 3. roots of the forest, marked at their first item
+  if (tid < m - 1 && L.complete[tid]) {
+    const int qL = L.qL[tid], qR = L.qR[tid];
+    const int a = qL, b = qR - 1;
+    if (!(L.il[a] == 0 && L.ir[b] == (int)N - 1)) {  // the mesh root has none
+      const int parent = L.dm[0][qR] > L.dm[0][qL] ? qR - 1 : qL - 1;
+      if (!(parent >= 0 && parent <= m - 2 && L.complete[parent]))
+        L.root_of[a] = tid;
+    }
+  }
+  if (tid < m && !(L.il[tid] == 0 && L.ir[tid] == (int)N - 1)) {
+    const int parent = L.dm[0][tid + 1] > L.dm[0][tid] ? tid : tid - 1;
+    if (!(parent >= 0 && parent <= m - 2 && L.complete[parent]))
+      L.root_of[tid] = T + tid;
+  }
+  __syncthreads();
+
+  // the frontier in leaf order: the root whose first item is tid
+  const int v = tid < m ? L.root_of[tid] : -1;
+  const bool out = v >= 0;
+  int fl = 0, fr = 0, fdl = 0, fdr = 0;
+  float fmn[3] = {0.f, 0.f, 0.f}, fmx[3] = {0.f, 0.f, 0.f};
+  if (out) {
+    int a, b;
+    if (v >= T) {
+      a = b = v - T;
+    } else {
+      a = L.qL[v];
+      b = L.qR[v] - 1;
+    }
+    item_range_box(L, a, b, fmn, fmx);
+    fl = L.il[a];
+    fr = L.ir[b];
+    fdl = L.dm[0][a];
+    fdr = L.dm[0][b + 1];
+  }
+  emit_frontier(out, fl, fr, fdl, fdr, fmn, fmx, L.wsum, gtile, items_out,
+                count_out, ctl, tid, T);
 }
 
 // ---------------------------------------------------------------------------
