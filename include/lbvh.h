@@ -228,7 +228,11 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
  * the budget completes with LBVH_ERR_OUT_OF_BUDGET; a refit of a mesh that
  * is not resident, or with a different triangle count, completes with
  * LBVH_ERR_NOT_FOUND / LBVH_ERR_SIZE_MISMATCH.  Resident results are
- * byte-identical to lbvh_build / lbvh_refit on the same input. */
+ * byte-identical to lbvh_build / lbvh_refit on the same input.
+ * The budget bounds all device memory the server holds: its mesh arena
+ * (device chunks of up to 4 GiB, allocated as the resident set grows and
+ * carved into mesh ranges) plus every slot's scratch; bytes_in_use in the
+ * stats reports that sum, resident_bytes the part holding meshes. */
 typedef struct lbvh_server lbvh_server;
 
 typedef enum {

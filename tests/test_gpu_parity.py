@@ -210,6 +210,35 @@ def test_stage_pipeline_matches_oracle(ctx, lb, ora):
             x.destroy()
 
 
+def test_stage_pipeline_heightfield_16m(ctx, lb, ora):
+    """The per-stage sequence over the 16,777,216-triangle heightfield: the up
+    pass meets most parents on chip and the ones on paths across its 1,024-leaf
+    blocks in HBM; the result is the oracle's."""
+    say("stage pipeline heightfield-16m")
+    tri = lb.workload("heightfield-16m")
+    n = len(tri)
+    b = lb.wl_bounds(tri)
+    d = ctx.to_device(tri)
+    aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx, par = ctx.alloc(4 * n), ctx.alloc(8 * n)
+    bvh, otri = ctx.alloc(64 * n), ctx.alloc(48 * n)
+    try:
+        ctx.aabb_morton(d, n, b, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.rearrange(d, idx, n, otri)
+        say("oracle heightfield-16m with caller bounds")
+        wb, wo, wsk, wpm = ora.build(tri, b, 0)
+        assert np.array_equal(keys.read(np.uint32, n), wsk)
+        assert np.array_equal(idx.read(np.int32, n), wpm)
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb)
+        assert np.array_equal(otri.read(np.uint32, 12 * n), wo.view(np.uint32).ravel())
+    finally:
+        for x in (d, aabb, keys, idx, par, bvh, otri):
+            x.destroy()
+
+
 # ---------------------------------------------------------------- fused build
 
 @pytest.mark.parametrize("div_mode", [0, 1])

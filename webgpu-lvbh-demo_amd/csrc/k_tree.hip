@@ -177,56 +177,125 @@ __device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
 
 // One thread per leaf climbs through the parents array.  Each child writes
 // its half of the parent node (Karras order: a child is the left one when
-// the child numbered one above it has the same parent) and publishes its box
-// through the same epoch-stamped exchange as the fused kernel (arrive_global,
-// keyed by the parent's id); the first arriver stops, the second merges and
-// climbs.  Unlike the reference there is no zero-sentinel spin
-// (bvh-up-pass.js:265-268): the stamps say when the sibling's box is there.
-__global__ __launch_bounds__(kMapThreads) void k_up_pass(
+// the child numbered one above it has the same parent); the first child to
+// arrive stops, the second merges the boxes and climbs on.  Unlike the
+// reference there is no zero-sentinel spin (bvh-up-pass.js:265-268).
+//
+// A block takes kUpTile consecutive leaves.  Parents whose id lies in the
+// block's range meet in LDS (per-side box slots and an arrival count); a
+// subtree that lies inside the range has all its node ids there, so it
+// completes on chip.  After one barrier, the climbs that stopped at a node
+// whose other child never came (it reaches outside the range), or at a
+// parent whose id is outside the range, continue through the epoch-stamped
+// exchange in HBM (arrive_global, keyed by the parent's id).  Only the nodes
+// on the paths across block boundaries go through HBM: every node did, at
+// a few device-scope atomics each, before.
+constexpr int kUpTile = 1024;
+
+struct UpLds {
+  float box[2][6][kUpTile];  // [side][min.xyz max.xyz][parent id - block start]
+  uint32_t cnt[kUpTile];     // arrivals per parent
+};
+
+__device__ __forceinline__ bool up_is_left(const int32_t *__restrict__ par,
+                                           uint32_t n, int64_t num,
+                                           int32_t p) {
+  const int64_t sib = num + 1;
+  return (sib <= (int64_t)n - 1 && par[n + sib] == p) ||
+         (sib <= (int64_t)n - 2 && par[sib] == p);
+}
+
+__global__ __launch_bounds__(kUpTile) void k_up_pass(
     const int32_t *__restrict__ idx, const int4 *__restrict__ aabb,
     const int32_t *__restrict__ par, uint32_t n, int4 *nodes, GSide *gside,
     uint32_t *tags, Ctl *ctl) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+  __shared__ UpLds L;
+  const int tid = threadIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t ts = (int64_t)tile * kUpTile;
+  const int64_t te = min(ts + kUpTile, (int64_t)n);
+  const int64_t g = ts + tid;
+  L.cnt[tid] = 0u;
+  __syncthreads();
   const uint32_t epoch = ctl->epoch;
-  const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
-  const int4 lo = aabb[2 * (size_t)src], hi = aabb[2 * (size_t)src + 1];
-  float mn[3] = {__int_as_float(lo.x), __int_as_float(lo.y), __int_as_float(lo.z)};
-  float mx[3] = {__int_as_float(hi.x), __int_as_float(hi.y), __int_as_float(hi.z)};
-  perturb(mn, mx);
-  if (n == 1) {
-    write_single_leaf_node(nodes, mn, mx);
-    return;
-  }
-  if (g == n - 1) {
-    const int4 z = make_int4(0, 0, 0, 0);
-    int4 *last = nodes + 4 * (size_t)(n - 1);
-    last[0] = z; last[1] = z; last[2] = z; last[3] = z;
-  }
+  // 0: done, 1: first to arrive at a parent in LDS, 2: continue in HBM
+  int mode = 0;
   bool leaf = true;
   int64_t num = g;  // leaf position or internal node id
-  int32_t p = par[n + g];
-  // A valid tree is at most 64 levels deep (each level consumes a bit of
-  // the 62-bit position-augmented key); the step cap and the range check
-  // keep a malformed parents array from running away.
-  for (int step = 0; step < 128; ++step) {
-    if (p < 0 || p >= (int32_t)n - 1) return;
-    const int64_t sib = num + 1;
-    const bool left = (sib <= (int64_t)n - 1 && par[n + sib] == p) ||
-                      (sib <= (int64_t)n - 2 && par[sib] == p);
-    const int id = leaf ? -(int)(num + 1) : (int)num;
-    int4 *node = nodes + 4 * (size_t)p;
-    if (left) {
-      node[0] = pack3(mn, id);
-      node[1] = pack3(mx, 2);
+  int32_t p = -1;
+  int side = 0;
+  float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+  if (g < (int64_t)n) {
+    const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
+    const int4 lo = aabb[2 * (size_t)src], hi = aabb[2 * (size_t)src + 1];
+    mn[0] = __int_as_float(lo.x); mn[1] = __int_as_float(lo.y);
+    mn[2] = __int_as_float(lo.z);
+    mx[0] = __int_as_float(hi.x); mx[1] = __int_as_float(hi.y);
+    mx[2] = __int_as_float(hi.z);
+    perturb(mn, mx);
+    if (n == 1) {
+      write_single_leaf_node(nodes, mn, mx);
     } else {
-      node[2] = pack3(mn, id);
-      node[3] = pack3(mx, 0);
+      if (g == (int64_t)n - 1) {
+        const int4 z = make_int4(0, 0, 0, 0);
+        int4 *last = nodes + 4 * (size_t)(n - 1);
+        last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+      }
+      p = par[n + g];
+      // A valid tree is at most 64 levels deep (each level consumes a bit of
+      // the 62-bit position-augmented key); the step cap and the range
+      // check keep a malformed parents array from running away.
+      for (int step = 0; step < 128; ++step) {
+        if (p < 0 || p >= (int32_t)n - 1) break;
+        const bool left = up_is_left(par, n, num, p);
+        side = left ? 0 : 1;
+        const int id = leaf ? -(int)(num + 1) : (int)num;
+        int4 *node = nodes + 4 * (size_t)p;
+        if (left) {
+          node[0] = pack3(mn, id);
+          node[1] = pack3(mx, 2);
+        } else {
+          node[2] = pack3(mn, id);
+          node[3] = pack3(mx, 0);
+        }
+        if (p < ts || p >= te) {  // the parent meets in HBM
+          mode = 2;
+          break;
+        }
+        const int slot = (int)(p - ts);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          L.box[side][k][slot] = mn[k];
+          L.box[side][3 + k][slot] = mx[k];
+        }
+        const uint32_t old = __hip_atomic_fetch_add(
+            &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0) {
+          mode = 1;
+          break;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          mn[k] = fminf(mn[k], L.box[1 - side][k][slot]);
+          mx[k] = fmaxf(mx[k], L.box[1 - side][3 + k][slot]);
+        }
+        if (p == 0) break;  // root merged
+        leaf = false;
+        num = p;
+        p = par[p];
+      }
     }
+  }
+  __syncthreads();
+  // the first arriver whose sibling did not come on chip goes to HBM too
+  if (mode == 1 && L.cnt[p - ts] == 1u) mode = 2;
+  if (mode != 2) return;
+  // Through HBM from parent p, whose half this child has already written.
+  for (int step = 0; step < 128; ++step) {
     float smn[3], smx[3];
     int unused;
-    if (!arrive_global(gside, tags, ctl, epoch, p, left ? 0 : 1, 0, mn, mx,
-                       smn, smx, unused))
+    if (!arrive_global(gside, tags, ctl, epoch, p, side, 0, mn, mx, smn, smx,
+                       unused))
       return;  // first arriver
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -234,24 +303,46 @@ __global__ __launch_bounds__(kMapThreads) void k_up_pass(
       mx[k] = fmaxf(mx[k], smx[k]);
     }
     if (p == 0) return;  // root merged
-    leaf = false;
     num = p;
     p = par[p];
+    if (p < 0 || p >= (int32_t)n - 1) return;
+    const bool left = up_is_left(par, n, num, p);
+    side = left ? 0 : 1;
+    int4 *node = nodes + 4 * (size_t)p;
+    if (left) {
+      node[0] = pack3(mn, (int)num);
+      node[1] = pack3(mx, 2);
+    } else {
+      node[2] = pack3(mn, (int)num);
+      node[3] = pack3(mx, 0);
+    }
   }
 }
 
-// rearrange_triangles (rearrange.js:162-169): whole 48-byte records.
+// rearrange_triangles (rearrange.js:162-169): whole 48-byte records, stored
+// through LDS 16 contiguous bytes per lane (a lane's own record would put 48
+// bytes between the lanes of each store).
 __global__ __launch_bounds__(kMapThreads) void k_rearrange(
     const int4 *__restrict__ tri, const int32_t *__restrict__ idx, uint32_t n,
     int4 *__restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t j = (uint32_t)idx[i];
-  const size_t s = 3 * (size_t)(j < n ? j : 0u);
-  const int4 a = tri[s], b = tri[s + 1], c = tri[s + 2];
-  out[3 * (size_t)i] = a;
-  out[3 * (size_t)i + 1] = b;
-  out[3 * (size_t)i + 2] = c;
+  __shared__ int4 s_rec[3 * kMapThreads];
+  const uint32_t i0 = blockIdx.x * kMapThreads;
+  const uint32_t i = i0 + threadIdx.x;
+  if (i < n) {
+    const uint32_t j = (uint32_t)idx[i];
+    const size_t s = 3 * (size_t)(j < n ? j : 0u);
+    s_rec[3 * threadIdx.x] = tri[s];
+    s_rec[3 * threadIdx.x + 1] = tri[s + 1];
+    s_rec[3 * threadIdx.x + 2] = tri[s + 2];
+  }
+  __syncthreads();
+  const uint32_t nq = 3 * (min(i0 + kMapThreads, n) - i0);
+  int4 *o = out + 3 * (size_t)i0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t e = k * kMapThreads + threadIdx.x;
+    if (e < nq) o[e] = s_rec[e];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -755,8 +846,8 @@ void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     const int32_t *parents, uint32_t n, float4 *nodes,
                     GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_up_pass, dim3(blocks_for(n, kMapThreads)),
-                     dim3(kMapThreads), 0, s, idx,
+  hipLaunchKernelGGL(k_up_pass, dim3(blocks_for(n, kUpTile)), dim3(kUpTile),
+                     0, s, idx,
                      reinterpret_cast<const int4 *>(aabb), parents, n,
                      reinterpret_cast<int4 *>(nodes), gside, tags, ctl);
 }
