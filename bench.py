@@ -373,6 +373,35 @@ def main():
         ctx.build(d_tri, n, hb, bvh, otri)
     hb_ms, _ = timed_builds(lb, ctx, hip, d_tri, n, hb, bvh, otri, args.steps)
     hb_ms /= args.steps
+    # The reference's five execute() shims called one after another
+    # (build-bvh.js:24-56) on the same mesh, each timed on the stream.
+    log("per-stage entry points")
+    aabb_b, keys_b = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx_b, par_b = ctx.alloc(4 * n), ctx.alloc(8 * n)
+    shim = {}
+
+    def time_stage(name, fn, reps=3):
+        fn()
+        ctx.synchronize()
+        ea, eb = hip.event(), hip.event()
+        hip.record(ea, ctx.stream)
+        for _ in range(reps):
+            fn()
+        hip.record(eb, ctx.stream)
+        shim[name] = round(hip.elapsed_ms(ea, eb) / reps, 4)
+
+    time_stage("aabb_morton", lambda: ctx.aabb_morton(d_tri, n, hb, aabb_b, keys_b))
+    # the sort is destructive: each timed call re-sorts freshly computed keys,
+    # and the Morton time is taken out again
+    time_stage("aabb_morton_sort", lambda: (ctx.aabb_morton(d_tri, n, hb, aabb_b, keys_b),
+                                            ctx.sort(keys_b, idx_b, n)))
+    shim["sort"] = round(shim.pop("aabb_morton_sort") - shim["aabb_morton"], 4)
+    time_stage("radix_tree", lambda: ctx.radix_tree(keys_b, n, par_b))
+    time_stage("up_pass", lambda: ctx.up_pass(idx_b, aabb_b, par_b, n, bvh))
+    time_stage("rearrange", lambda: ctx.rearrange(d_tri, idx_b, n, otri))
+    shim["total"] = round(sum(shim.values()), 4)
+    for x in (aabb_b, keys_b, idx_b, par_b):
+        x.destroy()
     log("host-inclusive builds")
     t_host = []
     for _ in range(3):
@@ -453,6 +482,7 @@ def main():
                           "note": "the same build given the exact scene bounds "
                                   "(as main.js passes MODEL_BOUNDS): no bounds "
                                   "reduction, Morton from the triangles"},
+        "per_stage_ms": shim,
         "host_inclusive": {"ms": round(host_ms, 3),
                            "Mtri_s": round(n / (host_ms * 1e-3) / 1e6, 1),
                            "note": "upload of the triangles from pageable host "
