@@ -84,9 +84,10 @@ int lbvh_get_div_mode(const lbvh_ctx *ctx);
  * are validated; results do not depend on them. */
 typedef enum {
   LBVH_TUNE_TREE_TILE = 0, /* leaves per block of the fused up pass: 256|512|1024 */
-  LBVH_TUNE_SORT_BITS = 1  /* radix-sort digit width: 8 (4 passes over the
-                              30-bit keys) or 10 (3 passes); the reference uses
-                              2-bit digits, radix-sort.js:486 */
+  LBVH_TUNE_SORT_BITS = 1  /* digit width of the builds' sort: 8 (4 passes
+                              over the 30-bit Morton keys) or 10 (3 passes);
+                              the reference uses 2-bit digits,
+                              radix-sort.js:486 */
 } lbvh_tunable;
 /* set_tuning fixes a value for every size (value 0: no fixed value, builds
  * use the autotuned table, else the size-based default: tree tile 1024, and
@@ -149,9 +150,12 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
                      uint32_t *keys_out);
 
 /* Replaces initRadixSortKernel(device).execute(keyBuf, N)
- * (radix-sort.js:161-500): stable LSD sort of u32 keys.  Destructive like
- * the reference: the sorted keys replace `keys`; idx_out[j] is the input
- * position of the key now at j. */
+ * (radix-sort.js:161-500): stable LSD sort of u32 keys over all 32 bits, as
+ * the reference's 16 passes of 2 bits sort them: 4 passes of 8 bits, or of
+ * 10 bits (10, 10, 10, 2) when LBVH_TUNE_SORT_BITS is fixed at 10 (the
+ * autotuned table is for the builds' 30-bit keys).  Destructive like the
+ * sorted keys replace `keys`; idx_out[j] is the input position of the key
+ * now at j. */
 int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n);
 
 /* Replaces initRadixTreeKernel(device).execute(sortedKeyBuf, N)

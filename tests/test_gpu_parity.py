@@ -115,12 +115,15 @@ class sort_bits:
         self.ctx.set_tuning(self.lb.TUNE_SORT_BITS, 0)
 
 
+@pytest.mark.parametrize("key_bits", [30, 32])
 @pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65536, 1000003, 4194304])
-def test_sort_is_stable(ctx, lb, n, bits):
-    say(f"sort n={n} bits={bits}")
-    rng = np.random.default_rng(n)
-    keys = sort_keys(rng, n)
+def test_sort_is_stable(ctx, lb, n, bits, key_bits):
+    """Morton-range keys and full u32 keys: the stand-alone sort orders all
+    32 bits, as the reference's 16 passes of 2 bits do."""
+    say(f"sort n={n} bits={bits} key_bits={key_bits}")
+    rng = np.random.default_rng(n + key_bits)
+    keys = sort_keys(rng, n, bits=key_bits)
     dk = ctx.to_device(keys)
     di = ctx.alloc(4 * n)
     with sort_bits(ctx, lb, bits):
@@ -160,7 +163,11 @@ def test_sort_heavy_duplicates_and_constant_digits(ctx, lb, bits):
                     (300000, np.full(300000, 0x3FFFFFFF, np.uint32)),
                     (123457, (np.arange(123457, dtype=np.uint32) % 3) << 24),
                     (123457, (np.arange(123457, dtype=np.uint32) % 5) << 20),
-                    (200000, np.arange(200000, dtype=np.uint32)[::-1].copy())]:
+                    (200000, np.arange(200000, dtype=np.uint32)[::-1].copy()),
+                    (300000, np.full(300000, 0xFFFFFFFF, np.uint32)),
+                    (123457, (np.arange(123457, dtype=np.uint32) % 4) << 30),
+                    (123457, ((np.arange(123457, dtype=np.uint32) % 7) << 29)
+                     | (np.arange(123457, dtype=np.uint32)[::-1] & 0xFF))]:
         say(f"sort special n={n} bits={bits}")
         dk = ctx.to_device(keys)
         di = ctx.alloc(4 * n)

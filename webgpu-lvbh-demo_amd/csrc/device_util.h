@@ -48,17 +48,25 @@ __device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
 
 // Lanes of the wave that are `valid` and hold the calling lane's digit d
 // (kBits bits), found with one ballot per bit.  Every lane of the wave must
-// call it.
+// call it.  Per bit, each half of the peer mask keeps the lanes whose vote
+// equals the lane's own bit: peers & ~(vote ^ s), s = the bit spread over
+// the word, one v_bitop3 per half (truth table 0x84 = S1 & ~(S0 ^ S2),
+// symmetric in S0 and S2).  The loop has no exit before kBits: a uniform
+// early exit (to skip digit bits that are zero in every key) measured
+// slower than the ballots it saves, as it keeps the items' matches from
+// interleaving.
 template <int kBits>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
-  uint64_t peers = __ballot(valid);
+  const uint64_t v0 = __ballot(valid);
+  uint32_t lo = (uint32_t)v0, hi = (uint32_t)(v0 >> 32);
 #pragma unroll
   for (int b = 0; b < kBits; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t vote = __ballot(bit);
-    peers &= bit ? vote : ~vote;
+    const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+    const uint64_t vote = __ballot(s != 0u);
+    lo = __builtin_amdgcn_bitop3_b32((uint32_t)vote, lo, s, 0x84);
+    hi = __builtin_amdgcn_bitop3_b32((uint32_t)(vote >> 32), hi, s, 0x84);
   }
-  return peers;
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // Triangles in chunks of kMapThreads: a chunk's 3 * kMapThreads float4 are

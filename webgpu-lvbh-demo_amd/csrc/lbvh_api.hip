@@ -303,10 +303,10 @@ struct SortPlan {
   int32_t *vdst[4];
   int passes;
 };
-SortPlan sort_plan(int bits, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-                   uint32_t *x, int32_t *vtmp, int32_t *vout) {
+SortPlan sort_plan(int bits, int key_bits, const uint32_t *in, uint32_t *tmp,
+                   uint32_t *out, uint32_t *x, int32_t *vtmp, int32_t *vout) {
   SortPlan P{};
-  P.passes = sort_passes(bits);
+  P.passes = sort_passes(bits, key_bits);
   uint32_t *chain[5];
   int32_t *vchain[5];
   if (P.passes % 2 == 0) {
@@ -349,14 +349,14 @@ int ensure_partials(lbvh_ctx *ctx) {
 // stand-alone sort is in place like the reference's; x is used only then,
 // with an odd pass count).  counted0: the Morton kernel already wrote pass
 // 0's per-tile counts into the digit buffer; the other passes count their
-// own.
+// own.  key_bits: significant bits of the keys (30 for Morton keys).
 void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
               uint32_t *x, int32_t *vtmp, int32_t *vout, uint32_t n,
-              bool counted0, int bits) {
+              bool counted0, int bits, int key_bits = kKeyBits) {
   const uint32_t tiles = sort_tiles(n);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
   uint32_t *off = cnt + (size_t)sort_radix(bits) * tiles;
-  const SortPlan P = sort_plan(bits, in, tmp, out, x, vtmp, vout);
+  const SortPlan P = sort_plan(bits, key_bits, in, tmp, out, x, vtmp, vout);
   for (int p = 0; p < P.passes; ++p) {
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
     launch_sort_pass(bits, p, P.src[p], P.vsrc[p], P.dst[p], P.vdst[p], n,
@@ -511,7 +511,8 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
                       ctx->stream);
   uint32_t *cnt = static_cast<uint32_t *>(ctx->b_digits.ptr);
   uint32_t *off = cnt + (size_t)sort_radix(P.bits) * P.sort_tiles;
-  const SortPlan SP = sort_plan(P.bits, keys, tmp, out, nullptr, vtmp, vout);
+  const SortPlan SP =
+      sort_plan(P.bits, kKeyBits, keys, tmp, out, nullptr, vtmp, vout);
   for (int p = 0; p < SP.passes; ++p)
     launch_sort_pass(P.bits, p, SP.src[p], SP.vsrc[p], SP.dst[p], SP.vdst[p],
                      P.total, P.sort_tiles, cnt, off, p == 0, ctx->ctl,
@@ -891,12 +892,15 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
     HIP_TRY(ctx, hipMemsetAsync(idx_out, 0, 4, ctx->stream));
     return LBVH_OK;
   }
-  const int bits = bits_for(ctx, n);
+  // Any u32 key, as the reference's 16 passes of 2 bits sort: all 32 bits.
+  // 10-bit digits take 4 passes here too, so they run only when fixed.
+  const int bits = ctx->sort_bits_fixed ? ctx->sort_bits : kRadixBits;
   if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   if ((st = begin_op(ctx))) return st;
   run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
            static_cast<uint32_t *>(ctx->keys_a.ptr),
-           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false, bits);
+           static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false, bits,
+           kFullKeyBits);
   return check_launch(ctx, "lbvh_sort");
 }
 

@@ -8,20 +8,28 @@ namespace lbvh {
 
 // ---- sizes and launch shapes --------------------------------------------
 
-// Sort digits: 8 bits (4 passes over the 30-bit keys: bits 0-7, 8-15,
-// 16-23, 24-29) or, as a tunable, 10 bits (3 passes: 0-9, 10-19, 20-29).
+// Sort digits: 8 bits (4 passes over the 30-bit Morton keys: bits 0-7,
+// 8-15, 16-23, 24-29) or, as a tunable, 10 bits (3 passes: 0-9, 10-19,
+// 20-29).  The stand-alone sort takes any u32 key: 4 passes of 8 bits, or of
+// 10 bits (the last one 2 bits wide) when that width is fixed.
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;  // 256 digit values per pass
 constexpr int kWideBits = 10;
 constexpr int kWideRadix = 1 << kWideBits;
-constexpr int kKeyBits = 30;
-constexpr int kHistWords = 3 * kWideRadix;  // >= passes x digits of either width
-__host__ __device__ constexpr int sort_passes(int bits) {
-  return (kKeyBits + bits - 1) / bits;
+constexpr int kKeyBits = 30;      // Morton keys
+constexpr int kFullKeyBits = 32;  // keys of the stand-alone sort
+constexpr int kHistWords = 4 * kWideRadix;  // >= passes x digits of any sort
+__host__ __device__ constexpr int sort_passes(int bits,
+                                             int key_bits = kKeyBits) {
+  return (key_bits + bits - 1) / bits;
 }
 __host__ __device__ constexpr int sort_radix(int bits) { return 1 << bits; }
 static_assert(sort_passes(kRadixBits) * kRadix <= kHistWords, "hist size");
 static_assert(sort_passes(kWideBits) * kWideRadix <= kHistWords, "hist size");
+static_assert(sort_passes(kRadixBits, kFullKeyBits) * kRadix <= kHistWords,
+              "hist size");
+static_assert(sort_passes(kWideBits, kFullKeyBits) * kWideRadix <= kHistWords,
+              "hist size");
 constexpr int kSortThreads = 256;        // 4 wave64 per tile
 constexpr int kSortItems = 16;           // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
