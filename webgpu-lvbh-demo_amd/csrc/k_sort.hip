@@ -13,8 +13,10 @@
 //   count    per-tile digit counts (pass 0's are written by the Morton
 //            kernel as it produces the keys), and their column sums;
 //   scan     one block per digit scans that digit's column of tile counts
-//            and adds the digit's base -> the first output position of every
-//            (digit, tile) run;
+//            -> each (digit, tile) run's position among the digit's keys,
+//            and the column total (the global histogram);
+//            the scatter adds the digit's base (an exclusive scan of the
+//            totals, which every tile computes while its keys load);
 //   scatter  each tile ranks its keys with wave64 ballot matching into
 //            per-wave LDS histograms (rank order = input order: stable),
 //            reorders them in LDS and writes each digit run contiguously.
@@ -25,7 +27,8 @@
 //
 // A pass's global digit histogram (the digit bases of a single mesh) is the
 // column sums of its per-tile counts, and a batched mesh's bases come from
-// its own tiles' counts: no device-scope atomics anywhere.  Counts and
+// its own tiles' counts (k_seg_bases, folded into the offsets by the column
+// scan): no device-scope atomics anywhere.  Counts and
 // offsets are digit-major: entry (d, t) at d * tiles + t, so the scan reads
 // and writes columns contiguously.  Tile ids are XCD-aware
 // (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
@@ -49,19 +52,6 @@ struct Digits {
   static constexpr int kDpt = kR / kSortThreads;
   static_assert(kDpt >= 1 && kR % kSortThreads == 0, "whole digits per thread");
 };
-
-// The global histogram of one pass's digit: column sums of its per-tile
-// counts (block d sums column d).  Plain stores into the pass's histogram:
-// no device-scope atomics, and each sort rewrites it.
-__global__ __launch_bounds__(kSortThreads) void k_digit_totals(
-    const uint32_t *__restrict__ cnt, uint32_t tiles, uint32_t *hist) {
-  __shared__ uint32_t s_wsum[16];
-  const uint32_t *col = cnt + (size_t)blockIdx.x * tiles;
-  uint32_t sum = 0;
-  for (uint32_t t = threadIdx.x; t < tiles; t += kSortThreads) sum += col[t];
-  const uint32_t ex = block_exclusive_scan(sum, s_wsum);
-  if (threadIdx.x == kSortThreads - 1) hist[blockIdx.x] = ex + sum;
-}
 
 // Per-tile digit counts of one pass -> cnt[d * tiles + tile].
 template <int kBits>
@@ -132,8 +122,9 @@ __device__ __forceinline__ void digit_prefix(const uint32_t (&h)[Digits<kBits>::
 
 // Column scan: block d turns column d of the tile counts into output
 // positions.  P(t) = sum of cnt[d][t'] over t' < t.
-//   one mesh:  off[d][t] = base(d) + P(t), base(d) = keys with a smaller
-//              digit (exclusive scan of the global histogram);
+//   one mesh:  off[d][t] = P(t), and hist[d] = the column total; the
+//              scatter adds base(d) = keys with a smaller digit (exclusive
+//              scan of hist over the digits), which needs every column;
 //   segments:  off[d][t] = base(s, d) + P(t) - P(t0(s)) for the segment s of
 //              tile t with first tile t0(s); seg_base[s][d] holds base(s, d)
 //              (the segment's first key + its keys with a smaller digit) and
@@ -146,7 +137,7 @@ __device__ __forceinline__ void digit_prefix(const uint32_t (&h)[Digits<kBits>::
 template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
-    uint32_t tiles, const uint32_t *__restrict__ hist, SortSegs segs,
+    uint32_t tiles, uint32_t *__restrict__ hist, SortSegs segs,
     uint32_t *seg_base) {
   using D = Digits<kBits>;
   constexpr int kChunk = kSortThreads * kScanPer;
@@ -155,19 +146,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
   __shared__ uint32_t s_carry;
   const uint32_t d = blockIdx.x;
   const bool segmented = segs.tile_seg != nullptr;
-  if (!segmented) {
-    uint32_t h[D::kDpt], below[D::kDpt];
-#pragma unroll
-    for (int i = 0; i < D::kDpt; ++i) h[i] = hist[threadIdx.x * D::kDpt + i];
-    digit_prefix<kBits>(h, below, s_wsum);
-#pragma unroll
-    for (int i = 0; i < D::kDpt; ++i)
-      if (threadIdx.x * D::kDpt + i == d) s_carry = below[i];
-  } else if (threadIdx.x == 0) {
-    s_carry = 0u;
-  }
-  __syncthreads();
-  uint32_t carry = s_carry;
+  uint32_t carry = 0;
   const uint32_t *col = cnt + (size_t)d * tiles;
   uint32_t *out = off + (size_t)d * tiles;
   const auto pad = [](uint32_t e) { return e + e / kScanPer; };
@@ -224,6 +203,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     }
     __syncthreads();  // s_col is refilled by the next chunk
   }
+  if (!segmented && threadIdx.x == 0) hist[d] = carry;
 }
 
 // Digit totals and column scan of a single mesh in one block, for sorts of
@@ -306,43 +286,91 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_bases(
         segs.seg_off[sg] + below[i];
 }
 
+// digit_prefix of two counts at once, packed (a << 32) | b: the b sums
+// (digit totals, at most 2^32 - 1 keys in all) never carry into a.
+template <int kBits>
+__device__ __forceinline__ void digit_prefix2(
+    const uint64_t (&h)[Digits<kBits>::kDpt],
+    uint64_t (&out)[Digits<kBits>::kDpt], uint64_t *s_wsum) {
+  using D = Digits<kBits>;
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) sum += h[i];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_wsum[wave] = x;
+  __syncthreads();
+  uint64_t run = x - sum;
+  for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < D::kDpt; ++i) {
+    out[i] = run;
+    run += h[i];
+  }
+}
+
 // One digit pass.  kFirst: the payload is the position in the segment (the
 // reference's init_idx, radix-sort.js:527-534, folded into the first pass).
+// hist (or null when off holds whole output positions): the pass's digit
+// totals; off then holds positions among the digit's keys and each tile
+// adds the digit bases, an exclusive scan of hist, computed in the same
+// block scan as the tile's own run starts.
 template <int kBits, bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
-    SortSegs segs) {
+    const uint32_t *__restrict__ hist, SortSegs segs) {
   using D = Digits<kBits>;
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   __shared__ uint32_t s_whist[kWaves][D::kR];
   __shared__ uint32_t s_spare[kWaves][64];  // targets of the no-op adds
   __shared__ uint32_t s_start[D::kR];  // tile-local start of each digit run
-  __shared__ uint32_t s_gbase[D::kR];  // output position minus local start
+  // the digit totals (hist), then output position minus local start
+  __shared__ uint32_t s_gbase[D::kR];
   __shared__ uint32_t s_wsum[16];
+  __shared__ uint64_t s_wsum2[16];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
   for (int k = tid; k < kWaves * D::kR; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
+  // The digit totals go straight from global memory into s_gbase (no
+  // registers held meanwhile): wave w's q-th load fills 64 consecutive
+  // words at (q * kWaves + w) * 64.
+  if (hist) {
+#pragma unroll
+    for (int q = 0; q < D::kDpt; ++q) {
+      const uint32_t e = (q * kWaves + wave) * 64;
+      __builtin_amdgcn_global_load_lds(hist + e + lane, s_gbase + e, 4, 0, 0);
+    }
+  }
   uint32_t goff[D::kDpt];
 #pragma unroll
   for (int i = 0; i < D::kDpt; ++i)
     goff[i] = off[(size_t)(tid * D::kDpt + i) * tiles + tile];
   const TileRange r = sort_tile_range(tile, n, segs);
-  __syncthreads();
+  // No barrier here: one waits for every outstanding global load, and the
+  // tile's own loads below should be in flight with the ones above.  The
+  // zeroed counters and s_gbase are first used after the next barrier.
 
   // Warp-striped keys: wave w owns keys [wbase, wbase + 64*kSortItems), item
   // j of lane l sits at wbase + 64*j + l, so (j, l) order is input order.
   // A whole 16-byte-aligned tile is loaded with 16-byte loads into the LDS
-  // reorder buffer (free until the reorder) and read back striped: 4 loads
-  // per thread instead of 16 (keys) + 16 (payloads).
+  // reorder buffer (free until the reorder) and its keys read back striped:
+  // 4 loads per thread instead of 16 (keys) + 16 (payloads).  Payloads stay
+  // in s_vals (at the striped position) until the ranking is done, so they
+  // hold no registers while it runs.
   const uint32_t wbase = r.begin + wave * 64 * kSortItems;
   uint32_t key[kSortItems];
-  int32_t val[kSortItems];
   const bool vec =
       r.end - r.begin == (uint32_t)kSortTile &&
       ((reinterpret_cast<uintptr_t>(keys_in + r.begin) |
@@ -366,22 +394,20 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
-      const uint32_t lp = wave * 64 * kSortItems + 64 * j + lane;
-      key[j] = s_keys[lp];
-      val[j] = kFirst ? (int32_t)(r.begin + lp - r.seg_begin) : s_vals[lp];
-    }
+    for (int j = 0; j < kSortItems; ++j)
+      key[j] = s_keys[wave * 64 * kSortItems + 64 * j + lane];
   } else {
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
       const uint32_t pos = wbase + 64 * j + lane;
       key[j] = 0u;
-      val[j] = 0;
       if (pos < r.end) {
         key[j] = keys_in[pos];
-        val[j] = kFirst ? (int32_t)(pos - r.seg_begin) : vals_in[pos];
+        if (!kFirst)
+          s_vals[wave * 64 * kSortItems + 64 * j + lane] = vals_in[pos];
       }
     }
+    __syncthreads();  // the zeroed counters
   }
 
   // Rank: for each item, the lanes holding the same digit are found with
@@ -431,12 +457,33 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
       total[i] += c;
     }
   }
-  digit_prefix<kBits>(total, start, s_wsum);
+  if (hist) {
+    // starts and digit bases in one scan; each thread reads and rewrites
+    // only its own digits' s_gbase words
+    uint64_t two[D::kDpt], pre[D::kDpt];
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i)
+      two[i] = ((uint64_t)total[i] << 32) | s_gbase[tid * D::kDpt + i];
+    digit_prefix2<kBits>(two, pre, s_wsum2);
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i) {
+      start[i] = (uint32_t)(pre[i] >> 32);
+      goff[i] += (uint32_t)pre[i];
+    }
+  } else {
+    digit_prefix<kBits>(total, start, s_wsum);
+  }
 #pragma unroll
   for (int i = 0; i < D::kDpt; ++i) {
     s_start[tid * D::kDpt + i] = start[i];
     s_gbase[tid * D::kDpt + i] = goff[i] - start[i];
   }
+  // payloads out of their striped slots before the reorder overwrites them
+  int32_t val[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j)
+    val[j] = kFirst ? (int32_t)(wbase + 64 * j + lane - r.seg_begin)
+                    : s_vals[wave * 64 * kSortItems + 64 * j + lane];
   __syncthreads();
 
   // Reorder the tile in LDS by (digit, input order).
@@ -478,6 +525,8 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   if (!counted)
     hipLaunchKernelGGL(k_count<kBits>, dim3(tiles), dim3(kSortThreads), 0, s,
                        keys_in, n, shift, cnt, sg);
+  // the digit totals the scatter scans into bases (null: off is complete)
+  const uint32_t *bases_from = nullptr;
   if (!segmented && tiles <= kSmallScanTiles) {
     hipLaunchKernelGGL(k_scan_small<kBits>, dim3(1), dim3(kR), 0, s, cnt, off,
                        tiles);
@@ -487,8 +536,7 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                          dim3(kSortThreads), 0, s, cnt, tiles, sg, num_segs,
                          seg_base);
     else
-      hipLaunchKernelGGL(k_digit_totals, dim3(kR), dim3(kSortThreads), 0, s,
-                         cnt, tiles, hist);
+      bases_from = hist;
     hipLaunchKernelGGL(k_scan_columns<kBits>, dim3(kR), dim3(kSortThreads), 0,
                        s, cnt, off, tiles, hist, sg, seg_base);
   }
@@ -496,11 +544,11 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
     hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in,
                        (const int32_t *)nullptr, keys_out, vals_out, n, shift,
-                       off, sg);
+                       off, bases_from, sg);
   else
     hipLaunchKernelGGL((k_scatter<kBits, false>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in, vals_in, keys_out,
-                       vals_out, n, shift, off, sg);
+                       vals_out, n, shift, off, bases_from, sg);
 }
 
 void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
