@@ -2,9 +2,11 @@
 the CPU, step for step: the gap deltas as a sparse table of range minima,
 each gap's nearest smaller delta on either side by binary lifting, child boxes
 from a segment tree over the item boxes, completeness flags and the forest's
-roots as the next frontier, with every array access bounds-checked.  Run
-level after level over small meshes, it must reproduce the oracle's nodes
-byte for byte (bvh-up-pass.js:200-280 with the Karras ids of
+roots as the next frontier, with every array access bounds-checked.  A block
+takes the frontiers of FAN tiles in rounds of at most T items, each round
+starting with the previous round's frontier.  Run level after level over
+small meshes (also with rounds far smaller than the kernel's, so that most
+blocks take several), it must reproduce the oracle's nodes byte for byte (bvh-up-pass.js:200-280 with the Karras ids of
 radix-tree.js:226-235, as the oracle restates them)."""
 import numpy as np
 import pytest
@@ -12,7 +14,7 @@ import pytest
 T = 1024      # items per block (kItemTile)
 LOG = 10      # sparse-table levels above 0
 FMAX = 128
-FAN = 8
+FAN = 32     # source tiles per block (kFanIn)
 INF = np.float32(np.inf)
 
 def clz(x): return 32 - int(x).bit_length()
@@ -27,7 +29,7 @@ class Arr:
     def __setitem__(self, i, v):
         assert 0 <= i < len(self.a), (i, len(self.a)); self.a[i] = v
 
-def item_level(items, skeys, n, nodes):
+def item_level(items, skeys, n, nodes, T=T, LOG=LOG):
     """items: list of (l, r, mn, mx, dl, dr) in leaf order (m <= T).
     Returns the frontier list in the same format."""
     m = len(items)
@@ -158,7 +160,23 @@ def leaf_level(tri, skeys, perm, n, Tl, nodes):
         fronts.append(item_level(items, skeys, n, nodes))
     return fronts
 
-def run(ora, tri, Tl):
+def item_block(flat, skeys, n, nodes, T_round):
+    """One block of an item level: rounds of at most T_round items, the
+    previous round's frontier first (k_items)."""
+    log = T_round.bit_length() - 1
+    assert 1 << log == T_round
+    carry, taken = [], 0
+    while True:
+        fresh = min(len(flat) - taken, T_round - len(carry))
+        assert fresh > 0 or taken == len(flat)
+        items = carry + flat[taken:taken + fresh]
+        front = item_level(items, skeys, n, nodes, T_round, log)
+        if taken + fresh == len(flat):
+            return front
+        taken += fresh
+        carry = front
+
+def run(ora, tri, Tl, T_round=T):
     bvh, otri, sk, pm = ora.build(tri)
     n = len(sk)
     nodes = np.zeros((max(n, 1), 16), np.int32)
@@ -170,7 +188,7 @@ def run(ora, tri, Tl):
             flat = [it for f in fronts[t:t + FAN] for it in f]
             for a, b in zip(flat, flat[1:]):
                 assert a[1] + 1 == b[0]
-            nxt.append(item_level(flat, sk, n, nodes))
+            nxt.append(item_block(flat, sk, n, nodes, T_round))
         fronts = nxt; levels += 1
         if len(fronts) == 1 and not fronts[0]: break
         assert levels < 20
@@ -180,17 +198,22 @@ def run(ora, tri, Tl):
 
 
 
-CASES = [("clustered", lambda lb: lb.wl_clustered(5, 6000, 0.3)),
-         ("uniform", lambda lb: lb.wl_uniform(6, 5000, 0.01)),
-         ("heightfield", lambda lb: lb.wl_heightfield(7, 32, 32)),
-         ("all-equal", lambda lb: np.tile(lb.wl_uniform(9, 1, 0.01), (700, 1))),
-         ("n2", lambda lb: lb.wl_uniform(3, 2, 0.01)),
-         ("n3", lambda lb: lb.wl_uniform(4, 3, 0.01)),
-         ("multi-level", lambda lb: lb.wl_uniform(11, 9 * 8 * 64 + 5, 0.01))]
+# (name, mesh, items per round)
+CASES = [("clustered", lambda lb: lb.wl_clustered(5, 6000, 0.3), T),
+         ("uniform", lambda lb: lb.wl_uniform(6, 5000, 0.01), T),
+         ("heightfield", lambda lb: lb.wl_heightfield(7, 32, 32), T),
+         ("all-equal", lambda lb: np.tile(lb.wl_uniform(9, 1, 0.01), (700, 1)), T),
+         ("n2", lambda lb: lb.wl_uniform(3, 2, 0.01), T),
+         ("n3", lambda lb: lb.wl_uniform(4, 3, 0.01), T),
+         ("multi-level", lambda lb: lb.wl_uniform(11, 33 * 32 * 64 + 5, 0.01), T),
+         ("rounds-uniform", lambda lb: lb.wl_uniform(12, 40000, 0.01), 128),
+         ("rounds-clustered", lambda lb: lb.wl_clustered(13, 30000, 0.3), 128),
+         ("rounds-all-equal", lambda lb: np.tile(lb.wl_uniform(14, 1, 0.01), (9000, 1)), 128)]
 
 
 @pytest.mark.parametrize("name", [c[0] for c in CASES])
 def test_parallel_item_levels_reproduce_oracle(lb, ora, name):
-    tri = dict(CASES)[name](lb)
-    ok, n, levels = run(ora, tri, 64)
+    gen, t_round = {c[0]: c[1:] for c in CASES}[name]
+    tri = gen(lb)
+    ok, n, levels = run(ora, tri, 64, t_round)
     assert ok, (name, n, levels)

@@ -363,9 +363,10 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // their count.
 //
 // Level L >= 1 builds the tree over items: one block of kItemTile threads
-// takes the frontiers of kFanIn consecutive tiles of level L-1 (at most
-// kItemTile items), until a single tile holds all of a mesh's remaining
-// items and completes its root.  Frontier items are the subtrees hanging off
+// takes the frontiers of kFanIn consecutive tiles of level L-1, in rounds
+// of at most kItemTile items (the previous round's frontier, then the next
+// source items), until a single tile holds all of a mesh's remaining items
+// and completes its root.  Frontier items are the subtrees hanging off
 // root paths, so climbing over them would be a serial chain of merges as
 // long as those paths; the item levels compute their nodes in parallel
 // instead (k_items).  The level plan depends only on
@@ -378,7 +379,11 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 
 constexpr int kFrontierMax = kTreeFrontier;
 constexpr int kItemTile = 1024;
-constexpr int kFanIn = kItemTile / kFrontierMax;  // level L-1 tiles per tile
+// level L-1 tiles per tile: up to 32 x 128 items, typically a few hundred
+// (a round holds at least kItemTile - kFrontierMax new items)
+constexpr int kFanIn = 32;
+static_assert(kFanIn < kItemTile && (kFanIn & (kFanIn - 1)) == 0,
+              "one thread per source tile; binary search over them");
 
 __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
@@ -387,17 +392,21 @@ __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
 // Frontier output shared by both level kinds: the stopped subtrees that
 // leave the tile, in item order, into the tile's kFrontierMax slots.  An item
 // is its leaf range, its box and the deltas at the gaps on either side of
-// the range, so the level above needs no sorted keys.
-__device__ __forceinline__ void emit_frontier(bool out, int l, int r, int dl,
-                                              int dr, const float mn[3],
-                                              const float mx[3],
-                                              uint32_t *wsum, uint32_t gtile,
-                                              int4 *items_out,
-                                              uint32_t *count_out, Ctl *ctl,
-                                              int tid, int T) {
+// the range, so the level above needs no sorted keys.  Returns the count
+// to every thread.
+__device__ __forceinline__ uint32_t emit_frontier(bool out, int l, int r,
+                                                  int dl, int dr,
+                                                  const float mn[3],
+                                                  const float mx[3],
+                                                  uint32_t *wsum,
+                                                  uint32_t gtile,
+                                                  int4 *items_out,
+                                                  uint32_t *count_out,
+                                                  Ctl *ctl, int tid, int T) {
   const uint32_t rank = block_exclusive_scan(out ? 1u : 0u, wsum);
+  uint32_t count = 0;  // the per-wave sums stay in wsum after the scan
+  for (int w = 0; w < (T >> 6); ++w) count += wsum[w];
   if (tid == T - 1) {
-    const uint32_t count = rank + (out ? 1u : 0u);
     count_out[gtile] = count;
     if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
   }
@@ -408,6 +417,7 @@ __device__ __forceinline__ void emit_frontier(bool out, int l, int r, int dl,
                       __float_as_int(mx[1]), __float_as_int(mx[2]));
     io[2] = make_int4(dl, dr, 0, 0);
   }
+  return count;
 }
 
 // ---- level 0: leaves ------------------------------------------------------
@@ -603,6 +613,14 @@ __global__ __launch_bounds__(T) void k_leaves(
 // level below.  Level L-1's tile g keeps its items in
 // items_in[g * kFrontierMax ...] and their number in count_in[g].
 //
+// The block sweeps them in rounds of m <= kItemTile items: the frontier the
+// previous round emitted (read back from the tile's own output slots; it is
+// the frontier of every leaf so far, at most kFrontierMax items) followed by
+// as many of the next source items as fit.  A round's last gap carries the
+// delta to the next round's first leaf, so the nodes it cannot complete
+// stay roots, exactly as at the block's own edge.  Typical frontiers are
+// small, and a level usually takes a single round.
+//
 // The tree over the block's m items is the min-Cartesian tree of the deltas
 // at the gaps between them: the node split at gap g spans, on each side, up
 // to the nearest gap with a smaller delta (equal deltas never face each
@@ -662,42 +680,18 @@ __device__ __forceinline__ void item_range_box(const ItemLds &L, int a, int b,
   }
 }
 
-__global__ __launch_bounds__(kItemTile) void k_items(
-    uint32_t n_all,
-    const int4 *__restrict__ items_in, const uint32_t *__restrict__ count_in,
-    uint32_t prev_tiles_all, int4 *__restrict__ items_out,
-    uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
-    TreeSegs segs) {
+// One round of k_items over m items: items [0, carry) are the tile's own
+// output slots (the previous round's frontier), items [carry, m) the source
+// items from `taken` on.  Writes the nodes it completes and leaves the
+// forest roots in L.root_of (gap, or kItemTile + item) with L.qL / L.qR.
+__device__ __forceinline__ void item_round(ItemLds &L, int m, int carry,
+                                           int taken, uint32_t src0,
+                                           const int4 *__restrict__ items_in,
+                                           const int4 *items_out,
+                                           uint32_t gtile, uint32_t N,
+                                           int4 *nodes) {
   constexpr int T = kItemTile;
-  __shared__ ItemLds L;
   const int tid = threadIdx.x;
-  const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
-  uint32_t tile0 = 0, leaf_base = 0, N = n_all, prev0 = 0,
-           prev_tiles = prev_tiles_all;
-  if (segs.tile_seg) {
-    const uint32_t sg = segs.tile_seg[gtile];
-    tile0 = segs.seg_tile0[sg];
-    leaf_base = segs.seg_off[sg];
-    N = segs.seg_off[sg + 1] - leaf_base;
-    prev0 = segs.prev_tile0[sg];
-    prev_tiles = segs.prev_tiles[sg];
-  }
-  const uint32_t tile = gtile - tile0;  // tile within the mesh
-  int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
-  const uint32_t src0 = prev0 + tile * kFanIn;  // first source tile
-  if (tid == 0) {
-    const uint32_t first = tile * kFanIn;
-    const uint32_t nsrc =
-        prev_tiles > first ? min((uint32_t)kFanIn, prev_tiles - first) : 0u;
-    uint32_t run = 0;
-#pragma unroll
-    for (int s = 0; s < kFanIn; ++s) {
-      L.pre[s] = run;
-      run += (uint32_t)s < nsrc
-                 ? min(count_in[src0 + s], (uint32_t)kFrontierMax) : 0u;
-    }
-    L.pre[kFanIn] = run;
-  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     L.smn[c][T + tid] = INFINITY;  // leaves past the items stay neutral
@@ -706,16 +700,23 @@ __global__ __launch_bounds__(kItemTile) void k_items(
   L.complete[tid] = 0;
   L.root_of[tid] = -1;
   __syncthreads();
-  const int m = (int)L.pre[kFanIn];
 
   // 1. items: leaf ranges, boxes (segment-tree leaves), gap deltas
   if (tid < m) {
-    int s = 0;
-    while (s + 1 < kFanIn && (int)L.pre[s + 1] <= tid) ++s;
-    const size_t it =
-        (size_t)(src0 + s) * kFrontierMax + (uint32_t)(tid - (int)L.pre[s]);
-    const int4 a = items_in[kItemInt4 * it], b = items_in[kItemInt4 * it + 1],
-               g = items_in[kItemInt4 * it + 2];
+    const int4 *src;
+    if (tid < carry) {
+      src = items_out + kItemInt4 * ((size_t)gtile * kFrontierMax + tid);
+    } else {
+      // the source tile holding item g: the last s with pre[s] <= g
+      const int g = taken + tid - carry;
+      int s = 0;
+#pragma unroll
+      for (int step = kFanIn / 2; step > 0; step >>= 1)
+        if ((int)L.pre[s + step] <= g) s += step;
+      src = items_in + kItemInt4 * ((size_t)(src0 + s) * kFrontierMax +
+                                    (uint32_t)(g - (int)L.pre[s]));
+    }
+    const int4 a = src[0], b = src[1], g = src[2];
     L.il[tid] = a.x;
     L.ir[tid] = a.y;
     L.smn[0][T + tid] = __int_as_float(a.z);
@@ -803,28 +804,80 @@ __global__ __launch_bounds__(kItemTile) void k_items(
       L.root_of[tid] = T + tid;
   }
   __syncthreads();
+}
 
-  // the frontier in leaf order: the root whose first item is tid
-  const int v = tid < m ? L.root_of[tid] : -1;
-  const bool out = v >= 0;
-  int fl = 0, fr = 0, fdl = 0, fdr = 0;
-  float fmn[3] = {0.f, 0.f, 0.f}, fmx[3] = {0.f, 0.f, 0.f};
-  if (out) {
-    int a, b;
-    if (v >= T) {
-      a = b = v - T;
-    } else {
-      a = L.qL[v];
-      b = L.qR[v] - 1;
-    }
-    item_range_box(L, a, b, fmn, fmx);
-    fl = L.il[a];
-    fr = L.ir[b];
-    fdl = L.dm[0][a];
-    fdr = L.dm[0][b + 1];
+__global__ __launch_bounds__(kItemTile) void k_items(
+    uint32_t n_all,
+    const int4 *__restrict__ items_in, const uint32_t *__restrict__ count_in,
+    uint32_t prev_tiles_all, int4 *items_out,  // written, then read back
+    uint32_t *__restrict__ count_out, int4 *nodes_all, Ctl *ctl,
+    TreeSegs segs) {
+  constexpr int T = kItemTile;
+  __shared__ ItemLds L;
+  const int tid = threadIdx.x;
+  const uint32_t gtile = xcd_tile(blockIdx.x, gridDim.x);
+  uint32_t tile0 = 0, leaf_base = 0, N = n_all, prev0 = 0,
+           prev_tiles = prev_tiles_all;
+  if (segs.tile_seg) {
+    const uint32_t sg = segs.tile_seg[gtile];
+    tile0 = segs.seg_tile0[sg];
+    leaf_base = segs.seg_off[sg];
+    N = segs.seg_off[sg + 1] - leaf_base;
+    prev0 = segs.prev_tile0[sg];
+    prev_tiles = segs.prev_tiles[sg];
   }
-  emit_frontier(out, fl, fr, fdl, fdr, fmn, fmx, L.wsum, gtile, items_out,
-                count_out, ctl, tid, T);
+  const uint32_t tile = gtile - tile0;  // tile within the mesh
+  int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
+  const uint32_t src0 = prev0 + tile * kFanIn;  // first source tile
+  {
+    // item prefix over the source tiles: L.pre[s] = first item of tile s
+    const uint32_t first = tile * kFanIn;
+    const uint32_t nsrc =
+        prev_tiles > first ? min((uint32_t)kFanIn, prev_tiles - first) : 0u;
+    const uint32_t c = (uint32_t)tid < nsrc
+                           ? min(count_in[src0 + tid], (uint32_t)kFrontierMax)
+                           : 0u;
+    const uint32_t ex = block_exclusive_scan(c, L.wsum);
+    if (tid <= kFanIn) L.pre[tid] = ex;  // pre[kFanIn] = the total
+  }
+  __syncthreads();
+  const int total = (int)L.pre[kFanIn];
+  int taken = 0;  // source items used by earlier rounds
+  int carry = 0;  // items of the previous round's frontier
+  while (true) {
+    const int fresh = min(total - taken, T - carry);
+    const bool last = taken + fresh == total;
+    const int m = carry + fresh;
+    item_round(L, m, carry, taken, src0, items_in, items_out, gtile, N,
+               nodes);
+    // the round's frontier: the next round's first items, or the output
+    const int v = tid < m ? L.root_of[tid] : -1;
+    const bool out = v >= 0;
+    int fl = 0, fr = 0, fdl = 0, fdr = 0;
+    float fmn[3] = {0.f, 0.f, 0.f}, fmx[3] = {0.f, 0.f, 0.f};
+    if (out) {
+      int a, b;
+      if (v >= T) {
+        a = b = v - T;
+      } else {
+        a = L.qL[v];
+        b = L.qR[v] - 1;
+      }
+      item_range_box(L, a, b, fmn, fmx);
+      fl = L.il[a];
+      fr = L.ir[b];
+      fdl = L.dm[0][a];
+      fdr = L.dm[0][b + 1];
+    }
+    __syncthreads();  // every read of this round's items is done
+    const uint32_t count = emit_frontier(out, fl, fr, fdl, fdr, fmn, fmx,
+                                         L.wsum, gtile, items_out, count_out,
+                                         ctl, tid, T);
+    if (last) break;
+    taken += fresh;
+    carry = (int)min(count, (uint32_t)kFrontierMax);
+    __syncthreads();  // the emitted items are visible to the whole block
+  }
 }
 
 // ---------------------------------------------------------------------------
