@@ -328,9 +328,16 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ hist, SortSegs segs) {
   using D = Digits<kBits>;
+  // Wide digits keep two 16-bit per-wave counters per word (a wave counts
+  // at most 64 * kSortItems = 1024 keys of a digit, so no carry crosses into
+  // the other half): 8 KB less LDS for 10-bit digits, a third block per CU.
+  constexpr bool kPack = kBits > kRadixBits;
+  constexpr uint32_t kHw = kPack ? D::kR / 2 : D::kR;  // counter words per wave
+  static_assert(!kPack || (64 * kSortItems < 65536 && D::kDpt % 2 == 0),
+                "16-bit counters, a thread's digits in whole words");
   __shared__ uint32_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
-  __shared__ uint32_t s_whist[kWaves][D::kR];
+  __shared__ uint32_t s_whist[kWaves][kHw];
   __shared__ uint32_t s_spare[kWaves][64];  // targets of the no-op adds
   __shared__ uint32_t s_start[D::kR];  // tile-local start of each digit run
   // the digit totals (hist), then output position minus local start
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  for (int k = tid; k < kWaves * D::kR; k += kSortThreads)
+  for (int k = tid; k < kWaves * (int)kHw; k += kSortThreads)
     (&s_whist[0][0])[k] = 0u;
   // The digit totals go straight from global memory into s_gbase (no
   // registers held meanwhile): wave w's q-th load fills 64 consecutive
@@ -429,8 +436,11 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     // result here): a group's first lane adds the group size to the digit
     // counter, the others add 0 to a word of their own.
     const bool lead = valid && below == 0;
-    uint32_t *slot = lead ? &s_whist[wave][d] : &s_spare[wave][lane];
-    prev[j] = atomicAdd(slot, lead ? (uint32_t)__popcll(peers) : 0u);
+    uint32_t *slot =
+        lead ? &s_whist[wave][kPack ? d >> 1 : d] : &s_spare[wave][lane];
+    prev[j] = atomicAdd(slot, lead ? (uint32_t)__popcll(peers)
+                                         << (kPack ? (d & 1) << 4 : 0)
+                                   : 0u);
     // a valid lane's group holds itself, so peers != 0
     const uint32_t leader =
         valid ? (uint32_t)__ffsll((unsigned long long)peers) - 1 : lane;
@@ -438,7 +448,8 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    const uint32_t p = __shfl(prev[j], (int)(rank[j] >> 8), 64);
+    uint32_t p = __shfl(prev[j], (int)(rank[j] >> 8), 64);
+    if (kPack) p = (p >> ((((key[j] >> shift) & 1u)) << 4)) & 0xffffu;
     rank[j] = (wbase + 64 * j + lane) < r.end ? p + (rank[j] & 0xffu) : 0u;
   }
   __syncthreads();
@@ -446,15 +457,31 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   // This tile's count of each of the thread's digits and per-wave exclusive
   // offsets, then the tile-local start of each digit run.
   uint32_t total[D::kDpt], start[D::kDpt];
+  if (kPack) {
+    // the thread's digits pair up in whole words
 #pragma unroll
-  for (int i = 0; i < D::kDpt; ++i) {
-    const uint32_t d = tid * D::kDpt + i;
-    total[i] = 0;
+    for (int i = 0; i < D::kDpt; i += 2) {
+      const uint32_t q = (tid * D::kDpt + i) >> 1;
+      total[i] = total[i + 1] = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      const uint32_t c = s_whist[w][d];
-      s_whist[w][d] = total[i];
-      total[i] += c;
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_whist[w][q];
+        s_whist[w][q] = total[i] | (total[i + 1] << 16);
+        total[i] += c & 0xffffu;
+        total[i + 1] += c >> 16;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i) {
+      const uint32_t d = tid * D::kDpt + i;
+      total[i] = 0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_whist[w][d];
+        s_whist[w][d] = total[i];
+        total[i] += c;
+      }
     }
   }
   if (hist) {
@@ -491,7 +518,10 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   for (int j = 0; j < kSortItems; ++j) {
     if ((wbase + 64 * j + lane) < r.end) {
       const uint32_t dj = (key[j] >> shift) & (D::kR - 1);
-      const uint32_t lp = s_start[dj] + s_whist[wave][dj] + rank[j];
+      const uint32_t wo =
+          kPack ? (s_whist[wave][dj >> 1] >> ((dj & 1) << 4)) & 0xffffu
+                : s_whist[wave][dj];
+      const uint32_t lp = s_start[dj] + wo + rank[j];
       s_keys[lp] = key[j];
       s_vals[lp] = val[j];
     }
