@@ -41,7 +41,8 @@ namespace lbvh {
 constexpr int kWaves = kSortThreads / 64;
 static_assert(kSortItems % 4 == 0, "16-byte tile loads");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
-// Single-mesh sorts of at most this many tiles (64K keys) scan in one block.
+// Single-mesh sorts of at most this many tiles (64K keys) launch no scan:
+// each scatter tile sums the count rows of its digits itself.
 constexpr uint32_t kSmallScanTiles = 16;
 
 // Digits of width kBits: kR values, kDpt of them per thread of a block of
@@ -206,60 +207,6 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
   if (!segmented && threadIdx.x == 0) hist[d] = carry;
 }
 
-// Digit totals and column scan of a single mesh in one block, for sorts of
-// at most kSmallScanTiles tiles: a small build is a chain of short
-// dependent kernels, and this replaces two per pass.  One thread per digit
-// (a block of 2^kBits threads).  The count matrix goes through LDS: loaded
-// with consecutive threads on consecutive words (a thread reading its own
-// column from global memory would put every lane on another cache line),
-// stored with rows padded to kSmallScanTiles + 1 words so that thread d's
-// walk along row d is free of bank conflicts.  The block scans the column
-// totals (the digit bases); each thread writes its row's running offsets
-// back (as k_scan_columns computes them) for a coalesced store.
-template <int kBits>
-__global__ __launch_bounds__(1 << kBits) void k_scan_small(
-    const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
-    uint32_t tiles) {
-  constexpr uint32_t kR = 1u << kBits;
-  constexpr uint32_t kStride = kSmallScanTiles + 1;
-  static_assert(kR <= 1024, "one block of one thread per digit");
-  __shared__ uint32_t s_m[kR * kStride];
-  __shared__ uint32_t s_wsum[16];
-  const uint32_t words = kR * tiles;
-  uint32_t v[kSmallScanTiles];
-#pragma unroll
-  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
-    const uint32_t e = threadIdx.x + j * kR;
-    v[j] = e < words ? cnt[e] : 0u;
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
-    const uint32_t e = threadIdx.x + j * kR;
-    if (e < words) s_m[(e / tiles) * kStride + e % tiles] = v[j];
-  }
-  __syncthreads();
-  uint32_t *row = s_m + threadIdx.x * kStride;
-  uint32_t total = 0;
-#pragma unroll
-  for (uint32_t t = 0; t < kSmallScanTiles; ++t)
-    if (t < tiles) total += row[t];
-  uint32_t run = block_exclusive_scan(total, s_wsum);
-#pragma unroll
-  for (uint32_t t = 0; t < kSmallScanTiles; ++t) {
-    if (t < tiles) {
-      const uint32_t c = row[t];
-      row[t] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (uint32_t j = 0; j < kSmallScanTiles; ++j) {
-    const uint32_t e = threadIdx.x + j * kR;
-    if (e < words) off[e] = s_m[(e / tiles) * kStride + e % tiles];
-  }
-}
-
 // Segment digit bases of one pass from its per-tile counts: block s sums
 // each digit's column over the segment's tiles, then seg_base[s][d] =
 // seg_off[s] + (keys of s whose digit is below d).
@@ -320,13 +267,16 @@ __device__ __forceinline__ void digit_prefix2(
 // hist (or null when off holds whole output positions): the pass's digit
 // totals; off then holds positions among the digit's keys and each tile
 // adds the digit bases, an exclusive scan of hist, computed in the same
-// block scan as the tile's own run starts.
+// block scan as the tile's own run starts.  small (a single mesh of at most
+// kSmallScanTiles tiles): off holds the per-tile counts themselves, and each
+// tile sums its digits' rows (from L2) into its column prefix and the digit
+// totals, so a small sort launches no scan kernel.
 template <int kBits, bool kFirst>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
-    const uint32_t *__restrict__ hist, SortSegs segs) {
+    const uint32_t *__restrict__ hist, int small, SortSegs segs) {
   using D = Digits<kBits>;
   // Wide digits keep two 16-bit per-wave counters per word (a wave counts
   // at most 64 * kSortItems = 1024 keys of a digit, so no carry crosses into
@@ -361,9 +311,28 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     }
   }
   uint32_t goff[D::kDpt];
+  if (small) {
+    // column prefix at this tile and the digit's total, from its count row
 #pragma unroll
-  for (int i = 0; i < D::kDpt; ++i)
-    goff[i] = off[(size_t)(tid * D::kDpt + i) * tiles + tile];
+    for (int i = 0; i < D::kDpt; ++i) {
+      const uint32_t d = tid * D::kDpt + i;
+      const uint32_t *row = off + (size_t)d * tiles;
+      uint32_t run = 0, pre = 0;
+#pragma unroll
+      for (uint32_t t = 0; t < kSmallScanTiles; ++t) {
+        if (t < tiles) {
+          if (t == tile) pre = run;
+          run += row[t];
+        }
+      }
+      goff[i] = pre;
+      s_gbase[d] = run;  // read back by this thread only
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i)
+      goff[i] = off[(size_t)(tid * D::kDpt + i) * tiles + tile];
+  }
   const TileRange r = sort_tile_range(tile, n, segs);
   // No barrier here: one waits for every outstanding global load, and the
   // tile's own loads below should be in flight with the ones above.  The
@@ -484,7 +453,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
       }
     }
   }
-  if (hist) {
+  if (hist || small) {
     // starts and digit bases in one scan; each thread reads and rewrites
     // only its own digits' s_gbase words
     uint64_t two[D::kDpt], pre[D::kDpt];
@@ -557,10 +526,10 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                        keys_in, n, shift, cnt, sg);
   // the digit totals the scatter scans into bases (null: off is complete)
   const uint32_t *bases_from = nullptr;
-  if (!segmented && tiles <= kSmallScanTiles) {
-    hipLaunchKernelGGL(k_scan_small<kBits>, dim3(1), dim3(kR), 0, s, cnt, off,
-                       tiles);
-  } else {
+  // a small single mesh: the scatter reads the counts themselves
+  const int small = !segmented && tiles <= kSmallScanTiles;
+  const uint32_t *offs = small ? cnt : off;
+  if (!small) {
     if (segmented)
       hipLaunchKernelGGL(k_seg_bases<kBits>, dim3(num_segs),
                          dim3(kSortThreads), 0, s, cnt, tiles, sg, num_segs,
@@ -574,11 +543,11 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
     hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in,
                        (const int32_t *)nullptr, keys_out, vals_out, n, shift,
-                       off, bases_from, sg);
+                       offs, bases_from, small, sg);
   else
     hipLaunchKernelGGL((k_scatter<kBits, false>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in, vals_in, keys_out,
-                       vals_out, n, shift, off, bases_from, sg);
+                       vals_out, n, shift, offs, bases_from, small, sg);
 }
 
 void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
