@@ -3,7 +3,8 @@ the CPU, step for step: the gap deltas as a sparse table of range minima,
 each gap's nearest smaller delta on either side by binary lifting, child boxes
 from a segment tree over the item boxes, completeness flags and the forest's
 roots as the next frontier, with every array access bounds-checked.  A block
-takes the frontiers of FAN tiles in rounds of at most T items, each round
+takes the frontiers of FAN tiles (all of them, in one last block, once at
+most FAN_MAX remain) in rounds of at most T items, each round
 starting with the previous round's frontier.  Run level after level over
 small meshes (also with rounds far smaller than the kernel's, so that most
 blocks take several), it must reproduce the oracle's nodes byte for byte (bvh-up-pass.js:200-280 with the Karras ids of
@@ -14,7 +15,8 @@ import pytest
 T = 1024      # items per block (kItemTile)
 LOG = 10      # sparse-table levels above 0
 FMAX = 128
-FAN = 32     # source tiles per block (kFanIn)
+FAN = 32     # source tiles per block (kFanIn) ...
+FAN_MAX = 64  # ... or all of them, up to this many, in one last block (fan_in)
 INF = np.float32(np.inf)
 
 def clz(x): return 32 - int(x).bit_length()
@@ -184,8 +186,9 @@ def run(ora, tri, Tl, T_round=T):
     levels = 1
     while len(fronts) > 1 or (fronts and fronts[0]):
         nxt = []
-        for t in range(0, len(fronts), FAN):
-            flat = [it for f in fronts[t:t + FAN] for it in f]
+        fan = FAN_MAX if len(fronts) <= FAN_MAX else FAN
+        for t in range(0, len(fronts), fan):
+            flat = [it for f in fronts[t:t + fan] for it in f]
             for a, b in zip(flat, flat[1:]):
                 assert a[1] + 1 == b[0]
             nxt.append(item_block(flat, sk, n, nodes, T_round))

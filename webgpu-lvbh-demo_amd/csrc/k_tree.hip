@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 // their count.
 //
 // Level L >= 1 builds the tree over items: one block of kItemTile threads
-// takes the frontiers of kFanIn consecutive tiles of level L-1, in rounds
+// takes the frontiers of fan_in(...) consecutive tiles of level L-1, in rounds
 // of at most kItemTile items (the previous round's frontier, then the next
 // source items), until a single tile holds all of a mesh's remaining items
 // and completes its root.  Frontier items are the subtrees hanging off
@@ -379,11 +379,19 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 
 constexpr int kFrontierMax = kTreeFrontier;
 constexpr int kItemTile = 1024;
-// level L-1 tiles per tile: up to 32 x 128 items, typically a few hundred
-// (a round holds at least kItemTile - kFrontierMax new items)
+// Level L-1 tiles per tile: kFanIn (up to 32 x 128 items, typically a few
+// hundred; a round holds at least kItemTile - kFrontierMax new items), or
+// all of them, up to kFanInMax, in one last tile.  Depends only on the
+// number of level L-1 tiles, so the host's level plan and the kernel agree.
 constexpr int kFanIn = 32;
-static_assert(kFanIn < kItemTile && (kFanIn & (kFanIn - 1)) == 0,
+constexpr int kFanInMax = 64;
+static_assert(kFanIn <= kFanInMax && kFanInMax < kItemTile &&
+                  (kFanInMax & (kFanInMax - 1)) == 0,
               "one thread per source tile; binary search over them");
+__host__ __device__ constexpr uint32_t fan_in(uint32_t prev_tiles) {
+  return prev_tiles <= (uint32_t)kFanInMax ? (uint32_t)kFanInMax
+                                           : (uint32_t)kFanIn;
+}
 
 __device__ __forceinline__ int delta_next(uint32_t a, uint32_t b, uint32_t i) {
   return (a != b) ? (int)__clz(a ^ b) : 32 + (int)__clz(i ^ (i + 1u));
@@ -609,7 +617,7 @@ __global__ __launch_bounds__(T) void k_leaves(
 
 // ---- levels >= 1: items ---------------------------------------------------
 
-// One item level: tile = the frontiers of kFanIn consecutive tiles of the
+// One item level: tile = the frontiers of fan_in consecutive tiles of the
 // level below.  Level L-1's tile g keeps its items in
 // items_in[g * kFrontierMax ...] and their number in count_in[g].
 //
@@ -649,7 +657,7 @@ struct ItemLds {
   unsigned char complete[kItemTile];  // per gap: node complete in the block
   int root_of[kItemTile];  // per first item: frontier root (gap, T + item)
   uint32_t wsum[16];
-  uint32_t pre[kFanIn + 1];  // first item of each source tile
+  uint32_t pre[kFanInMax + 1];  // first item of each source tile
 };
 
 // Box of items [a, b] from the segment tree (leaves at kItemTile + item).
@@ -711,7 +719,7 @@ __device__ __forceinline__ void item_round(ItemLds &L, int m, int carry,
       const int g = taken + tid - carry;
       int s = 0;
 #pragma unroll
-      for (int step = kFanIn / 2; step > 0; step >>= 1)
+      for (int step = kFanInMax / 2; step > 0; step >>= 1)
         if ((int)L.pre[s + step] <= g) s += step;
       src = items_in + kItemInt4 * ((size_t)(src0 + s) * kFrontierMax +
                                     (uint32_t)(g - (int)L.pre[s]));
@@ -828,20 +836,22 @@ __global__ __launch_bounds__(kItemTile) void k_items(
   }
   const uint32_t tile = gtile - tile0;  // tile within the mesh
   int4 *nodes = nodes_all + 4 * (size_t)leaf_base;
-  const uint32_t src0 = prev0 + tile * kFanIn;  // first source tile
+  const uint32_t fan = fan_in(prev_tiles);
+  const uint32_t src0 = prev0 + tile * fan;  // first source tile
   {
     // item prefix over the source tiles: L.pre[s] = first item of tile s
-    const uint32_t first = tile * kFanIn;
+    // (the total from s = nsrc on)
+    const uint32_t first = tile * fan;
     const uint32_t nsrc =
-        prev_tiles > first ? min((uint32_t)kFanIn, prev_tiles - first) : 0u;
+        prev_tiles > first ? min(fan, prev_tiles - first) : 0u;
     const uint32_t c = (uint32_t)tid < nsrc
                            ? min(count_in[src0 + tid], (uint32_t)kFrontierMax)
                            : 0u;
     const uint32_t ex = block_exclusive_scan(c, L.wsum);
-    if (tid <= kFanIn) L.pre[tid] = ex;  // pre[kFanIn] = the total
+    if (tid <= kFanInMax) L.pre[tid] = ex;
   }
   __syncthreads();
-  const int total = (int)L.pre[kFanIn];
+  const int total = (int)L.pre[kFanInMax];
   int taken = 0;  // source items used by earlier rounds
   int carry = 0;  // items of the previous round's frontier
   while (true) {
@@ -914,13 +924,13 @@ void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
 }
 
 // Level plan: ceil(n / tile) leaf tiles, then ceil(previous / kFanIn) item
-// tiles per level until one tile remains.
+// tiles per level, or one tile once at most kFanInMax remain (fan_in).
 int tree_levels(uint32_t n, int tile, uint32_t *grid) {
   uint64_t tiles = ((uint64_t)n + tile - 1) / tile;
   int levels = 0;
   grid[levels++] = (uint32_t)(tiles ? tiles : 1);
   while (tiles > 1 && levels < kMaxTreeLevels) {
-    tiles = (tiles + kFanIn - 1) / kFanIn;
+    tiles = (tiles + fan_in((uint32_t)tiles) - 1) / fan_in((uint32_t)tiles);
     grid[levels++] = (uint32_t)tiles;
   }
   return levels;
