@@ -117,7 +117,8 @@ class sort_bits:
 
 @pytest.mark.parametrize("key_bits", [30, 32])
 @pytest.mark.parametrize("bits", [8, 10])
-@pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65536, 1000003, 4194304])
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65535, 65536, 65537,
+                               1000003, 4194304])
 def test_sort_is_stable(ctx, lb, n, bits, key_bits):
     """Morton-range keys and full u32 keys: the stand-alone sort orders all
     32 bits, as the reference's 16 passes of 2 bits do."""
@@ -326,6 +327,28 @@ def test_build_wide_digits_match_oracle(ctx, lb, ora, keep_topology):
             if keep_topology:
                 assert np.array_equal(got[2], want[2]), name
                 assert np.array_equal(got[3], want[3]), name
+
+
+@pytest.mark.parametrize("n", [32 * 1024, 32 * 1024 + 1, 64 * 1024, 64 * 1024 + 1,
+                               2048 * 1024, 2048 * 1024 + 1])
+def test_build_at_plan_boundaries(ctx, lb, ora, n):
+    """Sizes where the plan changes shape (1,024-leaf tiles): 32, 33 and 64
+    leaf tiles take one item level (a single block over all of them), 65 take
+    two (3 blocks of 32 sources, then one); 16 sort tiles (the scatter sums
+    the counts itself) and 17 (a column-scan kernel); 2,048 leaf tiles (64,
+    then one) and 2,049 (65, then 3, then one).  Both digit widths."""
+    tri = lb.wl_uniform(n % 9973, n, 0.01)
+    want = ora.build(tri, None, 0)
+    ctx.set_tuning(lb.TUNE_TREE_TILE, 1024)
+    try:
+        for bits in (8, 10):
+            say(f"plan boundary n={n} bits={bits}")
+            with sort_bits(ctx, lb, bits):
+                got = ctx.build_arrays(tri, None)
+            assert np.array_equal(got[0], want[0]), bits
+            assert np.array_equal(got[1].view(np.uint32), want[1].view(np.uint32)), bits
+    finally:
+        ctx.set_tuning(lb.TUNE_TREE_TILE, 0)
 
 
 def test_build_is_deterministic(ctx, lb):
