@@ -390,6 +390,24 @@ def test_build_heightfield_16m_full_size(ctx, lb, ora, bits):
     assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
 
 
+def test_build_67m_against_oracle(ctx, lb, ora):
+    """2^26 + 3 triangles (3.2 GB in, 4.3 GB of nodes out): four times the
+    headline size, where any 32-bit index product (3 float4 per triangle, 4
+    int4 per node, 16,385 sort tiles) would wrap if it were computed
+    narrower.  Checked against the oracle's multi-threaded build, which the
+    CPU tests pin byte for byte to the scalar one."""
+    n = (1 << 26) + 3
+    say(f"build uniform n={n}")
+    tri = lb.wl_uniform(67, n, 0.01)
+    bvh, otri, sk, pm = ctx.build_arrays(tri, None, keep_topology=True)
+    say("oracle (multi-threaded)")
+    wb, wo, wsk, wpm, _ = ora.build_mt(tri, None, 0)
+    assert np.array_equal(sk, wsk)
+    assert np.array_equal(pm, wpm)
+    assert np.array_equal(bvh, wb)
+    assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
+
+
 # ---------------------------------------------------------------- refit
 
 def test_refit_matches_oracle(ctx, lb, ora):
