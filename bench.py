@@ -362,8 +362,9 @@ def main():
 
     # The JS entry point is handed host triangles (main.js passes a JS array):
     # the same build with the 48-byte triangles crossing PCIe first, from
-    # pageable host memory as a JS array's backing store is.  Reported beside
-    # `value`, which keeps the triangles resident.
+    # pageable host memory as a JS array's backing store is (lbvh_upload
+    # stages large copies through pinned buffers).  Reported beside `value`,
+    # which keeps the triangles resident.
     # The reference's caller passes MODEL_BOUNDS (main.js:88, from the OBJ
     # parser): the same build with the exact scene bounds given, so the GPU
     # bounds reduction is skipped and the Morton pass reads the triangles.

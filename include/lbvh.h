@@ -124,11 +124,14 @@ int lbvh_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out_dptr);
 /* Replaces GPUBuffer.destroy() (build-bvh.js:59-63). */
 int lbvh_free(lbvh_ctx *ctx, void *dptr);
 /* Host -> device copy (the mappedAtCreation upload, build-bvh.js:20-21).
- * Synchronous with respect to the host buffer. */
+ * Synchronous with respect to the host buffer.  Copies of 64 MB or more go
+ * through the context's pinned staging buffers (128 MB, allocated on the
+ * first such copy, freed by lbvh_destroy), filled by four host threads. */
 int lbvh_upload(lbvh_ctx *ctx, void *dst_dev, const void *src_host,
                 uint64_t bytes);
 /* Device -> host copy after all queued work (the reference never reads
- * back; tests and callers that want the bytes do).  Synchronises. */
+ * back; tests and callers that want the bytes do).  Synchronises.  Large
+ * copies are staged as in lbvh_upload. */
 int lbvh_download(lbvh_ctx *ctx, void *dst_host, const void *src_dev,
                   uint64_t bytes);
 
