@@ -119,10 +119,12 @@ __device__ void write_single_leaf_node(int4 *nodes, const float mn[3],
 // 64-bit words (box min.xyz, max.xyz, far end), each carrying the operation's
 // epoch in its high half and written with a relaxed agent-scope atomic store.
 // The arrival tag is swapped relaxed too, so no cache-wide release/acquire
-// fence is issued: the second arriver instead polls each sibling word until
-// it shows the current epoch.  The sibling stored those words before its own
-// swap, which this thread has already observed, so every poll ends; the cap
-// only bounds the loop should that ever break (ctl->fault is then set).
+// fence is issued: the second arriver instead polls the sibling's words until
+// all show the current epoch, the seven loads in flight together (one
+// round trip per poll, not seven: a chain of merges pays it at every
+// level).  The sibling stored those words before its own swap, which this
+// thread has already observed, so the polling ends; the cap only bounds the
+// loop should that ever break (ctl->fault is then set).
 // Returns true, with the sibling's box and far end, for the second arriver.
 __device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
                                               Ctl *ctl, uint32_t epoch,
@@ -148,21 +150,25 @@ __device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
   if (old != epoch) return false;
   const uint64_t *other = gside[2 * s + (1 - side)].w;
   uint32_t o[7];
+  uint32_t spins = 0;
+  while (true) {
+    uint64_t w[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    uint64_t w;
-    uint32_t spins = 0;
-    while (true) {
-      w = __hip_atomic_load(other + k, __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_AGENT);
-      if ((uint32_t)(w >> 32) == epoch) break;
-      if (++spins > kSpinCap) {
-        atomicOr(&ctl->fault, 2u);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+    for (int k = 0; k < 7; ++k)
+      w[k] = __hip_atomic_load(other + k, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    bool all = true;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      all = all && (uint32_t)(w[k] >> 32) == epoch;
+      o[k] = (uint32_t)w[k];
     }
-    o[k] = (uint32_t)w;
+    if (all) break;
+    if (++spins > kSpinCap) {
+      atomicOr(&ctl->fault, 2u);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
   }
   smn[0] = __uint_as_float(o[0]); smn[1] = __uint_as_float(o[1]);
   smn[2] = __uint_as_float(o[2]); smx[0] = __uint_as_float(o[3]);
