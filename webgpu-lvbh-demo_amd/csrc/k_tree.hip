@@ -200,15 +200,38 @@ constexpr int kUpTile = 1024;
 
 struct UpLds {
   float box[2][6][kUpTile];  // [side][min.xyz max.xyz][parent id - block start]
+  int32_t cid[2][kUpTile];   // [side][parent id - block start]: child id
   uint32_t cnt[kUpTile];     // arrivals per parent
+  // the block's parents entries, ids ts .. te (one past the block, for the
+  // sibling test at its edge): a climb reads these at every step, and from
+  // global memory they made each step a chain of dependent loads
+  int32_t par_leaf[kUpTile + 1];  // parents[n + id]
+  int32_t par_node[kUpTile + 1];  // parents[id]
 };
 
-__device__ __forceinline__ bool up_is_left(const int32_t *__restrict__ par,
-                                           uint32_t n, int64_t num,
-                                           int32_t p) {
+// parents[n + leaf] / parents[node], from LDS inside the block's staged range
+__device__ __forceinline__ int32_t up_par_leaf(const UpLds &L,
+                                               const int32_t *__restrict__ par,
+                                               uint32_t n, int64_t ts,
+                                               int64_t te, int64_t id) {
+  return (id >= ts && id <= te) ? L.par_leaf[id - ts] : par[n + id];
+}
+__device__ __forceinline__ int32_t up_par_node(const UpLds &L,
+                                               const int32_t *__restrict__ par,
+                                               int64_t ts, int64_t te,
+                                               int64_t id) {
+  return (id >= ts && id <= te) ? L.par_node[id - ts] : par[id];
+}
+
+// Node num (a leaf when leaf) is its parent p's left child when the child
+// numbered one above it has the same parent.
+__device__ __forceinline__ bool up_is_left(const UpLds &L,
+                                           const int32_t *__restrict__ par,
+                                           uint32_t n, int64_t ts, int64_t te,
+                                           int64_t num, int32_t p) {
   const int64_t sib = num + 1;
-  return (sib <= (int64_t)n - 1 && par[n + sib] == p) ||
-         (sib <= (int64_t)n - 2 && par[sib] == p);
+  return (sib <= (int64_t)n - 1 && up_par_leaf(L, par, n, ts, te, sib) == p) ||
+         (sib <= (int64_t)n - 2 && up_par_node(L, par, ts, te, sib) == p);
 }
 
 __global__ __launch_bounds__(kUpTile) void k_up_pass(
@@ -222,6 +245,11 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   const int64_t te = min(ts + kUpTile, (int64_t)n);
   const int64_t g = ts + tid;
   L.cnt[tid] = 0u;
+  for (int t = tid; t <= kUpTile; t += kUpTile) {
+    const int64_t id = ts + t;
+    L.par_leaf[t] = id < (int64_t)n ? par[n + id] : -1;
+    L.par_node[t] = id < (int64_t)n ? par[id] : -1;
+  }
   __syncthreads();
   const uint32_t epoch = ctl->epoch;
   // 0: done, 1: first to arrive at a parent in LDS, 2: continue in HBM
@@ -230,6 +258,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   int64_t num = g;  // leaf position or internal node id
   int32_t p = -1;
   int side = 0;
+  int self = 0;  // this climber's child id at p
   float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
   if (g < (int64_t)n) {
     const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
@@ -247,23 +276,14 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         int4 *last = nodes + 4 * (size_t)(n - 1);
         last[0] = z; last[1] = z; last[2] = z; last[3] = z;
       }
-      p = par[n + g];
+      p = L.par_leaf[tid];
       // A valid tree is at most 64 levels deep (each level consumes a bit of
       // the 62-bit position-augmented key); the step cap and the range
       // check keep a malformed parents array from running away.
       for (int step = 0; step < 128; ++step) {
         if (p < 0 || p >= (int32_t)n - 1) break;
-        const bool left = up_is_left(par, n, num, p);
-        side = left ? 0 : 1;
-        const int id = leaf ? -(int)(num + 1) : (int)num;
-        int4 *node = nodes + 4 * (size_t)p;
-        if (left) {
-          node[0] = pack3(mn, id);
-          node[1] = pack3(mx, 2);
-        } else {
-          node[2] = pack3(mn, id);
-          node[3] = pack3(mx, 0);
-        }
+        side = up_is_left(L, par, n, ts, te, num, p) ? 0 : 1;
+        self = leaf ? -(int)(num + 1) : (int)num;
         if (p < ts || p >= te) {  // the parent meets in HBM
           mode = 2;
           break;
@@ -274,6 +294,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
           L.box[side][k][slot] = mn[k];
           L.box[side][3 + k][slot] = mx[k];
         }
+        L.cid[side][slot] = self;
         const uint32_t old = __hip_atomic_fetch_add(
             &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0) {
@@ -288,13 +309,34 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         if (p == 0) break;  // root merged
         leaf = false;
         num = p;
-        p = par[p];
+        p = up_par_node(L, par, ts, te, p);
       }
     }
   }
   __syncthreads();
   // the first arriver whose sibling did not come on chip goes to HBM too
   if (mode == 1 && L.cnt[p - ts] == 1u) mode = 2;
+  if (mode == 2) {  // its half of p, which the sibling completes in HBM
+    int4 *node = nodes + 4 * (size_t)p;
+    node[2 * side] = pack3(mn, self);
+    node[2 * side + 1] = pack3(mx, side ? 0 : 2);
+  }
+  // The nodes both of whose children arrived on chip, in id order, four
+  // lanes per 64-byte node (scattered 16-byte half-node stores from the
+  // climbing lanes cost more than the rest of the pass).
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = j * kUpTile + tid;
+    const int kk = e >> 2, q = e & 3, sd = q >> 1;
+    if (ts + kk < te && L.cnt[kk] == 2u) {
+      const int c = (q & 1) * 3;  // min or max half of the side's box
+      const int w = (q & 1) ? (sd ? 0 : 2) : L.cid[sd][kk];
+      nodes[4 * (size_t)(ts + kk) + q] =
+          make_int4(__float_as_int(L.box[sd][c][kk]),
+                    __float_as_int(L.box[sd][c + 1][kk]),
+                    __float_as_int(L.box[sd][c + 2][kk]), w);
+    }
+  }
   if (mode != 2) return;
   // Through HBM from parent p, whose half this child has already written.
   for (int step = 0; step < 128; ++step) {
@@ -310,9 +352,9 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     }
     if (p == 0) return;  // root merged
     num = p;
-    p = par[p];
+    p = up_par_node(L, par, ts, te, p);
     if (p < 0 || p >= (int32_t)n - 1) return;
-    const bool left = up_is_left(par, n, num, p);
+    const bool left = up_is_left(L, par, n, ts, te, num, p);
     side = left ? 0 : 1;
     int4 *node = nodes + 4 * (size_t)p;
     if (left) {
