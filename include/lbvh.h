@@ -140,7 +140,8 @@ int lbvh_download(lbvh_ctx *ctx, void *dst_host, const void *src_dev,
 /* Scene bounds: fp32 min/max over every vertex coordinate, written to
  * bounds_out[0..5] (device) as min.xyz, max.xyz.  The reference takes
  * bounds from the caller (parse-obj.js:10-53 -> aabb-z-idx.js:113-124);
- * lbvh_build uses this reduction when the caller passes none. */
+ * lbvh_build uses this reduction when the caller passes none.  An empty
+ * mesh has no bounds: n = 0 fails with LBVH_ERR_INVALID_ARGUMENT. */
 int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
                 float *bounds_out);
 

@@ -351,6 +351,31 @@ def test_build_at_plan_boundaries(ctx, lb, ora, n):
         ctx.set_tuning(lb.TUNE_TREE_TILE, 0)
 
 
+def test_every_entry_point_with_no_triangles(ctx, lb):
+    """N = 0: zero-size work, no dispatch (build-bvh.js with NUM_TRIS = 0).
+    Every entry point returns OK and leaves its outputs untouched, except
+    the bounds of an empty mesh, which do not exist (invalid argument)."""
+    sentinel = np.full(64, 0x5A5A5A5A, np.uint32)
+    bufs = [ctx.to_device(sentinel) for _ in range(6)]
+    tri, a, b, c, d, e = bufs
+    with pytest.raises(lb.LBVHError) as err:
+        ctx.bounds(tri, 0, a)
+    assert err.value.code == lb.ERR_INVALID_ARGUMENT
+    ctx.aabb_morton(tri, 0, None, a, b)
+    ctx.aabb_morton(tri, 0, [0, 0, 0, 1, 1, 1], a, b)
+    ctx.sort(a, b, 0)
+    ctx.radix_tree(a, 0, b)
+    ctx.up_pass(a, b, c, 0, d)
+    ctx.rearrange(tri, a, 0, b)
+    ctx.build(tri, 0, None, a, b, c, d)
+    ctx.build(tri, 0, [0, 0, 0, 1, 1, 1], a, b)
+    ctx.refit(tri, 0, a, b, c, d)
+    ctx.synchronize()
+    for x in bufs:
+        assert np.array_equal(x.read(np.uint32, 64), sentinel)
+        x.destroy()
+
+
 def test_build_is_deterministic(ctx, lb):
     tri = lb.workload("uniform-64k")
     first = ctx.build_arrays(tri, None)[0]

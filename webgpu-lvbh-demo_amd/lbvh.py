@@ -26,6 +26,17 @@ DIV_IEEE = 1
 TUNE_TREE_TILE = 0
 TUNE_SORT_BITS = 1
 
+# lbvh_status (include/lbvh.h)
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_SIZE_MISMATCH = 2
+ERR_OUT_OF_BUDGET = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_DEVICE = 5
+ERR_NO_DEVICE = 6
+ERR_BUSY = 7
+ERR_NOT_FOUND = 8
+
 STATUS = {
     0: "ok",
     1: "invalid argument",
