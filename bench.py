@@ -260,6 +260,48 @@ def cpu_baseline(tri, reps=3):
     return min(times), times, threads
 
 
+def js_build_bench(tri, bounds, reps=5, log=print):
+    """initBVHBuild(device).build(TRI_ARRAY, N, MODEL_BOUNDS) from Node.js,
+    timed as main.js:87-90 times it: a wall clock around the awaited call,
+    so the upload from host memory, the output allocations and the build.
+    Runs bvh/time_build.js in a child process; None without node."""
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is None:
+        log("js build leg skipped: node not found")
+        return None
+    log("js build() wall time")
+    script = os.path.join(ROOT, "webgpu-lvbh-demo_amd", "bvh", "time_build.js")
+    tmp = tempfile.mkdtemp(prefix="lbvh_js_")
+    tri_path = os.path.join(tmp, "tri.f32")
+    bounds_path = os.path.join(tmp, "bounds.json")
+    try:
+        np.ascontiguousarray(tri, np.float32).tofile(tri_path)
+        with open(bounds_path, "w") as f:
+            json.dump({"min": bounds[:3], "max": bounds[3:]}, f)
+        r = subprocess.run([node, script, tri_path, str(len(tri)), bounds_path,
+                            str(reps)], capture_output=True, text=True, timeout=300)
+    finally:
+        for q in (tri_path, bounds_path):
+            if os.path.exists(q):
+                os.remove(q)
+        os.rmdir(tmp)
+    if r.returncode != 0:
+        raise RuntimeError("time_build.js failed: " + r.stdout + r.stderr)
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    best = out["best_ms"]
+    return {"ms": round(best, 3),
+            "Mtri_s": round(len(tri) / (best * 1e-3) / 1e6, 1),
+            "reps_ms": [round(x, 3) for x in out["ms"]],
+            "note": "Node.js initBVHBuild(device).build(Float32Array, N, "
+                    "MODEL_BOUNDS), wall clock around the awaited call as "
+                    "main.js:87-90 measures it: upload from host memory, "
+                    "output allocation, build; best of the repetitions after "
+                    "the first (which includes context set-up)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +316,8 @@ def main():
                     help="radix-sort digit width (8 or 10)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-js", action="store_true",
+                    help="skip the Node.js build() wall-time leg")
     ap.add_argument("--serving-requests", type=int, default=20000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB, the "
@@ -426,6 +470,10 @@ def main():
         b.destroy()
     ctx.close()
 
+    js = None
+    if not args.no_js:
+        js = js_build_bench(tri, hb, log=log)
+
     batched = None
     if not args.no_batched:
         batched = batched_bench(lb, hip, log=log)
@@ -490,6 +538,7 @@ def main():
                                    "memory + build, best of 3"},
         "profiled_ms_per_step": round(pms / args.steps, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "js_build": js,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,
