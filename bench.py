@@ -102,6 +102,21 @@ def timed_builds(lb, ctx, hip, d_tri, n, bounds, bvh, otri, steps):
     return ms, wall
 
 
+def per_build_ms(ctx, hip, d_tri, n, bounds, bvh, otri, runs):
+    """Device time of each of `runs` builds, enqueued back to back with an
+    event pair around every build; sorted milliseconds."""
+    ev = [hip.event() for _ in range(2 * runs)]
+    ctx.synchronize()
+    for i in range(runs):
+        hip.record(ev[2 * i], ctx.stream)
+        ctx.build(d_tri, n, bounds, bvh, otri)
+        hip.record(ev[2 * i + 1], ctx.stream)
+    out = sorted(hip.elapsed_ms(ev[2 * i], ev[2 * i + 1]) for i in range(runs))
+    for e in ev:
+        hip.h.hipEventDestroy(e)
+    return out
+
+
 def batched_bench(lb, hip, meshes=2048, steps=5, log=print):
     """2,048 independent meshes, log-uniform in [256, 262144] triangles, each a
     uniform mesh with its own seed: one graph-captured lbvh_build_batched vs a
@@ -382,6 +397,9 @@ def main():
     log(f"timed x{args.steps}")
     ms, wall = timed_builds(lb, ctx, hip, d_tri, n, None, bvh, otri, args.steps)
     ms_per_step = ms / args.steps
+    runs = max(20, args.steps)
+    log(f"per-build events x{runs}")
+    each = per_build_ms(ctx, hip, d_tri, n, None, bvh, otri, runs)
 
     log("profiled pass")
     ctx.profile_enable(True)
@@ -536,6 +554,10 @@ def main():
                            "Mtri_s": round(n / (host_ms * 1e-3) / 1e6, 1),
                            "note": "upload of the triangles from pageable host "
                                    "memory + build, best of 3"},
+        "per_build_ms": {"runs": runs, "median": round(float(np.median(each)), 4),
+                         "min": round(each[0], 4), "max": round(each[-1], 4),
+                         "note": "an event pair around each build, builds "
+                                 "enqueued back to back after the warm-ups"},
         "profiled_ms_per_step": round(pms / args.steps, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "js_build": js,
