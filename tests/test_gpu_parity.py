@@ -218,6 +218,53 @@ def test_stage_pipeline_matches_oracle(ctx, lb, ora):
             x.destroy()
 
 
+@pytest.mark.parametrize("n", [1024, 1025, 8192, 8193, 65536, 65537, 600001])
+def test_up_pass_at_level_boundaries(ctx, lb, ora, n):
+    """The per-stage up pass leaves each 1,024-leaf block's unfinished
+    subtrees as items and merges the items of 8 blocks per launch until one
+    block holds the root: 0, 1, 2, 3 item levels and ragged last groups."""
+    say(f"up pass levels n={n}")
+    tri = lb.wl_clustered(n, n, 0.05) if n % 2 else lb.wl_uniform(n, n, 0.01)
+    b = lb.wl_bounds(tri)
+    d = ctx.to_device(tri)
+    aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx, par, bvh = ctx.alloc(4 * n), ctx.alloc(8 * n), ctx.alloc(64 * n)
+    try:
+        ctx.aabb_morton(d, n, b, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.synchronize()
+        wb, _, _, _ = ora.build(tri, b, 0)
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb)
+    finally:
+        for x in (d, aabb, keys, idx, par, bvh):
+            x.destroy()
+
+
+def test_up_pass_all_equal_keys(ctx, lb, ora):
+    """20,000 copies of one triangle: every key equal, the tree splits by
+    position only."""
+    say("up pass all-equal n=20000")
+    tri = np.tile(lb.wl_uniform(31, 1, 0.01), (20000, 1))
+    n = len(tri)
+    b = lb.wl_bounds(tri)
+    d = ctx.to_device(tri)
+    aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx, par, bvh = ctx.alloc(4 * n), ctx.alloc(8 * n), ctx.alloc(64 * n)
+    try:
+        ctx.aabb_morton(d, n, b, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.synchronize()
+        wb, _, _, _ = ora.build(tri, b, 0)
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb)
+    finally:
+        for x in (d, aabb, keys, idx, par, bvh):
+            x.destroy()
+
+
 def test_stage_pipeline_heightfield_16m(ctx, lb, ora):
     """The per-stage sequence over the 16,777,216-triangle heightfield: the up
     pass meets most parents on chip and the ones on paths across its 1,024-leaf

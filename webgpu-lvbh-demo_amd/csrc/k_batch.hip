@@ -33,11 +33,10 @@ __device__ __forceinline__ uint32_t to_u32_sat(float v) {
   return (uint32_t)v;
 }
 
-// Bumps the epoch and (when the GPU computes the bounds) resets the
-// per-mesh bounds accumulators.
+// Resets the per-mesh bounds accumulators (when the GPU computes the
+// bounds).
 __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
-    uint32_t *seg_ord, uint32_t num_segs, int init_bounds, Ctl *ctl) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
+    uint32_t *seg_ord, uint32_t num_segs, int init_bounds) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (init_bounds)
@@ -185,7 +184,7 @@ void launch_batch_prologue(uint32_t *seg_ord, uint32_t num_segs,
   if (blocks > 1024) blocks = 1024;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_batch_prologue, dim3(blocks), dim3(kMapThreads), 0, s,
-                     seg_ord, num_segs, init_bounds ? 1 : 0, ctl);
+                     seg_ord, num_segs, init_bounds ? 1 : 0);
 }
 
 void launch_batch_bounds(const float4 *tri, const SortSegs &segs,

@@ -74,11 +74,10 @@ __device__ void block_minmax(float v[6]) {
 // lane.
 __global__ __launch_bounds__(kMapThreads) void k_bounds(
     const float4 *__restrict__ tri, uint32_t n, float4 *__restrict__ partials,
-    Ctl *ctl, int bump_epoch, float *__restrict__ cen) {
+    Ctl *ctl, float *__restrict__ cen) {
   static_assert(kMapThreads == 256, "a wave stores one component of a chunk");
   __shared__ float4 s_tri[3 * kMapThreads];
   __shared__ float s_cen[3][kMapThreads];
-  if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   const uint32_t t = threadIdx.x;
   const uint32_t chunks = (n + kMapThreads - 1) / kMapThreads;
@@ -161,13 +160,12 @@ template <int kBits>
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
-    uint32_t *__restrict__ keys, Ctl *ctl, int bump_epoch,
+    uint32_t *__restrict__ keys, Ctl *ctl,
     uint32_t *__restrict__ cnt) {
   constexpr int kR = 1 << kBits;
   static_assert(kSortTile % kMapThreads == 0, "tiles are whole chunks");
   __shared__ uint32_t s_hist[kR];
   __shared__ float4 s_tri[3 * kMapThreads];
-  if (bump_epoch && blockIdx.x == 0 && threadIdx.x == 0) ctl->epoch += 1;
   if (cnt)
     for (int d = threadIdx.x; d < kR; d += kMapThreads) s_hist[d] = 0u;
   float bmin[3], ext[3], rcp[3];
@@ -340,16 +338,14 @@ __global__ void k_copy_bounds(const Ctl *ctl, float *out) {
   else if (t < 6) out[t] = ctl->bounds[t + 1];
 }
 
-__global__ void k_bump_epoch(Ctl *ctl) { ctl->epoch += 1; }
-
 uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
-                       Ctl *ctl, bool bump_epoch, hipStream_t s, float *cen,
+                       Ctl *ctl, hipStream_t s, float *cen,
                        uint32_t fold_max) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kBoundsBlocks) blocks = kBoundsBlocks;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
-                     partials, ctl, bump_epoch ? 1 : 0, cen);
+                     partials, ctl, cen);
   if (blocks <= fold_max) return blocks;  // the consumer folds the partials
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kMapThreads), 0, s,
                      partials, blocks, ctl);
@@ -358,8 +354,7 @@ uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
 
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
-                        Ctl *ctl, bool bump_epoch, hipStream_t s,
-                        uint32_t *cnt, int bits) {
+                        Ctl *ctl, hipStream_t s, uint32_t *cnt, int bits) {
   uint32_t blocks = (n + kMapThreads - 1) / kMapThreads;
   if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
   if (cnt) blocks = sort_tiles(n);
@@ -367,13 +362,11 @@ void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
   if (cnt && bits == kWideBits)
     hipLaunchKernelGGL(k_aabb_morton<kWideBits>, dim3(blocks),
                        dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
-                       bump_epoch ? 1 : 0, cnt);
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt);
   else
     hipLaunchKernelGGL(k_aabb_morton<kRadixBits>, dim3(blocks),
                        dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl,
-                       bump_epoch ? 1 : 0, cnt);
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt);
 }
 
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
@@ -392,10 +385,6 @@ void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_bounds, dim3(1), dim3(64), 0, s, ctl, out);
-}
-
-void launch_bump_epoch(Ctl *ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_bump_epoch, dim3(1), dim3(1), 0, s, ctl);
 }
 
 }  // namespace lbvh

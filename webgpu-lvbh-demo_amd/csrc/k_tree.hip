@@ -115,68 +115,6 @@ __device__ void write_single_leaf_node(int4 *nodes, const float mn[3],
   nodes[3] = make_int4(0, 0, 0, 0);
 }
 
-// Exchanges this side of split s through HBM.  Each side record is seven
-// 64-bit words (box min.xyz, max.xyz, far end), each carrying the operation's
-// epoch in its high half and written with a relaxed agent-scope atomic store.
-// The arrival tag is swapped relaxed too, so no cache-wide release/acquire
-// fence is issued: the second arriver instead polls the sibling's words until
-// all show the current epoch, the seven loads in flight together (one
-// round trip per poll, not seven: a chain of merges pays it at every
-// level).  The sibling stored those words before its own swap, which this
-// thread has already observed, so the polling ends; the cap only bounds the
-// loop should that ever break (ctl->fault is then set).
-// Returns true, with the sibling's box and far end, for the second arriver.
-__device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
-                                              Ctl *ctl, uint32_t epoch,
-                                              int64_t s, int side,
-                                              int far_self, const float mn[3],
-                                              const float mx[3], float smn[3],
-                                              float smx[3], int &sfar) {
-  uint64_t *mine = gside[2 * s + side].w;
-  const uint64_t stamp = (uint64_t)epoch << 32;
-  const uint32_t v[7] = {__float_as_uint(mn[0]), __float_as_uint(mn[1]),
-                         __float_as_uint(mn[2]), __float_as_uint(mx[0]),
-                         __float_as_uint(mx[1]), __float_as_uint(mx[2]),
-                         (uint32_t)far_self};
-#pragma unroll
-  for (int k = 0; k < 7; ++k)
-    __hip_atomic_store(mine + k, stamp | v[k], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  // Stores issued and acknowledged before the swap (no cache maintenance at
-  // workgroup scope), so the sibling normally finds them on its first poll.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  const uint32_t old = __hip_atomic_exchange(gtag + s, epoch, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-  if (old != epoch) return false;
-  const uint64_t *other = gside[2 * s + (1 - side)].w;
-  uint32_t o[7];
-  uint32_t spins = 0;
-  while (true) {
-    uint64_t w[7];
-#pragma unroll
-    for (int k = 0; k < 7; ++k)
-      w[k] = __hip_atomic_load(other + k, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    bool all = true;
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-      all = all && (uint32_t)(w[k] >> 32) == epoch;
-      o[k] = (uint32_t)w[k];
-    }
-    if (all) break;
-    if (++spins > kSpinCap) {
-      atomicOr(&ctl->fault, 2u);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-  smn[0] = __uint_as_float(o[0]); smn[1] = __uint_as_float(o[1]);
-  smn[2] = __uint_as_float(o[2]); smx[0] = __uint_as_float(o[3]);
-  smx[1] = __uint_as_float(o[4]); smx[2] = __uint_as_float(o[5]);
-  sfar = (int)o[6];
-  return true;
-}
-
 // ---------------------------------------------------------------------------
 // Per-stage up pass (bvh_upward_pass, bvh-up-pass.js:200-280)
 // ---------------------------------------------------------------------------
@@ -190,18 +128,25 @@ __device__ __forceinline__ bool arrive_global(GSide *gside, uint32_t *gtag,
 // A block takes kUpTile consecutive leaves.  Parents whose id lies in the
 // block's range meet in LDS (per-side box slots and an arrival count); a
 // subtree that lies inside the range has all its node ids there, so it
-// completes on chip.  After one barrier, the climbs that stopped at a node
-// whose other child never came (it reaches outside the range), or at a
-// parent whose id is outside the range, continue through the epoch-stamped
-// exchange in HBM (arrive_global, keyed by the parent's id).  Only the nodes
-// on the paths across block boundaries go through HBM: every node did, at
-// a few device-scope atomics each, before.
+// completes on chip.  The subtrees that cannot complete there -- the
+// block's maximal complete subtrees, at most two per level of the tree, so
+// at most kUpFront -- leave as items, in leaf order, for k_up_items, which
+// merges the items of kUpFanIn neighbouring blocks the same way, level
+// after level, until one block holds the root.  No block ever waits for
+// another: kernel boundaries order the levels.
 constexpr int kUpTile = 1024;
+constexpr int kUpFront = 128;  // items a block leaves at most (2 per level)
+constexpr int kUpFanIn = 8;    // blocks of the level below per item block
+static_assert(kUpFanIn * kUpFront <= kUpTile, "one thread per item");
 
 struct UpLds {
   float box[2][6][kUpTile];  // [side][min.xyz max.xyz][parent id - block start]
   int32_t cid[2][kUpTile];   // [side][parent id - block start]: child id
   uint32_t cnt[kUpTile];     // arrivals per parent
+  // [side][parent id - block start]: the child's far leaf (its first leaf
+  // when it is the left child, its last when the right one), which the
+  // second arriver takes as the merged range's other end
+  int32_t far[2][kUpTile];
   // the block's parents entries, ids ts .. te (one past the block, for the
   // sibling test at its edge): a climb reads these at every step, and from
   // global memory they made each step a chain of dependent loads
@@ -234,11 +179,68 @@ __device__ __forceinline__ bool up_is_left(const UpLds &L,
          (sib <= (int64_t)n - 2 && up_par_node(L, par, ts, te, sib) == p);
 }
 
+// An item: a complete subtree that has still to arrive at its parent.
+//   [0] child id as the parent lists it, first leaf, last leaf, parent id
+//   [1] box min.xyz, side at the parent (0: left child, 1: right child)
+//   [2] box max.xyz, 0
+__device__ __forceinline__ void up_put_item(int4 *slot, int self, int32_t lo,
+                                            int32_t hi, int32_t p, int side,
+                                            const float mn[3],
+                                            const float mx[3]) {
+  slot[0] = make_int4(self, lo, hi, p);
+  slot[1] = pack3(mn, side);
+  slot[2] = pack3(mx, 0);
+}
+
+// up_is_left from global memory only.
+__device__ __forceinline__ bool up_is_left_g(const int32_t *__restrict__ par,
+                                             uint32_t n, int64_t num,
+                                             int32_t p) {
+  const int64_t sib = num + 1;
+  return (sib <= (int64_t)n - 1 && par[n + sib] == p) ||
+         (sib <= (int64_t)n - 2 && par[sib] == p);
+}
+
+// Writes the block's pending items in leaf order: each pending thread
+// passes its subtree's position in the block (0 .. kUpTile-1), a thread
+// that is not pending -1.  mark is kUpTile words of free LDS.  count_out
+// gets the number written; more than kUpFront, or any at all when `last`
+// (the block held the whole tree), sets ctl->fault.
+__device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
+                                        int at, int4 *items,
+                                        uint32_t *count_out, Ctl *ctl,
+                                        int last, int self, int32_t lo,
+                                        int32_t hi, int32_t p, int side,
+                                        const float mn[3],
+                                        const float mx[3]) {
+  const int tid = threadIdx.x;
+  mark[tid] = 0u;
+  __syncthreads();
+  if (at >= 0) mark[at] = 1u;
+  __syncthreads();
+  const uint32_t v = mark[tid];
+  const uint32_t pos = block_exclusive_scan(v, s_wsum);
+  mark[tid] = pos;
+  if (tid == kUpTile - 1) {
+    const uint32_t total = pos + v;
+    *count_out = total < (uint32_t)kUpFront ? total : (uint32_t)kUpFront;
+    if (total > (uint32_t)kUpFront || (last && total > 0)) atomicOr(&ctl->fault, 4u);
+  }
+  __syncthreads();
+  if (at >= 0) {
+    const uint32_t q = mark[at];
+    if (q < (uint32_t)kUpFront)
+      up_put_item(items + 3 * (size_t)q, self, lo, hi, p, side, mn, mx);
+  }
+}
+
 __global__ __launch_bounds__(kUpTile) void k_up_pass(
     const int32_t *__restrict__ idx, const int4 *__restrict__ aabb,
-    const int32_t *__restrict__ par, uint32_t n, int4 *nodes, GSide *gside,
-    uint32_t *tags, Ctl *ctl) {
+    const int32_t *__restrict__ par, uint32_t n, int4 *nodes,
+    int4 *__restrict__ items, uint32_t *__restrict__ item_cnt, Ctl *ctl,
+    int last) {
   __shared__ UpLds L;
+  __shared__ uint32_t s_wsum[16];
   const int tid = threadIdx.x;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t ts = (int64_t)tile * kUpTile;
@@ -251,30 +253,30 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     L.par_node[t] = id < (int64_t)n ? par[id] : -1;
   }
   __syncthreads();
-  const uint32_t epoch = ctl->epoch;
-  // 0: done, 1: first to arrive at a parent in LDS, 2: continue in HBM
+  // 0: done, 1: first to arrive at a parent in LDS, 2: pending (an item)
   int mode = 0;
   bool leaf = true;
   int64_t num = g;  // leaf position or internal node id
   int32_t p = -1;
   int side = 0;
   int self = 0;  // this climber's child id at p
+  int32_t lo = (int32_t)g, hi = (int32_t)g;  // the subtree's leaves
   float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
   if (g < (int64_t)n) {
     const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
-    const int4 lo = aabb[2 * (size_t)src], hi = aabb[2 * (size_t)src + 1];
-    mn[0] = __int_as_float(lo.x); mn[1] = __int_as_float(lo.y);
-    mn[2] = __int_as_float(lo.z);
-    mx[0] = __int_as_float(hi.x); mx[1] = __int_as_float(hi.y);
-    mx[2] = __int_as_float(hi.z);
+    const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
+    mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
+    mn[2] = __int_as_float(bl.z);
+    mx[0] = __int_as_float(bh.x); mx[1] = __int_as_float(bh.y);
+    mx[2] = __int_as_float(bh.z);
     perturb(mn, mx);
     if (n == 1) {
       write_single_leaf_node(nodes, mn, mx);
     } else {
       if (g == (int64_t)n - 1) {
         const int4 z = make_int4(0, 0, 0, 0);
-        int4 *last = nodes + 4 * (size_t)(n - 1);
-        last[0] = z; last[1] = z; last[2] = z; last[3] = z;
+        int4 *lastn = nodes + 4 * (size_t)(n - 1);
+        lastn[0] = z; lastn[1] = z; lastn[2] = z; lastn[3] = z;
       }
       p = L.par_leaf[tid];
       // A valid tree is at most 64 levels deep (each level consumes a bit of
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         if (p < 0 || p >= (int32_t)n - 1) break;
         side = up_is_left(L, par, n, ts, te, num, p) ? 0 : 1;
         self = leaf ? -(int)(num + 1) : (int)num;
-        if (p < ts || p >= te) {  // the parent meets in HBM
+        if (p < ts || p >= te) {  // the parent lies outside the block
           mode = 2;
           break;
         }
@@ -295,6 +297,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
           L.box[side][3 + k][slot] = mx[k];
         }
         L.cid[side][slot] = self;
+        L.far[side][slot] = side == 0 ? lo : hi;
         const uint32_t old = __hip_atomic_fetch_add(
             &L.cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0) {
@@ -306,6 +309,8 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
           mn[k] = fminf(mn[k], L.box[1 - side][k][slot]);
           mx[k] = fmaxf(mx[k], L.box[1 - side][3 + k][slot]);
         }
+        if (side == 0) hi = L.far[1][slot];
+        else lo = L.far[0][slot];
         if (p == 0) break;  // root merged
         leaf = false;
         num = p;
@@ -314,13 +319,10 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     }
   }
   __syncthreads();
-  // the first arriver whose sibling did not come on chip goes to HBM too
+  // a first arriver whose sibling did not come on chip is pending too (the
+  // sibling reaches outside the block); an item writes its half of the
+  // parent when it arrives there at a later level
   if (mode == 1 && L.cnt[p - ts] == 1u) mode = 2;
-  if (mode == 2) {  // its half of p, which the sibling completes in HBM
-    int4 *node = nodes + 4 * (size_t)p;
-    node[2 * side] = pack3(mn, self);
-    node[2 * side + 1] = pack3(mx, side ? 0 : 2);
-  }
   // The nodes both of whose children arrived on chip, in id order, four
   // lanes per 64-byte node (scattered 16-byte half-node stores from the
   // climbing lanes cost more than the rest of the pass).
@@ -337,34 +339,124 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
                     __float_as_int(L.box[sd][c + 2][kk]), w);
     }
   }
-  if (mode != 2) return;
-  // Through HBM from parent p, whose half this child has already written.
-  for (int step = 0; step < 128; ++step) {
-    float smn[3], smx[3];
-    int unused;
-    if (!arrive_global(gside, tags, ctl, epoch, p, side, 0, mn, mx, smn, smx,
-                       unused))
-      return;  // first arriver
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      mn[k] = fminf(mn[k], smn[k]);
-      mx[k] = fmaxf(mx[k], smx[k]);
+  // the parents entries are no longer read: their words order the items
+  up_emit(reinterpret_cast<uint32_t *>(L.par_node), s_wsum,
+          mode == 2 ? (int)(lo - ts) : -1,
+          items + 3 * (size_t)tile * kUpFront, item_cnt + tile, ctl, last,
+          self, lo, hi, p, side, mn, mx);
+}
+
+// One level of items: block b takes the items of blocks [kUpFanIn b,
+// kUpFanIn b + kUpFanIn) of the level below, in leaf order (at most
+// kUpFanIn * kUpFront, one thread each).  Those items partition a leaf
+// range, and the split of every node still to complete inside it is a gap
+// between two adjacent items: arrivals meet in that gap's LDS slot (the left
+// child at the gap after its last item, the right child at the gap before
+// its first).  Each arrival writes its half of the parent node; the second
+// one merges and climbs on.  Subtrees whose sibling reaches outside the
+// range are the level's items.
+struct UpItemLds {
+  float box[2][6][kUpTile];   // [side][min.xyz max.xyz][gap]
+  int32_t far_item[2][kUpTile];  // [side][gap]: far item index (as far)
+  int32_t far_leaf[2][kUpTile];  // [side][gap]: far leaf
+  uint32_t cnt[kUpTile];      // arrivals per gap
+  uint32_t mark[kUpTile];     // item ordering
+  uint32_t base[kUpFanIn + 1];
+};
+
+__global__ __launch_bounds__(kUpTile) void k_up_items(
+    const int4 *__restrict__ items_in, const uint32_t *__restrict__ cnt_in,
+    uint32_t tiles_in, int4 *__restrict__ items_out,
+    uint32_t *__restrict__ cnt_out, const int32_t *__restrict__ par,
+    uint32_t n, int4 *nodes, Ctl *ctl, int last) {
+  __shared__ UpItemLds S;
+  __shared__ uint32_t s_wsum[16];
+  const int tid = threadIdx.x;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t c0 = tile * kUpFanIn;
+  const uint32_t nc = min((uint32_t)kUpFanIn, tiles_in - c0);
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (uint32_t j = 0; j < nc; ++j) {
+      S.base[j] = run;
+      const uint32_t c = cnt_in[c0 + j];
+      run += c < (uint32_t)kUpFront ? c : (uint32_t)kUpFront;
     }
-    if (p == 0) return;  // root merged
-    num = p;
-    p = up_par_node(L, par, ts, te, p);
-    if (p < 0 || p >= (int32_t)n - 1) return;
-    const bool left = up_is_left(L, par, n, ts, te, num, p);
-    side = left ? 0 : 1;
-    int4 *node = nodes + 4 * (size_t)p;
-    if (left) {
-      node[0] = pack3(mn, (int)num);
-      node[1] = pack3(mx, 2);
-    } else {
-      node[2] = pack3(mn, (int)num);
-      node[3] = pack3(mx, 0);
+    S.base[nc] = run;
+  }
+  S.cnt[tid] = 0u;
+  __syncthreads();
+  const int m = (int)S.base[nc];
+  int mode = 0;  // 0: done, 1: first arriver, 2: pending (an item)
+  int self = 0, side = 0;
+  int32_t p = -1, lo = 0, hi = 0;
+  int i0 = tid, i1 = tid;  // the subtree's items
+  float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+  if (tid < m) {
+    uint32_t j = 0;
+    while (j + 1 < nc && S.base[j + 1] <= (uint32_t)tid) ++j;
+    const int4 *it =
+        items_in + 3 * ((size_t)(c0 + j) * kUpFront + (tid - S.base[j]));
+    const int4 a = it[0], b = it[1], c = it[2];
+    self = a.x; lo = a.y; hi = a.z; p = a.w;
+    mn[0] = __int_as_float(b.x); mn[1] = __int_as_float(b.y);
+    mn[2] = __int_as_float(b.z); side = b.w;
+    mx[0] = __int_as_float(c.x); mx[1] = __int_as_float(c.y);
+    mx[2] = __int_as_float(c.z);
+    mode = 2;
+    for (int step = 0; step < 128; ++step) {
+      if (p < 0 || p >= (int32_t)n - 1) {
+        mode = 0;
+        break;
+      }
+      const int gap = side == 0 ? i1 : i0 - 1;
+      if (gap < 0 || gap >= m - 1) {  // the sibling lies outside the range
+        mode = 2;
+        break;
+      }
+      int4 *node = nodes + 4 * (size_t)p;
+      node[2 * side] = pack3(mn, self);
+      node[2 * side + 1] = pack3(mx, side ? 0 : 2);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        S.box[side][k][gap] = mn[k];
+        S.box[side][3 + k][gap] = mx[k];
+      }
+      S.far_item[side][gap] = side == 0 ? i0 : i1;
+      S.far_leaf[side][gap] = side == 0 ? lo : hi;
+      const uint32_t old = __hip_atomic_fetch_add(
+          &S.cnt[gap], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) {
+        mode = 1;
+        break;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
+        mx[k] = fmaxf(mx[k], S.box[1 - side][3 + k][gap]);
+      }
+      if (side == 0) {
+        i1 = S.far_item[1][gap];
+        hi = S.far_leaf[1][gap];
+      } else {
+        i0 = S.far_item[0][gap];
+        lo = S.far_leaf[0][gap];
+      }
+      if (p == 0) {  // root merged
+        mode = 0;
+        break;
+      }
+      const int32_t num = p;
+      self = num;
+      p = par[num];
+      side = up_is_left_g(par, n, num, p) ? 0 : 1;
     }
   }
+  __syncthreads();
+  if (mode == 1 && S.cnt[side == 0 ? i1 : i0 - 1] == 1u) mode = 2;
+  up_emit(S.mark, s_wsum, mode == 2 ? i0 : -1,
+          items_out + 3 * (size_t)tile * kUpFront, cnt_out + tile, ctl, last,
+          self, lo, hi, p, side, mn, mx);
 }
 
 // rearrange_triangles (rearrange.js:162-169): whole 48-byte records, stored
@@ -954,13 +1046,42 @@ void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
                      dim3(kMapThreads), 0, s, keys, n, parents);
 }
 
+uint64_t up_pass_scratch_bytes(uint32_t n) {
+  uint64_t tiles = blocks_for(n, kUpTile), items = 0, counts = 0;
+  while (true) {
+    items += tiles * kUpFront;
+    counts += tiles;
+    if (tiles == 1) break;
+    tiles = (tiles + kUpFanIn - 1) / kUpFanIn;
+  }
+  return 48 * items + 4 * counts;
+}
+
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     const int32_t *parents, uint32_t n, float4 *nodes,
-                    GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s) {
-  hipLaunchKernelGGL(k_up_pass, dim3(blocks_for(n, kUpTile)), dim3(kUpTile),
-                     0, s, idx,
-                     reinterpret_cast<const int4 *>(aabb), parents, n,
-                     reinterpret_cast<int4 *>(nodes), gside, tags, ctl);
+                    void *scratch, Ctl *ctl, hipStream_t s) {
+  uint32_t tiles = blocks_for(n, kUpTile);
+  uint64_t total_items = 0;
+  for (uint32_t t = tiles;; t = (t + kUpFanIn - 1) / kUpFanIn) {
+    total_items += (uint64_t)t * kUpFront;
+    if (t == 1) break;
+  }
+  int4 *items = static_cast<int4 *>(scratch);
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(items + 3 * total_items);
+  int4 *nd = reinterpret_cast<int4 *>(nodes);
+  hipLaunchKernelGGL(k_up_pass, dim3(tiles), dim3(kUpTile), 0, s, idx,
+                     reinterpret_cast<const int4 *>(aabb), parents, n, nd,
+                     items, cnt, ctl, tiles == 1 ? 1 : 0);
+  while (tiles > 1) {
+    const uint32_t next = (tiles + kUpFanIn - 1) / kUpFanIn;
+    int4 *out = items + 3 * (size_t)tiles * kUpFront;
+    hipLaunchKernelGGL(k_up_items, dim3(next), dim3(kUpTile), 0, s, items,
+                       cnt, tiles, out, cnt + tiles, parents, n, nd, ctl,
+                       next == 1 ? 1 : 0);
+    items = out;
+    cnt += tiles;
+    tiles = next;
+  }
 }
 
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,

@@ -52,12 +52,10 @@ struct lbvh_ctx {
   std::recursive_mutex mu;
   // device state
   Ctl *ctl = nullptr;
-  uint64_t epoch_ops = 0;  // epoch bumps issued since the last rewind
   DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
   DevBuf digits;                        // sort tile counts + offsets
-  DevBuf gside;                         // per-stage up-pass exchange
+  DevBuf up_items;                      // per-stage up-pass items and counts
   DevBuf tree_items, tree_counts;       // agglomeration item slots, counts
-  DevBuf ctag;                          // per-stage up-pass tags (zeroed)
   DevBuf partials;                      // bounds partials
   DevBuf cen;                           // triangle box centres (GPU bounds)
   // batched builds: scratch, and the plan (tile tables + captured graph) of
@@ -252,9 +250,7 @@ void dev_release(lbvh_ctx *ctx, DevBuf &b) {
   b.bytes = 0;
 }
 
-// Grow-only scratch buffer.  Buffers whose contents carry epoch stamps are
-// zeroed when (re)allocated, which keeps every stale stamp older than the
-// current epoch.
+// Grow-only scratch buffer (zeroed when (re)allocated if `zero`).
 int ensure(lbvh_ctx *ctx, DevBuf &b, uint64_t bytes, bool zero) {
   if (b.bytes >= bytes) return LBVH_OK;
   // Grow with headroom so that slowly increasing sizes do not reallocate on
@@ -272,21 +268,6 @@ int ensure(lbvh_ctx *ctx, DevBuf &b, uint64_t bytes, bool zero) {
   b.ptr = p;
   b.bytes = want;
   if (zero) HIP_TRY(ctx, hipMemsetAsync(p, 0, want, ctx->stream));
-  return LBVH_OK;
-}
-
-// The per-stage up pass stamps its exchange words with the epoch.  Long
-// before it wraps, the stamped buffers are cleared and the epoch restarts.
-constexpr uint64_t kEpochRewind = 1ull << 26;
-
-int begin_op(lbvh_ctx *ctx) {
-  if (++ctx->epoch_ops < kEpochRewind) return LBVH_OK;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  for (DevBuf *b : {&ctx->ctag})
-    if (b->ptr) HIP_TRY(ctx, hipMemsetAsync(b->ptr, 0, b->bytes, ctx->stream));
-  HIP_TRY(ctx, hipMemsetAsync(&ctx->ctl->epoch, 0, sizeof(uint32_t),
-                              ctx->stream));
-  ctx->epoch_ops = 1;
   return LBVH_OK;
 }
 
@@ -325,8 +306,8 @@ struct StageScope {
 
 std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
   return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
-          &ctx->digits,  &ctx->gside,    &ctx->tree_items,
-          &ctx->tree_counts, &ctx->ctag, &ctx->partials, &ctx->cen,
+          &ctx->digits,  &ctx->up_items,    &ctx->tree_items,
+          &ctx->tree_counts, &ctx->partials, &ctx->cen,
           &ctx->b_keys_a,
           &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
           &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_tree_items,
@@ -875,8 +856,10 @@ int lbvh_synchronize(lbvh_ctx *ctx) {
   if (fault) {
     HIP_TRY(ctx, hipMemset(&ctx->ctl->fault, 0, 4));
     return fail(ctx, LBVH_ERR_DEVICE,
-                "a sort look-back wait exceeded its bound; results of the "
-                "operations since the last synchronize are invalid");
+                "a tree kernel found an inconsistent parents array or "
+                "frontier (a frontier overflowed or a tree did not close); "
+                "results of the operations since the last synchronize are "
+                "invalid");
   }
   return LBVH_OK;
 }
@@ -965,9 +948,8 @@ int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bounds of an empty mesh");
   if ((st = ensure_partials(ctx))) return st;
-  if ((st = begin_op(ctx))) return st;
   launch_bounds(reinterpret_cast<const float4 *>(tri), n,
-                static_cast<float4 *>(ctx->partials.ptr), ctx->ctl, true,
+                static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
                 ctx->stream);
   launch_copy_bounds(ctx->ctl, bounds_out, ctx->stream);
   return check_launch(ctx, "lbvh_bounds");
@@ -982,7 +964,6 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !keys_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = begin_op(ctx))) return st;
   MortonParams p{};
   if (bounds_host) {
     p = morton_params(ctx, bounds_host);
@@ -990,13 +971,13 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
     if ((st = ensure_partials(ctx))) return st;
     p.div_mode = ctx->div_mode;
     launch_bounds(reinterpret_cast<const float4 *>(tri), n,
-                  static_cast<float4 *>(ctx->partials.ptr), ctx->ctl, true,
+                  static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
                   ctx->stream);
   }
   launch_aabb_morton(reinterpret_cast<const float4 *>(tri), n, p,
                      bounds_host == nullptr,
                      reinterpret_cast<float4 *>(aabb_out), keys_out, ctx->ctl,
-                     bounds_host != nullptr, ctx->stream);
+                     ctx->stream);
   return check_launch(ctx, "lbvh_aabb_morton");
 }
 
@@ -1015,7 +996,6 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
   // 10-bit digits take 4 passes here too, so they run only when fixed.
   const int bits = ctx->sort_bits_fixed ? ctx->sort_bits : kRadixBits;
   if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
-  if ((st = begin_op(ctx))) return st;
   run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
            static_cast<uint32_t *>(ctx->keys_a.ptr),
            static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false, bits,
@@ -1043,14 +1023,11 @@ int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
   if (n == 0) return LBVH_OK;
   if (!idx || !aabb || !parents || !bvh_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = ensure(ctx, ctx->ctag, 4ull * n, true))) return st;
-  if ((st = ensure(ctx, ctx->gside, 2ull * sizeof(GSide) * n, false))) return st;
-  if ((st = begin_op(ctx))) return st;
-  launch_bump_epoch(ctx->ctl, ctx->stream);
+  if ((st = ensure(ctx, ctx->up_items, up_pass_scratch_bytes(n), false)))
+    return st;
   launch_up_pass(idx, reinterpret_cast<const float4 *>(aabb), parents, n,
-                 static_cast<float4 *>(bvh_out),
-                 static_cast<GSide *>(ctx->gside.ptr),
-                 static_cast<uint32_t *>(ctx->ctag.ptr), ctx->ctl, ctx->stream);
+                 static_cast<float4 *>(bvh_out), ctx->up_items.ptr, ctx->ctl,
+                 ctx->stream);
   return check_launch(ctx, "lbvh_up_pass");
 }
 
@@ -1089,7 +1066,6 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
   if (use_cen && (st = ensure(ctx, ctx->cen, 12ull * cen_stride(n), false)))
     return st;
-  if ((st = begin_op(ctx))) return st;
 
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
   float *cen = use_cen ? static_cast<float *>(ctx->cen.ptr) : nullptr;
@@ -1107,7 +1083,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   } else {
     p.div_mode = ctx->div_mode;
     StageScope sc(ctx, LBVH_STAGE_BOUNDS);
-    parts = launch_bounds(t4, n, partials, ctx->ctl, true, ctx->stream, cen,
+    parts = launch_bounds(t4, n, partials, ctx->ctl, ctx->stream, cen,
                           use_cen ? kFoldPartialsMax : 0);
   }
   {
@@ -1118,8 +1094,7 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
                         ctx->stream, partials, parts);
     else
       launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                         ctx->ctl, bounds_host != nullptr, ctx->stream, cnt,
-                         bits);
+                         ctx->ctl, ctx->stream, cnt, bits);
   }
   if (sort) {
     run_sort(ctx, keys, tmp, skeys, nullptr,
@@ -1196,7 +1171,6 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                                 24ull * num_meshes, hipMemcpyHostToDevice,
                                 ctx->stream));
   }
-  if ((st = begin_op(ctx))) return st;
   // Replay the captured graph when the batch layout, buffers and options
   // match the last call; otherwise capture the pipeline anew.
   std::vector<const void *> key = {

@@ -42,33 +42,23 @@ constexpr int kItemInt4 = 3;            // 48-byte items: range, box, gap deltas
 constexpr int kMapThreads = 256;  // triangle-parallel kernels
 constexpr int kBoundsBlocks = 2048;  // grid of the bounds reduction
 constexpr int kMortonBlocksMax = 2048;  // grid-stride Morton (per-stage entry)
-constexpr uint32_t kSpinCap = 1u << 22;  // polls of a bounded wait (per-stage up pass)
 
 // ---- device control block ------------------------------------------------
 //
 // One per context, zeroed at allocation.  Nothing in it needs a reset
 // between operations (or graph replays):
-//   epoch       bumped once per operation by its first kernel; stamps the
-//               arrival tags of the per-stage up pass
-//   fault       sticky flag: a bounded wait gave up or a frontier overflowed
+//   fault       sticky flag: a frontier overflowed or a tree did not close
 //               (never set in a correct run; read by lbvh_synchronize)
 //   bounds      scene bounds of the last reduction (k_bounds_final)
 //   hist        global digit counts of each pass of the current sort (pass
 //               p's at hist + p * radix)
 struct Ctl {
-  uint32_t epoch;
   uint32_t fault;
-  uint32_t pad0[6];
+  uint32_t pad0[7];
   float bounds[8];                       // min.xyz, 0, max.xyz, 0
   uint32_t hist[kHistWords];             // global digit counts per pass
 };
 
-// Exchange record of one side of a split in the fused up pass: the side's box
-// (min.xyz, max.xyz) and the far end of its leaf range, one value per 64-bit
-// word with the operation's epoch in the high half.  Two sides per split.
-struct __align__(64) GSide {
-  uint64_t w[8];  // 7 used
-};
 
 // Morton quantisation parameters (uniform block of aabb-z-idx.js:164-169).
 struct MortonParams {
@@ -86,8 +76,7 @@ struct MortonParams {
 // at most fold_max partials, the folding launch is skipped and the partial
 // count is returned (launch_morton_cen folds them); otherwise 0.
 uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
-                       Ctl *ctl, bool bump_epoch, hipStream_t s,
-                       float *cen = nullptr, uint32_t fold_max = 0);
+                       Ctl *ctl, hipStream_t s, float *cen = nullptr, uint32_t fold_max = 0);
 // Morton keys + per-tile digit counts (as launch_aabb_morton with cnt) from
 // the centres launch_bounds wrote and the bounds in ctl, or folded from
 // `parts` bounds partials when parts != 0.
@@ -107,10 +96,8 @@ __host__ __device__ constexpr uint32_t cen_stride(uint32_t n) {
 // first sort pass's per-tile digit counts for digits of `bits` bits.
 void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
                         bool bounds_from_ctl, float4 *aabb, uint32_t *keys,
-                        Ctl *ctl, bool bump_epoch, hipStream_t s,
-                        uint32_t *cnt = nullptr, int bits = kRadixBits);
+                        Ctl *ctl, hipStream_t s, uint32_t *cnt = nullptr, int bits = kRadixBits);
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s);
-void launch_bump_epoch(Ctl *ctl, hipStream_t s);
 
 // k_sort.hip
 // Segments of a batched sort (all null: one segment [0, n)).  Tiles of
@@ -143,9 +130,12 @@ inline uint32_t sort_tiles(uint32_t n) {
 // k_tree.hip
 void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
                        hipStream_t s);
+// Per-stage up pass: the level-1 kernel, then item levels until one block
+// holds the root; scratch of up_pass_scratch_bytes(n) (items, then counts).
+uint64_t up_pass_scratch_bytes(uint32_t n);
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
                     const int32_t *parents, uint32_t n, float4 *nodes,
-                    GSide *gside, uint32_t *tags, Ctl *ctl, hipStream_t s);
+                    void *scratch, Ctl *ctl, hipStream_t s);
 void launch_rearrange(const float4 *tri, const int32_t *idx, uint32_t n,
                       float4 *out, hipStream_t s);
 // Segments of one agglomeration level of a batched build (all null: one

@@ -438,9 +438,9 @@ void dispatcher(lbvh_server *s) {
         if (st == LBVH_OK) {
           const hipError_t q = hipEventQuery(w.ev);
           if (q == hipErrorNotReady) break;
-          // (Kernel-side bounded-wait faults are sticky per slot and
-          // reported by lbvh_server_drain, which keeps this path free of
-          // device reads.)
+          // (Kernel-side faults, such as a frontier overflow, are sticky
+          // per slot and reported by lbvh_server_drain, which keeps this
+          // path free of device reads.)
           st = q == hipSuccess ? LBVH_OK : LBVH_ERR_DEVICE;
         }
         finish(s, w, st, k);
