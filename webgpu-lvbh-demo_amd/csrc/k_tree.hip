@@ -192,15 +192,6 @@ __device__ __forceinline__ void up_put_item(int4 *slot, int self, int32_t lo,
   slot[2] = pack3(mx, 0);
 }
 
-// up_is_left from global memory only.
-__device__ __forceinline__ bool up_is_left_g(const int32_t *__restrict__ par,
-                                             uint32_t n, int64_t num,
-                                             int32_t p) {
-  const int64_t sib = num + 1;
-  return (sib <= (int64_t)n - 1 && par[n + sib] == p) ||
-         (sib <= (int64_t)n - 2 && par[sib] == p);
-}
-
 // Writes the block's pending items in leaf order: each pending thread
 // passes its subtree's position in the block (0 .. kUpTile-1), a thread
 // that is not pending -1.  mark is kUpTile words of free LDS.  count_out
@@ -446,10 +437,14 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
         mode = 0;
         break;
       }
+      // the parent and the two words of the left-child test (up_is_left)
+      // depend only on num: three loads in flight together
       const int32_t num = p;
+      const int32_t pl = num + 1 <= (int32_t)n - 1 ? par[n + num + 1] : -2;
+      const int32_t pn = num + 1 <= (int32_t)n - 2 ? par[num + 1] : -2;
       self = num;
       p = par[num];
-      side = up_is_left_g(par, n, num, p) ? 0 : 1;
+      side = (pl == p || pn == p) ? 0 : 1;
     }
   }
   __syncthreads();
