@@ -171,7 +171,9 @@ int lbvh_radix_tree(lbvh_ctx *ctx, const uint32_t *sorted_keys, uint32_t n,
 
 /* Replaces initBVHUpPassKernel(device).execute(IDX, AABB, PARENT, N)
  * (bvh-up-pass.js:70-156, WGSL :200-280): bottom-up node boxes; writes
- * N 64-byte nodes. */
+ * N 64-byte nodes.  Scratch: about 7 bytes per triangle.  A parents array
+ * that is not a radix tree makes the next lbvh_synchronize return
+ * LBVH_ERR_DEVICE. */
 int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
                  const int32_t *parents, uint32_t n, void *bvh_out);
 
