@@ -265,6 +265,36 @@ def test_up_pass_all_equal_keys(ctx, lb, ora):
             x.destroy()
 
 
+def test_up_pass_reports_a_parents_array_that_is_not_a_tree(ctx, lb, ora):
+    """All-zero parents (every leaf a child of the root): block frontiers
+    overflow and the last level is left with items, so the kernels flag the
+    context and lbvh_synchronize reports a device error; the flag is cleared
+    and the next valid call succeeds."""
+    say("up pass with all-zero parents n=5000")
+    n = 5000
+    tri = lb.wl_uniform(41, n, 0.01)
+    b = lb.wl_bounds(tri)
+    d = ctx.to_device(tri)
+    aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx, bvh = ctx.alloc(4 * n), ctx.alloc(64 * n)
+    par = ctx.to_device(np.zeros(2 * n, np.int32))
+    try:
+        ctx.aabb_morton(d, n, b, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        with pytest.raises(lb.LBVHError) as err:
+            ctx.synchronize()
+        assert err.value.code == lb.ERR_DEVICE
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.synchronize()
+        wb, _, _, _ = ora.build(tri, b, 0)
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb)
+    finally:
+        for x in (d, aabb, keys, idx, par, bvh):
+            x.destroy()
+
+
 def test_stage_pipeline_heightfield_16m(ctx, lb, ora):
     """The per-stage sequence over the 16,777,216-triangle heightfield: the up
     pass meets most parents on chip and the ones on paths across its 1,024-leaf
