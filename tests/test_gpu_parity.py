@@ -510,6 +510,33 @@ def test_build_67m_against_oracle(ctx, lb, ora):
     assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
 
 
+def test_stage_pipeline_67m_against_oracle(ctx, lb, ora):
+    """The per-stage sequence at 2^26 + 3 triangles: 65,537 up-pass blocks
+    and six item levels above them (8,193, 1,025, 129, 17, 3 and 1 blocks),
+    with the GPU bounds reduction, against the oracle's multi-threaded
+    build."""
+    n = (1 << 26) + 3
+    say(f"stage pipeline uniform n={n}")
+    tri = lb.wl_uniform(68, n, 0.01)
+    d = ctx.to_device(tri)
+    aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
+    idx, par, bvh = ctx.alloc(4 * n), ctx.alloc(8 * n), ctx.alloc(64 * n)
+    try:
+        ctx.aabb_morton(d, n, None, aabb, keys)
+        ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.up_pass(idx, aabb, par, n, bvh)
+        ctx.synchronize()
+        say("oracle (multi-threaded)")
+        wb, _, wsk, wpm, _ = ora.build_mt(tri, None, 0)
+        assert np.array_equal(keys.read(np.uint32, n), wsk)
+        assert np.array_equal(idx.read(np.int32, n), wpm)
+        assert np.array_equal(bvh.read(np.uint8, 64 * n), wb)
+    finally:
+        for x in (d, aabb, keys, idx, par, bvh):
+            x.destroy()
+
+
 # ---------------------------------------------------------------- refit
 
 def test_refit_matches_oracle(ctx, lb, ora):
