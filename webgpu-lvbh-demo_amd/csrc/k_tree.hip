@@ -237,6 +237,17 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   const int64_t ts = (int64_t)tile * kUpTile;
   const int64_t te = min(ts + kUpTile, (int64_t)n);
   const int64_t g = ts + tid;
+  // The leaf's box (a gather through the permutation) is loaded while the
+  // block stages its parents entries: both round trips before one barrier.
+  float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+  if (g < (int64_t)n) {
+    const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
+    const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
+    mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
+    mn[2] = __int_as_float(bl.z);
+    mx[0] = __int_as_float(bh.x); mx[1] = __int_as_float(bh.y);
+    mx[2] = __int_as_float(bh.z);
+  }
   L.cnt[tid] = 0u;
   for (int t = tid; t <= kUpTile; t += kUpTile) {
     const int64_t id = ts + t;
@@ -252,14 +263,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   int side = 0;
   int self = 0;  // this climber's child id at p
   int32_t lo = (int32_t)g, hi = (int32_t)g;  // the subtree's leaves
-  float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
   if (g < (int64_t)n) {
-    const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
-    const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
-    mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
-    mn[2] = __int_as_float(bl.z);
-    mx[0] = __int_as_float(bh.x); mx[1] = __int_as_float(bh.y);
-    mx[2] = __int_as_float(bh.z);
     perturb(mn, mx);
     if (n == 1) {
       write_single_leaf_node(nodes, mn, mx);
