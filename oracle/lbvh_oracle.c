@@ -1,7 +1,7 @@
 /*
  * lbvh_oracle.c -- CPU restatement of the reference LBVH build pipeline.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product).  See the header
+ * TEST INFRASTRUCTURE ONLY (the oracle, never the product).  See the header
  * for the buffer layouts.  Compiled with -ffp-contract=off so every fp32
  * operation rounds exactly once, as in the WGSL source.
  *

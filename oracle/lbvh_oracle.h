@@ -1,7 +1,7 @@
 /*
  * lbvh_oracle.h -- CPU restatement of the reference LBVH build pipeline.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle that the HIP
  * product path is compared against.  Only tests/, the smoke check and the
  * cpu_baseline leg of bench.py may load it; the product library never links
  * or calls it.

@@ -2,7 +2,7 @@
 reference pipeline.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, and only as the checker.  The product library
+cpu_baseline leg of bench.py, and only as the oracle results are compared with.  The product library
 (webgpu-lvbh-demo_amd/lib/liblbvh.so) never loads it.
 
 Parity status of each function (how it is pinned, see tests/test_oracle.py):
