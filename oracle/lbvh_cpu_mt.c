@@ -2,7 +2,7 @@
  * lbvh_cpu_mt.c -- multi-threaded CPU restatement of the reference LBVH
  * build (OpenMP), for the CPU baseline timed beside the GPU.
  *
- * TEST INFRASTRUCTURE ONLY (a baseline and a checker, never the product).
+ * TEST INFRASTRUCTURE ONLY (a baseline and an oracle, never the product).
  * Same results as oracle_build (lbvh_oracle.c), byte for byte, with the
  * stages parallelised the way a CPU implementation would be:
  *   bounds        per-thread min/max, then a fold    (parse-obj.js:10-53)

@@ -66,7 +66,7 @@ void oracle_raycast(const void *bvh, const float *tri, uint32_t n,
 void oracle_raycast_bruteforce(const float *tri, uint32_t n, const float *rays,
                                uint32_t count, float *hits);
 
-/* Independent brute-force checkers (no shared code with the restatement). */
+/* Independent brute-force references (no shared code with the restatement). */
 void oracle_bruteforce_tree(const uint32_t *sorted_keys, uint32_t n,
                             int32_t *parents_out);
 void oracle_bruteforce_bvh(const float *tri, const int32_t *perm,
