@@ -29,6 +29,15 @@
  *
  * Every function returns an lbvh_status; details via lbvh_last_error().
  * Nothing throws across this boundary.
+ *
+ * Extents: a device pointer that lies inside a buffer from lbvh_alloc must
+ * have room, from that address to the buffer's end, for every byte the
+ * call reads or writes through it (e.g. 48*N for triangles, 64*N for BVH
+ * nodes); otherwise the call fails with LBVH_ERR_SIZE_MISMATCH before any
+ * copy or launch.  WebGPU gives the reference the same guarantee by
+ * bounds-checking buffer accesses, and Float32Array.set throws RangeError
+ * on an oversize upload (build-bvh.js:14-21).  Pointers from the caller's
+ * own hipMalloc are not checked.
  */
 #ifndef LBVH_H
 #define LBVH_H

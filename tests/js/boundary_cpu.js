@@ -50,4 +50,60 @@ try {
   assert.strictEqual(e.code, 6)
 }
 assert.ok(threw, 'createServer on a missing device must throw')
-console.log('boundary_cpu ok')
+// Uploads follow mappedAtCreation + Float32Array.set (build-bvh.js:14-21):
+// data longer than the buffer throws RangeError before any allocation, a
+// shorter one leaves the rest zero.  None of these reach the device.
+const ue = bvh._internal.uploadExtent
+assert.deepStrictEqual(ue(48, new Float32Array(12)), { bytes: 48, zeros: 0 })
+assert.deepStrictEqual(ue(48, new Float32Array(6)), { bytes: 24, zeros: 24 })
+assert.deepStrictEqual(ue(48, new ArrayBuffer(10)), { bytes: 10, zeros: 38 })
+assert.deepStrictEqual(ue(48, undefined), { bytes: 0, zeros: 0 })
+assert.deepStrictEqual(ue(0, new Float32Array(0)), { bytes: 0, zeros: 0 })
+assert.throws(() => ue(48, new Float32Array(13)), RangeError)
+assert.throws(() => ue(0, new Uint8Array(1)), RangeError)
+assert.throws(() => ue(48, [1, 2, 3]), TypeError)
+
+// A device whose allocator must never be reached: every call below is
+// refused on the host first.
+const noDevice = Object.create(bvh.LBVHDevice.prototype)
+noDevice.ctx = null
+noDevice.queue = { onSubmittedWorkDone: () => Promise.resolve() }
+let warned = []
+const origWarn = console.warn
+console.warn = (msg) => { warned.push(String(msg)) }
+
+async function hostChecks () {
+  // build(): a Float32Array of more than 12 * N floats
+  await assert.rejects(bvh.initBVHBuild(noDevice).build(new Float32Array(13), 1, null), RangeError)
+  // and a JS Array of more than 12 * N numbers (Float32Array.set itself)
+  await assert.rejects(bvh.initBVHBuild(noDevice).build(new Array(25).fill(0), 2, null), RangeError)
+  // batched: a mesh array longer than its NUM_TRIS, or not 12 floats per triangle
+  await assert.rejects(bvh.initBVHBatchBuild(noDevice).build(
+    [{ TRI_ARRAY: new Float32Array(24), NUM_TRIS: 1 }], null), RangeError)
+  await assert.rejects(bvh.initBVHBatchBuild(noDevice).build([new Float32Array(13)], null),
+    RangeError)
+  // up pass: each input must hold `size` entries (warning + undefined, as
+  // the other shims report a mismatch)
+  const up = bvh.initBVHUpPassKernel(noDevice)
+  const buf = (size) => ({ size, ptr: 1n })
+  for (const [idx, aabb, par] of [[4, 64, 16], [8, 32, 16], [8, 64, 8]]) {
+    warned = []
+    const r = await up.execute(buf(idx), buf(aabb), buf(par), 2)
+    assert.strictEqual(r, undefined)
+    assert.strictEqual(warned.length, 1)
+    assert.ok(/too small for requested size \[ 2 \]/.test(warned[0]), warned[0])
+  }
+  // ray caster: BVH / triangle buffers too small for NUM_TRIS, ragged rays
+  await assert.rejects(bvh.raycast(noDevice, buf(64), buf(96), 2, new Float32Array(8)), RangeError)
+  await assert.rejects(bvh.raycast(noDevice, buf(128), buf(48), 2, new Float32Array(8)), RangeError)
+  await assert.rejects(bvh.raycast(noDevice, buf(128), buf(96), 2, new Float32Array(9)), RangeError)
+}
+
+hostChecks().then(() => {
+  console.warn = origWarn
+  console.log('boundary_cpu ok')
+}, (e) => {
+  console.warn = origWarn
+  console.error(e)
+  process.exit(1)
+})

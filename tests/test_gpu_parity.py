@@ -265,27 +265,51 @@ def test_up_pass_all_equal_keys(ctx, lb, ora):
             x.destroy()
 
 
-def test_up_pass_reports_a_parents_array_that_is_not_a_tree(ctx, lb, ora):
-    """All-zero parents (every leaf a child of the root): block frontiers
-    overflow and the last level is left with items, so the kernels flag the
-    context and lbvh_synchronize reports a device error; the flag is cleared
-    and the next valid call succeeds."""
-    say("up pass with all-zero parents n=5000")
-    n = 5000
+@pytest.mark.parametrize("kind", ["zeros", "minus_one", "self_loops", "index_out_of_range",
+                                  "one_block"])
+def test_up_pass_reports_a_parents_array_that_is_not_a_tree(ctx, lb, ora, kind):
+    """Inputs that are not a radix tree over the leaves end every climb in
+    bounds and flag the context, and lbvh_synchronize reports a device error;
+    the flag is cleared and the next valid call succeeds.
+      zeros: every leaf a child of the root (frontiers overflow, third children)
+      minus_one: every parent out of range (no climb starts, the root never merges)
+      self_loops: each internal node its own parent (third arrivals / step cap)
+      index_out_of_range: one permutation entry >= n (the load is clamped)
+      one_block: 1,000 leaves with all-(-1) parents, the single-block case"""
+    n = 1000 if kind == "one_block" else 5000
+    say(f"up pass with a malformed input: {kind} n={n}")
     tri = lb.wl_uniform(41, n, 0.01)
     b = lb.wl_bounds(tri)
     d = ctx.to_device(tri)
     aabb, keys = ctx.alloc(32 * n), ctx.alloc(4 * n)
     idx, bvh = ctx.alloc(4 * n), ctx.alloc(64 * n)
-    par = ctx.to_device(np.zeros(2 * n, np.int32))
+    par = ctx.alloc(8 * n)
     try:
         ctx.aabb_morton(d, n, b, aabb, keys)
         ctx.sort(keys, idx, n)
+        ctx.radix_tree(keys, n, par)
+        ctx.synchronize()
+        good_par = par.read(np.int32, 2 * n)
+        good_idx = idx.read(np.int32, n)
+        bad_par, bad_idx = good_par.copy(), good_idx
+        if kind == "zeros":
+            bad_par[:] = 0
+        elif kind in ("minus_one", "one_block"):
+            bad_par[:] = -1
+        elif kind == "self_loops":
+            bad_par[1:n - 1] = np.arange(1, n - 1, dtype=np.int32)
+        else:
+            bad_idx = good_idx.copy()
+            bad_idx[n // 2] = n + 7
+        par.write(bad_par)
+        idx.write(bad_idx)
         ctx.up_pass(idx, aabb, par, n, bvh)
         with pytest.raises(lb.LBVHError) as err:
             ctx.synchronize()
         assert err.value.code == lb.ERR_DEVICE
-        ctx.radix_tree(keys, n, par)
+        assert "not a radix tree" in str(err.value) or "more subtrees" in str(err.value)
+        par.write(good_par)
+        idx.write(good_idx)
         ctx.up_pass(idx, aabb, par, n, bvh)
         ctx.synchronize()
         wb, _, _, _ = ora.build(tri, b, 0)

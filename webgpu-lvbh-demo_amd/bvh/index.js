@@ -64,11 +64,23 @@ class LBVHDevice {
   }
 
   // GPUDevice.createBuffer({ size }) (build-bvh.js:14-18); `data` optionally
-  // fills it (the mappedAtCreation upload, build-bvh.js:20-21).
+  // fills it like the mappedAtCreation upload (build-bvh.js:20-21): data
+  // longer than the buffer throws RangeError before anything is allocated,
+  // as Float32Array.set does, and a shorter one leaves the rest zero, as a
+  // new mapped buffer is.
   createBuffer (descriptor, data) {
     const size = descriptor.size
+    const fill = uploadExtent(size, data)
     const buf = new LBVHBuffer(this, size, addon.alloc(this.ctx, size))
-    if (data !== undefined && size > 0) addon.upload(this.ctx, buf.ptr, data)
+    try {
+      if (fill.bytes > 0) addon.upload(this.ctx, buf.ptr, data)
+      if (fill.zeros > 0) {
+        addon.upload(this.ctx, buf.ptr + BigInt(fill.bytes), new Uint8Array(fill.zeros))
+      }
+    } catch (e) {
+      buf.destroy()
+      throw e
+    }
     return buf
   }
 
@@ -95,6 +107,22 @@ class LBVHDevice {
 
 function createDevice (options) {
   return new LBVHDevice(options)
+}
+
+// Bytes of `data` (a typed array or ArrayBuffer) copied into a new buffer of
+// `size` bytes and the zero bytes after them: { bytes, zeros }.  Throws
+// RangeError, as Float32Array.set does (build-bvh.js:20), when data does
+// not fit.  No data: nothing copied, nothing zeroed.
+function uploadExtent (size, data) {
+  if (data === undefined || data === null) return { bytes: 0, zeros: 0 }
+  const bytes = data.byteLength
+  if (typeof bytes !== 'number') {
+    throw new TypeError('buffer data must be a typed array or an ArrayBuffer')
+  }
+  if (bytes > size) {
+    throw new RangeError(`Source is too large: ${bytes} bytes for a buffer of ${size}`)
+  }
+  return { bytes: bytes, zeros: size - bytes }
 }
 
 // TRI_ARRAY (JS Array of 12*N numbers, parse-obj.js:55-59) -> Float32Array,
@@ -154,14 +182,24 @@ function initBVHBatchBuild (device) {
   async function build (MESHES, BOUNDS) {
     const S = MESHES.length
     const OFFSETS = new Uint32Array(S + 1)
+    // Each mesh fills 12 * n floats of its range like Float32Array.set: a
+    // longer array throws RangeError, a shorter one leaves zeros.
     const parts = MESHES.map(function (m) {
-      if (m instanceof Float32Array) return m
-      return toFloat32(m.TRI_ARRAY, m.NUM_TRIS)
+      if (m instanceof Float32Array) {
+        if (m.length % 12 !== 0) throw new RangeError('a mesh needs 12 floats per triangle')
+        return { tri: m, n: m.length / 12 }
+      }
+      return { tri: toFloat32(m.TRI_ARRAY, m.NUM_TRIS), n: m.NUM_TRIS }
     })
-    for (let k = 0; k < S; k++) OFFSETS[k + 1] = OFFSETS[k] + parts[k].length / 12
+    for (let k = 0; k < S; k++) {
+      if (parts[k].tri.length > 12 * parts[k].n) {
+        throw new RangeError(`Source is too large: mesh ${k} has ${parts[k].tri.length} floats for ${parts[k].n} triangles`)
+      }
+      OFFSETS[k + 1] = OFFSETS[k] + parts[k].n
+    }
     const total = OFFSETS[S]
     const tris = new Float32Array(12 * total)
-    for (let k = 0; k < S; k++) tris.set(parts[k], 12 * OFFSETS[k])
+    for (let k = 0; k < S; k++) tris.set(parts[k].tri, 12 * OFFSETS[k])
     let bounds = null
     if (BOUNDS !== undefined && BOUNDS !== null) {
       bounds = new Float32Array(6 * S)
@@ -186,6 +224,10 @@ function initBVHBatchBuild (device) {
 // dir.xyz, pad) through a built BVH; resolves to a Float32Array of 4 floats
 // per ray (normal.xyz facing the ray, distance; -1 for a miss).
 async function raycast (device, BVH_BUFFER, O_TRIANGLE_BUFFER, NUM_TRIS, RAYS) {
+  if (BVH_BUFFER.size < 64 * NUM_TRIS || O_TRIANGLE_BUFFER.size < 48 * NUM_TRIS) {
+    throw new RangeError(`raycast: BVH [ ${BVH_BUFFER.size} ] or triangle [ ${O_TRIANGLE_BUFFER.size} ] buffer is too small for ${NUM_TRIS} triangles`)
+  }
+  if (RAYS.length % 8 !== 0) throw new RangeError('raycast: RAYS must hold 8 floats per ray')
   const count = RAYS.length / 8
   const R = device.createBuffer({ size: RAYS.length * 4 }, RAYS)
   const H = device.createBuffer({ size: count * 16 })
@@ -319,8 +361,18 @@ function initRadixTreeKernel (device) {
 function initBVHUpPassKernel (device) {
   return { execute }
 
-  // bvh-up-pass.js:70-156 (the reference has no size guard here)
+  // bvh-up-pass.js:70-156.  The reference has no size guard here (WebGPU
+  // bounds-checks the shader's reads); buffers too small for `size` are
+  // refused the way the other shims refuse a mismatch.
   async function execute (IDX_BUFFER, AABB_BUFFER, PARENT_BUFFER, size) {
+    const need = [[IDX_BUFFER, 4, 'index'], [AABB_BUFFER, 32, 'AABB'],
+      [PARENT_BUFFER, 8, 'parent']]
+    for (const [buf, per, name] of need) {
+      if (buf.size < per * size) {
+        console.warn(`in BVH up pass: ${name} buffer size [ ${buf.size} ] is too small for requested size [ ${size} ]`)
+        return
+      }
+    }
     const BVH_BUFFER = device.createBuffer({ size: size * 64 })
     if (size > 0) {
       addon.upPass(device.ctx, IDX_BUFFER.ptr, AABB_BUFFER.ptr, PARENT_BUFFER.ptr,
@@ -383,5 +435,5 @@ module.exports = {
   LBVHBuffer,
   LBVHDevice,
   addon,
-  _internal: { toFloat32, toBounds }
+  _internal: { toFloat32, toBounds, uploadExtent }
 }

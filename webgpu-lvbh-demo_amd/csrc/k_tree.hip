@@ -6,11 +6,13 @@
 // Two code paths produce the same BVH bytes:
 //   * per-stage kernels mirroring the reference's execute() shims
 //     (k_radix_tree -> parents array, k_up_pass over parents, k_rearrange);
-//   * k_tree_fused, used by lbvh_build / lbvh_refit: one kernel that gathers
-//     the triangles into leaf order, computes leaf boxes, and builds the
-//     Karras tree bottom-up by agglomeration (Apetrei 2014) without a
-//     parents array, merging inside LDS whenever both children of a split
-//     lie in the block's tile and exchanging through HBM only at tile seams.
+//   * the agglomeration used by lbvh_build / lbvh_refit / batched builds:
+//     k_leaves gathers the triangles into leaf order, computes leaf boxes
+//     and builds the Karras tree bottom-up (Apetrei 2014) without a parents
+//     array, merging inside LDS wherever both children of a split lie in
+//     the block's tile; each tile passes the subtrees it cannot complete
+//     (its frontier) to k_items, whose levels merge the frontiers of 32
+//     tiles per block until one block completes the root.
 //
 // Common definitions.  Sorted position i, key k_i.  delta(i, j) = common
 // prefix length of the keys at i and j, with equal keys compared by their
@@ -215,7 +217,8 @@ __device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
   if (tid == kUpTile - 1) {
     const uint32_t total = pos + v;
     *count_out = total < (uint32_t)kUpFront ? total : (uint32_t)kUpFront;
-    if (total > (uint32_t)kUpFront || (last && total > 0)) atomicOr(&ctl->fault, 4u);
+    if (total > (uint32_t)kUpFront || (last && total > 0))
+      atomicOr(&ctl->fault, kFaultUpFrontier);
   }
   __syncthreads();
   if (at >= 0) {
@@ -232,6 +235,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     int last) {
   __shared__ UpLds L;
   __shared__ uint32_t s_wsum[16];
+  __shared__ int s_root;  // the root merged in this block
   const int tid = threadIdx.x;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t ts = (int64_t)tile * kUpTile;
@@ -240,8 +244,14 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   // The leaf's box (a gather through the permutation) is loaded while the
   // block stages its parents entries: both round trips before one barrier.
   float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+  // a malformed input (an index or parent out of range, a path longer than
+  // any tree's, a third arrival at a node) ends the thread's climb and
+  // sets ctl->fault; every access stays in bounds
+  bool bad = false;
   if (g < (int64_t)n) {
-    const uint32_t src = (uint32_t)idx[g] < n ? (uint32_t)idx[g] : 0u;
+    const uint32_t raw = (uint32_t)idx[g];
+    bad = raw >= n;
+    const uint32_t src = raw < n ? raw : 0u;
     const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
     mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
     mn[2] = __int_as_float(bl.z);
@@ -249,6 +259,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     mx[2] = __int_as_float(bh.z);
   }
   L.cnt[tid] = 0u;
+  if (tid == 0) s_root = 0;
   for (int t = tid; t <= kUpTile; t += kUpTile) {
     const int64_t id = ts + t;
     L.par_leaf[t] = id < (int64_t)n ? par[n + id] : -1;
@@ -277,8 +288,12 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
       // A valid tree is at most 64 levels deep (each level consumes a bit of
       // the 62-bit position-augmented key); the step cap and the range
       // check keep a malformed parents array from running away.
-      for (int step = 0; step < 128; ++step) {
-        if (p < 0 || p >= (int32_t)n - 1) break;
+      int step = 0;
+      for (; step < 128; ++step) {
+        if (p < 0 || p >= (int32_t)n - 1) {
+          bad = true;
+          break;
+        }
         side = up_is_left(L, par, n, ts, te, num, p) ? 0 : 1;
         self = leaf ? -(int)(num + 1) : (int)num;
         if (p < ts || p >= te) {  // the parent lies outside the block
@@ -299,6 +314,10 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
           mode = 1;
           break;
         }
+        if (old >= 2) {  // a third child at one node
+          bad = true;
+          break;
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           mn[k] = fminf(mn[k], L.box[1 - side][k][slot]);
@@ -306,14 +325,21 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         }
         if (side == 0) hi = L.far[1][slot];
         else lo = L.far[0][slot];
-        if (p == 0) break;  // root merged
+        if (p == 0) {  // root merged
+          s_root = 1;
+          break;
+        }
         leaf = false;
         num = p;
         p = up_par_node(L, par, ts, te, p);
       }
+      if (step == 128) bad = true;
     }
   }
+  if (bad) atomicOr(&ctl->fault, kFaultNotATree);
   __syncthreads();
+  // the block that holds the whole tree must have merged its root
+  if (last && tid == 0 && n > 1 && !s_root) atomicOr(&ctl->fault, kFaultNotATree);
   // a first arriver whose sibling did not come on chip is pending too (the
   // sibling reaches outside the block); an item writes its half of the
   // parent when it arrives there at a later level
@@ -366,6 +392,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
     uint32_t n, int4 *nodes, Ctl *ctl, int last) {
   __shared__ UpItemLds S;
   __shared__ uint32_t s_wsum[16];
+  __shared__ int s_root;  // the root merged in this block
   const int tid = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const uint32_t c0 = tile * kUpFanIn;
@@ -378,6 +405,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       run += c < (uint32_t)kUpFront ? c : (uint32_t)kUpFront;
     }
     S.base[nc] = run;
+    s_root = 0;
   }
   S.cnt[tid] = 0u;
   __syncthreads();
@@ -399,9 +427,12 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
     mx[0] = __int_as_float(c.x); mx[1] = __int_as_float(c.y);
     mx[2] = __int_as_float(c.z);
     mode = 2;
-    for (int step = 0; step < 128; ++step) {
+    bool bad = false;
+    int step = 0;
+    for (; step < 128; ++step) {
       if (p < 0 || p >= (int32_t)n - 1) {
         mode = 0;
+        bad = true;
         break;
       }
       const int gap = side == 0 ? i1 : i0 - 1;
@@ -425,6 +456,11 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
         mode = 1;
         break;
       }
+      if (old >= 2) {  // a third child at one node
+        mode = 0;
+        bad = true;
+        break;
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
@@ -439,6 +475,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       }
       if (p == 0) {  // root merged
         mode = 0;
+        s_root = 1;
         break;
       }
       // the parent and the two words of the left-child test (up_is_left)
@@ -450,8 +487,14 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       p = par[num];
       side = (pl == p || pn == p) ? 0 : 1;
     }
+    if (step == 128) {
+      mode = 0;
+      bad = true;
+    }
+    if (bad) atomicOr(&ctl->fault, kFaultNotATree);
   }
   __syncthreads();
+  if (last && tid == 0 && !s_root) atomicOr(&ctl->fault, kFaultNotATree);
   if (mode == 1 && S.cnt[side == 0 ? i1 : i0 - 1] == 1u) mode = 2;
   up_emit(S.mark, s_wsum, mode == 2 ? i0 : -1,
           items_out + 3 * (size_t)tile * kUpFront, cnt_out + tile, ctl, last,
@@ -555,7 +598,7 @@ __device__ __forceinline__ uint32_t emit_frontier(bool out, int l, int r,
   for (int w = 0; w < (T >> 6); ++w) count += wsum[w];
   if (tid == T - 1) {
     count_out[gtile] = count;
-    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, 8u);
+    if (count > (uint32_t)kFrontierMax) atomicOr(&ctl->fault, kFaultFrontier);
   }
   if (out && rank < (uint32_t)kFrontierMax) {
     int4 *io = items_out + kItemInt4 * ((size_t)gtile * kFrontierMax + rank);

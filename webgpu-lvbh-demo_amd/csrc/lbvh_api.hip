@@ -9,14 +9,15 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <new>
 #include <string>
 #include <thread>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/lbvh.h"
+#include "alloc_table.h"
 #include "lbvh_internal.h"
 
 namespace lbvh {
@@ -48,7 +49,7 @@ struct lbvh_ctx {
   bool sort_bits_fixed = false;
   int tune_bits[33] = {0};       // autotuned digit width per size bucket
   std::string last_error;
-  std::unordered_map<void *, uint64_t> user_allocs;
+  AllocTable user_allocs;  // buffers handed out by lbvh_alloc
   std::recursive_mutex mu;
   // device state
   Ctl *ctl = nullptr;
@@ -331,6 +332,27 @@ int check_n(lbvh_ctx *ctx, uint32_t n) {
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
                 "triangle count " + std::to_string(n) + " exceeds the maximum " +
                     std::to_string(LBVH_MAX_TRIS));
+  return LBVH_OK;
+}
+
+// A device pointer argument and the bytes an entry point reads or writes
+// from it.  Pointers inside an lbvh_alloc buffer must have room for those
+// bytes; other pointers (the caller's own allocations) are not judged.
+struct Span {
+  const void *p;
+  uint64_t bytes;
+  const char *what;
+};
+int check_spans(lbvh_ctx *ctx, std::initializer_list<Span> spans) {
+  for (const Span &s : spans) {
+    uint64_t room = 0;
+    if (s.p && s.bytes &&
+        ctx->user_allocs.check(s.p, s.bytes, &room) == AllocTable::kOverrun)
+      return fail(ctx, LBVH_ERR_SIZE_MISMATCH,
+                  std::string(s.what) + " needs " + std::to_string(s.bytes) +
+                      " bytes, but its lbvh_alloc buffer holds " +
+                      std::to_string(room) + " from that address");
+  }
   return LBVH_OK;
 }
 
@@ -694,7 +716,7 @@ int lbvh_destroy(lbvh_ctx *ctx) {
     }
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     release_staging(ctx);
-    for (auto &kv : ctx->user_allocs) (void)hipFree(kv.first);
+    ctx->user_allocs.for_each([](void *p, uint64_t) { (void)hipFree(p); });
     ctx->user_allocs.clear();
     (void)hipFree(ctx->ctl);
     (void)hipStreamDestroy(ctx->stream);
@@ -855,11 +877,19 @@ int lbvh_synchronize(lbvh_ctx *ctx) {
   HIP_TRY(ctx, hipMemcpy(&fault, &ctx->ctl->fault, 4, hipMemcpyDeviceToHost));
   if (fault) {
     HIP_TRY(ctx, hipMemset(&ctx->ctl->fault, 0, 4));
+    std::string what;
+    if (fault & kFaultNotATree)
+      what += "; the up pass's parents or index array is not a radix tree "
+              "over the leaves (an entry out of range, a path longer than "
+              "any tree's, a node with a third child, or no root)";
+    if (fault & kFaultUpFrontier)
+      what += "; an up pass block left more subtrees than a tree allows";
+    if (fault & kFaultFrontier)
+      what += "; a build tile's frontier overflowed";
     return fail(ctx, LBVH_ERR_DEVICE,
-                "a tree kernel found an inconsistent parents array or "
-                "frontier (a frontier overflowed or a tree did not close); "
-                "results of the operations since the last synchronize are "
-                "invalid");
+                "a tree kernel found inconsistent input" + what +
+                    "; results of the operations since the last synchronize "
+                    "are invalid");
   }
   return LBVH_OK;
 }
@@ -883,7 +913,7 @@ int lbvh_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out_dptr) {
   void *p = nullptr;
   int st = dev_alloc(ctx, bytes, &p);
   if (st != LBVH_OK) return st;
-  ctx->user_allocs[p] = bytes;
+  ctx->user_allocs.add(p, bytes);
   *out_dptr = p;
   return LBVH_OK;
 }
@@ -891,14 +921,17 @@ int lbvh_alloc(lbvh_ctx *ctx, uint64_t bytes, void **out_dptr) {
 int lbvh_free(lbvh_ctx *ctx, void *dptr) {
   CTX_GUARD(ctx);
   if (!dptr) return LBVH_OK;
-  auto it = ctx->user_allocs.find(dptr);
-  if (it == ctx->user_allocs.end())
+  if (ctx->user_allocs.check(dptr, 0) == AllocTable::kForeign)
     return fail(ctx, LBVH_ERR_NOT_FOUND, "pointer was not allocated by lbvh_alloc");
+  uint64_t bytes = 0;
+  if (!ctx->user_allocs.find(dptr, &bytes))
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                "pointer lies inside an lbvh_alloc buffer but is not its start");
   // Work already queued may still read the buffer.
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   HIP_TRY(ctx, hipFree(dptr));
-  ctx->in_use -= it->second;
-  ctx->user_allocs.erase(it);
+  ctx->in_use -= bytes;
+  ctx->user_allocs.remove(dptr, nullptr);
   return LBVH_OK;
 }
 
@@ -908,6 +941,9 @@ int lbvh_upload(lbvh_ctx *ctx, void *dst_dev, const void *src_host,
   if (bytes == 0) return LBVH_OK;
   if (!dst_dev || !src_host)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  int st;
+  if ((st = check_spans(ctx, {{dst_dev, bytes, "upload destination"}})))
+    return st;
   if (bytes >= kStageMin && ensure_staging(ctx)) {
     HIP_TRY(ctx, staged_copy(ctx, static_cast<char *>(dst_dev),
                              static_cast<const char *>(src_host), bytes, true));
@@ -925,6 +961,9 @@ int lbvh_download(lbvh_ctx *ctx, void *dst_host, const void *src_dev,
   if (bytes == 0) return LBVH_OK;
   if (!dst_host || !src_dev)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  int st;
+  if ((st = check_spans(ctx, {{src_dev, bytes, "download source"}})))
+    return st;
   if (bytes >= kStageMin && ensure_staging(ctx)) {
     HIP_TRY(ctx, staged_copy(ctx, static_cast<char *>(dst_host),
                              static_cast<const char *>(src_dev), bytes, false));
@@ -947,6 +986,9 @@ int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
   if (n == 0)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bounds of an empty mesh");
+  if ((st = check_spans(ctx, {{tri, 48ull * n, "triangles"},
+                              {bounds_out, 24, "bounds output"}})))
+    return st;
   if ((st = ensure_partials(ctx))) return st;
   launch_bounds(reinterpret_cast<const float4 *>(tri), n,
                 static_cast<float4 *>(ctx->partials.ptr), ctx->ctl,
@@ -964,6 +1006,10 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !keys_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{tri, 48ull * n, "triangles"},
+                              {aabb_out, 32ull * n, "AABB output"},
+                              {keys_out, 4ull * n, "key output"}})))
+    return st;
   MortonParams p{};
   if (bounds_host) {
     p = morton_params(ctx, bounds_host);
@@ -988,6 +1034,9 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
   if (n == 0) return LBVH_OK;
   if (!keys || !idx_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{keys, 4ull * n, "keys"},
+                              {idx_out, 4ull * n, "index output"}})))
+    return st;
   if (n == 1) {
     HIP_TRY(ctx, hipMemsetAsync(idx_out, 0, 4, ctx->stream));
     return LBVH_OK;
@@ -1011,6 +1060,9 @@ int lbvh_radix_tree(lbvh_ctx *ctx, const uint32_t *sorted_keys, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!sorted_keys || !parents_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{sorted_keys, 4ull * n, "sorted keys"},
+                              {parents_out, 8ull * n, "parents output"}})))
+    return st;
   launch_radix_tree(sorted_keys, n, parents_out, ctx->stream);
   return check_launch(ctx, "lbvh_radix_tree");
 }
@@ -1023,6 +1075,11 @@ int lbvh_up_pass(lbvh_ctx *ctx, const int32_t *idx, const float *aabb,
   if (n == 0) return LBVH_OK;
   if (!idx || !aabb || !parents || !bvh_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{idx, 4ull * n, "index"},
+                              {aabb, 32ull * n, "AABBs"},
+                              {parents, 8ull * n, "parents"},
+                              {bvh_out, 64ull * n, "BVH output"}})))
+    return st;
   if ((st = ensure(ctx, ctx->up_items, up_pass_scratch_bytes(n), false)))
     return st;
   launch_up_pass(idx, reinterpret_cast<const float4 *>(aabb), parents, n,
@@ -1039,6 +1096,10 @@ int lbvh_rearrange(lbvh_ctx *ctx, const float *tri_in, const int32_t *idx,
   if (n == 0) return LBVH_OK;
   if (!tri_in || !idx || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{tri_in, 48ull * n, "triangles"},
+                              {idx, 4ull * n, "index"},
+                              {tri_out, 48ull * n, "triangle output"}})))
+    return st;
   launch_rearrange(reinterpret_cast<const float4 *>(tri_in), idx, n,
                    reinterpret_cast<float4 *>(tri_out), ctx->stream);
   return check_launch(ctx, "lbvh_rearrange");
@@ -1055,6 +1116,12 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{tri, 48ull * n, "triangles"},
+                              {bvh_out, 64ull * n, "BVH output"},
+                              {tri_out, 48ull * n, "triangle output"},
+                              {sorted_keys_out, 4ull * n, "sorted-key output"},
+                              {perm_out, 4ull * n, "permutation output"}})))
+    return st;
   const int bits = bits_for(ctx, n);
   if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   const int tile = tile_for(ctx, n);
@@ -1118,6 +1185,12 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
   if (n == 0) return LBVH_OK;
   if (!tri || !sorted_keys || !perm || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{tri, 48ull * n, "triangles"},
+                              {sorted_keys, 4ull * n, "sorted keys"},
+                              {perm, 4ull * n, "permutation"},
+                              {bvh_out, 64ull * n, "BVH output"},
+                              {tri_out, 48ull * n, "triangle output"}})))
+    return st;
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
   run_tree(ctx, tri, sorted_keys, perm, n, tri_out, bvh_out, tile);
@@ -1145,6 +1218,12 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if (total == 0) return LBVH_OK;
   if (!tri || !bvh_out || !tri_out)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{tri, 48ull * total, "triangles"},
+                              {bvh_out, 64ull * total, "BVH output"},
+                              {tri_out, 48ull * total, "triangle output"},
+                              {sorted_keys_out, 4ull * total, "sorted-key output"},
+                              {perm_out, 4ull * total, "permutation output"}})))
+    return st;
   if ((st = make_plan(ctx, offsets, num_meshes))) return st;
   const auto &P = ctx->plan;
   if ((st = ensure(ctx, ctx->b_keys_a, 4ull * total, false))) return st;
@@ -1210,6 +1289,11 @@ int lbvh_raycast(lbvh_ctx *ctx, const void *bvh, const float *tri, uint32_t n,
   if (count == 0) return LBVH_OK;
   if (!rays || !hits || (n && (!bvh || !tri)))
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  if ((st = check_spans(ctx, {{bvh, 64ull * n, "BVH"},
+                              {tri, 48ull * n, "triangles"},
+                              {rays, 32ull * count, "rays"},
+                              {hits, 16ull * count, "hit output"}})))
+    return st;
   if (n == 0) {
     HIP_TRY(ctx, hipMemsetAsync(hits, 0, 16ull * count, ctx->stream));
     return LBVH_OK;

@@ -47,11 +47,20 @@ constexpr int kMortonBlocksMax = 2048;  // grid-stride Morton (per-stage entry)
 //
 // One per context, zeroed at allocation.  Nothing in it needs a reset
 // between operations (or graph replays):
-//   fault       sticky flag: a frontier overflowed or a tree did not close
-//               (never set in a correct run; read by lbvh_synchronize)
+//   fault       sticky bits (kFault*), never set in a correct run; read and
+//               cleared by lbvh_synchronize
 //   bounds      scene bounds of the last reduction (k_bounds_final)
 //   hist        global digit counts of each pass of the current sort (pass
 //               p's at hist + p * radix)
+constexpr uint32_t kFaultUpFrontier = 4u;  // per-stage up pass: a block left
+                                           // more than kUpFront items, or
+                                           // the last level left items
+constexpr uint32_t kFaultFrontier = 8u;    // agglomeration frontier overflow
+constexpr uint32_t kFaultNotATree = 16u;   // per-stage up pass: an index or
+                                           // parent out of range, a path
+                                           // longer than any tree's, a third
+                                           // child at a node, or a root that
+                                           // never merged
 struct Ctl {
   uint32_t fault;
   uint32_t pad0[7];
