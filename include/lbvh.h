@@ -58,7 +58,9 @@ typedef enum {
   LBVH_ERR_DEVICE = 5,           /* a HIP runtime call failed */
   LBVH_ERR_NO_DEVICE = 6,        /* no such device ordinal */
   LBVH_ERR_BUSY = 7,             /* serving queue full / no free slot */
-  LBVH_ERR_NOT_FOUND = 8         /* unknown handle / request id */
+  LBVH_ERR_NOT_FOUND = 8,        /* unknown handle / request id */
+  LBVH_ERR_CANCELLED = 9,        /* serving: cancelled before it ran */
+  LBVH_ERR_TIMEOUT = 10          /* serving: not admitted within its wait */
 } lbvh_status;
 
 /* Morton division mode for rel = (cen - min) / (max - min)
@@ -157,7 +159,12 @@ int lbvh_bounds(lbvh_ctx *ctx, const float *tri, uint32_t n,
 /* Replaces initAABB_ZidxKernel(device).execute(TRI, N, bounds)
  * (aabb-z-idx.js:57-148, WGSL :177-208): per-triangle AABB (32 B) and
  * 30-bit Morton key of the AABB centre.  bounds_host = 6 host floats
- * (min.xyz, max.xyz) or NULL to use the GPU bounds reduction. */
+ * (min.xyz, max.xyz) or NULL to use the GPU bounds reduction.  Caller bounds
+ * must be finite (else LBVH_ERR_INVALID_ARGUMENT; the reference would
+ * quantise against them into implementation-defined keys).  A flat axis
+ * (min == max, a planar mesh, from the caller or the reduction) is
+ * deterministic: 0/0 converts to coordinate 0, and centres off the plane
+ * saturate (0 below, 1023's bits above). */
 int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
                      const float *bounds_host, float *aabb_out,
                      uint32_t *keys_out);
@@ -246,8 +253,11 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
  * are evicted when a new build needs room.  A request that can never fit
  * the budget completes with LBVH_ERR_OUT_OF_BUDGET; a refit of a mesh that
  * is not resident, or with a different triangle count, completes with
- * LBVH_ERR_NOT_FOUND / LBVH_ERR_SIZE_MISMATCH.  Resident results are
- * byte-identical to lbvh_build / lbvh_refit on the same input.
+ * LBVH_ERR_NOT_FOUND / LBVH_ERR_SIZE_MISMATCH.  A request whose tree
+ * kernels flag inconsistent input (lbvh_synchronize's LBVH_ERR_DEVICE)
+ * completes with LBVH_ERR_DEVICE and its mesh stops being resident for
+ * refits.  Resident results are byte-identical to lbvh_build / lbvh_refit
+ * on the same input.
  * The budget bounds all device memory the server holds: its mesh arena
  * (device chunks of up to 4 GiB, allocated as the resident set grows and
  * carved into mesh ranges) plus every slot's scratch; bytes_in_use in the
@@ -266,7 +276,9 @@ typedef enum {
   LBVH_SRC_UNIFORM = 2, /* synthesised on the device: the uniform mesh of
                            lbvh_wl_uniform(seed, n, param) (bit-identical) */
   LBVH_SRC_JITTER = 3   /* the resident mesh's triangles + N(0, param^2) per
-                           coordinate, drawn on the device from seed (refits) */
+                           coordinate, drawn on the device from seed (refits):
+                           bit-identical to lbvh_wl_jitter(seed, n, param) of
+                           the triangles in input order */
 } lbvh_source;
 
 typedef struct {
@@ -279,6 +291,15 @@ typedef struct {
   double submit_s;   /* host steady-clock seconds: submitted (arrival) */
   double start_s;    /* admitted to a slot */
   double done_s;     /* device work observed complete */
+  /* With lbvh_server_set_digest on, digests of the request's results as
+   * they were when its device work ended, else 0: the sum over the u32
+   * words w_j of BVH_BUFFER (16 per triangle, j from 0) of
+   * w_j * (splitmix64(j) | 1) modulo 2^64, and the same over
+   * O_TRIANGLE_BUFFER (12 words per triangle).  A checker can compare
+   * every request's results with a host replay even after the mesh has
+   * been refitted again or evicted. */
+  uint64_t digest_bvh;
+  uint64_t digest_tri;
 } lbvh_completion;
 
 typedef struct {
@@ -292,9 +313,24 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
 /* Waits for in-flight work, then frees every resident mesh and slot. */
 int lbvh_server_destroy(lbvh_server *srv);
 int lbvh_server_set_div_mode(lbvh_server *srv, int div_mode);
+/* Requests admitted from now on also digest their results (one extra pass
+ * over the mesh's BVH and O_TRI, about 112 bytes per triangle); no
+ * counterpart in the reference: for checkers of the serving loop. */
+int lbvh_server_set_digest(lbvh_server *srv, int enable);
 int lbvh_server_submit(lbvh_server *srv, int kind, uint64_t mesh_id,
                        int source, const float *tri, uint32_t n,
                        uint64_t seed, double param, uint64_t *out_request_id);
+/* Cancels a request that is still waiting for admission: it completes with
+ * LBVH_ERR_CANCELLED and nothing of it runs.  LBVH_ERR_BUSY if it is already
+ * admitted (its device work runs to the end), LBVH_ERR_NOT_FOUND if it has
+ * completed or never existed.  No counterpart in the reference. */
+int lbvh_server_cancel(lbvh_server *srv, uint64_t request_id);
+/* Requests submitted from now on that are not admitted within max_wait_s
+ * seconds of their submission complete with LBVH_ERR_TIMEOUT without
+ * running (0: wait as long as it takes, the default).  Checked as the
+ * dispatcher scans the queue's head, within about a millisecond of the
+ * deadline.  No counterpart in the reference. */
+int lbvh_server_set_max_wait(lbvh_server *srv, double max_wait_s);
 /* Up to `max` completions not yet returned (oldest first). */
 int lbvh_server_poll(lbvh_server *srv, lbvh_completion *out, uint32_t max,
                      uint32_t *out_count);

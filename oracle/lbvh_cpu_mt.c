@@ -249,3 +249,72 @@ int oracle_build_mt(const float *tri, uint32_t n, const float *bounds_or_null,
   free(box); free(keys); free(idx);
   return threads;
 }
+
+/* Refit: the up pass over new vertices with the topology of an earlier
+ * build (its sorted keys and permutation), multi-threaded.  Byte-identical
+ * to oracle_refit (lbvh_oracle.c). */
+void oracle_refit_mt(const float *tri, uint32_t n, const uint32_t *sorted_keys,
+                     const int32_t *perm, void *bvh_out, float *tri_out) {
+  if (n == 0) return;
+  float *box = (float *)malloc(24 * (size_t)n);
+  const int64_t N = n;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    const float *t = tri + 12 * i;
+    for (int c = 0; c < 3; ++c) {
+      box[6 * i + c] = mt_fmin(t[c], mt_fmin(t[4 + c], t[8 + c]));
+      box[6 * i + 3 + c] = mt_fmax(t[c], mt_fmax(t[4 + c], t[8 + c]));
+    }
+  }
+  mt_node *bvh = (mt_node *)bvh_out;
+  memset(bvh, 0, sizeof(mt_node) * (size_t)n);
+  if (n == 1) {
+    for (int c = 0; c < 3; ++c) {
+      bvh[0].l_min[c] = box[c] == 0.0f ? -1e-8f : box[c];
+      bvh[0].l_max[c] = box[3 + c] == 0.0f ? 1e-8f : box[3 + c];
+    }
+    bvh[0].l_child = -1;
+    bvh[0].f_1 = 1;
+  } else {
+    int32_t *child = (int32_t *)malloc(8 * (size_t)n);
+    int32_t *par_node = (int32_t *)calloc(n, 4);
+    int32_t *par_leaf = (int32_t *)malloc(4 * (size_t)n);
+    uint32_t *arrive = (uint32_t *)calloc(n, 4);
+    mt_tree(sorted_keys, n, child, par_node, par_leaf);
+    mt_up(perm, box, child, par_node, par_leaf, n, bvh, arrive);
+    free(child); free(par_node); free(par_leaf); free(arrive);
+  }
+  if (tri_out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i)
+      memcpy(tri_out + 12 * i, tri + 12 * (size_t)perm[i], 48);
+  }
+  free(box);
+}
+
+static uint64_t mt_splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+/* Digests of a build's outputs: out[0] = sum over the u32 words w_j of the
+ * BVH (16 per triangle) of w_j * (splitmix64(j) | 1), out[1] the same over
+ * O_TRI (12 per triangle), both modulo 2^64.  The serving loop reports the
+ * same digests of its results (lbvh_completion.digest_bvh / digest_tri). */
+void oracle_digest(const void *bvh, const float *tri_out, uint32_t n,
+                   uint64_t out[2]) {
+  const uint32_t *wb = (const uint32_t *)bvh;
+  const uint32_t *wt = (const uint32_t *)tri_out;
+  const int64_t nb = 16 * (int64_t)n, nt = 12 * (int64_t)n;
+  uint64_t a = 0, b = 0;
+#pragma omp parallel for schedule(static) reduction(+ : a)
+  for (int64_t j = 0; j < nb; ++j)
+    a += (uint64_t)wb[j] * (mt_splitmix64((uint64_t)j) | 1ull);
+#pragma omp parallel for schedule(static) reduction(+ : b)
+  for (int64_t j = 0; j < nt; ++j)
+    b += (uint64_t)wt[j] * (mt_splitmix64((uint64_t)j) | 1ull);
+  out[0] = a;
+  out[1] = b;
+}

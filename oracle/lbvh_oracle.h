@@ -56,6 +56,14 @@ int oracle_build_mt(const float *tri, uint32_t n, const float *bounds_or_null,
                     uint32_t *sorted_keys_out, int32_t *perm_out);
 void oracle_refit(const float *tri, uint32_t n, const uint32_t *sorted_keys,
                   const int32_t *perm, void *bvh_out, float *tri_out);
+/* The refit, multi-threaded (lbvh_cpu_mt.c): byte-identical outputs. */
+void oracle_refit_mt(const float *tri, uint32_t n, const uint32_t *sorted_keys,
+                     const int32_t *perm, void *bvh_out, float *tri_out);
+/* Digests of a build's BVH and O_TRI (the serving loop's per-request
+ * digests): out[0] = sum_j w_j * (splitmix64(j) | 1) mod 2^64 over the BVH's
+ * u32 words, out[1] the same over O_TRI's (lbvh_cpu_mt.c). */
+void oracle_digest(const void *bvh, const float *tri_out, uint32_t n,
+                   uint64_t out[2]);
 
 /* Closest-hit ray casting as the demo's path tracer does it
  * (pathtracer.js:495-628).  rays: 8 floats each (origin.xyz, pad, dir.xyz,

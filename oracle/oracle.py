@@ -25,6 +25,10 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
                          on hits/misses and closest distances
   build_mt               the multi-threaded CPU build (the CPU baseline):
                          byte-identical to build() on every output
+  refit / refit_mt       brute force on jittered vertices; refit_mt
+                         byte-identical to refit
+  digest                 an independent numpy restatement of the weighted
+                         word sums; a single changed word changes it
 """
 from __future__ import annotations
 
@@ -71,6 +75,8 @@ def lib():
             "oracle_build": (None, [vp, u32, vp, i32, vp, vp, vp, vp]),
             "oracle_build_mt": (i32, [vp, u32, vp, i32, vp, vp, vp, vp]),
             "oracle_refit": (None, [vp, u32, vp, vp, vp, vp]),
+            "oracle_refit_mt": (None, [vp, u32, vp, vp, vp, vp]),
+            "oracle_digest": (None, [vp, vp, u32, vp]),
             "oracle_raycast": (None, [vp, vp, u32, vp, u32, vp]),
             "oracle_raycast_bruteforce": (None, [vp, u32, vp, u32, vp]),
             "oracle_bruteforce_tree": (None, [vp, u32, vp]),
@@ -197,6 +203,30 @@ def refit(tri, skeys, perm):
     otri = np.empty((n, 12), np.float32)
     lib().oracle_refit(_p(tri), n, _p(sk), _p(pm), _p(bvh), _p(otri))
     return bvh, otri
+
+
+def refit_mt(tri, skeys, perm):
+    """refit() on all host cores (OpenMP), byte-identical."""
+    tri = _tri(tri)
+    n = tri.shape[0]
+    sk = np.ascontiguousarray(skeys, dtype=np.uint32)
+    pm = np.ascontiguousarray(perm, dtype=np.int32)
+    bvh = np.zeros(64 * n, np.uint8)
+    otri = np.empty((n, 12), np.float32)
+    lib().oracle_refit_mt(_p(tri), n, _p(sk), _p(pm), _p(bvh), _p(otri))
+    return bvh, otri
+
+
+def digest(bvh, otri):
+    """(BVH digest, O_TRI digest) of a build's outputs, as the serving loop
+    reports them per request."""
+    b = np.ascontiguousarray(bvh).view(np.uint8)
+    t = np.ascontiguousarray(otri, dtype=np.float32).reshape(-1, 12)
+    n = t.shape[0]
+    assert b.size == 64 * n
+    out = np.zeros(2, np.uint64)
+    lib().oracle_digest(_p(b), _p(t), n, _p(out))
+    return int(out[0]), int(out[1])
 
 
 def bruteforce_tree(skeys):

@@ -13,7 +13,8 @@ for (const name of ['initBVHBuild', 'initAABB_ZidxKernel', 'initRadixSortKernel'
   assert.strictEqual(typeof bvh[name], 'function', name)
 }
 for (const name of ['buildBatched', 'autotune', 'raycast', 'createServer', 'serverSubmit',
-  'serverPoll', 'serverDrain', 'serverMesh', 'serverEvict', 'serverStats']) {
+  'serverPoll', 'serverDrain', 'serverMesh', 'serverEvict', 'serverStats',
+  'serverSetDigest', 'serverCancel', 'serverSetMaxWait']) {
   assert.strictEqual(typeof bvh.addon[name], 'function', 'addon.' + name)
 }
 

@@ -95,3 +95,52 @@ def test_batch_log_uniform_256(ctx, lb):
         wb, wo = ctx.build_arrays(meshes[k], None)
         assert np.array_equal(bvh[64 * a:64 * b], wb), k
         assert np.array_equal(otri[a:b], wo), k
+
+
+def test_batch_benchmark_shape_every_mesh_matches_oracle(lb, ora):
+    """The batched benchmark's exact workload: 2,048 uniform meshes, sizes
+    log-uniform in [256, 262144] (78.9M triangles), GPU bounds, on a fresh
+    untuned context as the benchmark runs it.  Every mesh's sorted keys,
+    permutation, BVH nodes and O_TRI equal the oracle's multi-threaded
+    build of that mesh alone, and the per-mesh lbvh_build loop the speedup
+    is measured against gives the same bytes."""
+    meshes = 2048
+    sizes = lb.wl_log_uniform_sizes(lb.BATCH_SEED, meshes, 256, 262144)
+    offsets = np.zeros(meshes + 1, np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    total = int(offsets[-1])
+    say(f"batch benchmark shape: {meshes} meshes, {total} triangles")
+    tri = np.empty((total, 12), np.float32)
+    for k in range(meshes):
+        tri[offsets[k]:offsets[k + 1]] = lb.wl_uniform(lb.BATCH_SEED * 100000 + k,
+                                                       int(sizes[k]), 0.01)
+    with lb.Context(0) as ctx:
+        d = ctx.to_device(tri)
+        bvh, otri = ctx.alloc(64 * total), ctx.alloc(48 * total)
+        sk, pm = ctx.alloc(4 * total), ctx.alloc(4 * total)
+        try:
+            ctx.build_batched(d, offsets, None, bvh, otri, sk, pm)
+            got_bvh = bvh.read(np.uint8, 64 * total)
+            got_otri = otri.read(np.uint32, 12 * total).reshape(-1, 12)
+            got_sk, got_pm = sk.read(np.uint32, total), pm.read(np.int32, total)
+            say("batch benchmark shape: per-mesh loop")
+            for k in range(meshes):
+                o = int(offsets[k])
+                ctx.build(d.ptr + 48 * o, int(sizes[k]), None, bvh.ptr + 64 * o,
+                          otri.ptr + 48 * o)
+            loop_bvh = bvh.read(np.uint8, 64 * total)
+            loop_otri = otri.read(np.uint32, 12 * total).reshape(-1, 12)
+        finally:
+            for b in (d, bvh, otri, sk, pm):
+                b.destroy()
+    assert np.array_equal(loop_bvh, got_bvh)
+    assert np.array_equal(loop_otri, got_otri)
+    del loop_bvh, loop_otri
+    say("batch benchmark shape: oracle per mesh")
+    for k in range(meshes):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        wb, wo, wsk, wpm, _ = ora.build_mt(tri[a:b], None, ora.DIV_MUL_RCP)
+        assert np.array_equal(got_sk[a:b], wsk), k
+        assert np.array_equal(got_pm[a:b], wpm), k
+        assert np.array_equal(got_bvh[64 * a:64 * b], wb), k
+        assert np.array_equal(got_otri[a:b], wo.view(np.uint32)), k

@@ -126,3 +126,57 @@ def test_js_server_build_and_refit(tmp_path, lb, ora):
     rb, ro = ora.refit(jit, wsk, wpm)
     assert np.array_equal(np.fromfile(str(tmp_path / "s.refit.bvh"), np.uint8), rb)
     assert not np.array_equal(jit, tri)
+
+
+def make_obj(nx=150, ny=100):
+    """An OBJ model as a modeller writes one: a height grid of quads (fan
+    triangulated by the parser), some faces as triangles with negative
+    indices and v/vt/vn references, comments, and a second object whose
+    vertices the demo ignores."""
+    rng = np.random.default_rng(5)
+    lines = ["# test model", "o ground"]
+    h = rng.uniform(0.0, 0.3, (ny + 1, nx + 1))
+    for j in range(ny + 1):
+        for i in range(nx + 1):
+            lines.append(f"v {i / nx:.6f} {h[j, i]:.6f} {j / ny * 0.7:.6f}")
+    lines += ["vt 0 0", "vn 0 1 0"]
+    vid = lambda i, j: j * (nx + 1) + i + 1
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)
+            if (i + j) % 7 == 0:
+                lines.append(f"f {a}/1/1 {b}/1/1 {c}/1/1")
+                lines.append(f"f {a} {c} {d}")
+            else:
+                lines.append(f"f {a} {b} {c} {d}")
+    # a pentagon from the last vertices, by negative index
+    lines.append("v 0.2 0.9 0.1")
+    lines.append("v 0.3 0.95 0.1")
+    lines.append("f -1 -2 -5 -7 -9")
+    lines += ["o props", "v 5 5 5", "v 6 5 5", "v 5 6 5", "f 1 2 3"]
+    return "\n".join(lines) + "\n"
+
+
+def test_js_obj_model_through_the_drop_in(tmp_path, ora):
+    """The demo's flow (main.js:72-88) on the GPU: parseObj of an OBJ text,
+    then initBVHBuild(device).build(TRI_ARR, NUM_TRIS, BOUNDS).  BOUNDS
+    excludes the two floor triangles, whose keys saturate; the result equals
+    the oracle's build of the same f32 triangles and bounds."""
+    node = shutil.which("node")
+    assert node, "node is required for the JS boundary test"
+    obj = tmp_path / "model.obj"
+    obj.write_text(make_obj())
+    prefix = str(tmp_path / "o")
+    print("node obj_build_cli", flush=True)
+    r = subprocess.run([node, os.path.join(ROOT, "tests", "js", "obj_build_cli.js"), str(obj),
+                        prefix], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n = json.loads(r.stdout.strip().splitlines()[-1])["NUM_TRIS"]
+    assert n == 2 * 150 * 100 + 3 + 2   # quads and split quads, pentagon fan, floor
+    tri = np.fromfile(prefix + ".tri", np.float32).reshape(-1, 12)
+    bounds = np.fromfile(prefix + ".bounds", np.float32)
+    # the floor lies outside the model's bounds
+    assert tri[-2:, 0].min() < bounds[0] and tri[-2:, 4].max() > bounds[3]
+    wb, wo, wsk, _ = ora.build(tri, bounds, ora.DIV_MUL_RCP)
+    assert np.array_equal(np.fromfile(prefix + ".bvh", np.uint8), wb)
+    assert np.array_equal(np.fromfile(prefix + ".otri", np.uint32), wo.view(np.uint32).ravel())

@@ -102,6 +102,55 @@ def test_aabb_morton_caller_bounds_narrower_than_mesh(ctx, lb, ora):
     ctx.set_div_mode(0)
 
 
+@pytest.mark.parametrize("bounds_from", ["gpu", "caller"])
+def test_planar_mesh_flat_axis(ctx, lb, ora, bounds_from):
+    """A planar mesh (every z equal): the flat axis has min == max, so its
+    quantised coordinate is 0/0 -> 0 for every triangle, in both division
+    modes, from GPU bounds and from caller bounds; a caller box flat on z
+    below the mesh saturates the other way."""
+    n = 30001
+    tri = lb.wl_uniform(57, n, 0.01)
+    tri[:, [2, 6, 10]] = np.float32(0.25)
+    say(f"planar mesh n={n} bounds={bounds_from}")
+    boxes = [None]
+    if bounds_from == "caller":
+        below = lb.wl_bounds(tri)
+        below[2] = below[5] = np.float32(0.125)   # flat box under the plane
+        boxes = [lb.wl_bounds(tri), below]
+    for b in boxes:
+        for mode in (0, 1):
+            ctx.set_div_mode(mode)
+            bvh, otri, sk, pm = ctx.build_arrays(tri, b, keep_topology=True)
+            wb, wo, wsk, wpm = ora.build(tri, b, mode)
+            assert np.array_equal(sk, wsk) and np.array_equal(pm, wpm)
+            assert np.array_equal(bvh, wb)
+            assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
+    ctx.set_div_mode(0)
+
+
+def test_non_finite_caller_bounds_are_refused(ctx, lb):
+    """NaN or infinite MODEL_BOUNDS are refused before any launch, by the
+    build and by the per-stage Morton entry point."""
+    tri = lb.wl_uniform(58, 1000, 0.01)
+    d = ctx.to_device(tri)
+    bvh, otri = ctx.alloc(64 * 1000), ctx.alloc(48 * 1000)
+    aabb, keys = ctx.alloc(32 * 1000), ctx.alloc(4 * 1000)
+    try:
+        for bad in (np.nan, np.inf, -np.inf):
+            b = lb.wl_bounds(tri)
+            b[4] = bad
+            with pytest.raises(lb.LBVHError) as err:
+                ctx.build(d, 1000, b, bvh, otri)
+            assert err.value.code == lb.ERR_INVALID_ARGUMENT
+            with pytest.raises(lb.LBVHError) as err:
+                ctx.aabb_morton(d, 1000, b, aabb, keys)
+            assert err.value.code == lb.ERR_INVALID_ARGUMENT
+        ctx.synchronize()
+    finally:
+        for x in (d, bvh, otri, aabb, keys):
+            x.destroy()
+
+
 class sort_bits:
     """Fixes the radix-sort digit width of ctx inside a with-block (the
     size-based default after)."""

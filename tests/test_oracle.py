@@ -341,3 +341,55 @@ def test_build_mt_matches_build(ora, n):
         for w, g in zip(want, got[:4]):
             assert np.array_equal(np.asarray(w).view(np.uint8), np.asarray(g).view(np.uint8))
         assert got[4] >= 1
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1000, 70001])
+def test_refit_mt_matches_refit(ora, lb, n):
+    """The multi-threaded refit (used to replay the serving loop) gives the
+    scalar refit's bytes on jittered vertices, zeros in leaf boxes included."""
+    tri = lb.wl_clustered(n + 5, n, 0.05) if n > 2 else lb.wl_uniform(n + 5, n, 0.01)
+    _, _, sk, pm = ora.build(tri)
+    moved = lb.wl_jitter(n + 6, tri, 0.001)
+    if n > 100:
+        moved[17, 0:3] = 0.0
+    want = ora.refit(moved, sk, pm)
+    got = ora.refit_mt(moved, sk, pm)
+    assert np.array_equal(want[0], got[0])
+    assert np.array_equal(want[1].view(np.uint32), got[1].view(np.uint32))
+
+
+def _splitmix64_np(x):
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _digest_np(words):
+    words = words.reshape(-1)
+    with np.errstate(over="ignore"):
+        j = np.arange(words.size, dtype=np.uint64)
+        w = _splitmix64_np(j) | np.uint64(1)
+        return int(np.sum(words.astype(np.uint64) * w, dtype=np.uint64))
+
+
+def test_digest_matches_numpy_and_sees_single_words(ora, lb):
+    """oracle.digest, the serving loop's per-request digest, against an
+    independent numpy restatement; changing any one word (or swapping two
+    different words) changes it."""
+    tri = lb.wl_clustered(3, 5000, 0.05)
+    bvh, otri, _, _ = ora.build(tri)
+    want = (_digest_np(bvh.view(np.uint32)), _digest_np(otri.view(np.uint32)))
+    assert ora.digest(bvh, otri) == want
+    # splitmix64 known answer: the first output of the reference generator
+    # seeded with 0 (Vigna's splitmix64.c) is 0xE220A8397B1DCDAF
+    assert int(_splitmix64_np(np.array([0], np.uint64))[0]) == 0xE220A8397B1DCDAF
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        b2 = bvh.copy().view(np.uint32)
+        j = int(rng.integers(0, b2.size))
+        b2[j] ^= np.uint32(1 << int(rng.integers(0, 32)))
+        assert ora.digest(b2.view(np.uint8), otri)[0] != want[0]
+    o2 = otri.copy().view(np.uint32).reshape(-1)
+    o2[[5, 9]] = o2[[9, 5]]
+    assert ora.digest(bvh, o2.view(np.float32).reshape(-1, 12))[1] != want[1]

@@ -575,6 +575,47 @@ napi_value ServerSetDivMode(napi_env env, napi_callback_info info) {
   return undefined(env);
 }
 
+// serverSetDigest(srv, enable): later requests digest their results.
+napi_value ServerSetDigest(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  bool on = false;
+  napi_get_value_bool(env, argv[1], &on);
+  if (throw_server(env, srv, lbvh_server_set_digest(srv, on ? 1 : 0),
+                   "lbvh_server_set_digest"))
+    return nullptr;
+  return undefined(env);
+}
+
+// serverCancel(srv, requestId): cancels a request still waiting.
+napi_value ServerCancel(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  uint64_t id = 0;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv) ||
+      !get_u64(env, argv[1], &id))
+    return nullptr;
+  if (throw_server(env, srv, lbvh_server_cancel(srv, id), "lbvh_server_cancel"))
+    return nullptr;
+  return undefined(env);
+}
+
+// serverSetMaxWait(srv, seconds): queueing limit of later requests.
+napi_value ServerSetMaxWait(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  lbvh_server *srv;
+  if (!get_args(env, info, 2, argv) || !get_server(env, argv[0], &srv))
+    return nullptr;
+  double w = 0.0;
+  napi_get_value_double(env, argv[1], &w);
+  if (throw_server(env, srv, lbvh_server_set_max_wait(srv, w),
+                   "lbvh_server_set_max_wait"))
+    return nullptr;
+  return undefined(env);
+}
+
 // serverSubmit(srv, kind, meshId, source, tri, n, seed, param) -> request id.
 // tri: a TypedArray (host source, copied at submit), a BigInt device pointer
 // (device source) or null (synthesised sources).
@@ -619,7 +660,7 @@ napi_value ServerSubmit(napi_env env, napi_callback_info info) {
 }
 
 // serverPoll(srv) -> [{requestId, meshId, kind, status, n, slot, submitS,
-// startS, doneS}]
+// startS, doneS, digestBvh, digestTri}] (digests as BigInt)
 napi_value ServerPoll(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   lbvh_server *srv;
@@ -645,6 +686,8 @@ napi_value ServerPoll(napi_env env, napi_callback_info info) {
       napi_set_named_property(env, o, "submitS", num_value(env, c[i].submit_s));
       napi_set_named_property(env, o, "startS", num_value(env, c[i].start_s));
       napi_set_named_property(env, o, "doneS", num_value(env, c[i].done_s));
+      napi_set_named_property(env, o, "digestBvh", u64_value(env, c[i].digest_bvh));
+      napi_set_named_property(env, o, "digestTri", u64_value(env, c[i].digest_tri));
       napi_set_element(env, arr, at++, o);
     }
   } while (got == 256);
@@ -860,6 +903,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"buildBatched", BuildBatched},   {"autotune", Autotune},
       {"raycast", Raycast},             {"createServer", CreateServer},
       {"serverSetDivMode", ServerSetDivMode},
+      {"serverSetDigest", ServerSetDigest},
+      {"serverCancel", ServerCancel},
+      {"serverSetMaxWait", ServerSetMaxWait},
       {"serverSubmit", ServerSubmit},   {"serverPoll", ServerPoll},
       {"serverDrain", ServerDrain},     {"serverMesh", ServerMesh},
       {"serverEvict", ServerEvict},     {"serverStats", ServerStats},

@@ -276,8 +276,19 @@ class BVHServer {
   }
 
   // Completions not yet returned: [{ requestId, meshId, kind, status, n,
-  // slot, submitS, startS, doneS }]; times in seconds (steady clock).
+  // slot, submitS, startS, doneS, digestBvh, digestTri }]; times in seconds
+  // (steady clock); digests (BigInt) only after setDigest(true).
   poll () { return addon.serverPoll(this.srv) }
+
+  // Later requests digest their results, for checkers of the service.
+  setDigest (enable) { addon.serverSetDigest(this.srv, !!enable) }
+
+  // Cancels a request still waiting for admission (it completes with status
+  // 9); throws with code 7 if it already runs, 8 if it has completed.
+  cancel (requestId) { addon.serverCancel(this.srv, BigInt(requestId)) }
+
+  // Later requests not admitted within `seconds` complete with status 10.
+  setMaxWait (seconds) { addon.serverSetMaxWait(this.srv, seconds) }
 
   drain () { return addon.serverDrain(this.srv) }
 

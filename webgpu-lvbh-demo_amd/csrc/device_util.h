@@ -32,18 +32,29 @@ __device__ __forceinline__ TileRange sort_tile_range(uint32_t tile, uint32_t n,
   return {b, min(b + (uint32_t)kSortTile, se), sb, se};
 }
 
-// Adds one digit to a block's LDS histogram; a wave whose active lanes all
-// hold the same digit adds its lane count once.
+// Adds one digit per active lane to a block's LDS histogram.  LDS atomics on
+// one address serialise, and tiles of coherent keys give a wave few distinct
+// digits (a tile of the 16M heightfield holds 8.7 distinct digits on
+// average in the last pass), so the wave peels off its most common digit
+// groups first: the group of the first remaining lane's digit adds its lane
+// count once, up to kPeels times while groups hold at least kPeelMin lanes;
+// the remaining lanes add one each.
 __device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
-  const uint64_t active = __ballot(1);
-  const int leader = __ffsll((unsigned long long)active) - 1;
-  const uint32_t d0 = __shfl(d, leader, 64);
-  if (__ballot(d == d0) == active) {
-    if ((int)(threadIdx.x & 63) == leader)
-      atomicAdd(&s_hist[d0], (uint32_t)__popcll(active));
-  } else {
-    atomicAdd(&s_hist[d], 1u);
+  constexpr int kPeels = 4, kPeelMin = 8;
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t left = __ballot(1);
+#pragma unroll
+  for (int it = 0; it < kPeels; ++it) {
+    const int leader = __ffsll((unsigned long long)left) - 1;
+    const uint32_t d0 = __shfl(d, leader, 64);
+    const uint64_t same = __ballot(d == d0) & left;
+    const uint32_t cnt = (uint32_t)__popcll(same);
+    if (cnt < (uint32_t)kPeelMin && same != left) break;  // a diverse wave
+    if (lane == leader) atomicAdd(&s_hist[d0], cnt);
+    left &= ~same;
+    if (!left) return;
   }
+  if ((left >> lane) & 1) atomicAdd(&s_hist[d], 1u);
 }
 
 // Lanes of the wave that are `valid` and hold the calling lane's digit d
@@ -130,6 +141,45 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
   for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
   __syncthreads();
   return pre + x - v;
+}
+
+// ---- Morton keys (aabb-z-idx.js:192-208) ------------------------------------
+//
+// Shared by the single-mesh and the batched kernels so that both produce the
+// same key bits.  Every fp32 operation is an explicitly rounded intrinsic
+// and rounds once, as in WGSL.
+
+__device__ __forceinline__ uint32_t split_3(uint32_t u) {
+  uint32_t x = u;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// WGSL u32(f32): truncate toward zero, saturate to [0, 2^32-1]; NaN -> 0.
+__device__ __forceinline__ uint32_t f32_to_u32_sat(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+// The key of a box centre: rel = (cen - min) / (max - min), divided
+// correctly rounded (ieee) or as d * fl(1 / ext), then u32(rel * 1023.99f)
+// per axis, interleaved.
+__device__ __forceinline__ uint32_t morton_key(const float ce[3],
+                                               const float bmin[3],
+                                               const float ext[3],
+                                               const float rcp[3], bool ieee) {
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float d = __fsub_rn(ce[k], bmin[k]);
+    const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
+    q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
+  }
+  return split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
 }
 
 }  // namespace lbvh

@@ -18,21 +18,6 @@ __device__ __forceinline__ float ord2f(uint32_t o) {
   return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-__device__ __forceinline__ uint32_t split3(uint32_t u) {
-  uint32_t x = u;
-  x = (x | (x << 16)) & 0x030000FFu;
-  x = (x | (x << 8)) & 0x0300F00Fu;
-  x = (x | (x << 4)) & 0x030C30C3u;
-  x = (x | (x << 2)) & 0x09249249u;
-  return x;
-}
-
-__device__ __forceinline__ uint32_t to_u32_sat(float v) {
-  if (!(v > 0.0f)) return 0u;
-  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
-  return (uint32_t)v;
-}
-
 // Resets the per-mesh bounds accumulators (when the GPU computes the
 // bounds).
 __global__ __launch_bounds__(kMapThreads) void k_batch_prologue(
@@ -161,14 +146,7 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
       __syncthreads();  // s_tri is refilled for the next chunk
     }
     if (i >= end) continue;
-    uint32_t q[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float d = __fsub_rn(ce[k], bmin[k]);
-      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
-      q[k] = to_u32_sat(__fmul_rn(rel, 1023.99f));
-    }
-    const uint32_t key = split3(q[0]) | (split3(q[1]) << 1) | (split3(q[2]) << 2);
+    const uint32_t key = morton_key(ce, bmin, ext, rcp, ieee);
     keys[i] = key;
     digit_add(s_hist, key & (kR - 1));
   }

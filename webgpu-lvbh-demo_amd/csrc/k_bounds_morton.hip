@@ -11,22 +11,6 @@
 
 namespace lbvh {
 
-__device__ __forceinline__ uint32_t split_3(uint32_t u) {
-  uint32_t x = u;
-  x = (x | (x << 16)) & 0x030000FFu;
-  x = (x | (x << 8)) & 0x0300F00Fu;
-  x = (x | (x << 4)) & 0x030C30C3u;
-  x = (x | (x << 2)) & 0x09249249u;
-  return x;
-}
-
-// WGSL u32(f32): truncate toward zero, saturate to [0, 2^32-1]; NaN -> 0.
-__device__ __forceinline__ uint32_t f32_to_u32_sat(float v) {
-  if (!(v > 0.0f)) return 0u;
-  if (v >= 4294967296.0f) return 0xFFFFFFFFu;
-  return (uint32_t)v;
-}
-
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
@@ -205,16 +189,11 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     if (i < n) {
       float mn[3], mx[3];
       staged_box(s_tri, t, mn, mx);
-      uint32_t q[3];
+      float ce[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float cen = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
-        const float d = __fsub_rn(cen, bmin[k]);
-        const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
-        q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
-      }
-      const uint32_t key =
-          split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+      for (int k = 0; k < 3; ++k)
+        ce[k] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
+      const uint32_t key = morton_key(ce, bmin, ext, rcp, ieee);
       keys[i] = key;
       if (aabb) {
         aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
@@ -271,14 +250,7 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
   const size_t cs = cen_stride(n);
   const auto key_of = [&](float x, float y, float z) {
     const float ce[3] = {x, y, z};
-    uint32_t q[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float d = __fsub_rn(ce[k], bmin[k]);
-      const float rel = ieee ? __fdiv_rn(d, ext[k]) : __fmul_rn(d, rcp[k]);
-      q[k] = f32_to_u32_sat(__fmul_rn(rel, 1023.99f));
-    }
-    return split_3(q[0]) | (split_3(q[1]) << 1) | (split_3(q[2]) << 2);
+    return morton_key(ce, bmin, ext, rcp, ieee);
   };
   // All of the thread's centres are loaded before any is used: one
   // iteration's loads per wave in flight leave the kernel latency-bound far

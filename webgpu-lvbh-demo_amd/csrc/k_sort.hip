@@ -550,6 +550,17 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                        vals_out, n, shift, offs, bases_from, small, sg);
 }
 
+__global__ void k_single_key(const uint32_t *keys, uint32_t *sorted,
+                             int32_t *perm) {
+  sorted[0] = keys[0];
+  perm[0] = 0;
+}
+
+void launch_single_key(const uint32_t *keys, uint32_t *sorted, int32_t *perm,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_single_key, dim3(1), dim3(1), 0, s, keys, sorted, perm);
+}
+
 void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       const int32_t *vals_in, uint32_t *keys_out,
                       int32_t *vals_out, uint32_t n, uint32_t tiles,

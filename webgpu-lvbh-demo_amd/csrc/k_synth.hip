@@ -1,26 +1,20 @@
 // k_synth.hip -- device-side synthetic meshes for the serving loop.
 //
 // The same counter-based generator as the host workloads library
-// (include/lbvh_workloads.h): SplitMix64 of (seed, element, draw) -> f64 in
-// [0,1).  Uniform meshes reproduce lbvh_wl_uniform bit for bit (only f64
-// adds and multiplies, each rounded once).  The jitter draws Gaussians by
-// Box-Muller with the device's f64 log/cos, so it is statistically, not
-// bitwise, the host lbvh_wl_jitter.
+// (include/lbvh_workloads.h, csrc/synth_math.h): SplitMix64 of (seed,
+// element, draw) -> f64 in [0,1).  Uniform meshes reproduce lbvh_wl_uniform
+// bit for bit (only f64 adds and multiplies, each rounded once), and so
+// does the jitter: its Gaussians come from synth_math.h's Box-Muller, whose
+// ln, cos and sqrt are plain f64 arithmetic, so lbvh_wl_jitter on the host
+// replays a jittered mesh exactly.
+#define LBVH_HD __host__ __device__
 #include "device_util.h"
+#include "synth_math.h"
 
 namespace lbvh {
 
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-
-__device__ __forceinline__ double u01(uint64_t hseed, uint64_t i, uint64_t k) {
-  const uint64_t h = splitmix64(hseed ^ (i * 64 + k));
-  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
-}
+using lbvh_synth::splitmix64;
+using lbvh_synth::u01;
 
 __global__ __launch_bounds__(kMapThreads) void k_synth_uniform(
     uint64_t seed, uint32_t n, double h, float4 *tri) {
@@ -54,10 +48,7 @@ __global__ __launch_bounds__(kMapThreads) void k_synth_jitter(
       float *c = &v.x;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const uint64_t draw = 2 * (3 * k + a);
-        double x = u01(hseed, i, draw), y = u01(hseed, i, draw + 1);
-        if (x < 1e-300) x = 1e-300;
-        const double g = sqrt(-2.0 * log(x)) * cos(6.283185307179586 * y);
+        const double g = lbvh_synth::gauss(hseed, i, 2 * (3 * k + a));
         c[a] = (float)((double)c[a] + sigma * g);
       }
       tri[3 * i + k] = v;

@@ -75,6 +75,274 @@ __global__ __launch_bounds__(kMapThreads) void k_radix_tree(
   par[(hi == gamma + 1) ? N + gamma + 1 : gamma + 1] = (int32_t)i;
 }
 
+// The same parents array by all nearest smaller values.  The Karras tree is
+// the min-Cartesian tree of the gap deltas d(g) = delta(g, g+1): the node
+// split at gap g covers the leaves between the nearest gaps with a smaller
+// delta on either side, L(g) and R(g) (sentinels -1 and n-1, delta -1; two
+// equal deltas never face each other across larger ones, so "smaller" is
+// unambiguous).  Its parent splits at whichever of L and R has the larger
+// delta: at R, the node is the left child and its Karras id is R; at L, the
+// right child with id L + 1; the root (both sentinels) is 0.  Its children
+// are numbered g and g + 1 (leaves when single), so gap g writes exactly the
+// parents slots g or n + g, and g + 1 or n + g + 1 (radix-tree.js:146-156).
+//
+// Three launches, no waiting between blocks:
+//   k_rt_tiles    a block takes 4,096 gaps: deltas from its keys in LDS, L
+//                 and R inside 64-gap segments from bit-plane ballots, then
+//                 across the tile's segments; it writes the parents slots of
+//                 every gap whose bounds it found.  Gaps with no smaller
+//                 delta on one side inside the tile (its left and right
+//                 spines, strictly monotone, so at most 64 each) are kept
+//                 as records, and the spines give the tile's summary: for
+//                 each delta value v, its last and its first gap with a
+//                 smaller delta.
+//   k_rt_scan     per value v, the summaries of the tiles before (after)
+//                 each tile: a prefix max (suffix min) over tiles.
+//   k_rt_resolve  each record takes its missing bound from the scans and
+//                 writes its slots.
+constexpr int kRtThreads = 256;
+constexpr int kRtPer = 16;                          // gaps per thread
+constexpr int kRtTile = kRtThreads * kRtPer;        // 4096 gaps per tile
+constexpr int kRtSegs = kRtTile / 64;               // 64-gap segments
+constexpr int kRtRecords = 128;                     // spine gaps per tile
+constexpr int kRtPlanes = 7;                        // delta bits (<= 64)
+constexpr int kRtNone = 64;                         // "no gap" delta
+constexpr int kRtScanThreads = 256;
+
+// Lanes of a segment whose delta is below v, from its bit planes.
+__device__ __forceinline__ uint64_t rt_below(const uint64_t *planes, int v) {
+  uint64_t lt = 0, eq = ~0ull;
+#pragma unroll
+  for (int b = kRtPlanes - 1; b >= 0; --b) {
+    const uint64_t pb = planes[b];
+    if ((v >> b) & 1) {
+      lt |= eq & ~pb;
+      eq &= pb;
+    } else {
+      eq &= ~pb;
+    }
+  }
+  return lt;
+}
+
+// Writes gap g's two child slots: its Karras id from its bounds.
+__device__ __forceinline__ void rt_write(int32_t *__restrict__ par, uint32_t n,
+                                         int64_t g, int64_t L, int dL,
+                                         int64_t R, int dR) {
+  const int32_t id =
+      (dL < 0 && dR < 0) ? 0 : (dL > dR ? (int32_t)(L + 1) : (int32_t)R);
+  par[(L + 1 == g) ? (int64_t)n + g : g] = id;
+  par[(g + 1 == R) ? (int64_t)n + g + 1 : g + 1] = id;
+}
+
+// A deferred gap: {g, v | flags << 8 | (dL + 1) << 16 | (dR + 1) << 24, L, R}
+// with flag 1: L missing, 2: R missing.
+__global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
+    const uint32_t *__restrict__ keys, uint32_t n, int32_t *__restrict__ par,
+    int4 *__restrict__ recs, uint32_t *__restrict__ rec_cnt,
+    unsigned long long *__restrict__ last,
+    unsigned long long *__restrict__ first, uint32_t tiles) {
+  __shared__ uint32_t s_key[kRtTile + 1];
+  __shared__ uint8_t s_d[kRtTile];
+  __shared__ uint64_t s_planes[kRtSegs][kRtPlanes];
+  __shared__ int s_min[kRtSegs];
+  __shared__ int s_lpos[64], s_rpos[64];  // spine gap of each delta value
+  __shared__ uint32_t s_nrec;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t base = (int64_t)t * kRtTile;
+  const int64_t m = (int64_t)n - 1;  // gaps
+  if (t == 0 && tid == 0) {
+    par[0] = 0;       // the root's slot: no gap has the root as a child
+    par[n - 1] = 0;   // no internal node n-1
+    if (n == 1) par[1] = 0;
+  }
+  for (int i = tid; i <= kRtTile; i += kRtThreads) {
+    const int64_t k = base + i;
+    s_key[i] = k < (int64_t)n ? keys[k] : 0u;
+  }
+  if (tid < 64) {
+    s_lpos[tid] = -1;
+    s_rpos[tid] = -1;
+  }
+  if (tid == 0) s_nrec = 0;
+  __syncthreads();
+  // deltas and the bit planes of each segment; wave w owns segments
+  // 16w .. 16w + 15, lane l gap l of each
+  int dv[kRtPer];
+#pragma unroll
+  for (int j = 0; j < kRtPer; ++j) {
+    const int sg = wave * kRtPer + j;
+    const int i = sg * 64 + lane;
+    const int64_t g = base + i;
+    int v = kRtNone;
+    if (g < m) {
+      const uint32_t a = s_key[i], b = s_key[i + 1];
+      v = (a != b) ? (int)__clz(a ^ b)
+                   : 32 + (int)__clz((uint32_t)g ^ (uint32_t)(g + 1));
+    }
+    dv[j] = v;
+    s_d[i] = (uint8_t)v;
+    uint64_t pl[kRtPlanes];
+#pragma unroll
+    for (int b = 0; b < kRtPlanes; ++b) pl[b] = __ballot((v >> b) & 1);
+    int mn = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o, 64));
+    if (lane < kRtPlanes) s_planes[sg][lane] = pl[lane];
+    if (lane == 0) s_min[sg] = mn;
+  }
+  __syncthreads();
+  const uint64_t below_me = (1ull << lane) - 1;
+  const uint64_t above_me = lane == 63 ? 0ull : ~((2ull << lane) - 1);
+#pragma unroll 1
+  for (int j = 0; j < kRtPer; ++j) {
+    const int sg = wave * kRtPer + j;
+    const int i = sg * 64 + lane;
+    const int64_t g = base + i;
+    const int v = dv[j];
+    if (g >= m) continue;
+    const uint64_t lt = rt_below(s_planes[sg], v);
+    int li = -1, ri = -1;  // tile-local bounds, -1: not in the tile
+    uint64_t ml = lt & below_me, mr = lt & above_me;
+    if (ml) {
+      li = sg * 64 + 63 - (int)__clzll((long long)ml);
+    } else {
+      for (int s2 = sg - 1; s2 >= 0; --s2)
+        if (s_min[s2] < v) {
+          const uint64_t x = rt_below(s_planes[s2], v);
+          li = s2 * 64 + 63 - (int)__clzll((long long)x);
+          break;
+        }
+    }
+    if (mr) {
+      ri = sg * 64 + (int)__ffsll((long long)mr) - 1;
+    } else {
+      for (int s2 = sg + 1; s2 < kRtSegs; ++s2)
+        if (s_min[s2] < v) {
+          const uint64_t x = rt_below(s_planes[s2], v);
+          ri = s2 * 64 + (int)__ffsll((long long)x) - 1;
+          break;
+        }
+    }
+    const int dL = li >= 0 ? (int)s_d[li] : -1;
+    const int dR = ri >= 0 ? (int)s_d[ri] : -1;
+    if (li >= 0 && ri >= 0) {
+      rt_write(par, n, g, base + li, dL, base + ri, dR);
+      continue;
+    }
+    // a spine gap: its value is unique on each spine it is on
+    if (li < 0) s_lpos[v] = i;
+    if (ri < 0) s_rpos[v] = i;
+    const uint32_t r = atomicAdd(&s_nrec, 1u);
+    if (r < (uint32_t)kRtRecords) {
+      const int flags = (li < 0 ? 1 : 0) | (ri < 0 ? 2 : 0);
+      recs[(size_t)t * kRtRecords + r] =
+          make_int4((int)g, v | (flags << 8) | ((dL + 1) << 16) | ((dR + 1) << 24),
+                    li >= 0 ? (int)(base + li) : 0, ri >= 0 ? (int)(base + ri) : 0);
+    }
+  }
+  __syncthreads();
+  // The tile's summary per delta value v (lane v): its last gap with a
+  // smaller delta is the right-spine gap with the largest value below v,
+  // its first such gap the left-spine gap with the largest value below v.
+  if (wave == 0) {
+    const uint64_t rv = __ballot(s_rpos[lane] >= 0);
+    const uint64_t lv = __ballot(s_lpos[lane] >= 0);
+    const uint64_t mr = rv & below_me, ml = lv & below_me;
+    unsigned long long e_last = 0ull, e_first = ~0ull;
+    if (mr) {
+      const int u = 63 - (int)__clzll((long long)mr);
+      e_last = ((unsigned long long)(base + s_rpos[u] + 1) << 8) | (unsigned)u;
+    }
+    if (ml) {
+      const int u = 63 - (int)__clzll((long long)ml);
+      e_first = ((unsigned long long)(base + s_lpos[u]) << 8) | (unsigned)u;
+    }
+    last[(size_t)lane * tiles + t] = e_last;
+    first[(size_t)lane * tiles + t] = e_first;
+    if (lane == 0) rec_cnt[t] = min(s_nrec, (uint32_t)kRtRecords);
+  }
+}
+
+// Exclusive scans of the tile summaries, in place: block c < 64, prefix max
+// of last[c]; block 64 + c, suffix min of first[c] (tiles in reverse).
+__global__ __launch_bounds__(kRtScanThreads) void k_rt_scan(
+    unsigned long long *__restrict__ last,
+    unsigned long long *__restrict__ first, uint32_t tiles) {
+  __shared__ unsigned long long s_w[kRtScanThreads / 64];
+  const int c = blockIdx.x & 63;
+  const bool suffix = blockIdx.x >= 64;
+  unsigned long long *col = (suffix ? first : last) + (size_t)c * tiles;
+  const unsigned long long id = suffix ? ~0ull : 0ull;
+  const auto op = [suffix](unsigned long long a, unsigned long long b) {
+    return suffix ? (a < b ? a : b) : (a > b ? a : b);
+  };
+  const uint32_t per = (tiles + kRtScanThreads - 1) / kRtScanThreads;
+  const uint32_t k0 = threadIdx.x * per, k1 = min(k0 + per, tiles);
+  // position q of the scan order -> tile (reversed for the suffix)
+  const auto at = [&](uint32_t q) -> unsigned long long & {
+    return col[suffix ? tiles - 1 - q : q];
+  };
+  unsigned long long agg = id;
+  for (uint32_t q = k0; q < k1; ++q) agg = op(agg, at(q));
+  // exclusive block scan of the per-thread aggregates
+  unsigned long long x = agg;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = __shfl_up(x, o, 64);
+    if (lane >= o) x = op(x, y);
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  unsigned long long pre = id;
+  for (int k = 0; k < w; ++k) pre = op(pre, s_w[k]);
+  unsigned long long ex = __shfl_up(x, 1, 64);
+  if (lane == 0) ex = id;
+  unsigned long long run = op(pre, ex);
+  for (uint32_t q = k0; q < k1; ++q) {
+    const unsigned long long v = at(q);
+    at(q) = run;
+    run = op(run, v);
+  }
+}
+
+// The spine gaps: missing bounds from the scans (none: the sentinels).
+__global__ __launch_bounds__(kRtRecords) void k_rt_resolve(
+    int32_t *__restrict__ par, uint32_t n, const int4 *__restrict__ recs,
+    const uint32_t *__restrict__ rec_cnt,
+    const unsigned long long *__restrict__ last,
+    const unsigned long long *__restrict__ first, uint32_t tiles) {
+  const uint32_t t = blockIdx.x;
+  if (threadIdx.x >= rec_cnt[t]) return;
+  const int4 r = recs[(size_t)t * kRtRecords + threadIdx.x];
+  const int v = r.y & 0xFF, flags = (r.y >> 8) & 3;
+  int dL = ((r.y >> 16) & 0xFF) - 1, dR = ((r.y >> 24) & 0xFF) - 1;
+  int64_t L = r.z, R = r.w;
+  if (flags & 1) {
+    const unsigned long long e = last[(size_t)v * tiles + t];
+    if (e == 0ull) {
+      L = -1;
+      dL = -1;
+    } else {
+      L = (int64_t)(e >> 8) - 1;
+      dL = (int)(e & 0xFF);
+    }
+  }
+  if (flags & 2) {
+    const unsigned long long e = first[(size_t)v * tiles + t];
+    if (e == ~0ull) {
+      R = (int64_t)n - 1;
+      dR = -1;
+    } else {
+      R = (int64_t)(e >> 8);
+      dR = (int)(e & 0xFF);
+    }
+  }
+  rt_write(par, n, r.x, L, dL, R, dR);
+}
+
 // ---------------------------------------------------------------------------
 // Shared leaf / node helpers
 // ---------------------------------------------------------------------------
@@ -1081,11 +1349,31 @@ static inline uint32_t blocks_for(uint32_t n, uint32_t per) {
   return b ? b : 1;
 }
 
+static uint32_t rt_tiles(uint32_t n) {
+  const uint32_t gaps = n > 1 ? n - 1 : 1;
+  return blocks_for(gaps, kRtTile);
+}
+
+uint64_t radix_tree_scratch_bytes(uint32_t n) {
+  const uint64_t t = rt_tiles(n);
+  return t * (16ull * kRtRecords + 4 + 2 * 64 * 8);
+}
+
 void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
-                       hipStream_t s) {
-  const uint32_t threads = n > 1 ? n - 1 : 1;
-  hipLaunchKernelGGL(k_radix_tree, dim3(blocks_for(threads, kMapThreads)),
-                     dim3(kMapThreads), 0, s, keys, n, parents);
+                       void *scratch, hipStream_t s) {
+  const uint32_t tiles = rt_tiles(n);
+  int4 *recs = static_cast<int4 *>(scratch);
+  unsigned long long *last =
+      reinterpret_cast<unsigned long long *>(recs + (size_t)tiles * kRtRecords);
+  unsigned long long *first = last + (size_t)64 * tiles;
+  uint32_t *cnt = reinterpret_cast<uint32_t *>(first + (size_t)64 * tiles);
+  hipLaunchKernelGGL(k_rt_tiles, dim3(tiles), dim3(kRtThreads), 0, s, keys, n,
+                     parents, recs, cnt, last, first, tiles);
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_rt_scan, dim3(128), dim3(kRtScanThreads), 0, s, last,
+                     first, tiles);
+  hipLaunchKernelGGL(k_rt_resolve, dim3(tiles), dim3(kRtRecords), 0, s,
+                     parents, n, recs, cnt, last, first, tiles);
 }
 
 uint64_t up_pass_scratch_bytes(uint32_t n) {

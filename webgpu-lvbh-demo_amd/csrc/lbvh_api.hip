@@ -56,6 +56,7 @@ struct lbvh_ctx {
   DevBuf keys_a, keys_b, idx_a, idx_b;  // sort ping-pong, Morton output
   DevBuf digits;                        // sort tile counts + offsets
   DevBuf up_items;                      // per-stage up-pass items and counts
+  DevBuf rt_scratch;                    // per-stage radix tree records, summaries
   DevBuf tree_items, tree_counts;       // agglomeration item slots, counts
   DevBuf partials;                      // bounds partials
   DevBuf cen;                           // triangle box centres (GPU bounds)
@@ -307,7 +308,7 @@ struct StageScope {
 
 std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
   return {&ctx->keys_a,  &ctx->keys_b,   &ctx->idx_a,    &ctx->idx_b,
-          &ctx->digits,  &ctx->up_items,    &ctx->tree_items,
+          &ctx->digits,  &ctx->up_items,    &ctx->rt_scratch, &ctx->tree_items,
           &ctx->tree_counts, &ctx->partials, &ctx->cen,
           &ctx->b_keys_a,
           &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
@@ -353,6 +354,21 @@ int check_spans(lbvh_ctx *ctx, std::initializer_list<Span> spans) {
                       " bytes, but its lbvh_alloc buffer holds " +
                       std::to_string(room) + " from that address");
   }
+  return LBVH_OK;
+}
+
+// Caller bounds (MODEL_BOUNDS, aabb-z-idx.js:113-124) must be finite: the
+// reference would quantise against NaN or infinite bounds into keys WGSL
+// leaves implementation-defined.  min == max on an axis is accepted and
+// deterministic: that axis's coordinate is 0 for centres on the plane (0/0
+// converts to 0) and saturates off it.
+int check_bounds(lbvh_ctx *ctx, const float *bounds, uint32_t count) {
+  if (!bounds) return LBVH_OK;
+  for (uint64_t i = 0; i < 6ull * count; ++i)
+    if (!std::isfinite(bounds[i]))
+      return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                  "caller bounds must be finite (entry " + std::to_string(i) +
+                      " is " + std::to_string(bounds[i]) + ")");
   return LBVH_OK;
 }
 
@@ -461,11 +477,12 @@ int ensure_partials(lbvh_ctx *ctx) {
 // with an odd pass count).  counted0: the Morton kernel already wrote pass
 // 0's per-tile counts into the digit buffer; the other passes count their
 // own.  key_bits: significant bits of the keys (30 for Morton keys).
-void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
-              uint32_t *x, int32_t *vtmp, int32_t *vout, uint32_t n,
-              bool counted0, int bits, int key_bits = kKeyBits) {
+void run_sort(lbvh_ctx *ctx, uint32_t *digits, const uint32_t *in,
+              uint32_t *tmp, uint32_t *out, uint32_t *x, int32_t *vtmp,
+              int32_t *vout, uint32_t n, bool counted0, int bits,
+              int key_bits = kKeyBits) {
   const uint32_t tiles = sort_tiles(n);
-  uint32_t *cnt = static_cast<uint32_t *>(ctx->digits.ptr);
+  uint32_t *cnt = digits;
   uint32_t *off = cnt + (size_t)sort_radix(bits) * tiles;
   const SortPlan P = sort_plan(bits, key_bits, in, tmp, out, x, vtmp, vout);
   for (int p = 0; p < P.passes; ++p) {
@@ -478,13 +495,11 @@ void run_sort(lbvh_ctx *ctx, const uint32_t *in, uint32_t *tmp, uint32_t *out,
 
 // The tree levels of one mesh (ensure_tree_scratch done): the leaf tiles,
 // then the item levels above them, each level its own profiling stage.
-void run_tree(lbvh_ctx *ctx, const float *tri, const uint32_t *skeys,
-              const int32_t *perm, uint32_t n, float *tri_out, void *bvh_out,
-              int tile) {
+void run_tree(lbvh_ctx *ctx, int4 *items, uint32_t *counts, const float *tri,
+              const uint32_t *skeys, const int32_t *perm, uint32_t n,
+              float *tri_out, void *bvh_out, int tile) {
   uint32_t grid[kMaxTreeLevels];
   const int levels = tree_levels(n, tile, grid);
-  int4 *items = static_cast<int4 *>(ctx->tree_items.ptr);
-  uint32_t *counts = static_cast<uint32_t *>(ctx->tree_counts.ptr);
   uint64_t base = 0, prev = 0;
   for (int L = 0; L < levels; ++L) {
     StageScope sc(ctx, L ? LBVH_STAGE_TREE_TOP : LBVH_STAGE_TREE);
@@ -517,6 +532,84 @@ MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
   std::lock_guard<std::recursive_mutex> lock_((ctx)->mu);     \
   (void)hipSetDevice((ctx)->device)
 
+
+// ---- the single-mesh build pipeline -------------------------------------------
+
+// Device scratch of one build: sort ping-pong buffers, per-tile digit
+// counts and offsets, agglomeration item slots and counts, bounds partials
+// and (GPU bounds) the box centres.
+struct BuildScratch {
+  uint32_t *keys_a = nullptr, *keys_b = nullptr;
+  int32_t *idx_a = nullptr, *idx_b = nullptr;
+  uint32_t *digits = nullptr;
+  int4 *tree_items = nullptr;
+  uint32_t *tree_counts = nullptr;
+  float4 *partials = nullptr;
+  float *cen = nullptr;
+};
+
+int check_build_args(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                     const void *bvh_out, const float *tri_out,
+                     const uint32_t *sorted_keys_out, const int32_t *perm_out) {
+  int st;
+  if ((st = check_n(ctx, n))) return st;
+  if (n == 0) return LBVH_OK;
+  if (!tri || !bvh_out || !tri_out)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
+  return check_spans(ctx, {{tri, 48ull * n, "triangles"},
+                           {bvh_out, 64ull * n, "BVH output"},
+                           {tri_out, 48ull * n, "triangle output"},
+                           {sorted_keys_out, 4ull * n, "sorted-key output"},
+                           {perm_out, 4ull * n, "permutation output"}});
+}
+
+// Enqueues the whole build of n >= 1 triangles: bounds (unless given),
+// Morton keys with the first pass's digit counts, the sort, the
+// agglomeration levels.  Enqueues kernels only: no allocation, copy or
+// memset.
+void build_pipeline(lbvh_ctx *ctx, const BuildScratch &S, const float *tri,
+                    uint32_t n, const float *bounds_host, void *bvh_out,
+                    float *tri_out, uint32_t *sorted_keys_out,
+                    int32_t *perm_out, int bits, int tile) {
+  const bool sort = n > 1;
+  // GPU bounds: the bounds pass also writes the box centres, and the Morton
+  // pass reads those (12 bytes per triangle) instead of the triangles
+  const bool use_cen = S.cen != nullptr;
+  const float4 *t4 = reinterpret_cast<const float4 *>(tri);
+  uint32_t *keys = S.keys_a;
+  uint32_t *tmp = S.keys_b;
+  uint32_t *skeys = sorted_keys_out ? sorted_keys_out : keys;
+  // an odd number of passes ends in the other buffer: sort into keys_b
+  if (sort && sort_passes(bits) % 2 && skeys == keys) std::swap(skeys, tmp);
+  int32_t *perm = perm_out ? perm_out : S.idx_a;
+  MortonParams p{};
+  uint32_t parts = 0;  // bounds partials the Morton pass folds itself
+  if (bounds_host) {
+    p = morton_params(ctx, bounds_host);
+  } else {
+    p.div_mode = ctx->div_mode;
+    StageScope sc(ctx, LBVH_STAGE_BOUNDS);
+    parts = launch_bounds(t4, n, S.partials, ctx->ctl, ctx->stream, S.cen,
+                          use_cen ? kFoldPartialsMax : 0);
+  }
+  {
+    StageScope sc(ctx, LBVH_STAGE_MORTON);
+    uint32_t *cnt = sort ? S.digits : nullptr;
+    if (use_cen)
+      launch_morton_cen(S.cen, n, ctx->div_mode, keys, ctx->ctl, cnt, bits,
+                        ctx->stream, S.partials, parts);
+    else
+      launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
+                         ctx->ctl, ctx->stream, cnt, bits);
+  }
+  if (sort)
+    run_sort(ctx, S.digits, keys, tmp, skeys, nullptr, S.idx_b, perm, n, true,
+             bits);
+  else
+    launch_single_key(keys, skeys, perm, ctx->stream);
+  run_tree(ctx, S.tree_items, S.tree_counts, tri, skeys, perm, n, tri_out,
+           bvh_out, tile);
+}
 
 // ---- batched builds ---------------------------------------------------------
 
@@ -650,6 +743,8 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
 
 // ---------------------------------------------------------------------------
 
+lbvh::Ctl *lbvh_context_ctl(lbvh_ctx *ctx) { return ctx ? ctx->ctl : nullptr; }
+
 extern "C" {
 
 const char *lbvh_status_string(int status) {
@@ -663,6 +758,8 @@ const char *lbvh_status_string(int status) {
     case LBVH_ERR_NO_DEVICE: return "no such device";
     case LBVH_ERR_BUSY: return "busy";
     case LBVH_ERR_NOT_FOUND: return "not found";
+    case LBVH_ERR_CANCELLED: return "cancelled";
+    case LBVH_ERR_TIMEOUT: return "timed out";
     default: return "unknown status";
   }
 }
@@ -1010,6 +1107,7 @@ int lbvh_aabb_morton(lbvh_ctx *ctx, const float *tri, uint32_t n,
                               {aabb_out, 32ull * n, "AABB output"},
                               {keys_out, 4ull * n, "key output"}})))
     return st;
+  if ((st = check_bounds(ctx, bounds_host, 1))) return st;
   MortonParams p{};
   if (bounds_host) {
     p = morton_params(ctx, bounds_host);
@@ -1038,14 +1136,15 @@ int lbvh_sort(lbvh_ctx *ctx, uint32_t *keys, int32_t *idx_out, uint32_t n) {
                               {idx_out, 4ull * n, "index output"}})))
     return st;
   if (n == 1) {
-    HIP_TRY(ctx, hipMemsetAsync(idx_out, 0, 4, ctx->stream));
-    return LBVH_OK;
+    launch_single_key(keys, keys, idx_out, ctx->stream);
+    return check_launch(ctx, "lbvh_sort");
   }
   // Any u32 key, as the reference's 16 passes of 2 bits sort: all 32 bits.
   // 10-bit digits take 4 passes here too, so they run only when fixed.
   const int bits = ctx->sort_bits_fixed ? ctx->sort_bits : kRadixBits;
   if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
-  run_sort(ctx, keys, static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
+  run_sort(ctx, static_cast<uint32_t *>(ctx->digits.ptr), keys,
+           static_cast<uint32_t *>(ctx->keys_b.ptr), keys,
            static_cast<uint32_t *>(ctx->keys_a.ptr),
            static_cast<int32_t *>(ctx->idx_a.ptr), idx_out, n, false, bits,
            kFullKeyBits);
@@ -1063,7 +1162,10 @@ int lbvh_radix_tree(lbvh_ctx *ctx, const uint32_t *sorted_keys, uint32_t n,
   if ((st = check_spans(ctx, {{sorted_keys, 4ull * n, "sorted keys"},
                               {parents_out, 8ull * n, "parents output"}})))
     return st;
-  launch_radix_tree(sorted_keys, n, parents_out, ctx->stream);
+  if ((st = ensure(ctx, ctx->rt_scratch, radix_tree_scratch_bytes(n), false)))
+    return st;
+  launch_radix_tree(sorted_keys, n, parents_out, ctx->rt_scratch.ptr,
+                    ctx->stream);
   return check_launch(ctx, "lbvh_radix_tree");
 }
 
@@ -1112,67 +1214,30 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
                uint32_t *sorted_keys_out, int32_t *perm_out) {
   CTX_GUARD(ctx);
   int st;
-  if ((st = check_n(ctx, n))) return st;
-  if (n == 0) return LBVH_OK;
-  if (!tri || !bvh_out || !tri_out)
-    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "null pointer");
-  if ((st = check_spans(ctx, {{tri, 48ull * n, "triangles"},
-                              {bvh_out, 64ull * n, "BVH output"},
-                              {tri_out, 48ull * n, "triangle output"},
-                              {sorted_keys_out, 4ull * n, "sorted-key output"},
-                              {perm_out, 4ull * n, "permutation output"}})))
+  if ((st = check_build_args(ctx, tri, n, bvh_out, tri_out, sorted_keys_out,
+                             perm_out)) || n == 0)
     return st;
+  if ((st = check_bounds(ctx, bounds_host, 1))) return st;
   const int bits = bits_for(ctx, n);
-  if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   const int tile = tile_for(ctx, n);
+  const bool use_cen = !bounds_host && n > 1;
+  if ((st = ensure_sort_scratch(ctx, n, bits))) return st;
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
-  const bool sort = n > 1;
-  // GPU bounds: the bounds pass also writes the box centres, and the Morton
-  // pass reads those (12 bytes per triangle) instead of the triangles
-  const bool use_cen = !bounds_host && sort;
   if (!bounds_host && (st = ensure_partials(ctx))) return st;
   if (use_cen && (st = ensure(ctx, ctx->cen, 12ull * cen_stride(n), false)))
     return st;
-
-  const float4 *t4 = reinterpret_cast<const float4 *>(tri);
-  float *cen = use_cen ? static_cast<float *>(ctx->cen.ptr) : nullptr;
-  uint32_t *keys = static_cast<uint32_t *>(ctx->keys_a.ptr);
-  uint32_t *tmp = static_cast<uint32_t *>(ctx->keys_b.ptr);
-  uint32_t *skeys = sorted_keys_out ? sorted_keys_out : keys;
-  // an odd number of passes ends in the other buffer: sort into keys_b
-  if (sort && sort_passes(bits) % 2 && skeys == keys) std::swap(skeys, tmp);
-  int32_t *perm = perm_out ? perm_out : static_cast<int32_t *>(ctx->idx_a.ptr);
-  MortonParams p{};
-  float4 *partials = static_cast<float4 *>(ctx->partials.ptr);
-  uint32_t parts = 0;  // bounds partials the Morton pass folds itself
-  if (bounds_host) {
-    p = morton_params(ctx, bounds_host);
-  } else {
-    p.div_mode = ctx->div_mode;
-    StageScope sc(ctx, LBVH_STAGE_BOUNDS);
-    parts = launch_bounds(t4, n, partials, ctx->ctl, ctx->stream, cen,
-                          use_cen ? kFoldPartialsMax : 0);
-  }
-  {
-    StageScope sc(ctx, LBVH_STAGE_MORTON);
-    uint32_t *cnt = sort ? static_cast<uint32_t *>(ctx->digits.ptr) : nullptr;
-    if (use_cen)
-      launch_morton_cen(cen, n, ctx->div_mode, keys, ctx->ctl, cnt, bits,
-                        ctx->stream, partials, parts);
-    else
-      launch_aabb_morton(t4, n, p, bounds_host == nullptr, nullptr, keys,
-                         ctx->ctl, ctx->stream, cnt, bits);
-  }
-  if (sort) {
-    run_sort(ctx, keys, tmp, skeys, nullptr,
-             static_cast<int32_t *>(ctx->idx_b.ptr), perm, n, true, bits);
-  } else {
-    HIP_TRY(ctx, hipMemsetAsync(perm, 0, 4, ctx->stream));
-    if (skeys != keys)
-      HIP_TRY(ctx, hipMemcpyAsync(skeys, keys, 4, hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-  }
-  run_tree(ctx, tri, skeys, perm, n, tri_out, bvh_out, tile);
+  BuildScratch S;
+  S.keys_a = static_cast<uint32_t *>(ctx->keys_a.ptr);
+  S.keys_b = static_cast<uint32_t *>(ctx->keys_b.ptr);
+  S.idx_a = static_cast<int32_t *>(ctx->idx_a.ptr);
+  S.idx_b = static_cast<int32_t *>(ctx->idx_b.ptr);
+  S.digits = static_cast<uint32_t *>(ctx->digits.ptr);
+  S.tree_items = static_cast<int4 *>(ctx->tree_items.ptr);
+  S.tree_counts = static_cast<uint32_t *>(ctx->tree_counts.ptr);
+  S.partials = static_cast<float4 *>(ctx->partials.ptr);
+  S.cen = use_cen ? static_cast<float *>(ctx->cen.ptr) : nullptr;
+  build_pipeline(ctx, S, tri, n, bounds_host, bvh_out, tri_out,
+                 sorted_keys_out, perm_out, bits, tile);
   return check_launch(ctx, "lbvh_build");
 }
 
@@ -1193,7 +1258,9 @@ int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,
     return st;
   const int tile = tile_for(ctx, n);
   if ((st = ensure_tree_scratch(ctx, n, tile))) return st;
-  run_tree(ctx, tri, sorted_keys, perm, n, tri_out, bvh_out, tile);
+  run_tree(ctx, static_cast<int4 *>(ctx->tree_items.ptr),
+           static_cast<uint32_t *>(ctx->tree_counts.ptr), tri, sorted_keys,
+           perm, n, tri_out, bvh_out, tile);
   return check_launch(ctx, "lbvh_refit");
 }
 
@@ -1224,6 +1291,7 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
                               {sorted_keys_out, 4ull * total, "sorted-key output"},
                               {perm_out, 4ull * total, "permutation output"}})))
     return st;
+  if ((st = check_bounds(ctx, bounds_host, num_meshes))) return st;
   if ((st = make_plan(ctx, offsets, num_meshes))) return st;
   const auto &P = ctx->plan;
   if ((st = ensure(ctx, ctx->b_keys_a, 4ull * total, false))) return st;

@@ -135,10 +135,18 @@ void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
+// The sort of a single key: perm[0] = 0 and sorted[0] = keys[0] (sorted may
+// be keys).  A one-thread kernel, so that a build stays kernels only.
+void launch_single_key(const uint32_t *keys, uint32_t *sorted, int32_t *perm,
+                       hipStream_t s);
 
 // k_tree.hip
+// The per-stage radix tree (parents array) in three launches over 4,096-gap
+// tiles; scratch of radix_tree_scratch_bytes(n) (spine records, per-tile
+// summaries), nothing in it needs clearing.
+uint64_t radix_tree_scratch_bytes(uint32_t n);
 void launch_radix_tree(const uint32_t *keys, uint32_t n, int32_t *parents,
-                       hipStream_t s);
+                       void *scratch, hipStream_t s);
 // Per-stage up pass: the level-1 kernel, then item levels until one block
 // holds the root; scratch of up_pass_scratch_bytes(n) (items, then counts).
 uint64_t up_pass_scratch_bytes(uint32_t n);
@@ -199,3 +207,8 @@ void launch_raycast(const void *nodes, const float *tri, uint32_t n,
                     hipStream_t s);
 
 }  // namespace lbvh
+
+// lbvh_api.hip: a context's device control block, for the server's
+// per-request epilogue (which reads and clears its fault word).
+struct lbvh_ctx;
+lbvh::Ctl *lbvh_context_ctl(lbvh_ctx *ctx);

@@ -27,9 +27,11 @@
 #include <unordered_set>
 #include <vector>
 
+#define LBVH_HD __host__ __device__
 #include "../../include/lbvh.h"
 #include "lbvh_internal.h"
 #include "range_arena.h"
+#include "synth_math.h"
 
 namespace lbvh {
 void launch_synth_uniform(uint64_t seed, uint32_t n, double half_extent,
@@ -39,6 +41,76 @@ void launch_synth_jitter(uint64_t seed, uint32_t n, double sigma, float *tri,
 }  // namespace lbvh
 
 namespace {
+
+// ---- per-request epilogue -----------------------------------------------------
+
+// What a request's device work leaves for the dispatcher, in pinned host
+// memory: the slot context's fault bits and the result digests.
+struct RequestResult {
+  uint32_t fault;
+  uint32_t pad;
+  unsigned long long digest[2];  // BVH words, O_TRI words
+};
+
+__device__ __forceinline__ unsigned long long digest_weight(uint64_t j) {
+  return lbvh_synth::splitmix64(j) | 1ull;
+}
+
+__device__ __forceinline__ unsigned long long digest4(uint4 v, uint64_t j) {
+  return (unsigned long long)v.x * digest_weight(j) +
+         (unsigned long long)v.y * digest_weight(j + 1) +
+         (unsigned long long)v.z * digest_weight(j + 2) +
+         (unsigned long long)v.w * digest_weight(j + 3);
+}
+
+constexpr int kDigestThreads = 256;
+
+// sum[0] += the BVH words' weighted sum, sum[1] += the O_TRI words' (both
+// modulo 2^64, so the order of the additions does not matter).
+__global__ __launch_bounds__(kDigestThreads) void k_digest(
+    const uint4 *__restrict__ bvh, const uint4 *__restrict__ otri, uint32_t n,
+    unsigned long long *sum) {
+  __shared__ unsigned long long s_part[2][kDigestThreads / 64];
+  const uint64_t nb = 4ull * n, no = 3ull * n;  // 16-byte vectors
+  unsigned long long a = 0, b = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kDigestThreads;
+  for (uint64_t q = (uint64_t)blockIdx.x * kDigestThreads + threadIdx.x;
+       q < nb; q += stride) {
+    a += digest4(bvh[q], 4 * q);
+    if (q < no) b += digest4(otri[q], 4 * q);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o);
+    b += __shfl_xor(b, o);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_part[0][w] = a;
+    s_part[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kDigestThreads / 64; ++k) {
+      a += s_part[0][k];
+      b += s_part[1][k];
+    }
+    atomicAdd(&sum[0], a);
+    atomicAdd(&sum[1], b);
+  }
+}
+
+// After a request's kernels: hands the fault bits and digests to the host
+// and clears them for the slot's next request (same stream, so they hold
+// exactly this request's).
+__global__ void k_request_done(lbvh::Ctl *ctl, unsigned long long *sum,
+                               RequestResult *out) {
+  out->fault = ctl->fault;
+  out->digest[0] = sum[0];
+  out->digest[1] = sum[1];
+  ctl->fault = 0u;
+  sum[0] = 0ull;
+  sum[1] = 0ull;
+}
 
 double now_s() {
   using namespace std::chrono;
@@ -101,6 +173,7 @@ struct Request {
   int kind = 0, source = 0;
   uint32_t n = 0;
   double param = 0.0, submit_s = 0.0;
+  double deadline_s = 0.0;  // admit by this time or time out (0: no limit)
   const float *dev = nullptr;
   std::vector<float> host;
 };
@@ -111,6 +184,8 @@ struct Work {
   Request req;
   Mesh mesh;          // copy of the mesh's buffer layout for the worker
   hipEvent_t ev = nullptr;
+  int ring = 0;       // the slot's event and result entry of this request
+  bool digest = false;
   double start_s = 0.0;
   bool launched = false;  // the worker has enqueued it (or failed to)
   int status = LBVH_OK;   // the enqueue's status
@@ -131,6 +206,11 @@ struct Slot {
   std::atomic<uint64_t> scratch_bytes{0};
   hipEvent_t events[kSlotDepth] = {};
   uint32_t next_event = 0;  // dispatcher only
+  // per event: what the request's epilogue left (pinned host memory, and
+  // its device address), and the device words its digest accumulates in
+  RequestResult *results = nullptr;
+  RequestResult *results_dev = nullptr;
+  unsigned long long *sums = nullptr;
   std::mutex m;             // guards mailbox, inflight and their Work
   std::deque<Work> mailbox;   // assigned, not yet taken by the worker
   std::deque<Work> inflight;  // taken by the worker, in stream order
@@ -152,6 +232,8 @@ struct lbvh_server {
   int device = 0;
   uint64_t budget = 0;
   int div_mode = LBVH_DIV_MUL_RCP;
+  bool digest = false;  // admitted requests digest their results
+  std::atomic<double> max_wait{0.0};  // queueing limit of new requests
   std::deque<Slot> slots;  // stable addresses: workers hold references
   MeshArena arena;
   std::mutex mu;
@@ -276,8 +358,12 @@ bool make_room(lbvh_server *s, const Need &need, uint64_t keep) {
 }
 
 void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
-              double start) {
+              double start, const RequestResult *res = nullptr) {
   lbvh_completion c{};
+  if (res) {
+    c.digest_bvh = res->digest[0];
+    c.digest_tri = res->digest[1];
+  }
   c.request_id = r.id;
   c.mesh_id = r.mesh_id;
   c.kind = r.kind;
@@ -295,14 +381,18 @@ void complete(lbvh_server *s, const Request &r, int status, uint32_t slot,
   if (status != LBVH_OK) s->st.failed += 1;
 }
 
-// Marks a finished (or failed) request's mesh idle and reports it.
-void finish(lbvh_server *s, const Work &w, int status, uint32_t slot) {
+// Marks a finished (or failed) request's mesh idle and reports it.  A
+// failed build leaves no tree, and a failed refit a broken one: either way
+// the mesh is no longer valid for refits.
+void finish(lbvh_server *s, const Work &w, int status, uint32_t slot,
+            const RequestResult *res = nullptr) {
   auto it = s->meshes.find(w.req.mesh_id);
   if (it != s->meshes.end()) {
     it->second.busy = false;
     if (w.req.kind == LBVH_REQ_BUILD) it->second.valid = (status == LBVH_OK);
+    else if (status != LBVH_OK) it->second.valid = false;
   }
-  complete(s, w.req, status, slot, w.start_s);
+  complete(s, w.req, status, slot, w.start_s, res);
 }
 
 // Under the lock: gives request r its mesh range (make_room has found room
@@ -359,6 +449,18 @@ int enqueue(Slot &sl, Work &w) {
     st = lbvh_refit(sl.ctx, m.tri(), r.n, m.keys(), m.perm(), m.bvh(),
                     m.otri());
   if (st != LBVH_OK) return st;
+  if (w.digest) {
+    const uint64_t vecs = 4ull * r.n;
+    const uint32_t grid =
+        (uint32_t)std::min<uint64_t>(1024, (vecs + kDigestThreads - 1) / kDigestThreads);
+    hipLaunchKernelGGL(k_digest, dim3(grid), dim3(kDigestThreads), 0, stream,
+                       static_cast<const uint4 *>(m.bvh()),
+                       reinterpret_cast<const uint4 *>(m.otri()), r.n, sl.sums);
+  }
+  hipLaunchKernelGGL(k_request_done, dim3(1), dim3(1), 0, stream,
+                     lbvh_context_ctl(sl.ctx), sl.sums,
+                     sl.results_dev + w.ring);
+  if (hipGetLastError() != hipSuccess) return LBVH_ERR_DEVICE;
   if (hipEventRecord(w.ev, stream) != hipSuccess) return LBVH_ERR_DEVICE;
   return LBVH_OK;
 }
@@ -435,17 +537,37 @@ void dispatcher(lbvh_server *s) {
       while (!sl.inflight.empty() && sl.inflight.front().launched) {
         Work &w = sl.inflight.front();
         int st = w.status;
+        const RequestResult *res = nullptr;
         if (st == LBVH_OK) {
           const hipError_t q = hipEventQuery(w.ev);
           if (q == hipErrorNotReady) break;
-          // (Kernel-side faults, such as a frontier overflow, are sticky
-          // per slot and reported by lbvh_server_drain, which keeps this
-          // path free of device reads.)
           st = q == hipSuccess ? LBVH_OK : LBVH_ERR_DEVICE;
+          // the request's epilogue has written its entry by now: a tree
+          // kernel that found inconsistent input fails this request only
+          if (st == LBVH_OK) {
+            res = &sl.results[w.ring];
+            if (res->fault) st = LBVH_ERR_DEVICE;
+          }
         }
-        finish(s, w, st, k);
+        finish(s, w, st, k, res);
         sl.inflight.pop_front();
         progress = true;
+      }
+    }
+    // Queueing deadlines, at the head of the queue whether or not a slot is
+    // free (with one maximum wait they pass in arrival order).
+    {
+      const double now = now_s();
+      size_t looked = 0;
+      for (auto qi = s->queue.begin();
+           qi != s->queue.end() && looked < kAdmitWindow; ++looked) {
+        if (qi->deadline_s > 0.0 && now > qi->deadline_s) {
+          complete(s, *qi, LBVH_ERR_TIMEOUT, UINT32_MAX, now);
+          qi = s->queue.erase(qi);
+          progress = true;
+        } else {
+          ++qi;
+        }
       }
     }
     // Admissions, in arrival order.  A mesh already in a slot holds back
@@ -479,8 +601,10 @@ void dispatcher(lbvh_server *s) {
         w.req = std::move(*qi);
         qi = s->queue.erase(qi);
         w.start_s = now_s();
+        w.ring = (int)sl.next_event;
         w.ev = sl.events[sl.next_event];
         sl.next_event = (sl.next_event + 1) % kSlotDepth;
+        w.digest = s->digest;
         st = prepare(s, w);
         if (st == LBVH_OK) {
           {
@@ -534,6 +658,16 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
       if (hipEventCreateWithFlags(&sl.events[e], hipEventDisableTiming) !=
           hipSuccess)
         st = LBVH_ERR_DEVICE;
+    if (st == LBVH_OK &&
+        (hipHostMalloc((void **)&sl.results, sizeof(RequestResult) * kSlotDepth,
+                       hipHostMallocCoherent) != hipSuccess ||
+         hipHostGetDevicePointer((void **)&sl.results_dev, sl.results, 0) !=
+             hipSuccess ||
+         hipMalloc((void **)&sl.sums, 2 * sizeof(unsigned long long)) !=
+             hipSuccess ||
+         hipMemset(sl.sums, 0, 2 * sizeof(unsigned long long)) != hipSuccess ||
+         hipDeviceSynchronize() != hipSuccess))
+      st = LBVH_ERR_DEVICE;
     if (st != LBVH_OK) break;
   }
   if (st != LBVH_OK) {
@@ -541,6 +675,8 @@ int lbvh_server_create(int device_ordinal, uint64_t hbm_budget_bytes,
     for (auto &x : s->slots) {
       for (hipEvent_t e : x.events)
         if (e) (void)hipEventDestroy(e);
+      if (x.results) (void)hipHostFree(x.results);
+      if (x.sums) (void)hipFree(x.sums);
       if (x.ctx) lbvh_destroy(x.ctx);
     }
     delete s;
@@ -569,6 +705,8 @@ int lbvh_server_destroy(lbvh_server *s) {
   for (auto &sl : s->slots) {
     lbvh_synchronize(sl.ctx);
     for (hipEvent_t e : sl.events) (void)hipEventDestroy(e);
+    (void)hipHostFree(sl.results);
+    (void)hipFree(sl.sums);
   }
   (void)hipSetDevice(s->device);
   for (auto &kv : s->meshes) free_mesh(s, kv.second);
@@ -586,6 +724,13 @@ int lbvh_server_set_div_mode(lbvh_server *s, int div_mode) {
     if (st != LBVH_OK) return st;
   }
   s->div_mode = div_mode;
+  return LBVH_OK;
+}
+
+int lbvh_server_set_digest(lbvh_server *s, int enable) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  s->digest = enable != 0;
   return LBVH_OK;
 }
 
@@ -614,6 +759,8 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
   r.seed = seed;
   r.param = param;
   r.submit_s = now_s();
+  const double wait = s->max_wait.load();
+  if (wait > 0.0) r.deadline_s = r.submit_s + wait;
   if (source == LBVH_SRC_HOST) r.host.assign(tri, tri + 12ull * n);
   if (source == LBVH_SRC_DEVICE) r.dev = tri;
   {
@@ -625,6 +772,55 @@ int lbvh_server_submit(lbvh_server *s, int kind, uint64_t mesh_id, int source,
   }
   s->cv.notify_all();
   return LBVH_OK;
+}
+
+int lbvh_server_set_max_wait(lbvh_server *s, double max_wait_s) {
+  if (!s || !(max_wait_s >= 0.0)) return LBVH_ERR_INVALID_ARGUMENT;
+  s->max_wait = max_wait_s;
+  return LBVH_OK;
+}
+
+int lbvh_server_cancel(lbvh_server *s, uint64_t request_id) {
+  if (!s) return LBVH_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(s->mu);
+  bool found = false;
+  Request r;
+  {
+    std::lock_guard<std::mutex> g(s->in_mu);
+    for (auto it = s->inbox.begin(); it != s->inbox.end(); ++it)
+      if (it->id == request_id) {
+        r = std::move(*it);
+        s->inbox.erase(it);
+        s->inbox_n -= 1;
+        s->st.submitted += 1;  // as if the dispatcher had moved it
+        found = true;
+        break;
+      }
+  }
+  if (!found)
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+      if (it->id == request_id) {
+        r = std::move(*it);
+        s->queue.erase(it);
+        found = true;
+        break;
+      }
+  if (found) {
+    complete(s, r, LBVH_ERR_CANCELLED, UINT32_MAX, now_s());
+    s->cv.notify_all();
+    return LBVH_OK;
+  }
+  for (auto &sl : s->slots) {
+    std::lock_guard<std::mutex> g(sl.m);
+    for (auto *d : {&sl.mailbox, &sl.inflight})
+      for (auto &w : *d)
+        if (w.req.id == request_id) {
+          s->last_error = "request is already admitted";
+          return LBVH_ERR_BUSY;
+        }
+  }
+  s->last_error = "no waiting request with this id";
+  return LBVH_ERR_NOT_FOUND;
 }
 
 int lbvh_server_poll(lbvh_server *s, lbvh_completion *out, uint32_t max,

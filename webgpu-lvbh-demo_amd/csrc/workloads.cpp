@@ -1,6 +1,7 @@
 // workloads.cpp -- seeded synthetic meshes for tests and benchmarks.
 // Host-only; see include/lbvh_workloads.h for the definitions.
 #include "../../include/lbvh_workloads.h"
+#include "synth_math.h"
 
 #include <algorithm>
 #include <cmath>
@@ -11,24 +12,17 @@ namespace {
 
 constexpr float kPad = 3.1415f;  // parse-obj.js:56-58
 
-inline uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
+using lbvh_synth::splitmix64;
 
 // Uniform double in [0,1) for draw k of element i under `seed`.
 inline double u01(uint64_t seed, uint64_t i, uint64_t k) {
-  const uint64_t h = splitmix64(splitmix64(seed) ^ (i * 64 + k));
-  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+  return lbvh_synth::u01(splitmix64(seed), i, k);
 }
 
+// Standard normal from draws k and k+1 (Box-Muller), bit-identical to the
+// device's jitter (k_synth.hip).
 inline double gauss(uint64_t seed, uint64_t i, uint64_t k) {
-  // Box-Muller on draws k and k+1.
-  double a = u01(seed, i, k), b = u01(seed, i, k + 1);
-  if (a < 1e-300) a = 1e-300;
-  return std::sqrt(-2.0 * std::log(a)) * std::cos(6.283185307179586 * b);
+  return lbvh_synth::gauss(splitmix64(seed), i, k);
 }
 
 template <class F>

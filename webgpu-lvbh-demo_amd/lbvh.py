@@ -36,6 +36,8 @@ ERR_DEVICE = 5
 ERR_NO_DEVICE = 6
 ERR_BUSY = 7
 ERR_NOT_FOUND = 8
+ERR_CANCELLED = 9
+ERR_TIMEOUT = 10
 
 STATUS = {
     0: "ok",
@@ -47,6 +49,8 @@ STATUS = {
     6: "no such device",
     7: "busy",
     8: "not found",
+    9: "cancelled",
+    10: "timed out",
 }
 
 TRI_BYTES = 48
@@ -87,6 +91,9 @@ SIGNATURES = {
     "lbvh_server_create": (_i32, [_i32, _u64, _u32, C.POINTER(_vp)]),
     "lbvh_server_destroy": (_i32, [_vp]),
     "lbvh_server_set_div_mode": (_i32, [_vp, _i32]),
+    "lbvh_server_set_digest": (_i32, [_vp, _i32]),
+    "lbvh_server_cancel": (_i32, [_vp, _u64]),
+    "lbvh_server_set_max_wait": (_i32, [_vp, C.c_double]),
     "lbvh_server_submit": (_i32, [_vp, _i32, _u64, _i32, _vp, _u32, _u64, C.c_double,
                                   C.POINTER(_u64)]),
     "lbvh_server_poll": (_i32, [_vp, _vp, _u32, C.POINTER(_u32)]),
@@ -387,7 +394,8 @@ class Completion(C.Structure):
     _fields_ = [("request_id", C.c_uint64), ("mesh_id", C.c_uint64),
                 ("kind", C.c_int), ("status", C.c_int), ("n", C.c_uint32),
                 ("slot", C.c_uint32), ("submit_s", C.c_double),
-                ("start_s", C.c_double), ("done_s", C.c_double)]
+                ("start_s", C.c_double), ("done_s", C.c_double),
+                ("digest_bvh", C.c_uint64), ("digest_tri", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -427,6 +435,19 @@ class Server:
 
     def set_div_mode(self, mode: int):
         self._check(self.lib.lbvh_server_set_div_mode(self.handle, mode))
+
+    def cancel(self, request_id: int):
+        """Cancels a request still waiting for admission (LBVHError BUSY if
+        admitted, NOT_FOUND if completed)."""
+        self._check(self.lib.lbvh_server_cancel(self.handle, request_id))
+
+    def set_max_wait(self, seconds: float):
+        """Later requests not admitted within `seconds` time out (0: none)."""
+        self._check(self.lib.lbvh_server_set_max_wait(self.handle, seconds))
+
+    def set_digest(self, enable: bool = True):
+        """Requests admitted from now on report digests of their results."""
+        self._check(self.lib.lbvh_server_set_digest(self.handle, 1 if enable else 0))
 
     def submit(self, kind: int, mesh_id: int, source: int, n: int, tri=None,
                seed: int = 0, param: float = 0.0) -> int:
@@ -530,6 +551,29 @@ PARITY_SEED = 1
 CLUSTERED_SEED = 2
 HEIGHTFIELD_SEED = 3
 BATCH_SEED = 4
+
+
+def wl_serving_plan(rng, requests: int, size_seed: int = 6, lo: int = 1024,
+                    hi: int = 4194304):
+    """The serving workload's request plan: sizes log-uniform in [lo, hi]
+    (seeded), 70% refits of one of the 32 most recently built meshes
+    (vertices + N(0, 0.001) drawn on the device), 30% builds of a new
+    uniform mesh synthesised on the device.  Returns a list of (kind,
+    mesh id, source, n, seed, param); `rng` (numpy Generator) draws the mix
+    and the refit targets."""
+    sizes = wl_log_uniform_sizes(size_seed, requests, lo, hi)
+    is_refit = rng.random(requests) < 0.7
+    plan, built, next_mesh = [], [], 0
+    for i in range(requests):
+        if is_refit[i] and built:
+            m, n = built[int(rng.integers(max(0, len(built) - 32), len(built)))]
+            plan.append((REQ_REFIT, m, SRC_JITTER, n, 1000 + i, 0.001))
+        else:
+            n = int(sizes[i])
+            plan.append((REQ_BUILD, next_mesh, SRC_UNIFORM, n, 2000 + i, 0.01))
+            built.append((next_mesh, n))
+            next_mesh += 1
+    return plan
 
 
 def workload(name: str) -> np.ndarray:
