@@ -32,29 +32,18 @@ __device__ __forceinline__ TileRange sort_tile_range(uint32_t tile, uint32_t n,
   return {b, min(b + (uint32_t)kSortTile, se), sb, se};
 }
 
-// Adds one digit per active lane to a block's LDS histogram.  LDS atomics on
-// one address serialise, and tiles of coherent keys give a wave few distinct
-// digits (a tile of the 16M heightfield holds 8.7 distinct digits on
-// average in the last pass), so the wave peels off its most common digit
-// groups first: the group of the first remaining lane's digit adds its lane
-// count once, up to kPeels times while groups hold at least kPeelMin lanes;
-// the remaining lanes add one each.
+// Adds one digit to a block's LDS histogram; a wave whose active lanes all
+// hold the same digit adds its lane count once.
 __device__ __forceinline__ void digit_add(uint32_t *s_hist, uint32_t d) {
-  constexpr int kPeels = 4, kPeelMin = 8;
-  const int lane = (int)(threadIdx.x & 63);
-  uint64_t left = __ballot(1);
-#pragma unroll
-  for (int it = 0; it < kPeels; ++it) {
-    const int leader = __ffsll((unsigned long long)left) - 1;
-    const uint32_t d0 = __shfl(d, leader, 64);
-    const uint64_t same = __ballot(d == d0) & left;
-    const uint32_t cnt = (uint32_t)__popcll(same);
-    if (cnt < (uint32_t)kPeelMin && same != left) break;  // a diverse wave
-    if (lane == leader) atomicAdd(&s_hist[d0], cnt);
-    left &= ~same;
-    if (!left) return;
+  const uint64_t active = __ballot(1);
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  const uint32_t d0 = __shfl(d, leader, 64);
+  if (__ballot(d == d0) == active) {
+    if ((int)(threadIdx.x & 63) == leader)
+      atomicAdd(&s_hist[d0], (uint32_t)__popcll(active));
+  } else {
+    atomicAdd(&s_hist[d], 1u);
   }
-  if ((left >> lane) & 1) atomicAdd(&s_hist[d], 1u);
 }
 
 // Lanes of the wave that are `valid` and hold the calling lane's digit d

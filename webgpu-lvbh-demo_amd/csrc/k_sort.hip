@@ -26,9 +26,10 @@
 // the pass time.  Kernel boundaries are the only global synchronisation.
 //
 // A pass's global digit histogram (the digit bases of a single mesh) is the
-// column sums of its per-tile counts, and a batched mesh's bases come from
-// its own tiles' counts (k_seg_bases, folded into the offsets by the column
-// scan): no device-scope atomics anywhere.  Counts and
+// column sums of its per-tile counts.  A batched sort's meshes are segments
+// of whole tiles, each sorted on its own: one block per mesh turns its
+// tiles' counts into their offsets (k_seg_scan).  No device-scope atomics
+// anywhere.  Counts and
 // offsets are digit-major: entry (d, t) at d * tiles + t, so the scan reads
 // and writes columns contiguously.  Tile ids are XCD-aware
 // (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
@@ -121,38 +122,28 @@ __device__ __forceinline__ void digit_prefix(const uint32_t (&h)[Digits<kBits>::
   }
 }
 
-// Column scan: block d turns column d of the tile counts into output
-// positions.  P(t) = sum of cnt[d][t'] over t' < t.
-//   one mesh:  off[d][t] = P(t), and hist[d] = the column total; the
-//              scatter adds base(d) = keys with a smaller digit (exclusive
-//              scan of hist over the digits), which needs every column;
-//   segments:  off[d][t] = base(s, d) + P(t) - P(t0(s)) for the segment s of
-//              tile t with first tile t0(s); seg_base[s][d] holds base(s, d)
-//              (the segment's first key + its keys with a smaller digit) and
-//              is rewritten to base(s, d) - P(t0(s)) (mod 2^32) by the thread
-//              owning the segment's first tile.  Only block d touches column
-//              d of seg_base, so the rewrite needs just this block's barrier.
-// Chunks of kSortThreads * kScanPer column entries go through LDS, so global
-// loads and stores are coalesced while each thread scans kScanPer consecutive
-// entries (one pad word per kScanPer keeps the per-thread reads conflict-free).
+// Column scan of a single mesh: block d turns column d of the tile counts
+// into output positions, off[d][t] = P(t) = sum of cnt[d][t'] over t' < t,
+// and hist[d] = the column total; the scatter adds base(d) = keys with a
+// smaller digit (an exclusive scan of hist over the digits, which needs
+// every column).  Chunks of kSortThreads * kScanPer column entries go
+// through LDS, so global loads and stores are coalesced while each thread
+// scans kScanPer consecutive entries (one pad word per kScanPer keeps the
+// per-thread reads conflict-free).
 template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
-    uint32_t tiles, uint32_t *__restrict__ hist, SortSegs segs,
-    uint32_t *seg_base) {
-  using D = Digits<kBits>;
+    uint32_t tiles, uint32_t *__restrict__ hist) {
   constexpr int kChunk = kSortThreads * kScanPer;
   __shared__ uint32_t s_col[kChunk + kChunk / kScanPer];
   __shared__ uint32_t s_wsum[16];
   __shared__ uint32_t s_carry;
   const uint32_t d = blockIdx.x;
-  const bool segmented = segs.tile_seg != nullptr;
   uint32_t carry = 0;
   const uint32_t *col = cnt + (size_t)d * tiles;
   uint32_t *out = off + (size_t)d * tiles;
   const auto pad = [](uint32_t e) { return e + e / kScanPer; };
   for (uint32_t c0 = 0; c0 < tiles; c0 += kChunk) {
-    const uint32_t t0 = c0 + threadIdx.x * kScanPer;
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
       const uint32_t e = j * kSortThreads + threadIdx.x;
@@ -168,52 +159,35 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
     }
     const uint32_t ex = block_exclusive_scan(sum, s_wsum);
     if (threadIdx.x == kSortThreads - 1) s_carry = ex + sum;
-    uint32_t p[kScanPer];
     uint32_t run = carry + ex;
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
-      p[j] = run;
+      s_col[pad(threadIdx.x * kScanPer + j)] = run;
       run += v[j];
     }
-    if (segmented) {
-#pragma unroll
-      for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t t = t0 + j;
-        if (t < tiles) {
-          const uint32_t sg = segs.tile_seg[t];
-          if (t == segs.seg_tile0[sg])
-            seg_base[(size_t)sg * D::kR + d] -= p[j];
-        }
-      }
-    }
-    __syncthreads();  // s_carry and the rewritten bases are visible
-#pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-      const uint32_t t = t0 + j;
-      uint32_t o = p[j];
-      if (segmented && t < tiles)
-        o += seg_base[(size_t)segs.tile_seg[t] * D::kR + d];
-      s_col[pad(threadIdx.x * kScanPer + j)] = o;
-    }
+    __syncthreads();  // s_carry and the staged offsets are visible
     carry += s_carry;
-    __syncthreads();  // offsets staged; s_carry is rewritten by the next chunk
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
       const uint32_t e = j * kSortThreads + threadIdx.x;
       if (c0 + e < tiles) out[c0 + e] = s_col[pad(e)];
     }
-    __syncthreads();  // s_col is refilled by the next chunk
+    __syncthreads();  // s_col and s_carry are rewritten by the next chunk
   }
-  if (!segmented && threadIdx.x == 0) hist[d] = carry;
+  if (threadIdx.x == 0) hist[d] = carry;
 }
 
-// Segment digit bases of one pass from its per-tile counts: block s sums
-// each digit's column over the segment's tiles, then seg_base[s][d] =
-// seg_off[s] + (keys of s whose digit is below d).
+// Offsets of a batched pass, one block per segment (mesh).  A segment's
+// tiles are consecutive and hold only its keys, so the run of digit d in
+// its tile t starts at seg_off + (the segment's keys with a smaller digit)
+// + (digit-d keys in the segment's earlier tiles).  Thread t owns digits
+// [t * kDpt, (t + 1) * kDpt): it sums their counts over the segment's
+// tiles, the block scans the sums into bases, and the thread writes the
+// running positions.  An empty segment has no tiles.
 template <int kBits>
-__global__ __launch_bounds__(kSortThreads) void k_seg_bases(
-    const uint32_t *__restrict__ cnt, uint32_t tiles, SortSegs segs,
-    uint32_t num_segs, uint32_t *__restrict__ seg_base) {
+__global__ __launch_bounds__(kSortThreads) void k_seg_scan(
+    const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
+    uint32_t tiles, SortSegs segs, uint32_t num_segs) {
   using D = Digits<kBits>;
   __shared__ uint32_t s_wsum[16];
   const uint32_t sg = blockIdx.x;
@@ -227,10 +201,17 @@ __global__ __launch_bounds__(kSortThreads) void k_seg_bases(
     for (uint32_t t = t0; t < t1; ++t) total[i] += col[t];
   }
   digit_prefix<kBits>(total, below, s_wsum);
+  const uint32_t first = segs.seg_off[sg];
 #pragma unroll
-  for (int i = 0; i < D::kDpt; ++i)
-    seg_base[(size_t)sg * D::kR + threadIdx.x * D::kDpt + i] =
-        segs.seg_off[sg] + below[i];
+  for (int i = 0; i < D::kDpt; ++i) {
+    const size_t row = (size_t)(threadIdx.x * D::kDpt + i) * tiles;
+    uint32_t run = first + below[i];
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t c = cnt[row + t];
+      off[row + t] = run;
+      run += c;
+    }
+  }
 }
 
 // digit_prefix of two counts at once, packed (a << 32) | b: the b sums
@@ -517,7 +498,7 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                uint32_t *keys_out, int32_t *vals_out, uint32_t n,
                uint32_t tiles, uint32_t *cnt, uint32_t *off, bool counted,
                Ctl *ctl, hipStream_t s, const SortSegs &sg, bool segmented,
-               uint32_t num_segs, uint32_t *seg_base) {
+               uint32_t num_segs) {
   constexpr int kR = Digits<kBits>::kR;
   const uint32_t shift = (uint32_t)pass * kBits;
   uint32_t *hist = ctl->hist + (size_t)pass * kR;
@@ -529,15 +510,13 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   // a small single mesh: the scatter reads the counts themselves
   const int small = !segmented && tiles <= kSmallScanTiles;
   const uint32_t *offs = small ? cnt : off;
-  if (!small) {
-    if (segmented)
-      hipLaunchKernelGGL(k_seg_bases<kBits>, dim3(num_segs),
-                         dim3(kSortThreads), 0, s, cnt, tiles, sg, num_segs,
-                         seg_base);
-    else
-      bases_from = hist;
+  if (segmented) {
+    hipLaunchKernelGGL(k_seg_scan<kBits>, dim3(num_segs), dim3(kSortThreads),
+                       0, s, cnt, off, tiles, sg, num_segs);
+  } else if (!small) {
+    bases_from = hist;
     hipLaunchKernelGGL(k_scan_columns<kBits>, dim3(kR), dim3(kSortThreads), 0,
-                       s, cnt, off, tiles, hist, sg, seg_base);
+                       s, cnt, off, tiles, hist);
   }
   if (pass == 0)
     hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
@@ -565,19 +544,19 @@ void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       const int32_t *vals_in, uint32_t *keys_out,
                       int32_t *vals_out, uint32_t n, uint32_t tiles,
                       uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
-                      hipStream_t s, const SortSegs *segs, uint32_t num_segs,
-                      uint32_t *seg_base) {
+                      hipStream_t s, const SortSegs *segs,
+                      uint32_t num_segs) {
   if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
   if (bits == kWideBits)
     sort_pass<kWideBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
                          cnt, off, counted, ctl, s, sg, segs != nullptr,
-                         num_segs, seg_base);
+                         num_segs);
   else
     sort_pass<kRadixBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
                           cnt, off, counted, ctl, s, sg, segs != nullptr,
-                          num_segs, seg_base);
+                          num_segs);
 }
 
 }  // namespace lbvh

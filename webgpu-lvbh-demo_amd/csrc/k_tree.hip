@@ -5,7 +5,7 @@
 //
 // Two code paths produce the same BVH bytes:
 //   * per-stage kernels mirroring the reference's execute() shims
-//     (k_radix_tree -> parents array, k_up_pass over parents, k_rearrange);
+//     (k_rt_* -> parents array, k_up_pass over parents, k_rearrange);
 //   * the agglomeration used by lbvh_build / lbvh_refit / batched builds:
 //     k_leaves gathers the triangles into leaf order, computes leaf boxes
 //     and builds the Karras tree bottom-up (Apetrei 2014) without a parents
@@ -28,54 +28,11 @@
 namespace lbvh {
 
 // ---------------------------------------------------------------------------
-// Per-stage radix tree (compute_radix_tree_pointers, radix-tree.js:137-157)
+// Per-stage radix tree (compute_radix_tree_pointers, radix-tree.js:137-157,
+// compute_child_index :188-238, dif/del :160-185)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int delta_at(const uint32_t *__restrict__ keys,
-                                        int64_t n, int64_t i, int64_t j) {
-  if (j < 0 || j >= n) return -1;
-  const uint32_t a = keys[i], b = keys[j];
-  return (a != b) ? (int)__clz(a ^ b)
-                  : 32 + (int)__clz((uint32_t)i ^ (uint32_t)j);
-}
-
-// compute_child_index (radix-tree.js:188-238) with count-leading-zeros for
-// dif() (:160-169) and the textbook split search t = ceil(l/2^k) ... 1 (the
-// reference's loop at :216-224 repeats t = 1 until its i32 step wraps;
-// those probes do not change s).
-__global__ __launch_bounds__(kMapThreads) void k_radix_tree(
-    const uint32_t *__restrict__ keys, uint32_t n, int32_t *__restrict__ par) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    par[0] = 0;                       // root: never written (zero buffer)
-    par[n - 1] = 0;                   // no internal node N-1
-    if (n == 1) par[1] = 0;
-  }
-  if (i + 1 >= (int64_t)n) return;
-  const int64_t N = n;
-  const int d = (delta_at(keys, N, i, i + 1) - delta_at(keys, N, i, i - 1)) > 0
-                    ? 1 : -1;
-  const int dmin = delta_at(keys, N, i, i - d);
-  int64_t lmax = 2;
-  while (delta_at(keys, N, i, i + lmax * d) > dmin) lmax *= 2;
-  int64_t l = 0;
-  for (int64_t t = lmax / 2; t >= 1; t /= 2)
-    if (delta_at(keys, N, i, i + (l + t) * d) > dmin) l += t;
-  const int64_t j = i + l * d;
-  const int dnode = delta_at(keys, N, i, j);
-  int64_t s = 0;
-  for (int64_t v = 2;; v *= 2) {
-    const int64_t t = (l + v - 1) / v;
-    if (delta_at(keys, N, i, i + (s + t) * d) > dnode) s += t;
-    if (t == 1) break;
-  }
-  const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-  const int64_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
-  par[(lo == gamma) ? N + gamma : gamma] = (int32_t)i;
-  par[(hi == gamma + 1) ? N + gamma + 1 : gamma + 1] = (int32_t)i;
-}
-
-// The same parents array by all nearest smaller values.  The Karras tree is
+// The parents array by all nearest smaller values.  The Karras tree is
 // the min-Cartesian tree of the gap deltas d(g) = delta(g, g+1): the node
 // split at gap g covers the leaves between the nearest gaps with a smaller
 // delta on either side, L(g) and R(g) (sentinels -1 and n-1, delta -1; two
@@ -109,18 +66,17 @@ constexpr int kRtPlanes = 7;                        // delta bits (<= 64)
 constexpr int kRtNone = 64;                         // "no gap" delta
 constexpr int kRtScanThreads = 256;
 
-// Lanes of a segment whose delta is below v, from its bit planes.
+// Lanes of a segment whose delta is below v, from its bit planes: from the
+// high bit down, lanes equal to v so far drop out where their bit differs,
+// and those with a 0 where v has a 1 are below it.
 __device__ __forceinline__ uint64_t rt_below(const uint64_t *planes, int v) {
   uint64_t lt = 0, eq = ~0ull;
 #pragma unroll
   for (int b = kRtPlanes - 1; b >= 0; --b) {
     const uint64_t pb = planes[b];
-    if ((v >> b) & 1) {
-      lt |= eq & ~pb;
-      eq &= pb;
-    } else {
-      eq &= ~pb;
-    }
+    const uint64_t vb = 0ull - (uint64_t)((v >> b) & 1);  // v's bit, spread
+    lt |= eq & vb & ~pb;
+    eq &= ~(pb ^ vb);
   }
   return lt;
 }
@@ -186,11 +142,20 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     uint64_t pl[kRtPlanes];
 #pragma unroll
     for (int b = 0; b < kRtPlanes; ++b) pl[b] = __ballot((v >> b) & 1);
-    int mn = v;
+    // the segment's minimum from its planes, high bit first (wave-uniform)
+    uint64_t cand = ~0ull;
+    int mn = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mn = min(mn, __shfl_xor(mn, o, 64));
-    if (lane < kRtPlanes) s_planes[sg][lane] = pl[lane];
-    if (lane == 0) s_min[sg] = mn;
+    for (int b = kRtPlanes - 1; b >= 0; --b) {
+      const uint64_t zero = cand & ~pl[b];
+      if (zero) cand = zero;
+      else mn |= 1 << b;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int b = 0; b < kRtPlanes; ++b) s_planes[sg][b] = pl[b];
+      s_min[sg] = mn;
+    }
   }
   __syncthreads();
   const uint64_t below_me = (1ull << lane) - 1;
@@ -496,14 +461,27 @@ __device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
   }
 }
 
+// The number of nodes that got exactly two children in this block, summed
+// over the block (kUpTile threads, each passing its own count); the block's
+// first thread gets the total.
+__device__ __forceinline__ uint32_t up_block_count(uint32_t c,
+                                                   uint32_t *s_sum) {
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (threadIdx.x == 0) *s_sum = 0u;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) atomicAdd(s_sum, c);
+  __syncthreads();
+  return *s_sum;
+}
+
 __global__ __launch_bounds__(kUpTile) void k_up_pass(
     const int32_t *__restrict__ idx, const int4 *__restrict__ aabb,
     const int32_t *__restrict__ par, uint32_t n, int4 *nodes,
-    int4 *__restrict__ items, uint32_t *__restrict__ item_cnt, Ctl *ctl,
-    int last) {
+    int4 *__restrict__ items, uint32_t *__restrict__ item_cnt,
+    uint32_t *__restrict__ done, Ctl *ctl, int last) {
   __shared__ UpLds L;
   __shared__ uint32_t s_wsum[16];
-  __shared__ int s_root;  // the root merged in this block
+  __shared__ uint32_t s_done;
   const int tid = threadIdx.x;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t ts = (int64_t)tile * kUpTile;
@@ -512,13 +490,15 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   // The leaf's box (a gather through the permutation) is loaded while the
   // block stages its parents entries: both round trips before one barrier.
   float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
-  // a malformed input (an index or parent out of range, a path longer than
-  // any tree's, a third arrival at a node) ends the thread's climb and
-  // sets ctl->fault; every access stays in bounds
-  bool bad = false;
+  // Malformed input is caught without touching the climb: an index out of
+  // range here, and a parents array that is not a tree by the count of
+  // completed nodes (every internal node gets exactly two children, once,
+  // in exactly one block of one level: n - 1 in all, checked by the last
+  // level).  A parent out of range or a path past the step cap just ends a
+  // climb; every access stays in bounds.
   if (g < (int64_t)n) {
     const uint32_t raw = (uint32_t)idx[g];
-    bad = raw >= n;
+    if (raw >= n) atomicOr(&ctl->fault, kFaultNotATree);
     const uint32_t src = raw < n ? raw : 0u;
     const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
     mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
@@ -527,7 +507,6 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
     mx[2] = __int_as_float(bh.z);
   }
   L.cnt[tid] = 0u;
-  if (tid == 0) s_root = 0;
   for (int t = tid; t <= kUpTile; t += kUpTile) {
     const int64_t id = ts + t;
     L.par_leaf[t] = id < (int64_t)n ? par[n + id] : -1;
@@ -556,12 +535,8 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
       // A valid tree is at most 64 levels deep (each level consumes a bit of
       // the 62-bit position-augmented key); the step cap and the range
       // check keep a malformed parents array from running away.
-      int step = 0;
-      for (; step < 128; ++step) {
-        if (p < 0 || p >= (int32_t)n - 1) {
-          bad = true;
-          break;
-        }
+      for (int step = 0; step < 128; ++step) {
+        if (p < 0 || p >= (int32_t)n - 1) break;
         side = up_is_left(L, par, n, ts, te, num, p) ? 0 : 1;
         self = leaf ? -(int)(num + 1) : (int)num;
         if (p < ts || p >= te) {  // the parent lies outside the block
@@ -582,10 +557,6 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
           mode = 1;
           break;
         }
-        if (old >= 2) {  // a third child at one node
-          bad = true;
-          break;
-        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           mn[k] = fminf(mn[k], L.box[1 - side][k][slot]);
@@ -593,21 +564,14 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         }
         if (side == 0) hi = L.far[1][slot];
         else lo = L.far[0][slot];
-        if (p == 0) {  // root merged
-          s_root = 1;
-          break;
-        }
+        if (p == 0) break;  // root merged
         leaf = false;
         num = p;
         p = up_par_node(L, par, ts, te, p);
       }
-      if (step == 128) bad = true;
     }
   }
-  if (bad) atomicOr(&ctl->fault, kFaultNotATree);
   __syncthreads();
-  // the block that holds the whole tree must have merged its root
-  if (last && tid == 0 && n > 1 && !s_root) atomicOr(&ctl->fault, kFaultNotATree);
   // a first arriver whose sibling did not come on chip is pending too (the
   // sibling reaches outside the block); an item writes its half of the
   // parent when it arrives there at a later level
@@ -615,11 +579,13 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   // The nodes both of whose children arrived on chip, in id order, four
   // lanes per 64-byte node (scattered 16-byte half-node stores from the
   // climbing lanes cost more than the rest of the pass).
+  uint32_t completed = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int e = j * kUpTile + tid;
     const int kk = e >> 2, q = e & 3, sd = q >> 1;
     if (ts + kk < te && L.cnt[kk] == 2u) {
+      completed += q == 0 ? 1u : 0u;
       const int c = (q & 1) * 3;  // min or max half of the side's box
       const int w = (q & 1) ? (sd ? 0 : 2) : L.cid[sd][kk];
       nodes[4 * (size_t)(ts + kk) + q] =
@@ -627,6 +593,11 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
                     __float_as_int(L.box[sd][c + 1][kk]),
                     __float_as_int(L.box[sd][c + 2][kk]), w);
     }
+  }
+  const uint32_t total = up_block_count(completed, &s_done);
+  if (tid == 0) {
+    done[tile] = total;
+    if (last && n > 1 && total != n - 1) atomicOr(&ctl->fault, kFaultNotATree);
   }
   // the parents entries are no longer read: their words order the items
   up_emit(reinterpret_cast<uint32_t *>(L.par_node), s_wsum,
@@ -655,12 +626,13 @@ struct UpItemLds {
 
 __global__ __launch_bounds__(kUpTile) void k_up_items(
     const int4 *__restrict__ items_in, const uint32_t *__restrict__ cnt_in,
-    uint32_t tiles_in, int4 *__restrict__ items_out,
-    uint32_t *__restrict__ cnt_out, const int32_t *__restrict__ par,
+    const uint32_t *__restrict__ done_in, uint32_t tiles_in,
+    int4 *__restrict__ items_out, uint32_t *__restrict__ cnt_out,
+    uint32_t *__restrict__ done_out, const int32_t *__restrict__ par,
     uint32_t n, int4 *nodes, Ctl *ctl, int last) {
   __shared__ UpItemLds S;
   __shared__ uint32_t s_wsum[16];
-  __shared__ int s_root;  // the root merged in this block
+  __shared__ uint32_t s_done;
   const int tid = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const uint32_t c0 = tile * kUpFanIn;
@@ -673,7 +645,6 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       run += c < (uint32_t)kUpFront ? c : (uint32_t)kUpFront;
     }
     S.base[nc] = run;
-    s_root = 0;
   }
   S.cnt[tid] = 0u;
   __syncthreads();
@@ -695,12 +666,9 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
     mx[0] = __int_as_float(c.x); mx[1] = __int_as_float(c.y);
     mx[2] = __int_as_float(c.z);
     mode = 2;
-    bool bad = false;
-    int step = 0;
-    for (; step < 128; ++step) {
+    for (int step = 0; step < 128; ++step) {
       if (p < 0 || p >= (int32_t)n - 1) {
         mode = 0;
-        bad = true;
         break;
       }
       const int gap = side == 0 ? i1 : i0 - 1;
@@ -724,11 +692,6 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
         mode = 1;
         break;
       }
-      if (old >= 2) {  // a third child at one node
-        mode = 0;
-        bad = true;
-        break;
-      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
@@ -743,7 +706,6 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       }
       if (p == 0) {  // root merged
         mode = 0;
-        s_root = 1;
         break;
       }
       // the parent and the two words of the left-child test (up_is_left)
@@ -755,14 +717,17 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
       p = par[num];
       side = (pl == p || pn == p) ? 0 : 1;
     }
-    if (step == 128) {
-      mode = 0;
-      bad = true;
-    }
-    if (bad) atomicOr(&ctl->fault, kFaultNotATree);
   }
   __syncthreads();
-  if (last && tid == 0 && !s_root) atomicOr(&ctl->fault, kFaultNotATree);
+  // nodes completed here (exactly two children met at the gap), plus those
+  // of the source blocks: the last level holds the count of the whole tree
+  uint32_t completed = (tid < m - 1 && S.cnt[tid] == 2u) ? 1u : 0u;
+  if (tid < (int)nc) completed += done_in[c0 + tid];
+  const uint32_t total = up_block_count(completed, &s_done);
+  if (tid == 0) {
+    done_out[tile] = total;
+    if (last && total != n - 1) atomicOr(&ctl->fault, kFaultNotATree);
+  }
   if (mode == 1 && S.cnt[side == 0 ? i1 : i0 - 1] == 1u) mode = 2;
   up_emit(S.mark, s_wsum, mode == 2 ? i0 : -1,
           items_out + 3 * (size_t)tile * kUpFront, cnt_out + tile, ctl, last,
@@ -1384,7 +1349,7 @@ uint64_t up_pass_scratch_bytes(uint32_t n) {
     if (tiles == 1) break;
     tiles = (tiles + kUpFanIn - 1) / kUpFanIn;
   }
-  return 48 * items + 4 * counts;
+  return 48 * items + 8 * counts;  // items, item counts, completed counts
 }
 
 void launch_up_pass(const int32_t *idx, const float4 *aabb,
@@ -1396,20 +1361,27 @@ void launch_up_pass(const int32_t *idx, const float4 *aabb,
     total_items += (uint64_t)t * kUpFront;
     if (t == 1) break;
   }
+  uint64_t total_tiles = 0;
+  for (uint32_t t = tiles;; t = (t + kUpFanIn - 1) / kUpFanIn) {
+    total_tiles += t;
+    if (t == 1) break;
+  }
   int4 *items = static_cast<int4 *>(scratch);
   uint32_t *cnt = reinterpret_cast<uint32_t *>(items + 3 * total_items);
+  uint32_t *done = cnt + total_tiles;
   int4 *nd = reinterpret_cast<int4 *>(nodes);
   hipLaunchKernelGGL(k_up_pass, dim3(tiles), dim3(kUpTile), 0, s, idx,
                      reinterpret_cast<const int4 *>(aabb), parents, n, nd,
-                     items, cnt, ctl, tiles == 1 ? 1 : 0);
+                     items, cnt, done, ctl, tiles == 1 ? 1 : 0);
   while (tiles > 1) {
     const uint32_t next = (tiles + kUpFanIn - 1) / kUpFanIn;
     int4 *out = items + 3 * (size_t)tiles * kUpFront;
     hipLaunchKernelGGL(k_up_items, dim3(next), dim3(kUpTile), 0, s, items,
-                       cnt, tiles, out, cnt + tiles, parents, n, nd, ctl,
-                       next == 1 ? 1 : 0);
+                       cnt, done, tiles, out, cnt + tiles, done + tiles,
+                       parents, n, nd, ctl, next == 1 ? 1 : 0);
     items = out;
     cnt += tiles;
+    done += tiles;
     tiles = next;
   }
 }

@@ -62,8 +62,7 @@ struct lbvh_ctx {
   DevBuf cen;                           // triangle box centres (GPU bounds)
   // batched builds: scratch, and the plan (tile tables + captured graph) of
   // the last batch layout
-  // b_hist: the segments' digit bases of the current sort pass
-  DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_hist, b_ord,
+  DevBuf b_keys_a, b_keys_b, b_idx_a, b_idx_b, b_digits, b_ord,
       b_bounds, b_tree_items, b_tree_counts, b_table, b_cen;
   struct BatchPlan {
     std::vector<uint32_t> offsets;
@@ -312,7 +311,7 @@ std::vector<DevBuf *> all_scratch(lbvh_ctx *ctx) {
           &ctx->tree_counts, &ctx->partials, &ctx->cen,
           &ctx->b_keys_a,
           &ctx->b_keys_b, &ctx->b_idx_a, &ctx->b_idx_b,  &ctx->b_digits,
-          &ctx->b_hist,  &ctx->b_ord,    &ctx->b_bounds, &ctx->b_tree_items,
+          &ctx->b_ord,    &ctx->b_bounds, &ctx->b_tree_items,
           &ctx->b_tree_counts, &ctx->b_table, &ctx->b_cen};
 }
 
@@ -693,7 +692,6 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
                    int32_t *perm) {
   const auto &P = ctx->plan;
   const uint32_t *T = static_cast<const uint32_t *>(ctx->b_table.ptr);
-  uint32_t *seg_base = static_cast<uint32_t *>(ctx->b_hist.ptr);
   uint32_t *ord = static_cast<uint32_t *>(ctx->b_ord.ptr);
   const float4 *t4 = reinterpret_cast<const float4 *>(tri);
   const SortSegs ss{T + P.o_tile_seg, T + P.o_seg_tile0, T + P.o_seg_off};
@@ -720,7 +718,7 @@ void enqueue_batch(lbvh_ctx *ctx, const float *tri, bool have_bounds,
   for (int p = 0; p < SP.passes; ++p)
     launch_sort_pass(P.bits, p, SP.src[p], SP.vsrc[p], SP.dst[p], SP.vdst[p],
                      P.total, P.sort_tiles, cnt, off, p == 0, ctx->ctl,
-                     ctx->stream, &ss, P.S, seg_base);
+                     ctx->stream, &ss, P.S);
   int4 *items = static_cast<int4 *>(ctx->b_tree_items.ptr);
   uint32_t *counts = static_cast<uint32_t *>(ctx->b_tree_counts.ptr);
   uint64_t base = 0, prev = 0;
@@ -1300,8 +1298,6 @@ int lbvh_build_batched(lbvh_ctx *ctx, const float *tri,
   if ((st = ensure(ctx, ctx->b_idx_b, 4ull * total, false))) return st;
   const uint64_t radix = sort_radix(P.bits);
   if ((st = ensure(ctx, ctx->b_digits, 8ull * radix * P.sort_tiles, false)))
-    return st;
-  if ((st = ensure(ctx, ctx->b_hist, 4ull * radix * num_meshes, false)))
     return st;
   if ((st = ensure(ctx, ctx->b_ord, 32ull * num_meshes, false))) return st;
   if ((st = ensure(ctx, ctx->b_tree_items,

@@ -123,15 +123,14 @@ struct SortSegs {
 // and ignores vals_in).  cnt and off hold sort_radix(bits) x tiles words
 // each; counted: cnt already holds this pass's per-tile digit counts (the
 // Morton kernels write pass 0's).  A single mesh takes its digit bases from
-// the column sums of cnt (into ctl->hist); segments from their own tiles'
-// sums (into seg_base, sort_radix(bits) words per segment).  keys_in and
-// keys_out must differ.
+// the column sums of cnt (into ctl->hist); each segment from its own tiles'
+// counts, in one block per segment.  keys_in and keys_out must differ.
 void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       const int32_t *vals_in, uint32_t *keys_out,
                       int32_t *vals_out, uint32_t n, uint32_t tiles,
                       uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
                       hipStream_t s, const SortSegs *segs = nullptr,
-                      uint32_t num_segs = 0, uint32_t *seg_base = nullptr);
+                      uint32_t num_segs = 0);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }

@@ -126,6 +126,9 @@ constexpr uint64_t kMeshBytesPerTri = 168;
 // tile, all levels).
 constexpr uint64_t kBuildScratchPerTri = 72;
 constexpr uint64_t kRefitScratchPerTri = 36;
+// plus what does not scale with n: the context's control block, the bounds
+// partials (80 KB with headroom), and tile tables rounded up to whole tiles
+constexpr uint64_t kSlotScratchFixed = 1ull << 20;
 
 struct Mesh {
   uint32_t n = 0;
@@ -303,7 +306,7 @@ Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
   }
   const uint64_t scratch =
       (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
-      r.n + sizeof(lbvh::Ctl);
+      r.n + kSlotScratchFixed;
   const uint64_t have = sl.scratch_bytes;
   if (scratch > have) need.scratch = scratch - have;
   return need;
@@ -491,7 +494,7 @@ void worker(lbvh_server *s, uint32_t k) {
 // Validates a queued request against the resident state.  LBVH_OK if it can
 // run, another status if it must fail.
 int check_request(lbvh_server *s, const Request &r) {
-  const uint64_t fixed = sizeof(lbvh::Ctl) * s->slots.size();
+  const uint64_t fixed = kSlotScratchFixed * s->slots.size();
   if (s->budget &&
       kMeshBytesPerTri * r.n + kBuildScratchPerTri * r.n + fixed > s->budget)
     return LBVH_ERR_OUT_OF_BUDGET;

@@ -67,7 +67,9 @@ LBVH_HD inline double ln_pos(double x) {
 
 // cos(2 pi y) for y in [0, 1): with r = |y - 1/2| (exact), cos(2 pi y) =
 // -cos(2 pi r); for r > 1/4, -cos(2 pi r) = cos(2 pi (1/2 - r)) (exact).  The
-// angle t <= pi/2 goes into the Taylor series in nested form to t^26.
+// angle t <= pi/2 goes into the Taylor series to t^24 (the next term is
+// below 2^-62), evaluated in t^2 by Horner with the coefficients
+// (-1)^k / (2k)! as f64 constants: multiplies and adds only.
 LBVH_HD inline double cos_2pi(double y) {
   double a = y - 0.5;
   if (a < 0.0) a = -a;
@@ -78,21 +80,35 @@ LBVH_HD inline double cos_2pi(double y) {
   }
   const double t = a * 6.283185307179586;
   const double t2 = t * t;
-  double p = 1.0;
-  for (int k = 13; k >= 1; --k)
-    p = 1.0 - t2 / (double)((2 * k - 1) * (2 * k)) * p;
+  double p = 1.6117375710961184e-24;  //  1/24!
+  p = p * t2 - 8.896791392450574e-22;  // -1/22!
+  p = p * t2 + 4.110317623312165e-19;  // 1/20!
+  p = p * t2 - 1.5619206968586225e-16;  // -1/18!
+  p = p * t2 + 4.779477332387385e-14;  // 1/16!
+  p = p * t2 - 1.1470745597729725e-11;  // -1/14!
+  p = p * t2 + 2.08767569878681e-09;  // 1/12!
+  p = p * t2 - 2.755731922398589e-07;  // -1/10!
+  p = p * t2 + 2.48015873015873e-05;  // 1/8!
+  p = p * t2 - 0.001388888888888889;  // -1/6!
+  p = p * t2 + 0.041666666666666664;  // 1/4!
+  p = p * t2 - 0.5;  // -1/2!
+  p = p * t2 + 1.0;
   return sign * p;
 }
 
-// sqrt of a positive normal v: a first guess with half the exponent, then
-// a fixed number of Newton steps (a guess within a factor of 2 converges in
-// six; eight leave margin).
+// sqrt of a positive normal v = m 2^e (m in [1, 2)): Newton steps for
+// 1/sqrt(v), multiplies only, from 2^-ceil(e/2), which is within a factor
+// of sqrt 2 of it and not above sqrt 3 times it, so the steps converge
+// (relative error 0.41 -> 1e-16 in seven; eight leave margin); then
+// s = v * (1/sqrt(v)) and one correcting step s + (v - s^2) * r / 2.
 LBVH_HD inline double sqrt_pos(double v) {
   const uint64_t b = bits_of(v);
   const int e = (int)((b >> 52) & 0x7FF) - 1023;
-  double y = from_bits((uint64_t)((e >> 1) + 1023) << 52);
-  for (int it = 0; it < 8; ++it) y = 0.5 * (y + v / y);
-  return y;
+  double r = from_bits((uint64_t)(1023 - ((e + 1) >> 1)) << 52);
+  const double h = 0.5 * v;
+  for (int it = 0; it < 8; ++it) r = r * (1.5 - h * r * r);
+  const double s = v * r;
+  return s + (v - s * s) * (0.5 * r);
 }
 
 // Standard normal from draws k and k + 1 of element i (Box-Muller).
