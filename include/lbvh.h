@@ -212,6 +212,22 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
                const float *bounds_host, void *bvh_out, float *tri_out,
                uint32_t *sorted_keys_out, int32_t *perm_out);
 
+/* lbvh_build with no allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as lbvh_build, for at most 1,048,576 triangles (larger n fails
+ * with LBVH_ERR_INVALID_ARGUMENT).  Its scratch is one static 64 MiB array
+ * per device in the library's code object, held from the library's load on
+ * that device until the process ends, not charged to the context's budget;
+ * the call enqueues kernels only (no hipMalloc, hipFree, copy or memset, no
+ * profiling events).  Every call on one device shares that array: calls
+ * that overlap on a device (two contexts or threads, streams not ordered)
+ * race and give undefined results; calls ordered on one stream are safe.
+ * Nothing in the array needs to start at zero. */
+int lbvh_build_static(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                      const float *bounds_host, void *bvh_out, float *tri_out,
+                      uint32_t *sorted_keys_out, int32_t *perm_out);
+
 /* Refit: no counterpart in the reference, which rebuilds from scratch; it
  * re-runs the up pass (bvh-up-pass.js:200-280) and rearrange
  * (rearrange.js:162-169) over new vertex positions `tri` (input order,

@@ -91,6 +91,9 @@ struct lbvh_ctx {
   // allocated on the first large copy
   std::vector<void *> stage_buf;
   std::vector<hipEvent_t> stage_ev;
+  // device address of this device's static scratch (lbvh_build_static), or
+  // null if the lookup failed
+  void *static_scratch = nullptr;
 };
 
 namespace {
@@ -103,6 +106,36 @@ namespace {
 constexpr uint64_t kStageMin = 64ull << 20;
 constexpr uint64_t kStageChunk = 16ull << 20;
 constexpr int kStageThreads = 4;
+
+// Scratch of lbvh_build_static: one array per device, in the code object,
+// allocated when the library's code is loaded on the device and held until
+// the process ends.  Host code never passes `scratch` itself (a host-side
+// stand-in) to a kernel: it looks up the array's device address once per
+// device (static_scratch_address) and uses that.
+constexpr uint32_t kStaticMaxTris = 1u << 20;
+constexpr uint64_t kStaticScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ptr[kMaxDevices] = {};
+bool g_static_looked_up[kMaxDevices] = {};
+
+// The calling thread's current device must be `device`.
+void *static_scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_static_mu);
+  if (!g_static_looked_up[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) {
+      g_static_ptr[device] = p;
+    } else {
+      (void)hipGetLastError();
+    }
+    g_static_looked_up[device] = true;
+  }
+  return g_static_ptr[device];
+}
 
 int fail(lbvh_ctx *ctx, int status, const std::string &msg) {
   if (ctx) ctx->last_error = msg;
@@ -536,7 +569,8 @@ MortonParams morton_params(const lbvh_ctx *ctx, const float *bounds) {
 
 // Device scratch of one build: sort ping-pong buffers, per-tile digit
 // counts and offsets, agglomeration item slots and counts, bounds partials
-// and (GPU bounds) the box centres.
+// and (GPU bounds) the box centres.  From the context's arena (lbvh_build)
+// or the library's static array (lbvh_build_static).
 struct BuildScratch {
   uint32_t *keys_a = nullptr, *keys_b = nullptr;
   int32_t *idx_a = nullptr, *idx_b = nullptr;
@@ -608,6 +642,31 @@ void build_pipeline(lbvh_ctx *ctx, const BuildScratch &S, const float *tri,
     launch_single_key(keys, skeys, perm, ctx->stream);
   run_tree(ctx, S.tree_items, S.tree_counts, tri, skeys, perm, n, tri_out,
            bvh_out, tile);
+}
+
+// Lays out one build's scratch from `base` (null: sizes only); returns the
+// bytes it spans.  Regions start 256-byte aligned.
+uint64_t layout_build_scratch(uint32_t n, int bits, int tile, bool use_cen,
+                              char *base, BuildScratch *S) {
+  uint64_t at = 0;
+  const auto take = [&](uint64_t bytes) -> void * {
+    void *p = base ? base + at : nullptr;
+    at += (bytes + 255) & ~255ull;
+    return p;
+  };
+  S->keys_a = static_cast<uint32_t *>(take(4ull * n));
+  S->keys_b = static_cast<uint32_t *>(take(4ull * n));
+  S->idx_a = static_cast<int32_t *>(take(4ull * n));
+  S->idx_b = static_cast<int32_t *>(take(4ull * n));
+  S->digits = static_cast<uint32_t *>(
+      take(8ull * sort_tiles(n) * sort_radix(bits)));
+  const uint64_t tiles = tree_tiles_total(n, tile);
+  S->tree_items =
+      static_cast<int4 *>(take(16ull * kItemInt4 * kTreeFrontier * tiles));
+  S->tree_counts = static_cast<uint32_t *>(take(4ull * tiles));
+  S->partials = static_cast<float4 *>(take(32ull * kBoundsBlocks));
+  S->cen = use_cen ? static_cast<float *>(take(12ull * cen_stride(n))) : nullptr;
+  return at;
 }
 
 // ---- batched builds ---------------------------------------------------------
@@ -793,6 +852,7 @@ int lbvh_create(int device_ordinal, uint64_t hbm_budget_bytes,
     return st != LBVH_OK ? st : LBVH_ERR_DEVICE;
   }
   ctx->ctl = static_cast<Ctl *>(p);
+  ctx->static_scratch = static_scratch_address(device_ordinal);
   *out_ctx = ctx;
   return LBVH_OK;
 }
@@ -1237,6 +1297,43 @@ int lbvh_build(lbvh_ctx *ctx, const float *tri, uint32_t n,
   build_pipeline(ctx, S, tri, n, bounds_host, bvh_out, tri_out,
                  sorted_keys_out, perm_out, bits, tile);
   return check_launch(ctx, "lbvh_build");
+}
+
+int lbvh_build_static(lbvh_ctx *ctx, const float *tri, uint32_t n,
+                      const float *bounds_host, void *bvh_out, float *tri_out,
+                      uint32_t *sorted_keys_out, int32_t *perm_out) {
+  CTX_GUARD(ctx);
+  int st;
+  if ((st = check_build_args(ctx, tri, n, bvh_out, tri_out, sorted_keys_out,
+                             perm_out)) || n == 0)
+    return st;
+  if ((st = check_bounds(ctx, bounds_host, 1))) return st;
+  if (n > kStaticMaxTris)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                "lbvh_build_static takes at most " +
+                    std::to_string(kStaticMaxTris) + " triangles (" +
+                    std::to_string(n) + " given); use lbvh_build");
+  if (!ctx->static_scratch)
+    return fail(ctx, LBVH_ERR_DEVICE,
+                "the static scratch has no address on this device");
+  const int bits = bits_for(ctx, n);
+  const int tile = tile_for(ctx, n);
+  BuildScratch S;
+  const uint64_t need =
+      layout_build_scratch(n, bits, tile, !bounds_host && n > 1,
+                           static_cast<char *>(ctx->static_scratch), &S);
+  if (need > kStaticScratchBytes)
+    return fail(ctx, LBVH_ERR_INVALID_ARGUMENT,
+                "the build needs " + std::to_string(need) +
+                    " bytes of scratch, more than the static " +
+                    std::to_string(kStaticScratchBytes));
+  // no profiling events: they would be created here on first use
+  const bool prof = ctx->prof;
+  ctx->prof = false;
+  build_pipeline(ctx, S, tri, n, bounds_host, bvh_out, tri_out,
+                 sorted_keys_out, perm_out, bits, tile);
+  ctx->prof = prof;
+  return check_launch(ctx, "lbvh_build_static");
 }
 
 int lbvh_refit(lbvh_ctx *ctx, const float *tri, uint32_t n,

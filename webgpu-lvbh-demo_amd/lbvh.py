@@ -85,6 +85,7 @@ SIGNATURES = {
     "lbvh_up_pass": (_i32, [_vp, _vp, _vp, _vp, _u32, _vp]),
     "lbvh_rearrange": (_i32, [_vp, _vp, _vp, _u32, _vp]),
     "lbvh_build": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "lbvh_build_static": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
     "lbvh_refit": (_i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "lbvh_raycast": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     "lbvh_build_batched": (_i32, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
@@ -328,6 +329,13 @@ class Context:
         b = None if bounds is None else np.asarray(bounds, dtype=np.float32)
         self._check(self.lib.lbvh_build(self.handle, _dp(tri), n, _ptr(b), _dp(bvh),
                                         _dp(otri), _dp(skeys), _dp(perm)))
+
+    def build_static(self, tri, n: int, bounds: Optional[Sequence[float]], bvh, otri,
+                     skeys=None, perm=None):
+        """lbvh_build_static: lbvh_build without allocating (n <= 2^20)."""
+        b = None if bounds is None else np.asarray(bounds, dtype=np.float32)
+        self._check(self.lib.lbvh_build_static(self.handle, _dp(tri), n, _ptr(b), _dp(bvh),
+                                               _dp(otri), _dp(skeys), _dp(perm)))
 
     def build_batched(self, tri, offsets: np.ndarray, bounds, bvh, otri,
                       skeys=None, perm=None):
