@@ -439,14 +439,19 @@ int tile_for(const lbvh_ctx *ctx, uint32_t n) {
   return tuned_for(ctx->tree_tile, ctx->tree_tile_fixed, ctx->tune_tile, n);
 }
 
-// Untuned, 10-bit digits up to 2^21 keys (one pass and four launches fewer,
-// 5-9% faster builds of uniform meshes of 2^16..2^21 triangles on MI355X)
-// and 8-bit digits above (the 10-bit scatter holds 57 KB of LDS per block,
-// half the 8-bit one's occupancy, and was 2-4% slower at 2^22..2^24).
+// Untuned, 10-bit digits from 2^16 to 2^21 keys (one pass and four
+// launches fewer, 4-9% faster builds of uniform meshes of 2^17..2^21
+// triangles on MI355X) and 8-bit digits elsewhere: at most 2^16 keys sort
+// without a scan kernel, each scatter tile summing its digits' count rows,
+// four times as many words with 10-bit digits (2^16: 0.098 against
+// 0.105 ms); above 2^21 the 10-bit scatter's LDS (half the 8-bit one's
+// occupancy) made builds 2-4% slower.
+constexpr uint32_t kWideDigitsMinKeys = (1u << 16) + 1;
 constexpr uint32_t kWideDigitsMaxKeys = 1u << 21;
 int bits_for(const lbvh_ctx *ctx, uint32_t n) {
+  const bool wide = n >= kWideDigitsMinKeys && n <= kWideDigitsMaxKeys;
   return tuned_for(ctx->sort_bits, ctx->sort_bits_fixed, ctx->tune_bits, n,
-                   n <= kWideDigitsMaxKeys ? kWideBits : kRadixBits);
+                   wide ? kWideBits : kRadixBits);
 }
 
 // The key buffers of each pass of a sort from `in` to `out`: each pass reads

@@ -195,6 +195,8 @@ struct Work {
 };
 
 constexpr int kSlotDepth = 2;  // requests in flight per slot
+// How long an idle slot worker polls its mailbox before it blocks.
+constexpr double kWorkerPollS = 200e-6;
 // Admission looks at this many requests at the head of the queue per round:
 // enough to find runnable work past held ones, and it keeps a round's cost
 // independent of a backlog (a scan of the whole queue made each round slower
@@ -216,6 +218,7 @@ struct Slot {
   unsigned long long *sums = nullptr;
   std::mutex m;             // guards mailbox, inflight and their Work
   std::deque<Work> mailbox;   // assigned, not yet taken by the worker
+  std::atomic<uint32_t> mail_n{0};  // mailbox.size(), readable without m
   std::deque<Work> inflight;  // taken by the worker, in stream order
   std::condition_variable wcv;
   std::thread worker;
@@ -477,10 +480,21 @@ void worker(lbvh_server *s, uint32_t k) {
   Slot &sl = s->slots[k];
   std::unique_lock<std::mutex> lock(sl.m);
   while (true) {
+    if (sl.mailbox.empty() && !s->stop) {
+      // A blocked thread takes tens of microseconds to wake: poll for a
+      // while first, while requests keep coming.
+      lock.unlock();
+      const double until = now_s() + kWorkerPollS;
+      while (sl.mail_n.load(std::memory_order_acquire) == 0 && !s->stop &&
+             now_s() < until)
+        std::this_thread::yield();
+      lock.lock();
+    }
     sl.wcv.wait(lock, [&] { return s->stop || !sl.mailbox.empty(); });
     if (s->stop) break;
     sl.inflight.push_back(std::move(sl.mailbox.front()));
     sl.mailbox.pop_front();
+    sl.mail_n.fetch_sub(1, std::memory_order_relaxed);
     Work &w = sl.inflight.back();  // deque: stays valid while others change
     lock.unlock();
     const int st = enqueue(sl, w);
@@ -613,6 +627,7 @@ void dispatcher(lbvh_server *s) {
           {
             std::lock_guard<std::mutex> g(sl.m);
             sl.mailbox.push_back(std::move(w));
+            sl.mail_n.fetch_add(1, std::memory_order_release);
           }
           sl.wcv.notify_one();
           s->peak = std::max(s->peak, in_use(s));
@@ -634,10 +649,17 @@ void dispatcher(lbvh_server *s) {
       lock.lock();
       continue;
     }
-    // submit notifies without taking `mu`; the timeout bounds a wake-up it
-    // might race past
-    if (s->inbox_n == 0)
-      s->cv.wait_for(lock, std::chrono::microseconds(idle ? 50 : 20));
+    // While device work is in flight, poll: a timed wait sleeps for at least
+    // the kernel's timer slack (about 50 us), which would be added to every
+    // request's latency.  When idle, submit notifies; the timeout bounds a
+    // wake-up it might race past.
+    if (!idle) {
+      lock.unlock();
+      std::this_thread::yield();
+      lock.lock();
+    } else if (s->inbox_n == 0) {
+      s->cv.wait_for(lock, std::chrono::microseconds(50));
+    }
   }
 }
 
