@@ -209,6 +209,10 @@ struct Slot {
   // dispatcher reads this instead of the context, whose lock the worker
   // holds while it enqueues
   std::atomic<uint64_t> scratch_bytes{0};
+  // dispatcher only: what the slot's admitted requests may grow its scratch
+  // to (their estimates), counted until the slot is idle again, so that
+  // growth admitted but not yet allocated is not handed out twice
+  uint64_t reserve = 0;
   hipEvent_t events[kSlotDepth] = {};
   uint32_t next_event = 0;  // dispatcher only
   // per event: what the request's epilogue left (pinned host memory, and
@@ -265,7 +269,7 @@ namespace {
 
 uint64_t slots_bytes(lbvh_server *s) {
   uint64_t b = 0;
-  for (auto &sl : s->slots) b += sl.scratch_bytes;
+  for (auto &sl : s->slots) b += std::max<uint64_t>(sl.scratch_bytes, sl.reserve);
   return b;
 }
 
@@ -298,6 +302,7 @@ void free_mesh(lbvh_server *s, Mesh &m) {
 // one of its size) and slot scratch beyond what the slot holds.
 struct Need {
   uint64_t mesh = 0, scratch = 0;
+  uint64_t slot_total = 0;  // the slot's scratch estimate with this request
   bool any() const { return mesh || scratch; }
 };
 Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
@@ -310,8 +315,9 @@ Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
   const uint64_t scratch =
       (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
       r.n + kSlotScratchFixed;
-  const uint64_t have = sl.scratch_bytes;
+  const uint64_t have = std::max<uint64_t>(sl.scratch_bytes, sl.reserve);
   if (scratch > have) need.scratch = scratch - have;
+  need.slot_total = std::max(scratch, have);
   return need;
 }
 
@@ -357,6 +363,7 @@ bool make_room(lbvh_server *s, const Need &need, uint64_t keep) {
         // hands out, so it does not touch the context meanwhile
         lbvh_trim(sl.ctx);
         sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
+        sl.reserve = 0;
         trimmed = true;
       }
     if (!trimmed) return false;
@@ -570,6 +577,8 @@ void dispatcher(lbvh_server *s) {
         sl.inflight.pop_front();
         progress = true;
       }
+      // an idle slot's scratch_bytes holds all it has grown to
+      if (sl.mailbox.empty() && sl.inflight.empty()) sl.reserve = 0;
     }
     // Queueing deadlines, at the head of the queue whether or not a slot is
     // free (with one maximum wait they pass in arrival order).
@@ -614,6 +623,7 @@ void dispatcher(lbvh_server *s) {
           ++qi;
           continue;
         }
+        sl.reserve = need.slot_total;
         Work w;
         w.req = std::move(*qi);
         qi = s->queue.erase(qi);
