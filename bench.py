@@ -125,7 +125,36 @@ def per_build_ms(ctx, hip, d_tri, n, bounds, bvh, otri, runs):
     return out
 
 
-def batched_bench(lb, hip, meshes=2048, steps=5, log=print):
+def small_builds_bench(lb, ctx, hip, runs=50, log=print):
+    """Latency of single builds of the small configurations (uniform-64k,
+    clustered-1m) on a context already tuned for their sizes: the median and
+    best device time of an event pair around each build, and the time per
+    build of back-to-back builds."""
+    out = {}
+    for name in ("uniform-64k", "clustered-1m"):
+        tri = lb.workload(name)
+        n = tri.shape[0]
+        d = ctx.to_device(tri)
+        bvh, otri = ctx.alloc(64 * n), ctx.alloc(48 * n)
+        for _ in range(5):
+            ctx.build(d, n, None, bvh, otri)
+        each = per_build_ms(ctx, hip, d, n, None, bvh, otri, runs)
+        ms, _ = timed_builds(lb, ctx, hip, d, n, None, bvh, otri, runs)
+        out[name] = {"triangles": n,
+                     "tree_tile": ctx.get_tuning_for(lb.TUNE_TREE_TILE, n),
+                     "sort_digit_bits": ctx.get_tuning_for(lb.TUNE_SORT_BITS, n),
+                     "median_ms": round(each[len(each) // 2], 4),
+                     "min_ms": round(each[0], 4),
+                     "back_to_back_ms": round(ms / runs, 4)}
+        log(f"small build {name}: median {out[name]['median_ms']} ms")
+        for x in (d, bvh, otri):
+            x.destroy()
+    out["note"] = ("an event pair around each build, builds enqueued back to back "
+                   "after 5 warm-ups; inputs resident, GPU bounds")
+    return out
+
+
+def batched_bench(lb, hip, meshes=2048, steps=5, log=print, loop=True):
     """2,048 independent meshes, log-uniform in [256, 262144] triangles, each a
     uniform mesh with its own seed: one graph-captured lbvh_build_batched vs a
     loop of per-mesh lbvh_build calls on the same stream."""
@@ -150,29 +179,33 @@ def batched_bench(lb, hip, meshes=2048, steps=5, log=print):
         ctx.build_batched(d, offsets, None, bvh, otri)
     hip.record(b, ctx.stream)
     batched_ms = hip.elapsed_ms(a, b) / steps
-    log("batched: per-mesh loop")
-    nodes, tris = bvh.ptr, otri.ptr
+    loop_ms = None
+    if loop:
+        log("batched: per-mesh loop")
+        nodes, tris = bvh.ptr, otri.ptr
 
-    def loop():
-        for k in range(meshes):
-            o = int(offsets[k])
-            ctx.build(d.ptr + 48 * o, int(sizes[k]), None, nodes + 64 * o, tris + 48 * o)
+        def run_loop():
+            for k in range(meshes):
+                o = int(offsets[k])
+                ctx.build(d.ptr + 48 * o, int(sizes[k]), None, nodes + 64 * o,
+                          tris + 48 * o)
 
-    loop()
-    ctx.synchronize()
-    hip.record(a, ctx.stream)
-    loop_steps = 2
-    for _ in range(loop_steps):
-        loop()
-    hip.record(b, ctx.stream)
-    loop_ms = hip.elapsed_ms(a, b) / loop_steps
+        run_loop()
+        ctx.synchronize()
+        hip.record(a, ctx.stream)
+        loop_steps = 2
+        for _ in range(loop_steps):
+            run_loop()
+        hip.record(b, ctx.stream)
+        loop_ms = hip.elapsed_ms(a, b) / loop_steps
     for x in (d, bvh, otri):
         x.destroy()
     ctx.close()
     return {"workload": f"{meshes} uniform meshes, log-uniform [256, 262144]",
             "meshes": meshes, "triangles": total,
-            "batched_ms": round(batched_ms, 3), "loop_ms": round(loop_ms, 3),
-            "speedup": round(loop_ms / batched_ms, 2),
+            "batched_ms": round(batched_ms, 3),
+            "loop_ms": round(loop_ms, 3) if loop_ms else None,
+            "speedup": round(loop_ms / batched_ms, 2) if loop_ms else None,
             "batched_Mtri_s": round(total / (batched_ms * 1e-3) / 1e6, 1)}
 
 
@@ -333,12 +366,18 @@ def main():
                     help="radix-sort digit width (8 or 10)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small-build latencies (uniform-64k, "
+                         "clustered-1m)")
     ap.add_argument("--no-js", action="store_true",
                     help="skip the Node.js build() wall-time leg")
     ap.add_argument("--serving-requests", type=int, default=20000)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched leg (for per-kernel profiles of "
                          "the batched pipeline) and print its JSON")
+    ap.add_argument("--no-batched-loop", action="store_true",
+                    help="with --batched-only: time the batched build alone, "
+                         "without the per-mesh loop it is compared with")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB, the "
                          "known-size reference for HBM traffic counters")
@@ -351,7 +390,9 @@ def main():
     import lbvh as lb
     hip = Hip()
     if args.batched_only:
-        print(json.dumps({"batched": batched_bench(lb, hip, log=log)}), flush=True)
+        print(json.dumps({"batched": batched_bench(lb, hip, log=log,
+                                                   loop=not args.no_batched_loop)}),
+              flush=True)
         return
     log(f"generating {args.workload}")
     tri = lb.workload(args.workload)
@@ -494,6 +535,9 @@ def main():
 
     for b in (d_tri, bvh, otri):
         b.destroy()
+    small = None
+    if not args.no_small and tuning is not None:
+        small = small_builds_bench(lb, ctx, hip, log=log)
     ctx.close()
 
     js = None
@@ -580,6 +624,7 @@ def main():
         "profiled_ms_per_step": round(pms / args.steps, 4),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 4),
         "js_build": js,
+        "small_builds": small,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -85,7 +85,8 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
 }
 
 // Per-mesh leaf AABB centre -> Morton key, plus the first sort pass's
-// per-tile digit counts (cnt0[d * tiles + tile]).  Bounds: host-provided fp32
+// per-tile digit counts (tile-major, cnt0[tile * R + d], as the batched
+// sort keeps them).  Bounds: host-provided fp32
 // (6 per mesh; centres from the triangles) or the GPU accumulators (centres
 // from k_batch_bounds, cen non-null).
 template <int kBits>
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kR; d += kSortThreads)
-    cnt0[(size_t)d * gridDim.x + tile] = s_hist[d];
+    cnt0[(size_t)tile * kR + d] = s_hist[d];
 }
 
 void launch_batch_prologue(uint32_t *seg_ord, uint32_t num_segs,

@@ -29,9 +29,12 @@
 // column sums of its per-tile counts.  A batched sort's meshes are segments
 // of whole tiles, each sorted on its own: one block per mesh turns its
 // tiles' counts into their offsets (k_seg_scan).  No device-scope atomics
-// anywhere.  Counts and
-// offsets are digit-major: entry (d, t) at d * tiles + t, so the scan reads
-// and writes columns contiguously.  Tile ids are XCD-aware
+// anywhere.  A single mesh's counts and offsets are digit-major, entry
+// (d, t) at d * tiles + t, so the column scan reads and writes columns
+// contiguously; a batched sort's are tile-major, (d, t) at t * R + d, so a
+// mesh's entries are one contiguous run however few tiles it has (digit-
+// major, a mesh of a tile or two touched a cache line per digit for a few
+// bytes each).  Tile ids are XCD-aware
 // (xcd_tile): consecutive tiles run on the same XCD, so the cache lines that
 // neighbouring tiles share -- count columns and the boundaries of their digit
 // runs in the output -- are merged in one L2.
@@ -55,8 +58,28 @@ struct Digits {
   static_assert(kDpt >= 1 && kR % kSortThreads == 0, "whole digits per thread");
 };
 
-// Per-tile digit counts of one pass -> cnt[d * tiles + tile].
-template <int kBits>
+// Adds the wave's valid digits to s_c.  kMatch: the lanes holding one
+// digit are matched by ballots and their lowest lane adds the group's size,
+// one LDS atomic per distinct digit.  Per-lane atomics serialise on a
+// counter that many lanes share, which the top digit of Morton keys makes
+// the common case (at most 64 values, a few per tile); with many distinct
+// digits the ballots cost more than the conflicts they save.  Every lane of
+// the wave must call it.
+template <int kBits, bool kMatch>
+__device__ __forceinline__ void count_digit(uint32_t *s_c, uint32_t d,
+                                            bool valid) {
+  if (kMatch) {
+    const uint64_t peers = match_digit<kBits>(d, valid);
+    if (valid && popc_below(peers) == 0)
+      atomicAdd(&s_c[d], (uint32_t)__popcll(peers));
+  } else if (valid) {
+    digit_add(s_c, d);
+  }
+}
+
+// Per-tile digit counts of one pass -> cnt (digit-major, or tile-major for a
+// batched sort).
+template <int kBits, bool kMatch>
 __global__ __launch_bounds__(kSortThreads) void k_count(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
     uint32_t *__restrict__ cnt, SortSegs segs) {
@@ -87,7 +110,7 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
     }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j)
-      digit_add(s_c, (k[j] >> shift) & (D::kR - 1));
+      count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1), true);
   } else {
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
@@ -96,12 +119,13 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
     }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j)
-      if (r.begin + threadIdx.x + j * kSortThreads < r.end)
-        digit_add(s_c, (k[j] >> shift) & (D::kR - 1));
+      count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1),
+                                 r.begin + threadIdx.x + j * kSortThreads < r.end);
   }
   __syncthreads();
+  const bool tile_major = segs.tile_seg != nullptr;  // batched
   for (int d = threadIdx.x; d < D::kR; d += kSortThreads)
-    cnt[(size_t)d * tiles + tile] = s_c[d];
+    cnt[tile_major ? (size_t)tile * D::kR + d : (size_t)d * tiles + tile] = s_c[d];
 }
 
 // Exclusive prefix of h over all kR digits at each of the calling thread's
@@ -180,36 +204,59 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_columns(
 // Offsets of a batched pass, one block per segment (mesh).  A segment's
 // tiles are consecutive and hold only its keys, so the run of digit d in
 // its tile t starts at seg_off + (the segment's keys with a smaller digit)
-// + (digit-d keys in the segment's earlier tiles).  Thread t owns digits
-// [t * kDpt, (t + 1) * kDpt): it sums their counts over the segment's
-// tiles, the block scans the sums into bases, and the thread writes the
-// running positions.  An empty segment has no tiles.
+// + (digit-d keys in the segment's earlier tiles).  The counts are
+// tile-major, so the segment's are one contiguous run of tiles * R words:
+// thread t owns digits [t * kDpt, (t + 1) * kDpt) and reads them tile after
+// tile (each tile's R words in one coalesced sweep of the block), kUnroll
+// tiles in flight; it sums them over the segment, the block scans the sums
+// into bases, and the thread writes its running positions.  An empty
+// segment has no tiles.
 template <int kBits>
 __global__ __launch_bounds__(kSortThreads) void k_seg_scan(
     const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
     uint32_t tiles, SortSegs segs, uint32_t num_segs) {
   using D = Digits<kBits>;
+  constexpr int kUnroll = 8;
   __shared__ uint32_t s_wsum[16];
   const uint32_t sg = blockIdx.x;
   const uint32_t t0 = segs.seg_tile0[sg];
   const uint32_t t1 = (sg + 1 < num_segs) ? segs.seg_tile0[sg + 1] : tiles;
-  uint32_t total[D::kDpt], below[D::kDpt];
+  const uint32_t d0 = threadIdx.x * D::kDpt;
+  uint32_t total[D::kDpt];
 #pragma unroll
-  for (int i = 0; i < D::kDpt; ++i) {
-    const uint32_t *col = cnt + (size_t)(threadIdx.x * D::kDpt + i) * tiles;
-    total[i] = 0;
-    for (uint32_t t = t0; t < t1; ++t) total[i] += col[t];
+  for (int i = 0; i < D::kDpt; ++i) total[i] = 0u;
+  for (uint32_t t = t0; t < t1; t += kUnroll) {
+    uint32_t c[kUnroll][D::kDpt];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+      for (int i = 0; i < D::kDpt; ++i)
+        c[u][i] = t + u < t1 ? cnt[(size_t)(t + u) * D::kR + d0 + i] : 0u;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+      for (int i = 0; i < D::kDpt; ++i) total[i] += c[u][i];
   }
-  digit_prefix<kBits>(total, below, s_wsum);
+  uint32_t run[D::kDpt];
+  digit_prefix<kBits>(total, run, s_wsum);
   const uint32_t first = segs.seg_off[sg];
 #pragma unroll
-  for (int i = 0; i < D::kDpt; ++i) {
-    const size_t row = (size_t)(threadIdx.x * D::kDpt + i) * tiles;
-    uint32_t run = first + below[i];
-    for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t c = cnt[row + t];
-      off[row + t] = run;
-      run += c;
+  for (int i = 0; i < D::kDpt; ++i) run[i] += first;
+  for (uint32_t t = t0; t < t1; t += kUnroll) {
+    uint32_t c[kUnroll][D::kDpt];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+      for (int i = 0; i < D::kDpt; ++i)
+        c[u][i] = t + u < t1 ? cnt[(size_t)(t + u) * D::kR + d0 + i] : 0u;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (t + u >= t1) break;
+#pragma unroll
+      for (int i = 0; i < D::kDpt; ++i) {
+        off[(size_t)(t + u) * D::kR + d0 + i] = run[i];
+        run[i] += c[u][i];
+      }
     }
   }
 }
@@ -309,6 +356,10 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
       goff[i] = pre;
       s_gbase[d] = run;  // read back by this thread only
     }
+  } else if (segs.tile_seg) {  // batched: tile-major
+#pragma unroll
+    for (int i = 0; i < D::kDpt; ++i)
+      goff[i] = off[(size_t)tile * D::kR + tid * D::kDpt + i];
   } else {
 #pragma unroll
     for (int i = 0; i < D::kDpt; ++i)
@@ -498,13 +549,18 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                uint32_t *keys_out, int32_t *vals_out, uint32_t n,
                uint32_t tiles, uint32_t *cnt, uint32_t *off, bool counted,
                Ctl *ctl, hipStream_t s, const SortSegs &sg, bool segmented,
-               uint32_t num_segs) {
+               uint32_t num_segs, int key_bits) {
   constexpr int kR = Digits<kBits>::kR;
   const uint32_t shift = (uint32_t)pass * kBits;
   uint32_t *hist = ctl->hist + (size_t)pass * kR;
-  if (!counted)
-    hipLaunchKernelGGL(k_count<kBits>, dim3(tiles), dim3(kSortThreads), 0, s,
-                       keys_in, n, shift, cnt, sg);
+  // a digit of at most 6 significant bits: few values, shared counters
+  const bool few = key_bits - (int)shift <= 6;
+  if (!counted && few)
+    hipLaunchKernelGGL((k_count<kBits, true>), dim3(tiles), dim3(kSortThreads),
+                       0, s, keys_in, n, shift, cnt, sg);
+  else if (!counted)
+    hipLaunchKernelGGL((k_count<kBits, false>), dim3(tiles),
+                       dim3(kSortThreads), 0, s, keys_in, n, shift, cnt, sg);
   // the digit totals the scatter scans into bases (null: off is complete)
   const uint32_t *bases_from = nullptr;
   // a small single mesh: the scatter reads the counts themselves
@@ -545,18 +601,18 @@ void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       int32_t *vals_out, uint32_t n, uint32_t tiles,
                       uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
                       hipStream_t s, const SortSegs *segs,
-                      uint32_t num_segs) {
+                      uint32_t num_segs, int key_bits) {
   if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
   if (bits == kWideBits)
     sort_pass<kWideBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
                          cnt, off, counted, ctl, s, sg, segs != nullptr,
-                         num_segs);
+                         num_segs, key_bits);
   else
     sort_pass<kRadixBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
                           cnt, off, counted, ctl, s, sg, segs != nullptr,
-                          num_segs);
+                          num_segs, key_bits);
 }
 
 }  // namespace lbvh

@@ -371,8 +371,9 @@ __device__ void write_single_leaf_node(int4 *nodes, const float mn[3],
 // another: kernel boundaries order the levels.
 constexpr int kUpTile = 1024;
 constexpr int kUpFront = 128;  // items a block leaves at most (2 per level)
-constexpr int kUpFanIn = 8;    // blocks of the level below per item block
-static_assert(kUpFanIn * kUpFront <= kUpTile, "one thread per item");
+constexpr int kUpFanIn = 32;   // blocks of the level below per item block
+static_assert(kUpFanIn < kUpTile && kUpFront < kUpTile,
+              "one thread per source block; a round takes new items");
 
 struct UpLds {
   float box[2][6][kUpTile];  // [side][min.xyz max.xyz][parent id - block start]
@@ -389,29 +390,32 @@ struct UpLds {
   int32_t par_node[kUpTile + 1];  // parents[id]
 };
 
-// parents[n + leaf] / parents[node], from LDS inside the block's staged range
-__device__ __forceinline__ int32_t up_par_leaf(const UpLds &L,
-                                               const int32_t *__restrict__ par,
-                                               uint32_t n, int64_t ts,
-                                               int64_t te, int64_t id) {
-  return (id >= ts && id <= te) ? L.par_leaf[id - ts] : par[n + id];
+// parents[n + id] / parents[id] for id in the block's staged range
+// [ts, ts + kUpTile].  A climb only asks for ids there: it climbs through
+// leaves and nodes of the block's range [ts, te) and reads their successor
+// (<= te) and, for a node inside the range, its parent entry.  The index is
+// clamped all the same, and the arrays are read from LDS only: a choice
+// between LDS and global memory per lane compiles to a generic (flat) load,
+// which waits on the vector memory path at every step of the climb.
+__device__ __forceinline__ int32_t up_par_leaf(const UpLds &L, int32_t ts,
+                                               int32_t id) {
+  const int32_t k = id - ts;
+  return L.par_leaf[k < 0 ? 0 : (k > kUpTile ? kUpTile : k)];
 }
-__device__ __forceinline__ int32_t up_par_node(const UpLds &L,
-                                               const int32_t *__restrict__ par,
-                                               int64_t ts, int64_t te,
-                                               int64_t id) {
-  return (id >= ts && id <= te) ? L.par_node[id - ts] : par[id];
+__device__ __forceinline__ int32_t up_par_node(const UpLds &L, int32_t ts,
+                                               int32_t id) {
+  const int32_t k = id - ts;
+  return L.par_node[k < 0 ? 0 : (k > kUpTile ? kUpTile : k)];
 }
 
 // Node num (a leaf when leaf) is its parent p's left child when the child
 // numbered one above it has the same parent.
-__device__ __forceinline__ bool up_is_left(const UpLds &L,
-                                           const int32_t *__restrict__ par,
-                                           uint32_t n, int64_t ts, int64_t te,
-                                           int64_t num, int32_t p) {
-  const int64_t sib = num + 1;
-  return (sib <= (int64_t)n - 1 && up_par_leaf(L, par, n, ts, te, sib) == p) ||
-         (sib <= (int64_t)n - 2 && up_par_node(L, par, ts, te, sib) == p);
+__device__ __forceinline__ bool up_is_left(const UpLds &L, uint32_t n,
+                                           int32_t ts, int32_t num,
+                                           int32_t p) {
+  const int32_t sib = num + 1;
+  return (sib <= (int32_t)n - 1 && up_par_leaf(L, ts, sib) == p) ||
+         (sib <= (int32_t)n - 2 && up_par_node(L, ts, sib) == p);
 }
 
 // An item: a complete subtree that has still to arrive at its parent.
@@ -431,8 +435,9 @@ __device__ __forceinline__ void up_put_item(int4 *slot, int self, int32_t lo,
 // passes its subtree's position in the block (0 .. kUpTile-1), a thread
 // that is not pending -1.  mark is kUpTile words of free LDS.  count_out
 // gets the number written; more than kUpFront, or any at all when `last`
-// (the block held the whole tree), sets ctl->fault.
-__device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
+// (the block held the whole tree), sets ctl->fault.  Returns the number of
+// pending items to every thread.
+__device__ __forceinline__ uint32_t up_emit(uint32_t *mark, uint32_t *s_wsum,
                                         int at, int4 *items,
                                         uint32_t *count_out, Ctl *ctl,
                                         int last, int self, int32_t lo,
@@ -447,8 +452,11 @@ __device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
   const uint32_t v = mark[tid];
   const uint32_t pos = block_exclusive_scan(v, s_wsum);
   mark[tid] = pos;
+  // the scan leaves the per-wave sums in s_wsum
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < kUpTile / 64; ++w) total += s_wsum[w];
   if (tid == kUpTile - 1) {
-    const uint32_t total = pos + v;
     *count_out = total < (uint32_t)kUpFront ? total : (uint32_t)kUpFront;
     if (total > (uint32_t)kUpFront || (last && total > 0))
       atomicOr(&ctl->fault, kFaultUpFrontier);
@@ -459,6 +467,7 @@ __device__ __forceinline__ void up_emit(uint32_t *mark, uint32_t *s_wsum,
     if (q < (uint32_t)kUpFront)
       up_put_item(items + 3 * (size_t)q, self, lo, hi, p, side, mn, mx);
   }
+  return total;
 }
 
 // The number of nodes that got exactly two children in this block, summed
@@ -537,7 +546,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
       // check keep a malformed parents array from running away.
       for (int step = 0; step < 128; ++step) {
         if (p < 0 || p >= (int32_t)n - 1) break;
-        side = up_is_left(L, par, n, ts, te, num, p) ? 0 : 1;
+        side = up_is_left(L, n, (int32_t)ts, (int32_t)num, p) ? 0 : 1;
         self = leaf ? -(int)(num + 1) : (int)num;
         if (p < ts || p >= te) {  // the parent lies outside the block
           mode = 2;
@@ -567,7 +576,7 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         if (p == 0) break;  // root merged
         leaf = false;
         num = p;
-        p = up_par_node(L, par, ts, te, p);
+        p = up_par_node(L, (int32_t)ts, p);
       }
     }
   }
@@ -607,29 +616,37 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
 }
 
 // One level of items: block b takes the items of blocks [kUpFanIn b,
-// kUpFanIn b + kUpFanIn) of the level below, in leaf order (at most
-// kUpFanIn * kUpFront, one thread each).  Those items partition a leaf
-// range, and the split of every node still to complete inside it is a gap
-// between two adjacent items: arrivals meet in that gap's LDS slot (the left
-// child at the gap after its last item, the right child at the gap before
-// its first).  Each arrival writes its half of the parent node; the second
-// one merges and climbs on.  Subtrees whose sibling reaches outside the
-// range are the level's items.
+// kUpFanIn b + kUpFanIn) of the level below, in leaf order.  Those items
+// partition a leaf range, and the split of every node still to complete
+// inside it is a gap between two adjacent items: arrivals meet in that
+// gap's LDS slot (the left child at the gap after its last item, the right
+// child at the gap before its first).  Each arrival writes its half of the
+// parent node; the second one merges and climbs on.  Subtrees whose sibling
+// reaches outside the range are the level's items.
+//
+// The block sweeps its items in rounds of at most kUpTile, one thread each:
+// the items the previous round left pending (read back from the block's own
+// output slots; they are the pending subtrees of every leaf so far, at most
+// kUpFront), then as many new source items as fit.  A subtree at a round's
+// right edge stays pending exactly as at the block's, and meets its sibling
+// in a later round.  Typical sources leave a few dozen items each, so a
+// level usually takes one round.
 struct UpItemLds {
   float box[2][6][kUpTile];   // [side][min.xyz max.xyz][gap]
   int32_t far_item[2][kUpTile];  // [side][gap]: far item index (as far)
   int32_t far_leaf[2][kUpTile];  // [side][gap]: far leaf
   uint32_t cnt[kUpTile];      // arrivals per gap
   uint32_t mark[kUpTile];     // item ordering
-  uint32_t base[kUpFanIn + 1];
+  uint32_t base[kUpFanIn + 1];  // first item of each source block
 };
 
 __global__ __launch_bounds__(kUpTile) void k_up_items(
     const int4 *__restrict__ items_in, const uint32_t *__restrict__ cnt_in,
     const uint32_t *__restrict__ done_in, uint32_t tiles_in,
-    int4 *__restrict__ items_out, uint32_t *__restrict__ cnt_out,
-    uint32_t *__restrict__ done_out, const int32_t *__restrict__ par,
-    uint32_t n, int4 *nodes, Ctl *ctl, int last) {
+    int4 *items_out,  // written, then read back by the next round
+    uint32_t *__restrict__ cnt_out, uint32_t *__restrict__ done_out,
+    const int32_t *__restrict__ par, uint32_t n, int4 *nodes, Ctl *ctl,
+    int last) {
   __shared__ UpItemLds S;
   __shared__ uint32_t s_wsum[16];
   __shared__ uint32_t s_done;
@@ -637,101 +654,120 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
   const uint32_t tile = blockIdx.x;
   const uint32_t c0 = tile * kUpFanIn;
   const uint32_t nc = min((uint32_t)kUpFanIn, tiles_in - c0);
-  if (tid == 0) {
-    uint32_t run = 0;
-    for (uint32_t j = 0; j < nc; ++j) {
-      S.base[j] = run;
-      const uint32_t c = cnt_in[c0 + j];
-      run += c < (uint32_t)kUpFront ? c : (uint32_t)kUpFront;
-    }
-    S.base[nc] = run;
+  {
+    const uint32_t c =
+        (uint32_t)tid < nc ? min(cnt_in[c0 + tid], (uint32_t)kUpFront) : 0u;
+    const uint32_t ex = block_exclusive_scan(c, s_wsum);
+    if (tid <= kUpFanIn) S.base[tid] = ex;  // the total from nc on
   }
-  S.cnt[tid] = 0u;
+  // nodes completed by the source blocks, then by each round here: the last
+  // level holds the count of the whole tree
+  uint32_t completed = (uint32_t)tid < nc ? done_in[c0 + tid] : 0u;
+  int4 *own = items_out + 3 * (size_t)tile * kUpFront;
   __syncthreads();
-  const int m = (int)S.base[nc];
-  int mode = 0;  // 0: done, 1: first arriver, 2: pending (an item)
-  int self = 0, side = 0;
-  int32_t p = -1, lo = 0, hi = 0;
-  int i0 = tid, i1 = tid;  // the subtree's items
-  float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
-  if (tid < m) {
-    uint32_t j = 0;
-    while (j + 1 < nc && S.base[j + 1] <= (uint32_t)tid) ++j;
-    const int4 *it =
-        items_in + 3 * ((size_t)(c0 + j) * kUpFront + (tid - S.base[j]));
-    const int4 a = it[0], b = it[1], c = it[2];
-    self = a.x; lo = a.y; hi = a.z; p = a.w;
-    mn[0] = __int_as_float(b.x); mn[1] = __int_as_float(b.y);
-    mn[2] = __int_as_float(b.z); side = b.w;
-    mx[0] = __int_as_float(c.x); mx[1] = __int_as_float(c.y);
-    mx[2] = __int_as_float(c.z);
-    mode = 2;
-    for (int step = 0; step < 128; ++step) {
-      if (p < 0 || p >= (int32_t)n - 1) {
-        mode = 0;
-        break;
-      }
-      const int gap = side == 0 ? i1 : i0 - 1;
-      if (gap < 0 || gap >= m - 1) {  // the sibling lies outside the range
-        mode = 2;
-        break;
-      }
-      int4 *node = nodes + 4 * (size_t)p;
-      node[2 * side] = pack3(mn, self);
-      node[2 * side + 1] = pack3(mx, side ? 0 : 2);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        S.box[side][k][gap] = mn[k];
-        S.box[side][3 + k][gap] = mx[k];
-      }
-      S.far_item[side][gap] = side == 0 ? i0 : i1;
-      S.far_leaf[side][gap] = side == 0 ? lo : hi;
-      const uint32_t old = __hip_atomic_fetch_add(
-          &S.cnt[gap], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) {
-        mode = 1;
-        break;
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
-        mx[k] = fmaxf(mx[k], S.box[1 - side][3 + k][gap]);
-      }
-      if (side == 0) {
-        i1 = S.far_item[1][gap];
-        hi = S.far_leaf[1][gap];
+  const int total = (int)S.base[kUpFanIn];
+  int taken = 0;  // source items used by earlier rounds
+  int carry = 0;  // items left pending by the previous round
+  while (true) {
+    const int fresh = min(total - taken, kUpTile - carry);
+    const bool fin = taken + fresh == total;
+    const int m = carry + fresh;
+    S.cnt[tid] = 0u;
+    __syncthreads();
+    int mode = 0;  // 0: done, 1: first arriver, 2: pending (an item)
+    int self = 0, side = 0;
+    int32_t p = -1, lo = 0, hi = 0;
+    int i0 = tid, i1 = tid;  // the subtree's items
+    float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+    if (tid < m) {
+      const int4 *it;
+      if (tid < carry) {
+        it = own + 3 * (size_t)tid;
       } else {
-        i0 = S.far_item[0][gap];
-        lo = S.far_leaf[0][gap];
+        // source block of item g: the last j < nc with base[j] <= g
+        const uint32_t g = (uint32_t)(taken + tid - carry);
+        uint32_t j = 0;
+#pragma unroll
+        for (uint32_t step = kUpFanIn / 2; step > 0; step >>= 1)
+          if (j + step < nc && S.base[j + step] <= g) j += step;
+        it = items_in + 3 * ((size_t)(c0 + j) * kUpFront + (g - S.base[j]));
       }
-      if (p == 0) {  // root merged
-        mode = 0;
-        break;
+      const int4 a = it[0], b = it[1], c = it[2];
+      self = a.x; lo = a.y; hi = a.z; p = a.w;
+      mn[0] = __int_as_float(b.x); mn[1] = __int_as_float(b.y);
+      mn[2] = __int_as_float(b.z); side = b.w;
+      mx[0] = __int_as_float(c.x); mx[1] = __int_as_float(c.y);
+      mx[2] = __int_as_float(c.z);
+      mode = 2;
+      for (int step = 0; step < 128; ++step) {
+        if (p < 0 || p >= (int32_t)n - 1) {
+          mode = 0;
+          break;
+        }
+        const int gap = side == 0 ? i1 : i0 - 1;
+        if (gap < 0 || gap >= m - 1) {  // the sibling lies outside the round
+          mode = 2;
+          break;
+        }
+        int4 *node = nodes + 4 * (size_t)p;
+        node[2 * side] = pack3(mn, self);
+        node[2 * side + 1] = pack3(mx, side ? 0 : 2);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          S.box[side][k][gap] = mn[k];
+          S.box[side][3 + k][gap] = mx[k];
+        }
+        S.far_item[side][gap] = side == 0 ? i0 : i1;
+        S.far_leaf[side][gap] = side == 0 ? lo : hi;
+        const uint32_t old = __hip_atomic_fetch_add(
+            &S.cnt[gap], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0) {
+          mode = 1;
+          break;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
+          mx[k] = fmaxf(mx[k], S.box[1 - side][3 + k][gap]);
+        }
+        if (side == 0) {
+          i1 = S.far_item[1][gap];
+          hi = S.far_leaf[1][gap];
+        } else {
+          i0 = S.far_item[0][gap];
+          lo = S.far_leaf[0][gap];
+        }
+        if (p == 0) {  // root merged
+          mode = 0;
+          break;
+        }
+        // the parent and the two words of the left-child test (up_is_left)
+        // depend only on num: three loads in flight together
+        const int32_t num = p;
+        const int32_t pl = num + 1 <= (int32_t)n - 1 ? par[n + num + 1] : -2;
+        const int32_t pn = num + 1 <= (int32_t)n - 2 ? par[num + 1] : -2;
+        self = num;
+        p = par[num];
+        side = (pl == p || pn == p) ? 0 : 1;
       }
-      // the parent and the two words of the left-child test (up_is_left)
-      // depend only on num: three loads in flight together
-      const int32_t num = p;
-      const int32_t pl = num + 1 <= (int32_t)n - 1 ? par[n + num + 1] : -2;
-      const int32_t pn = num + 1 <= (int32_t)n - 2 ? par[num + 1] : -2;
-      self = num;
-      p = par[num];
-      side = (pl == p || pn == p) ? 0 : 1;
     }
+    __syncthreads();
+    completed += (tid < m - 1 && S.cnt[tid] == 2u) ? 1u : 0u;
+    // a first arriver whose sibling never came in this round is pending
+    if (mode == 1 && S.cnt[side == 0 ? i1 : i0 - 1] == 1u) mode = 2;
+    const uint32_t count =
+        up_emit(S.mark, s_wsum, mode == 2 ? i0 : -1, own, cnt_out + tile, ctl,
+                last && fin, self, lo, hi, p, side, mn, mx);
+    if (fin) break;
+    taken += fresh;
+    carry = (int)min(count, (uint32_t)kUpFront);
+    __syncthreads();  // the emitted items are visible to the whole block
   }
-  __syncthreads();
-  // nodes completed here (exactly two children met at the gap), plus those
-  // of the source blocks: the last level holds the count of the whole tree
-  uint32_t completed = (tid < m - 1 && S.cnt[tid] == 2u) ? 1u : 0u;
-  if (tid < (int)nc) completed += done_in[c0 + tid];
-  const uint32_t total = up_block_count(completed, &s_done);
+  const uint32_t all = up_block_count(completed, &s_done);
   if (tid == 0) {
-    done_out[tile] = total;
-    if (last && total != n - 1) atomicOr(&ctl->fault, kFaultNotATree);
+    done_out[tile] = all;
+    if (last && all != n - 1) atomicOr(&ctl->fault, kFaultNotATree);
   }
-  if (mode == 1 && S.cnt[side == 0 ? i1 : i0 - 1] == 1u) mode = 2;
-  up_emit(S.mark, s_wsum, mode == 2 ? i0 : -1,
-          items_out + 3 * (size_t)tile * kUpFront, cnt_out + tile, ctl, last,
-          self, lo, hi, p, side, mn, mx);
 }
 
 // rearrange_triangles (rearrange.js:162-169): whole 48-byte records, stored

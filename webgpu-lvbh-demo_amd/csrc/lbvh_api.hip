@@ -526,7 +526,7 @@ void run_sort(lbvh_ctx *ctx, uint32_t *digits, const uint32_t *in,
     StageScope sc(ctx, LBVH_STAGE_SORT0 + p);
     launch_sort_pass(bits, p, P.src[p], P.vsrc[p], P.dst[p], P.vdst[p], n,
                      tiles, cnt, off, p == 0 && counted0, ctx->ctl,
-                     ctx->stream);
+                     ctx->stream, nullptr, 0, key_bits);
   }
 }
 

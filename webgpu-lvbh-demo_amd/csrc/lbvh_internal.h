@@ -125,12 +125,14 @@ struct SortSegs {
 // Morton kernels write pass 0's).  A single mesh takes its digit bases from
 // the column sums of cnt (into ctl->hist); each segment from its own tiles'
 // counts, in one block per segment.  keys_in and keys_out must differ.
+// key_bits: significant bits of the keys (a digit with few of them is
+// counted with matched lanes).
 void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       const int32_t *vals_in, uint32_t *keys_out,
                       int32_t *vals_out, uint32_t n, uint32_t tiles,
                       uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
                       hipStream_t s, const SortSegs *segs = nullptr,
-                      uint32_t num_segs = 0);
+                      uint32_t num_segs = 0, int key_bits = kKeyBits);
 inline uint32_t sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
