@@ -321,6 +321,24 @@ __device__ __forceinline__ void perturb(float mn[3], float mx[3]) {
   }
 }
 
+// Box merges of quiet values.  For fminf / fmaxf the compiler canonicalises
+// every operand it cannot prove quiet (a v_max_f32 x, x, x each: six per
+// merge of two boxes, on every step of a climb).  The boxes a climb merges
+// are always quiet -- results of fminf / fmaxf, or canonicalised where a
+// caller's array is loaded -- and on quiet operands the bare instructions
+// give exactly fminf's and fmaxf's results (the other operand for a quiet
+// NaN; -0 below +0, and perturb keeps exact zeros out of the boxes anyway).
+__device__ __forceinline__ float qmin(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float qmax(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
 __device__ __forceinline__ void tri_box(const int4 a, const int4 b,
                                         const int4 c, float mn[3],
                                         float mx[3]) {
@@ -505,21 +523,35 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   // in exactly one block of one level: n - 1 in all, checked by the last
   // level).  A parent out of range or a path past the step cap just ends a
   // climb; every access stays in bounds.
-  if (g < (int64_t)n) {
-    const uint32_t raw = (uint32_t)idx[g];
+  //
+  // The parents words and the permutation entry go out together, clamped
+  // into the arrays (n >= 1) so that no branch separates them: the block
+  // waits two round trips, those loads and then the gather through the
+  // permutation, instead of three.
+  const bool in = g < (int64_t)n;
+  const int64_t e = ts + kUpTile;  // thread 0 also stages the entry past the tile
+  const bool in_e = tid == 0 && e < (int64_t)n;
+  const int32_t pl = par[in ? (int64_t)n + g : 0], pn = par[in ? g : 0];
+  const int32_t pl_e = par[in_e ? (int64_t)n + e : 0], pn_e = par[in_e ? e : 0];
+  const uint32_t raw = (uint32_t)idx[in ? g : 0];
+  if (in) {
     if (raw >= n) atomicOr(&ctl->fault, kFaultNotATree);
     const uint32_t src = raw < n ? raw : 0u;
     const int4 bl = aabb[2 * (size_t)src], bh = aabb[2 * (size_t)src + 1];
-    mn[0] = __int_as_float(bl.x); mn[1] = __int_as_float(bl.y);
-    mn[2] = __int_as_float(bl.z);
-    mx[0] = __int_as_float(bh.x); mx[1] = __int_as_float(bh.y);
-    mx[2] = __int_as_float(bh.z);
+    // quiet from here on: the climb merges with qmin / qmax
+    mn[0] = __builtin_canonicalizef(__int_as_float(bl.x));
+    mn[1] = __builtin_canonicalizef(__int_as_float(bl.y));
+    mn[2] = __builtin_canonicalizef(__int_as_float(bl.z));
+    mx[0] = __builtin_canonicalizef(__int_as_float(bh.x));
+    mx[1] = __builtin_canonicalizef(__int_as_float(bh.y));
+    mx[2] = __builtin_canonicalizef(__int_as_float(bh.z));
   }
   L.cnt[tid] = 0u;
-  for (int t = tid; t <= kUpTile; t += kUpTile) {
-    const int64_t id = ts + t;
-    L.par_leaf[t] = id < (int64_t)n ? par[n + id] : -1;
-    L.par_node[t] = id < (int64_t)n ? par[id] : -1;
+  L.par_leaf[tid] = in ? pl : -1;
+  L.par_node[tid] = in ? pn : -1;
+  if (tid == 0) {
+    L.par_leaf[kUpTile] = in_e ? pl_e : -1;
+    L.par_node[kUpTile] = in_e ? pn_e : -1;
   }
   __syncthreads();
   // 0: done, 1: first to arrive at a parent in LDS, 2: pending (an item)
@@ -568,8 +600,8 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          mn[k] = fminf(mn[k], L.box[1 - side][k][slot]);
-          mx[k] = fmaxf(mx[k], L.box[1 - side][3 + k][slot]);
+          mn[k] = qmin(mn[k], L.box[1 - side][k][slot]);
+          mx[k] = qmax(mx[k], L.box[1 - side][3 + k][slot]);
         }
         if (side == 0) hi = L.far[1][slot];
         else lo = L.far[0][slot];
@@ -727,8 +759,8 @@ __global__ __launch_bounds__(kUpTile) void k_up_items(
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          mn[k] = fminf(mn[k], S.box[1 - side][k][gap]);
-          mx[k] = fmaxf(mx[k], S.box[1 - side][3 + k][gap]);
+          mn[k] = qmin(mn[k], S.box[1 - side][k][gap]);
+          mx[k] = qmax(mx[k], S.box[1 - side][3 + k][gap]);
         }
         if (side == 0) {
           i1 = S.far_item[1][gap];
@@ -949,8 +981,8 @@ __device__ __forceinline__ void climb_leaves(Climb &c, LeafLds<T> &L, int n,
                          __int_as_float(smx.z)};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      c.mn[k] = is_left ? fminf(c.mn[k], sm[k]) : fminf(sm[k], c.mn[k]);
-      c.mx[k] = is_left ? fmaxf(c.mx[k], sx[k]) : fmaxf(sx[k], c.mx[k]);
+      c.mn[k] = qmin(c.mn[k], sm[k]);
+      c.mx[k] = qmax(c.mx[k], sx[k]);
     }
     if (is_left) c.r = smx.w;
     else c.l = smx.w;
@@ -996,7 +1028,14 @@ __global__ __launch_bounds__(T) void k_leaves(
   L.cnt[tid] = 0u;
   L.gap_of[tid] = -1;
   if (active) {
+    // Every load that does not depend on the permutation goes out with it
+    // (the keys at the leaf's gaps, clamped to the mesh so no branch
+    // separates them): the tile waits two round trips, the permutation and
+    // then the gather it names, instead of three.
     const uint32_t pi = (uint32_t)perm_all[leaf_base + k];
+    const uint32_t k_here = skeys[k];
+    const uint32_t k_next = skeys[(uint32_t)k + 1 < N ? k + 1 : k];
+    const uint32_t k_prev = skeys[k > 0 ? k - 1 : k];  // thread 0's gap
     const size_t src = 3 * (size_t)(leaf_base + (pi < N ? pi : 0u));
     const int4 a = tri_all[src], b = tri_all[src + 1], d = tri_all[src + 2];
     // O_TRI goes out through LDS (the node slots are free until the climb):
@@ -1008,11 +1047,12 @@ __global__ __launch_bounds__(T) void k_leaves(
     stage[3 * tid + 2] = d;
     tri_box(a, b, d, c.mn, c.mx);
     perturb(c.mn, c.mx);
-    L.dgap[tid + 1] = ((uint32_t)k + 1 < N)
-        ? delta_next(skeys[k], skeys[k + 1], (uint32_t)k) : -1;
-    if (tid == 0)
-      L.dgap[0] = (k > 0) ? delta_next(skeys[k - 1], skeys[k], (uint32_t)(k - 1))
-                          : -1;
+    // both deltas computed unconditionally and selected, so the compiler
+    // does not sink their loads into branches behind the gather
+    const int d_next = delta_next(k_here, k_next, (uint32_t)k);
+    const int d_prev = delta_next(k_prev, k_here, (uint32_t)(k - 1));
+    L.dgap[tid + 1] = ((uint32_t)k + 1 < N) ? d_next : -1;
+    if (tid == 0) L.dgap[0] = k > 0 ? d_prev : -1;
   }
   __syncthreads();
   {
@@ -1121,8 +1161,8 @@ __device__ __forceinline__ void item_range_box(const ItemLds &L, int a, int b,
     if (lo & 1) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        mn[c] = fminf(mn[c], L.smn[c][lo]);
-        mx[c] = fmaxf(mx[c], L.smx[c][lo]);
+        mn[c] = qmin(mn[c], L.smn[c][lo]);
+        mx[c] = qmax(mx[c], L.smx[c][lo]);
       }
       ++lo;
     }
@@ -1130,8 +1170,8 @@ __device__ __forceinline__ void item_range_box(const ItemLds &L, int a, int b,
       --hi;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        mn[c] = fminf(mn[c], L.smn[c][hi]);
-        mx[c] = fmaxf(mx[c], L.smx[c][hi]);
+        mn[c] = qmin(mn[c], L.smn[c][hi]);
+        mx[c] = qmax(mx[c], L.smx[c][hi]);
       }
     }
     lo >>= 1;
@@ -1201,8 +1241,8 @@ __device__ __forceinline__ void item_round(ItemLds &L, int m, int carry,
       const int i = (T >> lv) + tid;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        L.smn[c][i] = fminf(L.smn[c][2 * i], L.smn[c][2 * i + 1]);
-        L.smx[c][i] = fmaxf(L.smx[c][2 * i], L.smx[c][2 * i + 1]);
+        L.smn[c][i] = qmin(L.smn[c][2 * i], L.smn[c][2 * i + 1]);
+        L.smx[c][i] = qmax(L.smx[c][2 * i], L.smx[c][2 * i + 1]);
       }
     }
     __syncthreads();
