@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
   __shared__ uint8_t s_d[kRtTile];
   __shared__ uint64_t s_planes[kRtSegs][kRtPlanes];
   __shared__ int s_min[kRtSegs];
+  __shared__ uint64_t s_segplanes[kRtPlanes];  // bit planes of the minima
   __shared__ int s_lpos[64], s_rpos[64];  // spine gap of each delta value
   __shared__ uint32_t s_nrec;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -158,6 +159,19 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     }
   }
   __syncthreads();
+  // The segment minima are 64 values, one per lane of a wave: their bit
+  // planes find the nearest segment holding a smaller delta the way a
+  // segment's planes find the nearest gap, with no walk over the segments.
+  static_assert(kRtSegs == 64, "one lane per segment");
+  if (wave == 0) {
+    const int smn = s_min[lane];
+#pragma unroll
+    for (int b = 0; b < kRtPlanes; ++b) {
+      const uint64_t pb = __ballot((smn >> b) & 1);
+      if (lane == 0) s_segplanes[b] = pb;
+    }
+  }
+  __syncthreads();
   const uint64_t below_me = (1ull << lane) - 1;
   const uint64_t above_me = lane == 63 ? 0ull : ~((2ull << lane) - 1);
 #pragma unroll 1
@@ -170,26 +184,25 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     const uint64_t lt = rt_below(s_planes[sg], v);
     int li = -1, ri = -1;  // tile-local bounds, -1: not in the tile
     uint64_t ml = lt & below_me, mr = lt & above_me;
-    if (ml) {
-      li = sg * 64 + 63 - (int)__clzll((long long)ml);
-    } else {
-      for (int s2 = sg - 1; s2 >= 0; --s2)
-        if (s_min[s2] < v) {
-          const uint64_t x = rt_below(s_planes[s2], v);
-          li = s2 * 64 + 63 - (int)__clzll((long long)x);
-          break;
-        }
+    if (!ml || !mr) {
+      // segments other than this one whose minimum is below v
+      const uint64_t segs = rt_below(s_segplanes, v);
+      const uint64_t sl = ml ? 0ull : segs & ((1ull << sg) - 1);
+      const uint64_t sr = mr ? 0ull
+                             : (sg == 63 ? 0ull : segs & ~((2ull << sg) - 1));
+      if (sl) {
+        const int s2 = 63 - (int)__clzll((long long)sl);
+        ml = rt_below(s_planes[s2], v);
+        li = s2 * 64 + 63 - (int)__clzll((long long)ml);
+      }
+      if (sr) {
+        const int s2 = (int)__ffsll((long long)sr) - 1;
+        mr = rt_below(s_planes[s2], v);
+        ri = s2 * 64 + (int)__ffsll((long long)mr) - 1;
+      }
     }
-    if (mr) {
-      ri = sg * 64 + (int)__ffsll((long long)mr) - 1;
-    } else {
-      for (int s2 = sg + 1; s2 < kRtSegs; ++s2)
-        if (s_min[s2] < v) {
-          const uint64_t x = rt_below(s_planes[s2], v);
-          ri = s2 * 64 + (int)__ffsll((long long)x) - 1;
-          break;
-        }
-    }
+    if (li < 0 && ml) li = sg * 64 + 63 - (int)__clzll((long long)ml);
+    if (ri < 0 && mr) ri = sg * 64 + (int)__ffsll((long long)mr) - 1;
     const int dL = li >= 0 ? (int)s_d[li] : -1;
     const int dR = ri >= 0 ? (int)s_d[ri] : -1;
     if (li >= 0 && ri >= 0) {
