@@ -166,8 +166,8 @@ class sort_bits:
 
 @pytest.mark.parametrize("key_bits", [30, 32])
 @pytest.mark.parametrize("bits", [8, 10])
-@pytest.mark.parametrize("n", [1, 2, 3, 64, 4095, 4096, 4097, 65535, 65536, 65537,
-                               1000003, 4194304])
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 1023, 1024, 1025, 4095, 4096, 4097, 16384,
+                               16385, 65535, 65536, 65537, 1000003, 4194304])
 def test_sort_is_stable(ctx, lb, n, bits, key_bits):
     """Morton-range keys and full u32 keys: the stand-alone sort orders all
     32 bits, as the reference's 16 passes of 2 bits do."""
@@ -267,11 +267,12 @@ def test_stage_pipeline_matches_oracle(ctx, lb, ora):
             x.destroy()
 
 
-@pytest.mark.parametrize("n", [1024, 1025, 8192, 8193, 65536, 65537, 600001])
+@pytest.mark.parametrize("n", [1024, 1025, 32768, 32769, 65537, 600001, 1048577])
 def test_up_pass_at_level_boundaries(ctx, lb, ora, n):
     """The per-stage up pass leaves each 1,024-leaf block's unfinished
-    subtrees as items and merges the items of 8 blocks per launch until one
-    block holds the root: 0, 1, 2, 3 item levels and ragged last groups."""
+    subtrees as items and merges the items of 32 blocks per launch (in
+    rounds of at most 1,024 items) until one block holds the root: 0, 1, 2
+    and 3 item levels and ragged last groups."""
     say(f"up pass levels n={n}")
     tri = lb.wl_clustered(n, n, 0.05) if n % 2 else lb.wl_uniform(n, n, 0.01)
     b = lb.wl_bounds(tri)
@@ -479,14 +480,16 @@ def test_build_wide_digits_match_oracle(ctx, lb, ora, keep_topology):
                 assert np.array_equal(got[3], want[3]), name
 
 
-@pytest.mark.parametrize("n", [32 * 1024, 32 * 1024 + 1, 64 * 1024, 64 * 1024 + 1,
-                               2048 * 1024, 2048 * 1024 + 1])
+@pytest.mark.parametrize("n", [16 * 1024, 16 * 1024 + 1, 32 * 1024, 32 * 1024 + 1,
+                               64 * 1024, 64 * 1024 + 1, 2048 * 1024, 2048 * 1024 + 1])
 def test_build_at_plan_boundaries(ctx, lb, ora, n):
     """Sizes where the plan changes shape (1,024-leaf tiles): 32, 33 and 64
     leaf tiles take one item level (a single block over all of them), 65 take
-    two (3 blocks of 32 sources, then one); 16 sort tiles (the scatter sums
-    the counts itself) and 17 (a column-scan kernel); 2,048 leaf tiles (64,
-    then one) and 2,049 (65, then 3, then one).  Both digit widths."""
+    two (3 blocks of 32 sources, then one); 2,048 leaf tiles (64, then one)
+    and 2,049 (65, then 3, then one).  The sort: up to 2^16 keys in tiles of
+    1,024 keys, 16 of them (2^14 keys; the scatter sums the counts itself)
+    and 17 (a column-scan kernel); 2^16 + 1 keys in 17 tiles of 4,096.
+    Both digit widths."""
     tri = lb.wl_uniform(n % 9973, n, 0.01)
     want = ora.build(tri, None, 0)
     ctx.set_tuning(lb.TUNE_TREE_TILE, 1024)

@@ -15,16 +15,18 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
   return x * q + (x < r ? x : r) + b / kXcds;
 }
 
-// Keys [begin, end) of a sort tile inside its segment [seg_begin, seg_end)
-// (one segment [0, n) when segs.tile_seg is null).
+// Keys [begin, end) of a sort tile of tile_keys keys inside its segment
+// [seg_begin, seg_end) (one segment [0, n) when segs.tile_seg is null;
+// segments use kSortTile).
 struct TileRange {
   uint32_t begin, end, seg_begin, seg_end;
 };
-__device__ __forceinline__ TileRange sort_tile_range(uint32_t tile, uint32_t n,
-                                                     const SortSegs &segs) {
+__device__ __forceinline__ TileRange sort_tile_range(
+    uint32_t tile, uint32_t n, const SortSegs &segs,
+    uint32_t tile_keys = kSortTile) {
   if (!segs.tile_seg) {
-    const uint32_t b = tile * (uint32_t)kSortTile;
-    return {b, min(b + (uint32_t)kSortTile, n), 0u, n};
+    const uint32_t b = tile * tile_keys;
+    return {b, min(b + tile_keys, n), 0u, n};
   }
   const uint32_t sg = segs.tile_seg[tile];
   const uint32_t sb = segs.seg_off[sg], se = segs.seg_off[sg + 1];

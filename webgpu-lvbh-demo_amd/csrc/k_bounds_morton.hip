@@ -145,9 +145,11 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
     uint32_t *__restrict__ keys, Ctl *ctl,
-    uint32_t *__restrict__ cnt) {
+    uint32_t *__restrict__ cnt, uint32_t tile_chunks) {
   constexpr int kR = 1 << kBits;
-  static_assert(kSortTile % kMapThreads == 0, "tiles are whole chunks");
+  static_assert(kSortTile % kMapThreads == 0 &&
+                    kSmallSortTile % kMapThreads == 0,
+                "tiles are whole chunks");
   __shared__ uint32_t s_hist[kR];
   __shared__ float4 s_tri[3 * kMapThreads];
   if (cnt)
@@ -172,11 +174,10 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
   const uint32_t t = threadIdx.x;
   const uint32_t chunks = (n + kMapThreads - 1) / kMapThreads;
   uint32_t c0 = blockIdx.x, c1 = chunks, cstep = gridDim.x, tile = 0;
-  if (cnt) {
-    constexpr uint32_t kPerTile = kSortTile / kMapThreads;
+  if (cnt) {  // one sort tile of tile_chunks chunks per block
     tile = xcd_tile(blockIdx.x, gridDim.x);
-    c0 = tile * kPerTile;
-    c1 = min(c0 + kPerTile, chunks);
+    c0 = tile * tile_chunks;
+    c1 = min(c0 + tile_chunks, chunks);
     cstep = 1;
   }
   TriChunks tc{tri, 3 * (size_t)n, {}};
@@ -212,11 +213,11 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
 
 // compute_aabb_z_idx from the box centres k_bounds wrote (aabb-z-idx.js
 // :192-195 onward; the centre of :191 is already rounded the same way), plus
-// the first sort pass's per-tile digit counts.  One block per sort tile, as
-// k_aabb_morton with cnt.  Bounds from ctl, or (parts != 0) folded by every
+// the first sort pass's per-tile digit counts.  One block per sort tile of
+// kTileKeys keys, as k_aabb_morton with cnt.  Bounds from ctl, or (parts != 0) folded by every
 // block from the bounds partials as k_bounds_final folds them, which saves
 // small builds that launch.
-template <int kBits>
+template <int kBits, int kTileKeys>
 __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
     const float *__restrict__ cen, uint32_t n, int div_mode,
     uint32_t *__restrict__ keys, const Ctl *ctl, uint32_t *__restrict__ cnt,
@@ -245,8 +246,8 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
   }
   const bool ieee = (div_mode == 1);
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t first = tile * (uint32_t)kSortTile;
-  const uint32_t end = min(first + (uint32_t)kSortTile, n);
+  const uint32_t first = tile * (uint32_t)kTileKeys;
+  const uint32_t end = min(first + (uint32_t)kTileKeys, n);
   const size_t cs = cen_stride(n);
   const auto key_of = [&](float x, float y, float z) {
     const float ce[3] = {x, y, z};
@@ -257,9 +258,9 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
   // below HBM bandwidth.  A whole tile goes 16 bytes per lane: thread t takes
   // triangles first + 1024 g + 4 t .. + 3 of each group g (counting is
   // indifferent to which thread holds which key).
-  constexpr int kPer = kSortTile / kMapThreads;
+  constexpr int kPer = kTileKeys / kMapThreads;
   static_assert(kPer % 4 == 0, "whole float4 groups per thread");
-  if (end - first == (uint32_t)kSortTile) {
+  if (end - first == (uint32_t)kTileKeys) {
     float4 c[3][kPer / 4];
 #pragma unroll
     for (int g = 0; g < kPer / 4; ++g) {
@@ -331,14 +332,19 @@ void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
   if (blocks > (uint32_t)kMortonBlocksMax) blocks = kMortonBlocksMax;
   if (cnt) blocks = sort_tiles(n);
   if (blocks == 0) blocks = 1;
+  // chunks of kMapThreads triangles per sort tile
+  const uint32_t tile_chunks =
+      (uint32_t)(kSortThreads * sort_items(n)) / kMapThreads;
   if (cnt && bits == kWideBits)
     hipLaunchKernelGGL(k_aabb_morton<kWideBits>, dim3(blocks),
                        dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt);
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt,
+                       tile_chunks);
   else
     hipLaunchKernelGGL(k_aabb_morton<kRadixBits>, dim3(blocks),
                        dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt);
+                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt,
+                       tile_chunks);
 }
 
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
@@ -346,13 +352,18 @@ void launch_morton_cen(const float *cen, uint32_t n, int div_mode,
                        hipStream_t s, const float4 *partials, uint32_t parts) {
   const uint32_t tiles = sort_tiles(n);
   if (tiles == 0) return;
-  if (bits == kWideBits)
-    hipLaunchKernelGGL(k_morton_cen<kWideBits>, dim3(tiles), dim3(kMapThreads),
-                       0, s, cen, n, div_mode, keys, ctl, cnt, partials, parts);
-  else
-    hipLaunchKernelGGL(k_morton_cen<kRadixBits>, dim3(tiles),
-                       dim3(kMapThreads), 0, s, cen, n, div_mode, keys, ctl,
-                       cnt, partials, parts);
+  const bool small = sort_items(n) == kSmallSortItems;
+#define LBVH_MORTON_CEN(B, T)                                              \
+  hipLaunchKernelGGL((k_morton_cen<B, T>), dim3(tiles), dim3(kMapThreads), \
+                     0, s, cen, n, div_mode, keys, ctl, cnt, partials, parts)
+  if (bits == kWideBits) {
+    if (small) LBVH_MORTON_CEN(kWideBits, kSmallSortTile);
+    else LBVH_MORTON_CEN(kWideBits, kSortTile);
+  } else {
+    if (small) LBVH_MORTON_CEN(kRadixBits, kSmallSortTile);
+    else LBVH_MORTON_CEN(kRadixBits, kSortTile);
+  }
+#undef LBVH_MORTON_CEN
 }
 
 void launch_copy_bounds(const Ctl *ctl, float *out, hipStream_t s) {

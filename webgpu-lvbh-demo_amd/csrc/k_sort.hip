@@ -9,7 +9,8 @@
 // passes and launches, narrower ones less LDS per block and fewer count
 // words per tile).
 //
-// Each pass is reduce-then-scan over tiles of kSortTile keys:
+// Each pass is reduce-then-scan over tiles of kSortTile keys (kSmallSortTile
+// for a single mesh of at most kSmallSortMax keys):
 //   count    per-tile digit counts (pass 0's are written by the Morton
 //            kernel as it produces the keys), and their column sums;
 //   scan     one block per digit scans that digit's column of tile counts
@@ -43,7 +44,6 @@
 namespace lbvh {
 
 constexpr int kWaves = kSortThreads / 64;
-static_assert(kSortItems % 4 == 0, "16-byte tile loads");
 constexpr int kScanPer = 16;  // column entries per thread per scan chunk
 // Single-mesh sorts of at most this many tiles (64K keys) launch no scan:
 // each scatter tile sums the count rows of its digits itself.
@@ -79,7 +79,7 @@ __device__ __forceinline__ void count_digit(uint32_t *s_c, uint32_t d,
 
 // Per-tile digit counts of one pass -> cnt (digit-major, or tile-major for a
 // batched sort).
-template <int kBits, bool kMatch>
+template <int kBits, bool kMatch, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_count(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
     uint32_t *__restrict__ cnt, SortSegs segs) {
@@ -89,19 +89,19 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
   for (int d = threadIdx.x; d < D::kR; d += kSortThreads) s_c[d] = 0u;
   __syncthreads();
-  const TileRange r = sort_tile_range(tile, n, segs);
+  const TileRange r = sort_tile_range(tile, n, segs, kSortThreads * kItems);
   // All of the thread's keys are loaded before any is counted, so a wave
   // has its whole share of the tile in flight; a whole 16-byte-aligned tile
   // with 16-byte loads (counting does not depend on which thread holds
   // which key).
-  uint32_t k[kSortItems];
+  uint32_t k[kItems];
   const bool vec =
-      r.end - r.begin == (uint32_t)kSortTile &&
+      r.end - r.begin == (uint32_t)(kSortThreads * kItems) &&
       (reinterpret_cast<uintptr_t>(keys + r.begin) & 15u) == 0;
   if (vec) {
     const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + r.begin);
 #pragma unroll
-    for (int q = 0; q < kSortItems / 4; ++q) {
+    for (int q = 0; q < kItems / 4; ++q) {
       const uint4 v = k4[threadIdx.x + q * kSortThreads];
       k[4 * q] = v.x;
       k[4 * q + 1] = v.y;
@@ -109,16 +109,16 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
       k[4 * q + 3] = v.w;
     }
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j)
+    for (int j = 0; j < kItems; ++j)
       count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1), true);
   } else {
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
+    for (int j = 0; j < kItems; ++j) {
       const uint32_t i = r.begin + threadIdx.x + j * kSortThreads;
       k[j] = i < r.end ? keys[i] : 0u;
     }
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j)
+    for (int j = 0; j < kItems; ++j)
       count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1),
                                  r.begin + threadIdx.x + j * kSortThreads < r.end);
   }
@@ -299,22 +299,24 @@ __device__ __forceinline__ void digit_prefix2(
 // kSmallScanTiles tiles): off holds the per-tile counts themselves, and each
 // tile sums its digits' rows (from L2) into its column prefix and the digit
 // totals, so a small sort launches no scan kernel.
-template <int kBits, bool kFirst>
+template <int kBits, bool kFirst, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
     uint32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out,
     uint32_t n, uint32_t shift, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ hist, int small, SortSegs segs) {
   using D = Digits<kBits>;
+  constexpr int kTile = kSortThreads * kItems;
+  static_assert(kItems % 4 == 0, "16-byte tile loads");
   // Wide digits keep two 16-bit per-wave counters per word (a wave counts
-  // at most 64 * kSortItems = 1024 keys of a digit, so no carry crosses into
+  // at most 64 * kItems = 1024 keys of a digit, so no carry crosses into
   // the other half): 8 KB less LDS for 10-bit digits, a third block per CU.
   constexpr bool kPack = kBits > kRadixBits;
   constexpr uint32_t kHw = kPack ? D::kR / 2 : D::kR;  // counter words per wave
-  static_assert(!kPack || (64 * kSortItems < 65536 && D::kDpt % 2 == 0),
+  static_assert(!kPack || (64 * kItems < 65536 && D::kDpt % 2 == 0),
                 "16-bit counters, a thread's digits in whole words");
-  __shared__ uint32_t s_keys[kSortTile];
-  __shared__ int32_t s_vals[kSortTile];
+  __shared__ uint32_t s_keys[kTile];
+  __shared__ int32_t s_vals[kTile];
   __shared__ uint32_t s_whist[kWaves][kHw];
   __shared__ uint32_t s_spare[kWaves][64];  // targets of the no-op adds
   __shared__ uint32_t s_start[D::kR];  // tile-local start of each digit run
@@ -365,54 +367,54 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     for (int i = 0; i < D::kDpt; ++i)
       goff[i] = off[(size_t)(tid * D::kDpt + i) * tiles + tile];
   }
-  const TileRange r = sort_tile_range(tile, n, segs);
+  const TileRange r = sort_tile_range(tile, n, segs, kTile);
   // No barrier here: one waits for every outstanding global load, and the
   // tile's own loads below should be in flight with the ones above.  The
   // zeroed counters and s_gbase are first used after the next barrier.
 
-  // Warp-striped keys: wave w owns keys [wbase, wbase + 64*kSortItems), item
+  // Warp-striped keys: wave w owns keys [wbase, wbase + 64*kItems), item
   // j of lane l sits at wbase + 64*j + l, so (j, l) order is input order.
   // A whole 16-byte-aligned tile is loaded with 16-byte loads into the LDS
   // reorder buffer (free until the reorder) and its keys read back striped:
   // 4 loads per thread instead of 16 (keys) + 16 (payloads).  Payloads stay
   // in s_vals (at the striped position) until the ranking is done, so they
   // hold no registers while it runs.
-  const uint32_t wbase = r.begin + wave * 64 * kSortItems;
-  uint32_t key[kSortItems];
+  const uint32_t wbase = r.begin + wave * 64 * kItems;
+  uint32_t key[kItems];
   const bool vec =
-      r.end - r.begin == (uint32_t)kSortTile &&
+      r.end - r.begin == (uint32_t)kTile &&
       ((reinterpret_cast<uintptr_t>(keys_in + r.begin) |
         (kFirst ? 0u : reinterpret_cast<uintptr_t>(vals_in + r.begin))) &
        15u) == 0;
   if (vec) {
     const uint4 *k4 = reinterpret_cast<const uint4 *>(keys_in + r.begin);
     const int4 *v4 = reinterpret_cast<const int4 *>(vals_in + r.begin);
-    uint4 kq[kSortItems / 4];
-    int4 vq[kSortItems / 4];
+    uint4 kq[kItems / 4];
+    int4 vq[kItems / 4];
 #pragma unroll
-    for (int q = 0; q < kSortItems / 4; ++q) {
+    for (int q = 0; q < kItems / 4; ++q) {
       kq[q] = k4[tid + q * kSortThreads];
       if (!kFirst) vq[q] = v4[tid + q * kSortThreads];
     }
 #pragma unroll
-    for (int q = 0; q < kSortItems / 4; ++q) {
+    for (int q = 0; q < kItems / 4; ++q) {
       reinterpret_cast<uint4 *>(s_keys)[tid + q * kSortThreads] = kq[q];
       if (!kFirst)
         reinterpret_cast<int4 *>(s_vals)[tid + q * kSortThreads] = vq[q];
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j)
-      key[j] = s_keys[wave * 64 * kSortItems + 64 * j + lane];
+    for (int j = 0; j < kItems; ++j)
+      key[j] = s_keys[wave * 64 * kItems + 64 * j + lane];
   } else {
 #pragma unroll
-    for (int j = 0; j < kSortItems; ++j) {
+    for (int j = 0; j < kItems; ++j) {
       const uint32_t pos = wbase + 64 * j + lane;
       key[j] = 0u;
       if (pos < r.end) {
         key[j] = keys_in[pos];
         if (!kFirst)
-          s_vals[wave * 64 * kSortItems + 64 * j + lane] = vals_in[pos];
+          s_vals[wave * 64 * kItems + 64 * j + lane] = vals_in[pos];
       }
     }
     __syncthreads();  // the zeroed counters
@@ -426,9 +428,9 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   // all the items' atomics are issued first and their results read
   // afterwards (rank[j] holds the position in bits 0-7 and the group's first
   // lane in bits 8-13 meanwhile).
-  uint32_t rank[kSortItems], prev[kSortItems];
+  uint32_t rank[kItems], prev[kItems];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < kItems; ++j) {
     const bool valid = (wbase + 64 * j + lane) < r.end;
     const uint32_t d = (key[j] >> shift) & (D::kR - 1);
     const uint64_t peers = match_digit<kBits>(d, valid);
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     rank[j] = below | (leader << 8);
   }
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < kItems; ++j) {
     uint32_t p = __shfl(prev[j], (int)(rank[j] >> 8), 64);
     if (kPack) p = (p >> ((((key[j] >> shift) & 1u)) << 4)) & 0xffffu;
     rank[j] = (wbase + 64 * j + lane) < r.end ? p + (rank[j] & 0xffu) : 0u;
@@ -507,16 +509,16 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
     s_gbase[tid * D::kDpt + i] = goff[i] - start[i];
   }
   // payloads out of their striped slots before the reorder overwrites them
-  int32_t val[kSortItems];
+  int32_t val[kItems];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j)
+  for (int j = 0; j < kItems; ++j)
     val[j] = kFirst ? (int32_t)(wbase + 64 * j + lane - r.seg_begin)
-                    : s_vals[wave * 64 * kSortItems + 64 * j + lane];
+                    : s_vals[wave * 64 * kItems + 64 * j + lane];
   __syncthreads();
 
   // Reorder the tile in LDS by (digit, input order).
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < kItems; ++j) {
     if ((wbase + 64 * j + lane) < r.end) {
       const uint32_t dj = (key[j] >> shift) & (D::kR - 1);
       const uint32_t wo =
@@ -544,7 +546,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   }
 }
 
-template <int kBits>
+template <int kBits, int kItems>
 void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                uint32_t *keys_out, int32_t *vals_out, uint32_t n,
                uint32_t tiles, uint32_t *cnt, uint32_t *off, bool counted,
@@ -556,10 +558,10 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   // a digit of at most 6 significant bits: few values, shared counters
   const bool few = key_bits - (int)shift <= 6;
   if (!counted && few)
-    hipLaunchKernelGGL((k_count<kBits, true>), dim3(tiles), dim3(kSortThreads),
-                       0, s, keys_in, n, shift, cnt, sg);
+    hipLaunchKernelGGL((k_count<kBits, true, kItems>), dim3(tiles),
+                       dim3(kSortThreads), 0, s, keys_in, n, shift, cnt, sg);
   else if (!counted)
-    hipLaunchKernelGGL((k_count<kBits, false>), dim3(tiles),
+    hipLaunchKernelGGL((k_count<kBits, false, kItems>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in, n, shift, cnt, sg);
   // the digit totals the scatter scans into bases (null: off is complete)
   const uint32_t *bases_from = nullptr;
@@ -575,12 +577,12 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
                        s, cnt, off, tiles, hist);
   }
   if (pass == 0)
-    hipLaunchKernelGGL((k_scatter<kBits, true>), dim3(tiles),
+    hipLaunchKernelGGL((k_scatter<kBits, true, kItems>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in,
                        (const int32_t *)nullptr, keys_out, vals_out, n, shift,
                        offs, bases_from, small, sg);
   else
-    hipLaunchKernelGGL((k_scatter<kBits, false>), dim3(tiles),
+    hipLaunchKernelGGL((k_scatter<kBits, false, kItems>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in, vals_in, keys_out,
                        vals_out, n, shift, offs, bases_from, small, sg);
 }
@@ -605,14 +607,20 @@ void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
   if (tiles == 0) return;
   const SortSegs one{};
   const SortSegs &sg = segs ? *segs : one;
-  if (bits == kWideBits)
-    sort_pass<kWideBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
-                         cnt, off, counted, ctl, s, sg, segs != nullptr,
-                         num_segs, key_bits);
-  else
-    sort_pass<kRadixBits>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles,
-                          cnt, off, counted, ctl, s, sg, segs != nullptr,
-                          num_segs, key_bits);
+  // a batched sort keeps kSortTile tiles; a single mesh's tile follows n
+  const bool small = !segs && sort_items(n) == kSmallSortItems;
+#define LBVH_SORT_PASS(B, I)                                                 \
+  sort_pass<B, I>(pass, keys_in, vals_in, keys_out, vals_out, n, tiles, cnt, \
+                  off, counted, ctl, s, sg, segs != nullptr, num_segs,       \
+                  key_bits)
+  if (bits == kWideBits) {
+    if (small) LBVH_SORT_PASS(kWideBits, kSmallSortItems);
+    else LBVH_SORT_PASS(kWideBits, kSortItems);
+  } else {
+    if (small) LBVH_SORT_PASS(kRadixBits, kSmallSortItems);
+    else LBVH_SORT_PASS(kRadixBits, kSortItems);
+  }
+#undef LBVH_SORT_PASS
 }
 
 }  // namespace lbvh

@@ -712,7 +712,7 @@ int make_plan(lbvh_ctx *ctx, const uint32_t *offsets, uint32_t S) {
   for (uint32_t k = 0; k < S; ++k) {
     const uint32_t n = offsets[k + 1] - offsets[k];
     seg_tile0[k] = (uint32_t)tile_seg.size();
-    for (uint32_t j = 0; j < sort_tiles(n); ++j) tile_seg.push_back(k);
+    for (uint32_t j = 0; j < batch_sort_tiles(n); ++j) tile_seg.push_back(k);
     if (n) {
       grid_k[k].resize(kMaxTreeLevels);
       levels_k[k] = tree_levels(n, P.tile, grid_k[k].data());

@@ -33,6 +33,12 @@ static_assert(sort_passes(kWideBits, kFullKeyBits) * kWideRadix <= kHistWords,
 constexpr int kSortThreads = 256;        // 4 wave64 per tile
 constexpr int kSortItems = 16;           // keys per thread
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 keys per tile
+// A single-mesh sort of at most kSmallSortMax keys uses tiles of
+// kSmallSortTile keys (4 per thread): a small sort is a chain of one-tile
+// latencies, and a quarter of the serial work per thread shortens each.
+constexpr int kSmallSortItems = 4;
+constexpr int kSmallSortTile = kSortThreads * kSmallSortItems;  // 1024 keys
+constexpr uint32_t kSmallSortMax = 1u << 16;
 
 constexpr int kTreeTileDefault = 1024;  // leaves per tile of the fused up pass
 constexpr int kMaxTreeLevels = 16;      // agglomeration levels (6 at 2^24, T=1024)
@@ -133,7 +139,16 @@ void launch_sort_pass(int bits, int pass, const uint32_t *keys_in,
                       uint32_t *cnt, uint32_t *off, bool counted, Ctl *ctl,
                       hipStream_t s, const SortSegs *segs = nullptr,
                       uint32_t num_segs = 0, int key_bits = kKeyBits);
+// Keys per thread and tiles of a single-mesh sort of n keys (batched sorts
+// always use kSortTile: batch_sort_tiles).
+inline int sort_items(uint32_t n) {
+  return n <= kSmallSortMax ? kSmallSortItems : kSortItems;
+}
 inline uint32_t sort_tiles(uint32_t n) {
+  const uint32_t t = (uint32_t)(kSortThreads * sort_items(n));
+  return (n + t - 1) / t;
+}
+inline uint32_t batch_sort_tiles(uint32_t n) {
   return (n + kSortTile - 1) / kSortTile;
 }
 // The sort of a single key: perm[0] = 0 and sorted[0] = keys[0] (sorted may

@@ -120,10 +120,11 @@ double now_s() {
 // Bytes a resident mesh holds: triangles 48, BVH 64, O_TRI 48, sorted keys
 // 4, permutation 4 per triangle.
 constexpr uint64_t kMeshBytesPerTri = 168;
-// Scratch a slot may need to grow to for a build / refit, with the arena's
-// 25% headroom: keys and indices 16, box centres 12, sort tile counts and
-// offsets 1.25, agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf
-// tile, all levels).
+// Scratch a slot may need to grow to for a build / refit, with headroom for
+// the arena's rounding: keys and indices 16, box centres 12, sort tile
+// counts and offsets 2 (8 for sorts of at most 2^16 keys, whose tiles hold
+// 1,024 keys), agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf
+// tile, all levels): at most 63.5.
 constexpr uint64_t kBuildScratchPerTri = 72;
 constexpr uint64_t kRefitScratchPerTri = 36;
 // plus what does not scale with n: the context's control block, the bounds
