@@ -167,7 +167,8 @@ class sort_bits:
 @pytest.mark.parametrize("key_bits", [30, 32])
 @pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("n", [1, 2, 3, 64, 1023, 1024, 1025, 4095, 4096, 4097, 16384,
-                               16385, 65535, 65536, 65537, 1000003, 4194304])
+                               16385, 65535, 65536, 65537, 1000003, 1048576, 1048577,
+                               4194304])
 def test_sort_is_stable(ctx, lb, n, bits, key_bits):
     """Morton-range keys and full u32 keys: the stand-alone sort orders all
     32 bits, as the reference's 16 passes of 2 bits do."""
@@ -481,15 +482,16 @@ def test_build_wide_digits_match_oracle(ctx, lb, ora, keep_topology):
 
 
 @pytest.mark.parametrize("n", [16 * 1024, 16 * 1024 + 1, 32 * 1024, 32 * 1024 + 1,
-                               64 * 1024, 64 * 1024 + 1, 2048 * 1024, 2048 * 1024 + 1])
+                               64 * 1024, 64 * 1024 + 1, 1024 * 1024, 1024 * 1024 + 1,
+                               2048 * 1024, 2048 * 1024 + 1])
 def test_build_at_plan_boundaries(ctx, lb, ora, n):
     """Sizes where the plan changes shape (1,024-leaf tiles): 32, 33 and 64
     leaf tiles take one item level (a single block over all of them), 65 take
     two (3 blocks of 32 sources, then one); 2,048 leaf tiles (64, then one)
-    and 2,049 (65, then 3, then one).  The sort: up to 2^16 keys in tiles of
+    and 2,049 (65, then 3, then one).  The sort: up to 2^20 keys in tiles of
     1,024 keys, 16 of them (2^14 keys; the scatter sums the counts itself)
-    and 17 (a column-scan kernel); 2^16 + 1 keys in 17 tiles of 4,096.
-    Both digit widths."""
+    and 17 (a column-scan kernel); 2^20 + 1 keys in tiles of 4,096.  Both
+    digit widths."""
     tri = lb.wl_uniform(n % 9973, n, 0.01)
     want = ora.build(tri, None, 0)
     ctx.set_tuning(lb.TUNE_TREE_TILE, 1024)

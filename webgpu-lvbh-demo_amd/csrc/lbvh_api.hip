@@ -439,17 +439,21 @@ int tile_for(const lbvh_ctx *ctx, uint32_t n) {
   return tuned_for(ctx->tree_tile, ctx->tree_tile_fixed, ctx->tune_tile, n);
 }
 
-// Untuned, 10-bit digits from 2^16 to 2^21 keys (one pass and four
-// launches fewer, 4-9% faster builds of uniform meshes of 2^17..2^21
-// triangles on MI355X) and 8-bit digits elsewhere: at most 2^16 keys sort
-// without a scan kernel, each scatter tile summing its digits' count rows,
-// four times as many words with 10-bit digits (2^16: 0.098 against
-// 0.105 ms); above 2^21 the 10-bit scatter's LDS (half the 8-bit one's
-// occupancy) made builds 2-4% slower.
-constexpr uint32_t kWideDigitsMinKeys = (1u << 16) + 1;
-constexpr uint32_t kWideDigitsMaxKeys = 1u << 21;
+// Untuned digit widths, from builds of uniform meshes on MI355X (best
+// leaf tile each; sorts of at most 2^20 keys use 1,024-key tiles):
+//   keys        8-bit     10-bit
+//   2^16        0.077     0.076 ms
+//   2^17        0.087     0.087
+//   2^18        0.100     0.103
+//   2^19        0.125     0.131
+//   2^20        0.170     0.188
+//   2^21        0.260     0.256   (4,096-key tiles from here on)
+//   2^22        0.411     0.425
+// 10-bit digits save a pass but carry four times the count words per tile,
+// which small tiles multiply; so 10-bit up to 2^17 keys and from 2^20 + 1 to
+// 2^21, 8-bit elsewhere.
 int bits_for(const lbvh_ctx *ctx, uint32_t n) {
-  const bool wide = n >= kWideDigitsMinKeys && n <= kWideDigitsMaxKeys;
+  const bool wide = n <= (1u << 17) || (n > (1u << 20) && n <= (1u << 21));
   return tuned_for(ctx->sort_bits, ctx->sort_bits_fixed, ctx->tune_bits, n,
                    wide ? kWideBits : kRadixBits);
 }

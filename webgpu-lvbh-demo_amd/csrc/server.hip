@@ -122,7 +122,7 @@ double now_s() {
 constexpr uint64_t kMeshBytesPerTri = 168;
 // Scratch a slot may need to grow to for a build / refit, with headroom for
 // the arena's rounding: keys and indices 16, box centres 12, sort tile
-// counts and offsets 2 (8 for sorts of at most 2^16 keys, whose tiles hold
+// counts and offsets 2 (8 for sorts of at most 2^20 keys, whose tiles hold
 // 1,024 keys), agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf
 // tile, all levels): at most 63.5.
 constexpr uint64_t kBuildScratchPerTri = 72;
