@@ -168,7 +168,7 @@ class sort_bits:
 @pytest.mark.parametrize("bits", [8, 10])
 @pytest.mark.parametrize("n", [1, 2, 3, 64, 1023, 1024, 1025, 4095, 4096, 4097, 16384,
                                16385, 65535, 65536, 65537, 1000003, 1048576, 1048577,
-                               4194304])
+                               4194303, 4194304, 4194305, 4198400, 4202497])
 def test_sort_is_stable(ctx, lb, n, bits, key_bits):
     """Morton-range keys and full u32 keys: the stand-alone sort orders all
     32 bits, as the reference's 16 passes of 2 bits do."""
@@ -218,7 +218,9 @@ def test_sort_heavy_duplicates_and_constant_digits(ctx, lb, bits):
                     (300000, np.full(300000, 0xFFFFFFFF, np.uint32)),
                     (123457, (np.arange(123457, dtype=np.uint32) % 4) << 30),
                     (123457, ((np.arange(123457, dtype=np.uint32) % 7) << 29)
-                     | (np.arange(123457, dtype=np.uint32)[::-1] & 0xFF))]:
+                     | (np.arange(123457, dtype=np.uint32)[::-1] & 0xFF)),
+                    (4194309, np.zeros(4194309, np.uint32)),
+                    (4202497, (np.arange(4202497, dtype=np.uint32) % 255) << 8)]:
         say(f"sort special n={n} bits={bits}")
         dk = ctx.to_device(keys)
         di = ctx.alloc(4 * n)
