@@ -46,7 +46,9 @@ namespace lbvh {
 // Three launches, no waiting between blocks:
 //   k_rt_tiles    a block takes 4,096 gaps: deltas from its keys in LDS, L
 //                 and R inside 64-gap segments from bit-plane ballots, then
-//                 across the tile's segments; it writes the parents slots of
+//                 across the tile's segments from per-segment tables of the
+//                 last and first gap below each delta value (so a search
+//                 costs a few LDS reads); it writes the parents slots of
 //                 every gap whose bounds it found.  Gaps with no smaller
 //                 delta on one side inside the tile (its left and right
 //                 spines, strictly monotone, so at most 64 each) are kept
@@ -100,9 +102,11 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     unsigned long long *__restrict__ first, uint32_t tiles) {
   __shared__ uint32_t s_key[kRtTile + 1];
   __shared__ uint8_t s_d[kRtTile];
-  __shared__ uint64_t s_planes[kRtSegs][kRtPlanes];
+  // [segment][value u]: the segment's last / first gap with a delta below u
+  // (any byte where the segment has none)
+  __shared__ uint8_t s_last[kRtSegs][64], s_first[kRtSegs][64];
   __shared__ int s_min[kRtSegs];
-  __shared__ uint64_t s_segplanes[kRtPlanes];  // bit planes of the minima
+  __shared__ uint64_t s_segbelow[64];  // [u]: segments whose minimum is below u
   __shared__ int s_lpos[64], s_rpos[64];  // spine gap of each delta value
   __shared__ uint32_t s_nrec;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -124,9 +128,13 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
   }
   if (tid == 0) s_nrec = 0;
   __syncthreads();
-  // deltas and the bit planes of each segment; wave w owns segments
-  // 16w .. 16w + 15, lane l gap l of each
+  // Deltas, and per segment (wave w owns segments 16w .. 16w + 15, lane l
+  // gap l of each) its bit planes and its minimum.  With lane u standing for
+  // the value u, the planes give the mask of the segment's gaps below u:
+  // the tables of its last and first such gap come from it, and it stays in
+  // registers for the segment's own gaps (a gap of delta v reads lane v's).
   int dv[kRtPer];
+  uint64_t below_u[kRtPer];
 #pragma unroll
   for (int j = 0; j < kRtPer; ++j) {
     const int sg = wave * kRtPer + j;
@@ -152,24 +160,22 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
       if (zero) cand = zero;
       else mn |= 1 << b;
     }
-    if (lane == 0) {
-#pragma unroll
-      for (int b = 0; b < kRtPlanes; ++b) s_planes[sg][b] = pl[b];
-      s_min[sg] = mn;
-    }
+    const uint64_t tu = rt_below(pl, lane);
+    below_u[j] = tu;
+    s_last[sg][lane] = (uint8_t)(tu ? 63 - (int)__clzll((long long)tu) : 0);
+    s_first[sg][lane] = (uint8_t)(tu ? (int)__ffsll((long long)tu) - 1 : 0);
+    if (lane == 0) s_min[sg] = mn;
   }
   __syncthreads();
   // The segment minima are 64 values, one per lane of a wave: their bit
-  // planes find the nearest segment holding a smaller delta the way a
-  // segment's planes find the nearest gap, with no walk over the segments.
+  // planes give, for each value u, the segments holding a delta below u.
   static_assert(kRtSegs == 64, "one lane per segment");
   if (wave == 0) {
     const int smn = s_min[lane];
+    uint64_t sp[kRtPlanes];
 #pragma unroll
-    for (int b = 0; b < kRtPlanes; ++b) {
-      const uint64_t pb = __ballot((smn >> b) & 1);
-      if (lane == 0) s_segplanes[b] = pb;
-    }
+    for (int b = 0; b < kRtPlanes; ++b) sp[b] = __ballot((smn >> b) & 1);
+    s_segbelow[lane] = rt_below(sp, lane);
   }
   __syncthreads();
   const uint64_t below_me = (1ull << lane) - 1;
@@ -180,29 +186,36 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     const int i = sg * 64 + lane;
     const int64_t g = base + i;
     const int v = dv[j];
+    // every lane takes part in the exchange (a gap past the end asks lane 63)
+    const int src = v < 64 ? v : 63;
+    const uint64_t tu = below_u[j];
+    const uint64_t lt =
+        (uint64_t)(uint32_t)__shfl((int)(uint32_t)tu, src, 64) |
+        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(tu >> 32), src, 64) << 32);
     if (g >= m) continue;
-    const uint64_t lt = rt_below(s_planes[sg], v);
-    int li = -1, ri = -1;  // tile-local bounds, -1: not in the tile
-    uint64_t ml = lt & below_me, mr = lt & above_me;
-    if (!ml || !mr) {
-      // segments other than this one whose minimum is below v
-      const uint64_t segs = rt_below(s_segplanes, v);
-      const uint64_t sl = ml ? 0ull : segs & ((1ull << sg) - 1);
-      const uint64_t sr = mr ? 0ull
-                             : (sg == 63 ? 0ull : segs & ~((2ull << sg) - 1));
-      if (sl) {
-        const int s2 = 63 - (int)__clzll((long long)sl);
-        ml = rt_below(s_planes[s2], v);
-        li = s2 * 64 + 63 - (int)__clzll((long long)ml);
+    const uint64_t ml = lt & below_me, mr = lt & above_me;
+    // tile-local bounds, -1: not in the tile
+    int li = ml ? sg * 64 + 63 - (int)__clzll((long long)ml) : -1;
+    int ri = mr ? sg * 64 + (int)__ffsll((long long)mr) - 1 : -1;
+    if (li < 0 || ri < 0) {
+      // the nearest other segment whose minimum is below v, then its last
+      // (first) gap below v
+      const uint64_t segs = s_segbelow[v];
+      if (li < 0) {
+        const uint64_t sl = segs & ((1ull << sg) - 1);
+        if (sl) {
+          const int s2 = 63 - (int)__clzll((long long)sl);
+          li = s2 * 64 + (int)s_last[s2][v];
+        }
       }
-      if (sr) {
-        const int s2 = (int)__ffsll((long long)sr) - 1;
-        mr = rt_below(s_planes[s2], v);
-        ri = s2 * 64 + (int)__ffsll((long long)mr) - 1;
+      if (ri < 0) {
+        const uint64_t sr = sg == 63 ? 0ull : segs & ~((2ull << sg) - 1);
+        if (sr) {
+          const int s2 = (int)__ffsll((long long)sr) - 1;
+          ri = s2 * 64 + (int)s_first[s2][v];
+        }
       }
     }
-    if (li < 0 && ml) li = sg * 64 + 63 - (int)__clzll((long long)ml);
-    if (ri < 0 && mr) ri = sg * 64 + (int)__ffsll((long long)mr) - 1;
     const int dL = li >= 0 ? (int)s_d[li] : -1;
     const int dR = ri >= 0 ? (int)s_d[ri] : -1;
     if (li >= 0 && ri >= 0) {
