@@ -533,6 +533,44 @@ def test_every_entry_point_with_no_triangles(ctx, lb):
         x.destroy()
 
 
+@pytest.mark.parametrize("n", [1, 2, 1000, 65536, 300001])
+@pytest.mark.parametrize("bounds_from", ["gpu", "caller"])
+def test_repeated_builds_into_the_same_buffers(ctx, lb, ora, n, bounds_from):
+    """The same build (buffers, size, options, bounds) six times in a row,
+    the triangles rewritten in place between builds and another build in
+    between once: every result is the oracle's, whatever the scratch and
+    outputs held from the build before."""
+    meshes = [lb.wl_uniform(100 + k, n, 0.01) for k in range(2)]
+    b = None
+    if bounds_from == "caller":
+        lo = np.minimum(lb.wl_bounds(meshes[0]), lb.wl_bounds(meshes[1]))
+        hi = np.maximum(lb.wl_bounds(meshes[0]), lb.wl_bounds(meshes[1]))
+        b = np.concatenate([lo[:3], hi[3:]]).astype(np.float32)
+    want = [ora.build(m, b, 0) for m in meshes]
+    dtri = ctx.to_device(meshes[0])
+    bvh, otri = ctx.alloc(64 * n), ctx.alloc(48 * n)
+    sk, pm = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    other = lb.wl_uniform(7, 777, 0.01)
+    try:
+        for rep in range(6):
+            k = rep % 2
+            say(f"repeated build n={n} bounds={bounds_from} rep={rep}")
+            ctx.upload(dtri, meshes[k])
+            ctx.build(dtri, n, b, bvh, otri, sk, pm)
+            wb, wo, wsk, wpm = want[k]
+            assert np.array_equal(sk.read(np.uint32, n), wsk), rep
+            assert np.array_equal(pm.read(np.int32, n), wpm), rep
+            assert np.array_equal(bvh.read(np.uint8, 64 * n), wb), rep
+            assert np.array_equal(otri.read(np.uint32, 12 * n),
+                                  wo.view(np.uint32).reshape(-1)), rep
+            if rep == 3:  # a different build between two replays
+                assert np.array_equal(ctx.build_arrays(other, None)[0],
+                                      ora.build(other, None, 0)[0])
+    finally:
+        for x in (dtri, bvh, otri, sk, pm):
+            x.destroy()
+
+
 def test_build_is_deterministic(ctx, lb):
     tri = lb.workload("uniform-64k")
     first = ctx.build_arrays(tri, None)[0]
