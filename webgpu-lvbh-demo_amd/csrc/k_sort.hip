@@ -63,13 +63,16 @@ struct Digits {
 // one LDS atomic per distinct digit.  Per-lane atomics serialise on a
 // counter that many lanes share, which the top digit of Morton keys makes
 // the common case (at most 64 values, a few per tile); with many distinct
-// digits the ballots cost more than the conflicts they save.  Every lane of
-// the wave must call it.
+// digits the ballots cost more than the conflicts they save.  The matched
+// form serves digits of at most kFewBits significant bits (the higher ones
+// are zero), so it takes kFewBits ballots.  Every lane of the wave must call
+// it.
+constexpr int kFewBits = 6;
 template <int kBits, bool kMatch>
 __device__ __forceinline__ void count_digit(uint32_t *s_c, uint32_t d,
                                             bool valid) {
   if (kMatch) {
-    const uint64_t peers = match_digit<kBits>(d, valid);
+    const uint64_t peers = match_digit<kFewBits>(d, valid);
     if (valid && popc_below(peers) == 0)
       atomicAdd(&s_c[d], (uint32_t)__popcll(peers));
   } else if (valid) {
@@ -555,8 +558,9 @@ void sort_pass(int pass, const uint32_t *keys_in, const int32_t *vals_in,
   constexpr int kR = Digits<kBits>::kR;
   const uint32_t shift = (uint32_t)pass * kBits;
   uint32_t *hist = ctl->hist + (size_t)pass * kR;
-  // a digit of at most 6 significant bits: few values, shared counters
-  const bool few = key_bits - (int)shift <= 6;
+  // a digit of at most kFewBits significant bits: few values, shared
+  // counters
+  const bool few = key_bits - (int)shift <= kFewBits;
   if (!counted && few)
     hipLaunchKernelGGL((k_count<kBits, true, kItems>), dim3(tiles),
                        dim3(kSortThreads), 0, s, keys_in, n, shift, cnt, sg);
