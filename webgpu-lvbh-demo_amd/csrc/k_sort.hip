@@ -58,39 +58,44 @@ struct Digits {
   static_assert(kDpt >= 1 && kR % kSortThreads == 0, "whole digits per thread");
 };
 
-// Adds the wave's valid digits to s_c.  kMatch: the lanes holding one
-// digit are matched by ballots and their lowest lane adds the group's size,
-// one LDS atomic per distinct digit.  Per-lane atomics serialise on a
-// counter that many lanes share, which the top digit of Morton keys makes
-// the common case (at most 64 values, a few per tile); with many distinct
-// digits the ballots cost more than the conflicts they save.  The matched
-// form serves digits of at most kFewBits significant bits (the higher ones
-// are zero), so it takes kFewBits ballots.  Every lane of the wave must call
-// it.
+// A digit of at most kFewBits significant bits (the last one of Morton
+// keys) takes at most kFewVals values, and few of them per tile: counters
+// that the lanes of a wave share would serialise its atomics.  Each lane
+// then counts into a row of its own (rows of kFewVals + 1 words, so lanes
+// adding to one digit hit distinct banks), and the block sums the rows.
 constexpr int kFewBits = 6;
-template <int kBits, bool kMatch>
+constexpr int kFewVals = 1 << kFewBits;
+constexpr int kFewRow = kFewVals + 1;
+
+// Adds a valid digit to the block's counters (kFew: to the calling lane's
+// row; the mask only keeps the row in bounds, the digit has no higher bits).
+template <int kBits, bool kFew>
 __device__ __forceinline__ void count_digit(uint32_t *s_c, uint32_t d,
                                             bool valid) {
-  if (kMatch) {
-    const uint64_t peers = match_digit<kFewBits>(d, valid);
-    if (valid && popc_below(peers) == 0)
-      atomicAdd(&s_c[d], (uint32_t)__popcll(peers));
+  if (kFew) {
+    if (valid)
+      atomicAdd(&s_c[(threadIdx.x & 63) * kFewRow + (d & (kFewVals - 1))], 1u);
   } else if (valid) {
     digit_add(s_c, d);
   }
 }
 
 // Per-tile digit counts of one pass -> cnt (digit-major, or tile-major for a
-// batched sort).
-template <int kBits, bool kMatch, int kItems>
+// batched sort).  kFew: a digit of at most kFewBits significant bits.
+template <int kBits, bool kFew, int kItems>
 __global__ __launch_bounds__(kSortThreads) void k_count(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
     uint32_t *__restrict__ cnt, SortSegs segs) {
   using D = Digits<kBits>;
-  __shared__ uint32_t s_c[D::kR];
+  static_assert(!kFew || (kFewVals <= D::kR && kSortThreads % kFewVals == 0),
+                "a few-bit digit's values are digits; whole row groups");
+  constexpr int kWords = kFew ? 64 * kFewRow : D::kR;
+  __shared__ uint32_t s_c[kWords];
+  __shared__ uint32_t s_few[kFewVals];  // the rows' sums (kFew)
   const uint32_t tiles = gridDim.x;
   const uint32_t tile = xcd_tile(blockIdx.x, tiles);
-  for (int d = threadIdx.x; d < D::kR; d += kSortThreads) s_c[d] = 0u;
+  for (int w = threadIdx.x; w < kWords; w += kSortThreads) s_c[w] = 0u;
+  if (kFew && threadIdx.x < kFewVals) s_few[threadIdx.x] = 0u;
   __syncthreads();
   const TileRange r = sort_tile_range(tile, n, segs, kSortThreads * kItems);
   // All of the thread's keys are loaded before any is counted, so a wave
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
-      count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1), true);
+      count_digit<kBits, kFew>(s_c, (k[j] >> shift) & (D::kR - 1), true);
   } else {
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -122,13 +127,27 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
-      count_digit<kBits, kMatch>(s_c, (k[j] >> shift) & (D::kR - 1),
-                                 r.begin + threadIdx.x + j * kSortThreads < r.end);
+      count_digit<kBits, kFew>(s_c, (k[j] >> shift) & (D::kR - 1),
+                               r.begin + threadIdx.x + j * kSortThreads < r.end);
   }
   __syncthreads();
+  if (kFew) {
+    // thread t sums rows [g * kRows, (g + 1) * kRows) of value t % kFewVals,
+    // g = t / kFewVals
+    constexpr int kGroups = kSortThreads / kFewVals;
+    constexpr int kRows = 64 / kGroups;
+    const int v = threadIdx.x % kFewVals, g = threadIdx.x / kFewVals;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int l = 0; l < kRows; ++l) sum += s_c[(g * kRows + l) * kFewRow + v];
+    atomicAdd(&s_few[v], sum);
+    __syncthreads();
+  }
   const bool tile_major = segs.tile_seg != nullptr;  // batched
-  for (int d = threadIdx.x; d < D::kR; d += kSortThreads)
-    cnt[tile_major ? (size_t)tile * D::kR + d : (size_t)d * tiles + tile] = s_c[d];
+  for (int d = threadIdx.x; d < D::kR; d += kSortThreads) {
+    const uint32_t c = kFew ? (d < kFewVals ? s_few[d] : 0u) : s_c[d];
+    cnt[tile_major ? (size_t)tile * D::kR + d : (size_t)d * tiles + tile] = c;
+  }
 }
 
 // Exclusive prefix of h over all kR digits at each of the calling thread's
