@@ -4,7 +4,7 @@
 // lbvh_alloc records each buffer here.  Before an entry point copies into or
 // launches over a pointer, it asks whether [p, p + bytes) stays inside the
 // buffer that contains p.  A pointer inside no recorded buffer (the caller's
-// own hipMalloc) is not judged.  The reference never needs this check:
+// own hipMalloc) is not checked.  The reference never needs this check:
 // WebGPU bounds-checks every buffer access, and mappedAtCreation +
 // Float32Array.set throws RangeError on an oversize copy (build-bvh.js:14-21).
 // Not thread-safe: the context calls it under its lock.

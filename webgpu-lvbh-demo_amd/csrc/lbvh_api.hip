@@ -370,7 +370,7 @@ int check_n(lbvh_ctx *ctx, uint32_t n) {
 
 // A device pointer argument and the bytes an entry point reads or writes
 // from it.  Pointers inside an lbvh_alloc buffer must have room for those
-// bytes; other pointers (the caller's own allocations) are not judged.
+// bytes; other pointers (the caller's own allocations) are not checked.
 struct Span {
   const void *p;
   uint64_t bytes;
