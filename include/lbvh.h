@@ -102,7 +102,8 @@ typedef enum {
 } lbvh_tunable;
 /* set_tuning fixes a value for every size (value 0: no fixed value, builds
  * use the autotuned table, else the size-based default: tree tile 1024, and
- * 10-bit digits up to 2^21 triangles, 8-bit above); get_tuning_for returns
+ * 10-bit digits up to 2^17 triangles and from 2^20 + 1 to 2^21, 8-bit
+ * elsewhere); get_tuning_for returns
  * the value a build of n triangles would use (n = 0: the fixed/default
  * value). */
 int lbvh_set_tuning(lbvh_ctx *ctx, int param, int value);
@@ -111,7 +112,8 @@ int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n);
 /* Autotuner: for every power-of-two size bucket 2^k in [n_min, n_max] it
  * builds a device-synthesised uniform mesh of 2^k triangles with each
  * variant of every tunable, times them with HIP events (median of `reps`
- * after a warm-up) and keeps the fastest per bucket; later builds pick the
+ * after a warm-up) and keeps the fastest per bucket if it beats the
+ * size-based default by more than 2 %, else the default; later builds pick the
  * entry of their size's bucket (or the nearest tuned one).  A fixed value
  * from lbvh_set_tuning is cleared.  ms_out (optional) receives, per bucket
  * and variant in order, the measured milliseconds: buckets x 6 variants,

@@ -452,10 +452,14 @@ int tile_for(const lbvh_ctx *ctx, uint32_t n) {
 // 10-bit digits save a pass but carry four times the count words per tile,
 // which small tiles multiply; so 10-bit up to 2^17 keys and from 2^20 + 1 to
 // 2^21, 8-bit elsewhere.
-int bits_for(const lbvh_ctx *ctx, uint32_t n) {
+int untuned_bits(uint32_t n) {
   const bool wide = n <= (1u << 17) || (n > (1u << 20) && n <= (1u << 21));
+  return wide ? kWideBits : kRadixBits;
+}
+
+int bits_for(const lbvh_ctx *ctx, uint32_t n) {
   return tuned_for(ctx->sort_bits, ctx->sort_bits_fixed, ctx->tune_bits, n,
-                   wide ? kWideBits : kRadixBits);
+                   untuned_bits(n));
 }
 
 // The key buffers of each pass of a sort from `in` to `out`: each pass reads
@@ -950,14 +954,18 @@ int lbvh_get_tuning_for(const lbvh_ctx *ctx, int param, uint32_t n) {
   }
 }
 
+// A tuned variant must beat the untuned one by this fraction.
+constexpr float kTuneMargin = 0.02f;
+
 int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
                   float *ms_out) {
   CTX_GUARD(ctx);
   int st;
   if (n_min == 0 || n_max < n_min || n_max > LBVH_MAX_TRIS || reps < 1)
     return fail(ctx, LBVH_ERR_INVALID_ARGUMENT, "bad autotune range");
-  static const int tiles[3] = {256, 512, 1024};
+  static const int tiles[3] = {256, 512, kTreeTileDefault};
   static const int widths[2] = {kRadixBits, kWideBits};
+  static_assert(kTreeTileDefault == 1024, "the untuned tile is variant 2");
   const int b0 = bucket_of(n_min), b1 = bucket_of(n_max);
   const uint32_t nbig = 1u << b1;
   void *tri = nullptr, *bvh = nullptr, *otri = nullptr;
@@ -985,8 +993,7 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
     const uint32_t n = 1u << k;
     launch_synth_uniform(0x5EED0000ull + k, n, 0.01, static_cast<float *>(tri),
                          ctx->stream);
-    float best = 1e30f;
-    int best_tile = kTreeTileDefault, best_bits = kRadixBits;
+    float med_v[6];
     for (int v = 0; v < 6 && st == LBVH_OK; ++v) {
       ctx->tree_tile = tiles[v / 2];
       ctx->sort_bits = widths[v % 2];
@@ -1006,18 +1013,24 @@ int lbvh_autotune(lbvh_ctx *ctx, uint32_t n_min, uint32_t n_max, int reps,
       if (!ms.empty()) {
         std::sort(ms.begin(), ms.end());
         med = ms[ms.size() / 2];
-        if (med < best) {
-          best = med;
-          best_tile = tiles[v / 2];
-          best_bits = widths[v % 2];
-        }
       }
+      med_v[v] = med;
       if (ms_out) ms_out[out_i] = med;
       ++out_i;
     }
     if (st == LBVH_OK) {
-      ctx->tune_tile[k] = best_tile;
-      ctx->tune_bits[k] = best_bits;
+      // The untuned choice stays unless a variant beats it by more than
+      // kTuneMargin: on these uniform meshes neighbouring variants are often
+      // within run-to-run noise, and a choice made on noise does not carry to
+      // other meshes (leaf tile 1024 built clustered-1m 4 % faster than 512,
+      // which the uniform 2^20 mesh had picked by 0.3 %).
+      const int v0 = 2 * 2 + (untuned_bits(n) == kWideBits ? 1 : 0);
+      int pick = v0;
+      for (int v = 0; v < 6; ++v)
+        if (med_v[v] < med_v[pick]) pick = v;
+      if (!(med_v[pick] < med_v[v0] * (1.0f - kTuneMargin))) pick = v0;
+      ctx->tune_tile[k] = tiles[pick / 2];
+      ctx->tune_bits[k] = widths[pick % 2];
     }
   }
   ctx->tree_tile = saved_tile;
