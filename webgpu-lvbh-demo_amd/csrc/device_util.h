@@ -71,6 +71,24 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Non-temporal 16-byte loads and stores (the `nt` bit): for streams that
+// are read or written once, so that they do not displace lines that are
+// read again.
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef int nt_i4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_nt(const float4 *p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_nt(float4 *p, float4 v) {
+  __builtin_nontemporal_store(nt_f4{v.x, v.y, v.z, v.w},
+                              reinterpret_cast<nt_f4 *>(p));
+}
+__device__ __forceinline__ void store_nt(int4 *p, int4 v) {
+  __builtin_nontemporal_store(nt_i4{v.x, v.y, v.z, v.w},
+                              reinterpret_cast<nt_i4 *>(p));
+}
+
 // Triangles in chunks of kMapThreads: a chunk's 3 * kMapThreads float4 are
 // loaded contiguously (each load instruction covers 4 KB, where a triangle
 // per lane would stride 48 bytes between lanes), staged in LDS, and thread t
@@ -87,7 +105,7 @@ struct TriChunks {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const size_t q = q0 + (size_t)j * kMapThreads;
-      r[j] = q < q_end ? tri[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+      r[j] = q < q_end ? load_nt(tri + q) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
   __device__ void stage(float4 *s_tri) const {

@@ -89,9 +89,9 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds(
       const uint32_t i0 = c * kMapThreads + q;
       float *dst = cen + k * cs + i0;
       if (i0 + 3 < n) {
-        *reinterpret_cast<float4 *>(dst) =
-            make_float4(s_cen[k][q], s_cen[k][q + 1], s_cen[k][q + 2],
-                        s_cen[k][q + 3]);
+        store_nt(reinterpret_cast<float4 *>(dst),
+                 make_float4(s_cen[k][q], s_cen[k][q + 1], s_cen[k][q + 2],
+                             s_cen[k][q + 3]));
       } else {
         for (uint32_t e = 0; e < 4 && i0 + e < n; ++e) dst[e] = s_cen[k][q + e];
       }
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
       const size_t i = first + (size_t)g * 4 * kMapThreads + 4 * threadIdx.x;
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        c[k][g] = *reinterpret_cast<const float4 *>(cen + k * cs + i);
+        c[k][g] = load_nt(reinterpret_cast<const float4 *>(cen + k * cs + i));
     }
 #pragma unroll
     for (int g = 0; g < kPer / 4; ++g) {

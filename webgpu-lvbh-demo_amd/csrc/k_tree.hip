@@ -1088,7 +1088,7 @@ __global__ __launch_bounds__(T) void k_leaves(
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int e = j * T + tid;
-      if (e < nq) o[e] = stage[e];
+      if (e < nq) store_nt(o + e, stage[e]);
     }
   }
   __syncthreads();  // the node slots are the climb's from here
@@ -1120,7 +1120,7 @@ __global__ __launch_bounds__(T) void k_leaves(
         int4 v = L.node[g][q];
         if (q == 1) v.w = 2;
         if (q == 3) v.w = 0;
-        nodes[4 * (size_t)(ts + kk) + q] = v;
+        store_nt(nodes + 4 * (size_t)(ts + kk) + q, v);
       }
     }
   }
