@@ -458,7 +458,13 @@ def test_autotune_table_never_changes_results(lb, ora):
         for bucket, per_variant in times.items():
             assert set(per_variant) == {(t, b) for t in (256, 512, 1024) for b in (8, 10)}
             assert all(t > 0 for t in per_variant.values())
+            # the fastest variant if it beats the untuned one (tile 1024, the
+            # size rule's digit width) by more than 2 %, else the untuned one
+            wide = bucket <= 1 << 17 or (1 << 20) < bucket <= 1 << 21
+            untuned = (1024, 10 if wide else 8)
             best = min(per_variant, key=per_variant.get)
+            if not per_variant[best] < per_variant[untuned] * np.float32(0.98):
+                best = untuned
             assert c.get_tuning_for(lb.TUNE_TREE_TILE, bucket) == best[0]
             assert c.get_tuning_for(lb.TUNE_SORT_BITS, bucket) == best[1]
         for n in (1, 700, 5000, 70000):
