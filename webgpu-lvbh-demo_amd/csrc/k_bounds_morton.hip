@@ -135,12 +135,13 @@ __global__ __launch_bounds__(kMapThreads) void k_bounds_final(
 //   cen  = (max + min) * .5f                          (:191)
 //   rel  = (cen - bmin) / (bmax - bmin)               (:192, per div mode)
 //   key  = morton(u32(rel * 1023.99f))                (:194-208)
-// Optional outputs: the 32-byte AABB (per-stage entry point) and, for the
-// fused build, the first sort pass's per-tile digit counts,
+// Optional outputs: the 32-byte AABB (kBoxes, the per-stage entry point;
+// staged in LDS so that each store instruction writes whole lines) and,
+// for the fused build, the first sort pass's per-tile digit counts,
 // cnt[d * tiles + tile] (saving the sort a read of the keys).  With cnt a
 // block covers one sort tile (XCD-aware ids, as the sort's), without it a
 // grid-stride share; either way in TriChunks chunks.
-template <int kBits>
+template <int kBits, bool kBoxes>
 __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
     const float4 *__restrict__ tri, uint32_t n, MortonParams p,
     int bounds_from_ctl, float4 *__restrict__ aabb,
@@ -152,6 +153,7 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
                 "tiles are whole chunks");
   __shared__ uint32_t s_hist[kR];
   __shared__ float4 s_tri[3 * kMapThreads];
+  __shared__ float4 s_box[kBoxes ? 2 * kMapThreads : 1];
   if (cnt)
     for (int d = threadIdx.x; d < kR; d += kMapThreads) s_hist[d] = 0u;
   float bmin[3], ext[3], rcp[3];
@@ -195,14 +197,26 @@ __global__ __launch_bounds__(kMapThreads) void k_aabb_morton(
       for (int k = 0; k < 3; ++k)
         ce[k] = __fmul_rn(__fadd_rn(mx[k], mn[k]), 0.5f);
       const uint32_t key = morton_key(ce, bmin, ext, rcp, ieee);
-      keys[i] = key;
-      if (aabb) {
-        aabb[2 * (size_t)i] = make_float4(mn[0], mn[1], mn[2], 0.0f);
-        aabb[2 * (size_t)i + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
+      __builtin_nontemporal_store(key, keys + i);
+      if (kBoxes) {
+        s_box[2 * t] = make_float4(mn[0], mn[1], mn[2], 0.0f);
+        s_box[2 * t + 1] = make_float4(mx[0], mx[1], mx[2], 0.0f);
       }
       if (cnt) digit_add(s_hist, key & (kR - 1));
     }
-    __syncthreads();  // s_tri is refilled for the next chunk
+    __syncthreads();  // s_tri is refilled for the next chunk; s_box is full
+    if (kBoxes) {
+      // the chunk's boxes, 16 contiguous bytes per lane (read back before
+      // the next chunk's boxes are staged, past the next barrier)
+      const uint32_t nb = 2 * (min(c * kMapThreads + kMapThreads, n) -
+                               c * kMapThreads);
+      float4 *o = aabb + 2 * (size_t)c * kMapThreads;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t e = j * kMapThreads + t;
+        if (e < nb) store_nt(o + e, s_box[e]);
+      }
+    }
   }
   if (cnt) {
     __syncthreads();  // also when the block had no chunk
@@ -296,7 +310,7 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
       const uint32_t i = first + threadIdx.x + j * kMapThreads;
       if (i >= end) continue;
       const uint32_t key = key_of(c[0][j], c[1][j], c[2][j]);
-      keys[i] = key;
+      __builtin_nontemporal_store(key, keys + i);
       digit_add(s_hist, key & (kR - 1));
     }
   }
@@ -335,16 +349,19 @@ void launch_aabb_morton(const float4 *tri, uint32_t n, const MortonParams &p,
   // chunks of kMapThreads triangles per sort tile
   const uint32_t tile_chunks =
       (uint32_t)(kSortThreads * sort_items(n)) / kMapThreads;
-  if (cnt && bits == kWideBits)
-    hipLaunchKernelGGL(k_aabb_morton<kWideBits>, dim3(blocks),
-                       dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt,
-                       tile_chunks);
-  else
-    hipLaunchKernelGGL(k_aabb_morton<kRadixBits>, dim3(blocks),
-                       dim3(kMapThreads), 0, s, tri, n, p,
-                       bounds_from_ctl ? 1 : 0, aabb, keys, ctl, cnt,
-                       tile_chunks);
+  const bool wide = cnt && bits == kWideBits;  // digits of the counts
+#define LBVH_AABB_MORTON(B, X)                                              \
+  hipLaunchKernelGGL((k_aabb_morton<B, X>), dim3(blocks), dim3(kMapThreads), \
+                     0, s, tri, n, p, bounds_from_ctl ? 1 : 0, aabb, keys,  \
+                     ctl, cnt, tile_chunks)
+  if (aabb) {
+    if (wide) LBVH_AABB_MORTON(kWideBits, true);
+    else LBVH_AABB_MORTON(kRadixBits, true);
+  } else {
+    if (wide) LBVH_AABB_MORTON(kWideBits, false);
+    else LBVH_AABB_MORTON(kRadixBits, false);
+  }
+#undef LBVH_AABB_MORTON
 }
 
 void launch_morton_cen(const float *cen, uint32_t n, int div_mode,

@@ -557,9 +557,10 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
   const bool in = g < (int64_t)n;
   const int64_t e = ts + kUpTile;  // thread 0 also stages the entry past the tile
   const bool in_e = tid == 0 && e < (int64_t)n;
-  const int32_t pl = par[in ? (int64_t)n + g : 0], pn = par[in ? g : 0];
+  const int32_t pl = __builtin_nontemporal_load(par + (in ? (int64_t)n + g : 0));
+  const int32_t pn = __builtin_nontemporal_load(par + (in ? g : 0));
   const int32_t pl_e = par[in_e ? (int64_t)n + e : 0], pn_e = par[in_e ? e : 0];
-  const uint32_t raw = (uint32_t)idx[in ? g : 0];
+  const uint32_t raw = (uint32_t)__builtin_nontemporal_load(idx + (in ? g : 0));
   if (in) {
     if (raw >= n) atomicOr(&ctl->fault, kFaultNotATree);
     const uint32_t src = raw < n ? raw : 0u;
@@ -655,10 +656,10 @@ __global__ __launch_bounds__(kUpTile) void k_up_pass(
       completed += q == 0 ? 1u : 0u;
       const int c = (q & 1) * 3;  // min or max half of the side's box
       const int w = (q & 1) ? (sd ? 0 : 2) : L.cid[sd][kk];
-      nodes[4 * (size_t)(ts + kk) + q] =
-          make_int4(__float_as_int(L.box[sd][c][kk]),
-                    __float_as_int(L.box[sd][c + 1][kk]),
-                    __float_as_int(L.box[sd][c + 2][kk]), w);
+      store_nt(nodes + 4 * (size_t)(ts + kk) + q,
+               make_int4(__float_as_int(L.box[sd][c][kk]),
+                         __float_as_int(L.box[sd][c + 1][kk]),
+                         __float_as_int(L.box[sd][c + 2][kk]), w));
     }
   }
   const uint32_t total = up_block_count(completed, &s_done);
@@ -838,7 +839,7 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
   const uint32_t i0 = blockIdx.x * kMapThreads;
   const uint32_t i = i0 + threadIdx.x;
   if (i < n) {
-    const uint32_t j = (uint32_t)idx[i];
+    const uint32_t j = (uint32_t)__builtin_nontemporal_load(idx + i);
     const size_t s = 3 * (size_t)(j < n ? j : 0u);
     s_rec[3 * threadIdx.x] = tri[s];
     s_rec[3 * threadIdx.x + 1] = tri[s + 1];
@@ -850,7 +851,7 @@ __global__ __launch_bounds__(kMapThreads) void k_rearrange(
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const uint32_t e = k * kMapThreads + threadIdx.x;
-    if (e < nq) o[e] = s_rec[e];
+    if (e < nq) store_nt(o + e, s_rec[e]);
   }
 }
 
