@@ -57,8 +57,8 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_bounds(
       for (int c = 0; c < 3; ++c) {
         v[c] = fminf(v[c], mn[c]);
         v[3 + c] = fmaxf(v[3 + c], mx[c]);
-        cen[(size_t)c * cen_stride(total) + i] =
-            __fmul_rn(__fadd_rn(mx[c], mn[c]), 0.5f);
+        __builtin_nontemporal_store(__fmul_rn(__fadd_rn(mx[c], mn[c]), 0.5f),
+                                    cen + (size_t)c * cen_stride(total) + i);
       }
     }
     __syncthreads();  // s_tri is refilled for the next chunk
@@ -131,7 +131,8 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     if (cen) {
       if (i < end) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ce[c] = cen[(size_t)c * cen_stride(total) + i];
+        for (int c = 0; c < 3; ++c)
+          ce[c] = __builtin_nontemporal_load(cen + (size_t)c * cen_stride(total) + i);
       }
     } else {
       tc.stage(s_tri);
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kSortThreads) void k_batch_morton(
     }
     if (i >= end) continue;
     const uint32_t key = morton_key(ce, bmin, ext, rcp, ieee);
-    keys[i] = key;
+    __builtin_nontemporal_store(key, keys + i);
     digit_add(s_hist, key & (kR - 1));
   }
   __syncthreads();
