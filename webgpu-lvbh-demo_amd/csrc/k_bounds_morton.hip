@@ -291,7 +291,8 @@ __global__ __launch_bounds__(kMapThreads) void k_morton_cen(
       kq.y = key_of(c[0][g].y, c[1][g].y, c[2][g].y);
       kq.z = key_of(c[0][g].z, c[1][g].z, c[2][g].z);
       kq.w = key_of(c[0][g].w, c[1][g].w, c[2][g].w);
-      store_nt(reinterpret_cast<uint4 *>(keys + i), kq);
+      // plain stores: the sort's first scatter reads the keys next
+      *reinterpret_cast<uint4 *>(keys + i) = kq;
       digit_add(s_hist, kq.x & (kR - 1));
       digit_add(s_hist, kq.y & (kR - 1));
       digit_add(s_hist, kq.z & (kR - 1));
