@@ -210,10 +210,15 @@ struct Slot {
   // dispatcher reads this instead of the context, whose lock the worker
   // holds while it enqueues
   std::atomic<uint64_t> scratch_bytes{0};
-  // dispatcher only: what the slot's admitted requests may grow its scratch
-  // to (their estimates), counted until the slot is idle again, so that
-  // growth admitted but not yet allocated is not handed out twice
-  uint64_t reserve = 0;
+  // dispatcher only: a bound on the scratch the slot's context may hold,
+  // from the largest build and the largest refit admitted to it since its
+  // scratch was last released.  The context keeps every buffer at the
+  // largest size any request asked of it, and a build and a refit use
+  // different buffers, so the bound adds the two estimates; it also covers
+  // growth admitted but not yet allocated, which must not be handed out
+  // twice.
+  uint64_t max_build_n = 0, max_refit_n = 0;
+  uint64_t bound = 0;
   hipEvent_t events[kSlotDepth] = {};
   uint32_t next_event = 0;  // dispatcher only
   // per event: what the request's epilogue left (pinned host memory, and
@@ -270,7 +275,7 @@ namespace {
 
 uint64_t slots_bytes(lbvh_server *s) {
   uint64_t b = 0;
-  for (auto &sl : s->slots) b += std::max<uint64_t>(sl.scratch_bytes, sl.reserve);
+  for (auto &sl : s->slots) b += std::max<uint64_t>(sl.scratch_bytes, sl.bound);
   return b;
 }
 
@@ -303,7 +308,8 @@ void free_mesh(lbvh_server *s, Mesh &m) {
 // one of its size) and slot scratch beyond what the slot holds.
 struct Need {
   uint64_t mesh = 0, scratch = 0;
-  uint64_t slot_total = 0;  // the slot's scratch estimate with this request
+  uint64_t slot_total = 0;  // the slot's scratch bound with this request
+  uint64_t max_build_n = 0, max_refit_n = 0;  // the slot's, with it
   bool any() const { return mesh || scratch; }
 };
 Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
@@ -313,12 +319,18 @@ Need request_need(lbvh_server *s, const Request &r, const Slot &sl) {
     if (it == s->meshes.end() || !it->second.buf || it->second.n != r.n)
       need.mesh = MeshArena::round(kMeshBytesPerTri * r.n);
   }
-  const uint64_t scratch =
-      (r.kind == LBVH_REQ_BUILD ? kBuildScratchPerTri : kRefitScratchPerTri) *
-      r.n + kSlotScratchFixed;
-  const uint64_t have = std::max<uint64_t>(sl.scratch_bytes, sl.reserve);
-  if (scratch > have) need.scratch = scratch - have;
-  need.slot_total = std::max(scratch, have);
+  need.max_build_n = sl.max_build_n;
+  need.max_refit_n = sl.max_refit_n;
+  if (r.kind == LBVH_REQ_BUILD)
+    need.max_build_n = std::max<uint64_t>(need.max_build_n, r.n);
+  else
+    need.max_refit_n = std::max<uint64_t>(need.max_refit_n, r.n);
+  const uint64_t bound = kBuildScratchPerTri * need.max_build_n +
+                         kRefitScratchPerTri * need.max_refit_n +
+                         kSlotScratchFixed;
+  const uint64_t have = std::max<uint64_t>(sl.scratch_bytes, sl.bound);
+  if (bound > have) need.scratch = bound - have;
+  need.slot_total = std::max(bound, have);
   return need;
 }
 
@@ -359,12 +371,13 @@ bool make_room(lbvh_server *s, const Need &need, uint64_t keep) {
     if (s->arena.release_empty()) continue;
     bool trimmed = false;
     for (auto &sl : s->slots)
-      if (sl.idle() && sl.scratch_bytes > sizeof(lbvh::Ctl)) {
+      if (sl.idle() && (sl.scratch_bytes > sizeof(lbvh::Ctl) || sl.bound)) {
         // an idle slot's worker waits for work that only this thread
         // hands out, so it does not touch the context meanwhile
         lbvh_trim(sl.ctx);
         sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
-        sl.reserve = 0;
+        sl.bound = 0;
+        sl.max_build_n = sl.max_refit_n = 0;
         trimmed = true;
       }
     if (!trimmed) return false;
@@ -578,8 +591,6 @@ void dispatcher(lbvh_server *s) {
         sl.inflight.pop_front();
         progress = true;
       }
-      // an idle slot's scratch_bytes holds all it has grown to
-      if (sl.mailbox.empty() && sl.inflight.empty()) sl.reserve = 0;
     }
     // Queueing deadlines, at the head of the queue whether or not a slot is
     // free (with one maximum wait they pass in arrival order).
@@ -624,7 +635,9 @@ void dispatcher(lbvh_server *s) {
           ++qi;
           continue;
         }
-        sl.reserve = need.slot_total;
+        sl.bound = need.slot_total;
+        sl.max_build_n = need.max_build_n;
+        sl.max_refit_n = need.max_refit_n;
         Work w;
         w.req = std::move(*qi);
         qi = s->queue.erase(qi);
