@@ -120,12 +120,13 @@ double now_s() {
 // Bytes a resident mesh holds: triangles 48, BVH 64, O_TRI 48, sorted keys
 // 4, permutation 4 per triangle.
 constexpr uint64_t kMeshBytesPerTri = 168;
-// Scratch a slot may need to grow to for a build / refit, with headroom for
-// the arena's rounding: keys and indices 16, box centres 12, sort tile
-// counts and offsets 2 (8 for sorts of at most 2^20 keys, whose tiles hold
-// 1,024 keys), agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf
-// tile, all levels): at most 63.5.
-constexpr uint64_t kBuildScratchPerTri = 72;
+// Scratch a slot may need to grow to for a build / refit.  A build's
+// buffers: keys and indices 16, box centres 12, sort tile counts and
+// offsets 2 (8 for sorts of at most 2^20 keys, whose tiles hold 1,024
+// keys), agglomeration item slots up to 27.5 (128 x 48 B per 256-leaf tile,
+// all levels): at most 63.5.  A refit's: the item slots, 27.5.  A context
+// grows a buffer to 1.25 x what it is asked for (ensure), so 79.4 and 34.4.
+constexpr uint64_t kBuildScratchPerTri = 80;
 constexpr uint64_t kRefitScratchPerTri = 36;
 // plus what does not scale with n: the context's control block, the bounds
 // partials (80 KB with headroom), and tile tables rounded up to whole tiles
