@@ -100,6 +100,27 @@ def test_completion_digests_are_the_oracle_digests(ctx, lb, ora):
         assert done[c]["status"] == 0 and done[c]["digest_bvh"] == 0
 
 
+@pytest.mark.parametrize("seed", [6, 7, 8, 9, 10, 11])
+def test_serving_plan_stays_within_budget(lb, seed):
+    """The serving plan of the test below under other seeds, checked for the
+    budget only: the device memory the server holds (mesh arena and every
+    slot's scratch, counted at each admission) never exceeds it.  A slot's
+    scratch may be trimmed while a request is being admitted to it, and a
+    build and a refit grow different buffers of one context."""
+    requests = 240
+    budget = 3 << 30
+    plan = lb.wl_serving_plan(np.random.default_rng(seed), requests)
+    say(f"serving budget: seed {seed}, {requests} requests")
+    with lb.Server(0, budget, 4) as srv:
+        for kind, m, src, n, s, p in plan:
+            srv.submit(kind, m, src, n, seed=s, param=p)
+        srv.drain()
+        done = srv.poll()
+        st = srv.stats()
+    assert len(done) == requests
+    assert st["evictions"] > 0 and st["peak_bytes"] <= budget, st
+
+
 def test_serving_plan_every_request_matches_oracle(ctx, lb, ora):
     """The serving workload at a reduced request count: the benchmark's size
     law (log-uniform up to 4,194,304 triangles) and 70/30 refit/rebuild mix,

@@ -350,13 +350,19 @@ bool evict_lru(lbvh_server *s, uint64_t keep) {
   return true;
 }
 
-// Frees idle memory until a free arena range holds need.mesh bytes (or a
-// new chunk for it fits the budget) and need.scratch more slot bytes fit the
-// budget: chunks left empty go first when the budget is short, then least
-// recently used idle meshes other than `keep`, then idle slots' scratch.
-// True if the request fits.
-bool make_room(lbvh_server *s, const Need &need, uint64_t keep) {
+// Frees idle memory until request r fits on slot sl: a free arena range
+// holds its mesh (or a new chunk for it fits the budget) and its slot
+// scratch fits the budget.  Chunks left empty go first when the budget is
+// short, then least recently used idle meshes other than r's, then idle
+// slots' scratch.  The need is recomputed after every step (a trim of sl
+// itself changes it); *out gets the need the request fits with.  True if it
+// fits.
+bool make_room(lbvh_server *s, const Request &r, Slot &target, Need *out) {
+  const uint64_t keep = r.mesh_id;
   while (true) {
+    const Need need = request_need(s, r, target);
+    *out = need;
+    if (!need.any()) return true;
     const bool fit = !need.mesh || s->arena.can_fit(need.mesh);
     const uint64_t grow =
         fit ? 0 : MeshArena::chunk_for(need.mesh, s->budget);
@@ -629,8 +635,8 @@ void dispatcher(lbvh_server *s) {
       int st = check_request(s, *qi);
       if (st == LBVH_OK) {
         Slot &sl = s->slots[k];
-        const Need need = request_need(s, *qi, sl);
-        if (need.any() && (memory_blocked || !make_room(s, need, qi->mesh_id))) {
+        Need need = request_need(s, *qi, sl);
+        if (need.any() && (memory_blocked || !make_room(s, *qi, sl, &need))) {
           memory_blocked = true;
           held.insert(qi->mesh_id);
           ++qi;
