@@ -97,10 +97,6 @@ __device__ __forceinline__ int4 load_nt(const int4 *p) {
   const nt_i4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_i4 *>(p));
   return make_int4(v.x, v.y, v.z, v.w);
 }
-__device__ __forceinline__ void store_nt(uint4 *p, uint4 v) {
-  __builtin_nontemporal_store(nt_u4{v.x, v.y, v.z, v.w},
-                              reinterpret_cast<nt_u4 *>(p));
-}
 
 // Triangles in chunks of kMapThreads: a chunk's 3 * kMapThreads float4 are
 // loaded contiguously (each load instruction covers 4 KB, where a triangle

@@ -58,11 +58,6 @@ struct Digits {
   static_assert(kDpt >= 1 && kR % kSortThreads == 0, "whole digits per thread");
 };
 
-// A digit of at most kFewBits significant bits (the last one of Morton
-// keys) takes at most kFewVals values, and few of them per tile: counters
-// that the lanes of a wave share would serialise its atomics.  Each lane
-// then counts into a row of its own (rows of kFewVals + 1 words, so lanes
-// adding to one digit hit distinct banks), and the block sums the rows.
 // Tile loads of a large sort are non-temporal: the keys and payloads are
 // read once per pass, from a buffer far larger than the L2s, and would
 // push out lines that are still read.  A sort of small tiles (at most
@@ -71,6 +66,11 @@ struct Digits {
 template <int kItems>
 constexpr bool kStreamNt = kItems == kSortItems;
 
+// A digit of at most kFewBits significant bits (the last one of Morton
+// keys) takes at most kFewVals values, and few of them per tile: counters
+// that the lanes of a wave share would serialise its atomics.  Each lane
+// then counts into a row of its own (rows of kFewVals + 1 words, so lanes
+// adding to one digit hit distinct banks), and the block sums the rows.
 constexpr int kFewBits = 6;
 constexpr int kFewVals = 1 << kFewBits;
 constexpr int kFewRow = kFewVals + 1;
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(kSortThreads) void k_count(
     const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + r.begin);
 #pragma unroll
     for (int q = 0; q < kItems / 4; ++q) {
-      const uint4 v = kStreamNt<kItems> ? load_nt(k4 + threadIdx.x + q * kSortThreads)
-                                         : k4[threadIdx.x + q * kSortThreads];
+      const uint4 *src = k4 + threadIdx.x + q * kSortThreads;
+      const uint4 v = kStreamNt<kItems> ? load_nt(src) : *src;
       k[4 * q] = v.x;
       k[4 * q + 1] = v.y;
       k[4 * q + 2] = v.z;
@@ -567,10 +567,10 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
 
   // Write out: consecutive threads take consecutive tile positions, so each
   // digit run goes to consecutive addresses.  Plain stores: the line at each
-  // end of a run is shared with a neighbouring tile's run, and the two halves
-  // merge in L2 (non-temporal stores took pass 1 from 62 to 187 us).  Positions outside the segment
-  // cannot occur when the counts are right; the guard keeps a corrupted
-  // count from writing outside it.
+  // end of a run is shared with a neighbouring tile's run, and the two
+  // halves merge in L2 (non-temporal stores took pass 1 from 62 to 187 us).
+  // Positions outside the segment cannot occur when the counts are right;
+  // the guard keeps a corrupted count from writing outside it.
   const uint32_t tile_n = r.end - r.begin;
   for (uint32_t p = tid; p < tile_n; p += kSortThreads) {
     const uint32_t k = s_keys[p];
