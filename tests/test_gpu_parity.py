@@ -577,6 +577,67 @@ def test_repeated_builds_into_the_same_buffers(ctx, lb, ora, n, bounds_from):
             x.destroy()
 
 
+def sweep_mesh(lb, rng, n):
+    """A mesh of about n triangles, of a kind drawn from rng: uniform boxes of
+    a random size, clustered with a random duplicate share, a small
+    heightfield, or a handful of distinct triangles repeated."""
+    kind = ("uniform", "clustered", "heightfield", "repeated")[rng.integers(4)]
+    seed = int(rng.integers(1 << 30))
+    if kind == "uniform":
+        tri = lb.wl_uniform(seed, n, float(10 ** rng.uniform(-4, -1)))
+    elif kind == "clustered":
+        tri = lb.wl_clustered(seed, n, float(rng.uniform(0.0, 0.5)))
+    elif kind == "heightfield":
+        nx = max(1, int(np.sqrt(n / 2) * rng.uniform(0.5, 2.0)))
+        tri = lb.wl_heightfield(seed, nx, max(1, n // (2 * nx)))
+    else:
+        base = lb.wl_uniform(seed, int(rng.integers(1, 9)), 0.05)
+        tri = base[rng.integers(0, base.shape[0], size=n)]
+    return kind, np.ascontiguousarray(tri)
+
+
+@pytest.mark.parametrize("seed", range(32))
+def test_build_sweep_matches_oracle(ctx, lb, ora, seed):
+    """Seeded draws over what the fused build's code paths depend on: the
+    size (log-uniform over 1 .. 2,000,000, so tile, sort-plan and item-level
+    boundaries fall anywhere), the mesh kind, the leaf tile, the digit
+    width, the division mode, and the bounds (GPU, the mesh's own, a box
+    narrower than the mesh so keys saturate, or a wider one).  Keys,
+    permutation, nodes and O_TRI must be the oracle's, bit for bit."""
+    rng = np.random.default_rng(4000 + seed)
+    n = max(1, int(round(np.exp(rng.uniform(0.0, np.log(2000000.0))))))
+    kind, tri = sweep_mesh(lb, rng, n)
+    n = tri.shape[0]
+    tile = int(rng.choice([256, 512, 1024]))
+    bits = int(rng.choice([8, 10]))
+    div_mode = int(rng.integers(2))
+    bounds_from = ("gpu", "own", "narrow", "wide")[rng.integers(4)]
+    b = None
+    if bounds_from != "gpu":
+        own = lb.wl_bounds(tri).astype(np.float64)
+        mid, half = (own[:3] + own[3:]) / 2, (own[3:] - own[:3]) / 2
+        scale = {"own": None, "narrow": 0.8, "wide": 1.5}[bounds_from]
+        b = own if scale is None else np.concatenate([mid - scale * half,
+                                                      mid + scale * half])
+        b = b.astype(np.float32)
+    say(f"sweep seed={seed} n={n} kind={kind} tile={tile} bits={bits} "
+        f"div={div_mode} bounds={bounds_from}")
+    ctx.set_tuning(lb.TUNE_TREE_TILE, tile)
+    ctx.set_tuning(lb.TUNE_SORT_BITS, bits)
+    ctx.set_div_mode(div_mode)
+    try:
+        bvh, otri, sk, pm = ctx.build_arrays(tri, b, keep_topology=True)
+    finally:
+        ctx.set_tuning(lb.TUNE_TREE_TILE, 0)
+        ctx.set_tuning(lb.TUNE_SORT_BITS, 0)
+        ctx.set_div_mode(0)
+    wb, wo, wsk, wpm = ora.build(tri, b, div_mode)
+    assert np.array_equal(sk, wsk)
+    assert np.array_equal(pm, wpm)
+    assert np.array_equal(bvh, wb)
+    assert np.array_equal(otri.view(np.uint32), wo.view(np.uint32))
+
+
 def test_build_is_deterministic(ctx, lb):
     tri = lb.workload("uniform-64k")
     first = ctx.build_arrays(tri, None)[0]
