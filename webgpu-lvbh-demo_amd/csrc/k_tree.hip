@@ -64,8 +64,13 @@ constexpr int kRtPer = 16;                          // gaps per thread
 constexpr int kRtTile = kRtThreads * kRtPer;        // 4096 gaps per tile
 constexpr int kRtSegs = kRtTile / 64;               // 64-gap segments
 constexpr int kRtRecords = 128;                     // spine gaps per tile
-constexpr int kRtPlanes = 7;                        // delta bits (<= 64)
-constexpr int kRtNone = 64;                         // "no gap" delta
+// Deltas are at most 63 (32 + clz of (g xor g + 1) when the keys are equal),
+// so six bit planes hold them.  A gap past the end takes 63 too: no lane u
+// (a value 0 .. 63) counts it as below u, so it never bounds a real gap, it
+// is skipped before it writes anything, and a segment of such gaps has
+// minimum 63, below no u.
+constexpr int kRtPlanes = 6;                        // delta bits
+constexpr int kRtNone = 63;                         // "no gap" delta
 constexpr int kRtScanThreads = 256;
 
 // Lanes of a segment whose delta is below v, from its bit planes: from the
@@ -187,11 +192,10 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_tiles(
     const int64_t g = base + i;
     const int v = dv[j];
     // every lane takes part in the exchange (a gap past the end asks lane 63)
-    const int src = v < 64 ? v : 63;
     const uint64_t tu = below_u[j];
     const uint64_t lt =
-        (uint64_t)(uint32_t)__shfl((int)(uint32_t)tu, src, 64) |
-        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(tu >> 32), src, 64) << 32);
+        (uint64_t)(uint32_t)__shfl((int)(uint32_t)tu, v, 64) |
+        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(tu >> 32), v, 64) << 32);
     if (g >= m) continue;
     const uint64_t ml = lt & below_me, mr = lt & above_me;
     // tile-local bounds, -1: not in the tile

@@ -376,16 +376,20 @@ bool make_room(lbvh_server *s, const Request &r, Slot &target, Need *out) {
     }
     if (evict_lru(s, keep)) continue;
     if (s->arena.release_empty()) continue;
+    // A trim is progress only if it released bytes or a bound, so the loop
+    // ends even if a context keeps more than its control block.
     bool trimmed = false;
     for (auto &sl : s->slots)
       if (sl.idle() && (sl.scratch_bytes > sizeof(lbvh::Ctl) || sl.bound)) {
         // an idle slot's worker waits for work that only this thread
         // hands out, so it does not touch the context meanwhile
+        const uint64_t before = sl.scratch_bytes;
+        const bool bounded = sl.bound != 0;
         lbvh_trim(sl.ctx);
         sl.scratch_bytes = lbvh_bytes_in_use(sl.ctx);
         sl.bound = 0;
         sl.max_build_n = sl.max_refit_n = 0;
-        trimmed = true;
+        if (bounded || sl.scratch_bytes < before) trimmed = true;
       }
     if (!trimmed) return false;
   }
