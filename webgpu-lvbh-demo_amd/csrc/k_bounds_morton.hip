@@ -22,9 +22,10 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Block-wide min/max of 6 values; result valid in thread 0.
+// Block-wide min/max of 6 values (up to 1,024 threads); result valid in
+// thread 0.
 __device__ void block_minmax(float v[6]) {
-  __shared__ float s[4][6];
+  __shared__ float s[16][6];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -118,7 +119,10 @@ __device__ void fold_partials(const float4 *__restrict__ partials,
   block_minmax(w);
 }
 
-__global__ __launch_bounds__(kMapThreads) void k_bounds_final(
+// One block of kFoldThreads: two partials per thread at the largest grid,
+// so the fold waits on one round of loads.
+constexpr int kFoldThreads = 1024;
+__global__ __launch_bounds__(kFoldThreads) void k_bounds_final(
     const float4 *__restrict__ partials, uint32_t blocks, Ctl *ctl) {
   float w[6];
   fold_partials(partials, blocks, w);
@@ -335,7 +339,7 @@ uint32_t launch_bounds(const float4 *tri, uint32_t n, float4 *partials,
   hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kMapThreads), 0, s, tri, n,
                      partials, ctl, cen);
   if (blocks <= fold_max) return blocks;  // the consumer folds the partials
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kMapThreads), 0, s,
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kFoldThreads), 0, s,
                      partials, blocks, ctl);
   return 0;
 }

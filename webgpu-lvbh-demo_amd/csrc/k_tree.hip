@@ -71,7 +71,7 @@ constexpr int kRtRecords = 128;                     // spine gaps per tile
 // minimum 63, below no u.
 constexpr int kRtPlanes = 6;                        // delta bits
 constexpr int kRtNone = 63;                         // "no gap" delta
-constexpr int kRtScanThreads = 256;
+constexpr int kRtScanThreads = 1024;  // four tiles per thread at 16M
 
 // Lanes of a segment whose delta is below v, from its bit planes: from the
 // high bit down, lanes equal to v so far drop out where their bit differs,
